@@ -1,0 +1,333 @@
+"""cloud-renderer_amd: MI355X (gfx950) billboard-cloud voxelize / mip / cone-trace path.
+
+Python host layer over the C ABI in include/cloudvox.h (libcloudvox.so).  The
+classes mirror the reference's host objects so that code written against
+Cloud-Renderer's render passes reads the same:
+
+  CloudVolume      src/CloudVolume.hpp:12-55   (position, x/y/zBounds, dimension,
+                                                levels, voxelSize, billboards, fluffiness)
+  Sun              src/Sun.hpp:12-19           (position; plus a directional flag)
+  Camera           src/Camera.hpp:25-45        (eye/target/up, fovy, near/far)
+  VoxelizeShader   src/Shaders/VoxelizeShader.hpp:18   voxelize(volume)
+  ConeTraceShader  src/Shaders/ConeTraceShader.hpp:12  coneTrace(volume), with the
+                   vct* parameters of src/Shaders/ConeTraceShader.hpp:28-32
+
+All work runs on the GPU through the HIP library; there is no CPU fallback.
+Output images are device buffers (a torch CUDA tensor or a raw device pointer).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import (CR_COMPOSITE_FTB, CR_COMPOSITE_GL_BLEND, CR_CONE_FTB, CR_CONE_FTB_DENSITY,  # noqa: F401
+                   CR_CONE_WEIGHTED, CR_FORMAT_R8, CR_FORMAT_RGBA16F, CR_FORMAT_RGBA32F, CR_MODE_REF,
+                   CR_MODE_RGBA, CR_PIXEL_RGBA8, CR_PIXEL_RGBA32F, CloudVoxError, cr_camera, cr_cone_params,
+                   cr_light, cr_stats)
+
+__all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "CloudVoxError",
+           "lib", "scenes"]
+
+
+def lib():
+    return _abi.lib()
+
+
+def _check(rc: int, ctx_handle=None) -> None:
+    if rc != 0:
+        msg = lib().cr_last_error(ctx_handle)
+        raise CloudVoxError(rc, msg.decode() if msg else "")
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+def _f3(v):
+    return _abi.F3(*[float(x) for x in v])
+
+
+class Context:
+    """One HIP device (cr_context)."""
+
+    def __init__(self, device: int = 0, profiling: bool = False):
+        h = C.c_void_p()
+        _check(lib().cr_context_create(int(device), 0, C.byref(h)))
+        self.handle = h
+        self.device = device
+        if profiling:
+            self.set_profiling(True)
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().cr_context_set_profiling(self.handle, 1 if on else 0), self.handle)
+
+    def stats(self) -> dict:
+        s = cr_stats()
+        _check(lib().cr_get_stats(self.handle, C.byref(s)), self.handle)
+        return {name: getattr(s, name) for name, _ in cr_stats._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            lib().cr_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class Billboards:
+    """CloudVolume::Billboards (src/CloudVolume.hpp:14-23)."""
+    positions: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    scales: np.ndarray = field(default_factory=lambda: np.zeros((0,), np.float32))
+
+    @property
+    def count(self) -> int:
+        return int(self.positions.shape[0])
+
+
+class CloudVolume:
+    """CloudVolume(dim, bounds, position, mips) (src/CloudVolume.cpp:7-55) on the GPU.
+
+    bounds is (lo, hi) applied to x, y and z like the reference's glm::vec2, or a
+    pair of 3-vectors.  fmt: CR_FORMAT_*; mode: CR_MODE_REF (shipped shaders) or
+    CR_MODE_RGBA (chord fill + lit shell).
+    """
+
+    def __init__(self, dim: int, bounds=(-1.0, 1.0), position=(0.0, 0.0, 0.0), mips: int = 4,
+                 fmt: int = CR_FORMAT_RGBA32F, mode: int = CR_MODE_RGBA, context: Optional[Context] = None):
+        self.ctx = context or default_context()
+        b = np.asarray(bounds, np.float32)
+        if b.shape == (2,):
+            lo, hi = np.full(3, b[0], np.float32), np.full(3, b[1], np.float32)
+        else:
+            lo, hi = b[0].astype(np.float32), b[1].astype(np.float32)
+        self.position = np.asarray(position, np.float32)
+        self.xBounds = (float(lo[0]), float(hi[0]))
+        self.yBounds = (float(lo[1]), float(hi[1]))
+        self.zBounds = (float(lo[2]), float(hi[2]))
+        self.dimension = int(dim)
+        self.levels = int(mips)
+        self.format = int(fmt)
+        self.mode = int(mode)
+        rng = hi - lo
+        self.range = rng
+        self.voxelSize = rng / np.float32(dim)
+        self.billboards = Billboards()
+        self.fluffiness = 1.0
+        h = C.c_void_p()
+        lo3, hi3, p3 = _f3(lo), _f3(hi), _f3(self.position)
+        _check(lib().cr_volume_create(self.ctx.handle, self.dimension, lo3, hi3, p3, self.levels, self.format,
+                                      self.mode, C.byref(h)), self.ctx.handle)
+        self.handle = h
+
+    # -- billboards -------------------------------------------------------
+    def set_billboards(self, positions, scales) -> None:
+        self.billboards = Billboards(np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+                                     np.ascontiguousarray(scales, np.float32).reshape(-1))
+        self.update()
+
+    def update(self) -> None:
+        """CloudVolume::update -> uploadBillboards (src/CloudVolume.cpp:84-93, 139-164)."""
+        bb = self.billboards
+        if bb.positions.shape[0] != bb.scales.shape[0]:
+            raise ValueError("positions and scales differ in length")
+        xyz = np.ascontiguousarray(bb.positions, np.float32)
+        sc = np.ascontiguousarray(bb.scales, np.float32)
+        _check(lib().cr_volume_set_billboards(self.handle, _abi.fptr(xyz), _abi.fptr(sc), bb.count,
+                                              float(self.fluffiness)), self.ctx.handle)
+
+    def set_noise(self, octaves=4, seed=0, base_freq=4.0, lacunarity=2.0, gain=0.5) -> None:
+        _check(lib().cr_volume_set_noise(self.handle, int(octaves), int(seed) & 0xffffffff, float(base_freq),
+                                         float(lacunarity), float(gain)), self.ctx.handle)
+
+    # -- readback ----------------------------------------------------------
+    def occupancy(self, stream=None):
+        cnt, h = C.c_uint64(), C.c_uint64()
+        _check(lib().cr_volume_occupancy(self.handle, C.byref(cnt), C.byref(h), _stream_handle(stream)),
+               self.ctx.handle)
+        return int(cnt.value), int(h.value)
+
+    def level_dim(self, k: int) -> int:
+        return max(1, self.dimension >> k)
+
+    def read_level(self, k: int, stream=None) -> np.ndarray:
+        n = self.level_dim(k)
+        if self.format == CR_FORMAT_R8:
+            out = np.empty((n, n, n), np.uint8)
+        elif self.format == CR_FORMAT_RGBA32F:
+            out = np.empty((n, n, n, 4), np.float32)
+        else:
+            out = np.empty((n, n, n, 4), np.uint16)
+        _check(lib().cr_volume_read_level(self.handle, k, out.ctypes.data_as(C.c_void_p), out.nbytes,
+                                          _stream_handle(stream)), self.ctx.handle)
+        return out  # indexed [z, y, x, (c)]
+
+    def read_occupancy(self, stream=None) -> np.ndarray:
+        n = self.dimension
+        out = np.empty((n, n, n), np.uint8)
+        _check(lib().cr_volume_read_occupancy(self.handle, out.ctypes.data_as(C.c_void_p), out.nbytes,
+                                              _stream_handle(stream)), self.ctx.handle)
+        return out
+
+    def read_position_map(self, w: int, h: int, stream=None) -> np.ndarray:
+        out = np.empty((h, w, 4), np.float32)
+        _check(lib().cr_volume_read_position_map(self.handle, out.ctypes.data_as(C.c_void_p), out.nbytes,
+                                                 _stream_handle(stream)), self.ctx.handle)
+        return out
+
+    def read_draw_order(self, stream=None) -> np.ndarray:
+        out = np.empty(max(self.billboards.count, 1), np.int32)
+        _check(lib().cr_volume_read_draw_order(self.handle, out.ctypes.data_as(C.c_void_p), out.shape[0],
+                                               _stream_handle(stream)), self.ctx.handle)
+        return out[:self.billboards.count]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().cr_volume_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Sun:
+    """Sun (src/Sun.hpp:12-19).  position drives the light view; the cone goes
+    toward `position` (point light, reference) or along `direction`."""
+    position: tuple = (5.0, 20.0, -5.0)   # src/main.cpp:37
+    direction: tuple = (0.0, 1.0, 0.0)
+    directional: bool = False
+
+    def to_c(self) -> cr_light:
+        return cr_light(_f3(self.position), _f3(self.direction), 1 if self.directional else 0)
+
+
+@dataclass
+class Camera:
+    eye: tuple = (0.0, 0.0, 3.0)
+    target: tuple = (0.0, 0.0, 0.0)
+    up: tuple = (0.0, 1.0, 0.0)
+    fovy: float = math.radians(60.0)
+    znear: float = 0.01
+    zfar: float = 2500.0
+
+    def to_c(self) -> cr_camera:
+        return cr_camera(_f3(self.eye), _f3(self.target), _f3(self.up), float(self.fovy), float(self.znear),
+                         float(self.zfar))
+
+
+class VoxelizeShader:
+    """VoxelizeShader::voxelize (src/Shaders/VoxelizeShader.cpp:18-31).  The
+    position map is light_w x light_h (the reference uses the window size)."""
+
+    def __init__(self, light_w: int = 1280, light_h: int = 720):
+        self.width = light_w
+        self.height = light_h
+
+    def voxelize(self, volume: CloudVolume, sun: Sun, stream=None) -> None:
+        light = sun.to_c()
+        _check(lib().cr_voxelize(volume.handle, C.byref(light), int(self.width), int(self.height),
+                                 _stream_handle(stream)), volume.ctx.handle)
+
+
+class ConeTraceShader:
+    """ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82)."""
+
+    def __init__(self, **kw):
+        # src/Shaders/ConeTraceShader.hpp:28-32
+        self.vctSteps = 16
+        self.vctConeAngle = 0.9
+        self.vctConeInitialHeight = 0.1
+        self.vctLodOffset = 0.0
+        self.vctDownScaling = 1.0
+        self.coneMode = CR_CONE_WEIGHTED
+        self.alphaStop = 0.99
+        self.sigma = 24.0
+        self.composite = CR_COMPOSITE_GL_BLEND
+        self.quantizeRGBA8 = True
+        self.compensated = False
+        self.clearColor = (0.2, 0.3, 0.5, 1.0)   # src/main.cpp:112
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(k)
+            setattr(self, k, v)
+
+    def params(self) -> cr_cone_params:
+        return cr_cone_params(int(self.vctSteps), float(self.vctConeAngle), float(self.vctConeInitialHeight),
+                              float(self.vctLodOffset), float(self.vctDownScaling), int(self.coneMode),
+                              float(self.alphaStop), float(self.sigma), int(self.composite),
+                              1 if self.quantizeRGBA8 else 0, 1 if self.compensated else 0,
+                              _abi.F4(*[float(c) for c in self.clearColor]))
+
+    def coneTrace(self, volume: CloudVolume, sun: Sun, camera: Camera, image, width: int, height: int,
+                  pixel_format: int = CR_PIXEL_RGBA32F, pitch: Optional[int] = None, stream=None) -> None:
+        """Render into `image`: a torch CUDA tensor (H, W, 4) or a raw device pointer (int)."""
+        if hasattr(image, "data_ptr"):
+            ptr = image.data_ptr()
+            if pitch is None:
+                pitch = image.stride(0) * image.element_size()
+            if stream is None:
+                import torch
+                stream = torch.cuda.current_stream(image.device)
+        else:
+            ptr = int(image)
+            if pitch is None:
+                pitch = width * (16 if pixel_format == CR_PIXEL_RGBA32F else 4)
+        light = sun.to_c()
+        cam = camera.to_c()
+        p = self.params()
+        _check(lib().cr_cone_trace(volume.handle, C.byref(light), C.byref(cam), C.byref(p), C.c_void_p(ptr),
+                                   int(width), int(height), C.c_size_t(pitch), int(pixel_format),
+                                   _stream_handle(stream)), volume.ctx.handle)
+
+
+from . import scenes  # noqa: E402
+
+
+def scene_objects(scene, camera_index: int = 0, context: Optional[Context] = None):
+    """Build (volume, sun, camera, voxelizer, tracer) for a scenes.Scene."""
+    vol = CloudVolume(scene.N, (scene.bounds_lo, scene.bounds_hi), scene.position, scene.levels, scene.fmt,
+                      scene.mode, context=context)
+    vol.fluffiness = scene.fluffiness
+    vol.set_billboards(scene.xyz, scene.scales)
+    if scene.noise is not None:
+        n = scene.noise
+        vol.set_noise(n.octaves, n.seed, n.base_freq, n.lacunarity, n.gain)
+    sun = Sun(position=tuple(float(x) for x in scene.sun_pos), direction=tuple(float(x) for x in scene.sun_dir),
+              directional=bool(scene.directional))
+    c = scene.cameras[camera_index]
+    cam = Camera(eye=c.eye, target=c.target, up=c.up, fovy=c.fovy, znear=c.znear, zfar=c.zfar)
+    vox = VoxelizeShader(scene.light_w, scene.light_h)
+    k = scene.cone
+    tr = ConeTraceShader(vctSteps=k.steps, vctConeAngle=k.cone_angle, vctConeInitialHeight=k.initial_height,
+                         vctLodOffset=k.lod_offset, vctDownScaling=k.down_scaling, coneMode=k.cone_mode,
+                         alphaStop=k.alpha_stop, sigma=k.sigma, composite=k.composite,
+                         quantizeRGBA8=bool(k.quantize_rgba8), compensated=bool(k.compensated),
+                         clearColor=k.clear_color)
+    return vol, sun, cam, vox, tr

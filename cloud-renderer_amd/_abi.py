@@ -1,0 +1,117 @@
+"""ctypes view of the C ABI in include/cloudvox.h.
+
+The library is loaded from this package directory.  There is no fallback:
+if libcloudvox.so is missing or a symbol is absent, import fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libcloudvox.so"
+
+CR_OK = 0
+CR_EINVAL = -1
+CR_ENOMEM = -2
+CR_EDEVICE = -3
+CR_ESTATE = -4
+CR_EUNSUPPORTED = -5
+
+CR_FORMAT_R8 = 0
+CR_FORMAT_RGBA32F = 1
+CR_FORMAT_RGBA16F = 2
+
+CR_MODE_REF = 0
+CR_MODE_RGBA = 1
+
+CR_CONE_WEIGHTED = 0
+CR_CONE_FTB = 1
+CR_CONE_FTB_DENSITY = 2
+
+CR_COMPOSITE_GL_BLEND = 0
+CR_COMPOSITE_FTB = 1
+
+CR_PIXEL_RGBA32F = 0
+CR_PIXEL_RGBA8 = 1
+
+F3 = C.c_float * 3
+F4 = C.c_float * 4
+
+
+class cr_light(C.Structure):
+    _fields_ = [("position", F3), ("direction", F3), ("directional", C.c_int)]
+
+
+class cr_camera(C.Structure):
+    _fields_ = [("eye", F3), ("target", F3), ("up", F3), ("fovy", C.c_float),
+                ("znear", C.c_float), ("zfar", C.c_float)]
+
+
+class cr_cone_params(C.Structure):
+    _fields_ = [("steps", C.c_int), ("cone_angle", C.c_float), ("initial_height", C.c_float),
+                ("lod_offset", C.c_float), ("down_scaling", C.c_float), ("cone_mode", C.c_int),
+                ("alpha_stop", C.c_float), ("sigma", C.c_float), ("composite", C.c_int),
+                ("quantize_rgba8", C.c_int), ("compensated", C.c_int), ("clear_color", F4)]
+
+
+class cr_stats(C.Structure):
+    _fields_ = [("light_fragments", C.c_int64), ("shell_texels", C.c_int64), ("fragments", C.c_int64),
+                ("samples", C.c_int64), ("two_level", C.c_int64),
+                ("ms_clear", C.c_float), ("ms_light_raster", C.c_float), ("ms_splat", C.c_float),
+                ("ms_mip", C.c_float), ("ms_setup", C.c_float), ("ms_sort", C.c_float),
+                ("ms_bin", C.c_float), ("ms_cone", C.c_float)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/cloudvox.h
+SIGNATURES = {
+    "cr_abi_version": (C.c_int, []),
+    "cr_last_error": (C.c_char_p, [C.c_void_p]),
+    "cr_context_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(C.c_void_p)]),
+    "cr_context_destroy": (C.c_int, [C.c_void_p]),
+    "cr_context_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cr_get_stats": (C.c_int, [C.c_void_p, C.POINTER(cr_stats)]),
+    "cr_volume_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "cr_volume_destroy": (C.c_int, [C.c_void_p]),
+    "cr_volume_set_noise": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "cr_volume_set_billboards": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                           C.c_float]),
+    "cr_voxelize": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.c_int, C.c_int, C.c_void_p]),
+    "cr_cone_trace": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_camera), C.POINTER(cr_cone_params),
+                                C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_void_p]),
+    "cr_volume_occupancy": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
+    "cr_volume_read_level": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "cr_volume_read_occupancy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "cr_volume_read_position_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "cr_volume_read_draw_order": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libcloudvox.so (built by cloud-renderer_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python cloud-renderer_amd/build.py` "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def fptr(a):
+    """float* view of a contiguous float32 numpy array."""
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class CloudVoxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cloudvox error {code}: {msg}")
+        self.code = code

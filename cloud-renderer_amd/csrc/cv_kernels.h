@@ -1,0 +1,114 @@
+// Shared declarations between the host runtime (cv_runtime.hip) and the
+// gfx950 kernels (cv_kernels.hip).  Plain structs passed by value as kernel
+// arguments; no pointers to host memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cv {
+
+constexpr int kMaxLevels = 16;
+constexpr int kMaxSteps = 256;
+constexpr int kTile = 8;           // camera tiles are 8x8 pixels = one wavefront
+constexpr int kLdsSortMax = 16384; // billboards sorted in one workgroup's LDS
+
+// One mip level in the bricked layout: 4x4x4 bricks, texels Morton-ordered
+// inside a brick (bit order x0 y0 z0 x1 y1 z1), bricks x-fastest.
+struct LevelDesc {
+    uint32_t n;      // texels per side
+    uint32_t nb;     // bricks per side = ceil(n / 4)
+    uint64_t base;   // first texel of the level (in texels)
+};
+
+// Voxel grid of the volume (world bounds, index map)
+struct GridDesc {
+    float lo[3];     // world bounds low corner = position + bounds_lo
+    float range[3];  // world bounds extent
+    float vpos[3];   // CloudVolume::position (billboard centres are relative)
+    float stepSize;  // min(voxelSize)
+    int N;           // voxels per side
+    uint32_t nb0;    // level-0 bricks per side
+};
+
+// Light view of pass 1 (Sun::update output + derived billboard basis)
+struct LightView {
+    float V[16];
+    float S[3], U[3], B[3];
+    float np[3];
+    float clip;
+    float lsx, lox, lsy, loy;  // x_view = xn * lsx + lox
+    float znear, zfar;
+    int W, H;
+};
+
+struct VolumeDesc {
+    LevelDesc lv[kMaxLevels];
+    int L;
+    int format;    // CR_FORMAT_*
+    int mode;      // CR_MODE_*
+    int noise_octaves;
+    uint32_t noise_seed;
+    float noise_base_freq, noise_lacunarity, noise_gain;
+};
+
+struct CameraView {
+    float V[16];
+    float S[3], U[3], B[3];
+    float eye[3];
+    float kx, ky;      // tan(fovy/2) * aspect, tan(fovy/2)
+    float znear, zfar;
+    int W, H;
+    int tiles_x, tiles_y;
+};
+
+struct ConeDesc {
+    float h[kMaxSteps];    // cone height before step i (voxels)
+    float lam[kMaxSteps];  // lod + lodOffset at step i
+    int steps;
+    float tanHalf;
+    float den;             // steps * downScaling
+    float alpha_stop;
+    float kscale;          // sigma * log2(e) / N
+    int cone_mode;
+    int composite;
+    int quantize;
+    int compensated;
+    float clear[4];
+    float lpos[3];
+    float ldir[3];         // normalize(direction) / N
+    int directional;
+    int pixfmt;
+    uint64_t pitch;
+};
+
+// ---- launch wrappers (cv_kernels.hip) ----
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s);
+void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
+                         unsigned long long* keys, unsigned long long* fillbits, unsigned long long* counters,
+                         hipStream_t s);
+void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards,
+                        const unsigned long long* keys, float4* posmap, unsigned long long* shellbits,
+                        unsigned long long* counters, hipStream_t s);
+void launch_level0(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
+                   const unsigned long long* fillbits, void* levels, hipStream_t s);
+void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s);
+void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boards, int S, float4* sph_world,
+                      float4* sph_view, int4* sph_tiles, unsigned long long* keys, hipStream_t s);
+int launch_sort(unsigned long long* keys, int S, const float4* sph_world, const CameraView& cv, int32_t* order,
+                int* tie_flag, hipStream_t s);
+void launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
+                unsigned long long* masks, hipStream_t s);
+void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
+                 const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
+                 const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                 hipStream_t s);
+void launch_occupancy(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,
+                      unsigned long long* out2, hipStream_t s);
+void launch_read_state(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,
+                       uint8_t* out, hipStream_t s);
+void launch_debrick(const VolumeDesc& vd, int k, const void* levels, void* out, hipStream_t s);
+
+size_t texel_bytes(int format);
+
+}  // namespace cv

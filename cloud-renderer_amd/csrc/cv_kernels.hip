@@ -1,0 +1,1022 @@
+// gfx950 kernels of the billboard voxelize -> mip -> cone-trace path.
+//
+// Every floating-point expression is written in the operation order of the
+// reference shaders (cited per kernel) and the file is compiled with
+// -ffp-contract=off and IEEE division/sqrt, so FP32 results are reproducible
+// bit for bit on the host.
+//
+// Memory layout (HBM):
+//   * volume levels: 4x4x4 bricks, Morton order inside a brick (x0 y0 z0 x1 y1 z1),
+//     bricks x-fastest.  A 2x2x2 trilinear footprint is one 128 B line for
+//     RGBA32F texels whenever it is brick-aligned, and the 8 corner addresses
+//     are sums of per-axis terms.
+//   * occupancy: one 64-bit word per level-0 brick for the lit shell and one
+//     for the chord fill, set with integer atomicOr (order-free, idempotent).
+//   * pass-1 depth: one 64-bit key per light texel, (depth bits << 32 | draw
+//     index) reduced with atomicMin: nearest wins, ties go to the first drawn
+//     billboard exactly like the GL depth test LESS.
+//   * camera binning: per 8x8 tile a bitmask over draw ranks, so a tile's
+//     fragments are visited in draw order without a per-tile sort.
+
+#include "cv_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace cv {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread2(uint32_t v) { return (v & 1u) | ((v & 2u) << 2); }
+__device__ __forceinline__ uint32_t morton6(uint32_t x, uint32_t y, uint32_t z) {
+    return spread2(x) | (spread2(y) << 1) | (spread2(z) << 2);
+}
+__device__ __forceinline__ void demorton6(uint32_t m, uint32_t* x, uint32_t* y, uint32_t* z) {
+    *x = (m & 1u) | ((m >> 2) & 2u);
+    *y = ((m >> 1) & 1u) | ((m >> 3) & 2u);
+    *z = ((m >> 2) & 1u) | ((m >> 4) & 2u);
+}
+
+__device__ __forceinline__ float h2f(uint32_t bits) {
+    unsigned short s = (unsigned short)bits;
+    _Float16 h;
+    __builtin_memcpy(&h, &s, 2);
+    return (float)h;
+}
+__device__ __forceinline__ uint32_t f2h(float f) {
+    _Float16 h = (_Float16)f;  // v_cvt_f16_f32, round to nearest even
+    unsigned short s;
+    __builtin_memcpy(&s, &h, 2);
+    return s;
+}
+
+// pixel centre -> NDC: (2 * (p + 0.5)) / n - 1
+__device__ __forceinline__ float pixel_ndc(int p, int n) { return (2.0f * ((float)p + 0.5f)) / (float)n - 1.0f; }
+
+// sphereContrib (res/first_voxelize.glsl:42-48, res/conetrace_frag.glsl:177-182)
+__device__ __forceinline__ float sphere_contrib(float du, float dv, float s) {
+    float dist = sqrtf(du * du + dv * dv);
+    float q = dist / s;
+    return sqrtf(fmaxf(0.0f, 1.0f - q * q));
+}
+
+// calculateVoxelIndex with image-store bounds (res/first_voxelize.glsl:23-37)
+__device__ __forceinline__ bool voxel_word_bit(const GridDesc& g, float px, float py, float pz, uint32_t* word,
+                                               uint32_t* bit) {
+    float fN = (float)g.N;
+    float lx = fN * ((px - g.lo[0]) / g.range[0]);
+    float ly = fN * ((py - g.lo[1]) / g.range[1]);
+    float lz = fN * ((pz - g.lo[2]) / g.range[2]);
+    if (!(lx > -1.0f && lx < fN && ly > -1.0f && ly < fN && lz > -1.0f && lz < fN)) return false;
+    uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
+    *word = ((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2);
+    *bit = morton6(ix & 3u, iy & 3u, iz & 3u);
+    return true;
+}
+
+__device__ __forceinline__ float cv_exp2_neg(float x) {
+    // 2^x for x <= 0: n = round(x), f = x - n, e^(f ln2) by Taylor degree 6
+    if (!(x > -126.0f)) return 0.0f;
+    float n = floorf(x + 0.5f);
+    float f = x - n;
+    float z = f * 0.693147180559945309f;
+    float p = 1.0f / 720.0f;
+    p = p * z + 1.0f / 120.0f;
+    p = p * z + 1.0f / 24.0f;
+    p = p * z + 1.0f / 6.0f;
+    p = p * z + 0.5f;
+    p = p * z + 1.0f;
+    p = p * z + 1.0f;
+    int ni = (int)n;
+    float scale = __uint_as_float((uint32_t)(ni + 127) << 23);
+    return p * scale;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t value) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: light-view raster (res/first_voxelize.glsl:39-64)
+// ---------------------------------------------------------------------------
+struct LightSphere {
+    float cx, cy, cz, s;
+    float vx, vy, vz;
+    bool live;
+};
+
+__device__ __forceinline__ LightSphere light_sphere(const GridDesc& g, const LightView& lv, float4 b) {
+    LightSphere o;
+    o.cx = g.vpos[0] + b.x;
+    o.cy = g.vpos[1] + b.y;
+    o.cz = g.vpos[2] + b.z;
+    o.s = b.w;
+    o.vx = ((lv.V[0] * o.cx + lv.V[4] * o.cy) + lv.V[8] * o.cz) + lv.V[12];
+    o.vy = ((lv.V[1] * o.cx + lv.V[5] * o.cy) + lv.V[9] * o.cz) + lv.V[13];
+    o.vz = ((lv.V[2] * o.cx + lv.V[6] * o.cy) + lv.V[10] * o.cz) + lv.V[14];
+    float dz = -o.vz;
+    o.live = (o.s > 0.0f) && (dz >= lv.znear && dz <= lv.zfar);
+    return o;
+}
+
+// fragment of sphere o at light pixel (px, py): fragPos and sphereContrib
+__device__ __forceinline__ bool light_fragment(const LightView& lv, const LightSphere& o, int px, int py, float* fx,
+                                               float* fy, float* fz, float* sc) {
+    float hx = pixel_ndc(px, lv.W) * lv.lsx + lv.lox;
+    float hy = pixel_ndc(py, lv.H) * lv.lsy + lv.loy;
+    float du = hx - o.vx;
+    float dv = hy - o.vy;
+    float c = sphere_contrib(du, dv, o.s);
+    *sc = c;
+    if (c < 0.01f) return false;  // discard
+    *fx = (o.cx + du * lv.S[0]) + dv * lv.U[0];
+    *fy = (o.cy + du * lv.S[1]) + dv * lv.U[1];
+    *fz = (o.cz + du * lv.S[2]) + dv * lv.U[2];
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, int chord,
+                                                      const float4* __restrict__ boards,
+                                                      unsigned long long* __restrict__ keys,
+                                                      unsigned long long* __restrict__ fillbits,
+                                                      unsigned long long* __restrict__ counters) {
+    const int j = blockIdx.x;
+    LightSphere o = light_sphere(g, lv, boards[j]);
+    if (!o.live) return;
+    // conservative pixel box of the billboard square
+    float xa = (((o.vx - o.s) - lv.lox) / lv.lsx + 1.0f) * 0.5f * (float)lv.W - 0.5f;
+    float xb = (((o.vx + o.s) - lv.lox) / lv.lsx + 1.0f) * 0.5f * (float)lv.W - 0.5f;
+    float ya = (((o.vy - o.s) - lv.loy) / lv.lsy + 1.0f) * 0.5f * (float)lv.H - 0.5f;
+    float yb = (((o.vy + o.s) - lv.loy) / lv.lsy + 1.0f) * 0.5f * (float)lv.H - 0.5f;
+    float fx0 = floorf(fminf(xa, xb)) - 1.0f, fx1 = ceilf(fmaxf(xa, xb)) + 1.0f;
+    float fy0 = floorf(fminf(ya, yb)) - 1.0f, fy1 = ceilf(fmaxf(ya, yb)) + 1.0f;
+    if (!(fx1 >= 0.0f && fy1 >= 0.0f && fx0 <= (float)(lv.W - 1) && fy0 <= (float)(lv.H - 1))) return;
+    int x0 = (int)fmaxf(fx0, 0.0f), x1 = (int)fminf(fx1, (float)(lv.W - 1));
+    int y0 = (int)fmaxf(fy0, 0.0f), y1 = (int)fminf(fy1, (float)(lv.H - 1));
+    int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+    int npix = bw * bh;
+    unsigned long long nfrag = 0, nchord = 0;
+    for (int t = threadIdx.x; t < npix; t += blockDim.x) {
+        int px = x0 + t % bw, py = y0 + t / bw;
+        float fx, fy, fz, sc;
+        if (!light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc)) continue;
+        ++nfrag;
+        float dist = o.s * sc;  // radius * sphereContrib
+        if (chord) {
+            // res/first_voxelize.glsl:53-58: start = fragPos - dir * dist;
+            // for (float i = 0; i < 2 * dist; i += stepSize) store (0,0,0,1)
+            float sx = fx - lv.B[0] * dist, sy = fy - lv.B[1] * dist, sz = fz - lv.B[2] * dist;
+            float lim = 2.0f * dist;
+            for (float i = 0.0f; i < lim; i += g.stepSize) {
+                float qx = sx + lv.B[0] * i, qy = sy + lv.B[1] * i, qz = sz + lv.B[2] * i;
+                uint32_t w, b;
+                ++nchord;
+                if (voxel_word_bit(g, qx, qy, qz, &w, &b)) atomicOr(&fillbits[w], 1ull << b);
+            }
+        }
+        float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
+        float ex = wx - lv.np[0], ey = wy - lv.np[1], ez = wz - lv.np[2];
+        float d = sqrtf((ex * ex + ey * ey) + ez * ez) / lv.clip;
+        d = fminf(fmaxf(d, 0.0f), 1.0f);
+        if (d < 1.0f) {
+            unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned long long)(uint32_t)j;
+            atomicMin(&keys[(size_t)py * lv.W + px], key);
+        }
+    }
+    if (counters) {
+        atomicAdd(&counters[0], nfrag);
+        atomicAdd(&counters[1], nchord);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// depth resolve + pass 2 (res/second_voxelize.glsl:34-51)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                     const unsigned long long* __restrict__ keys,
+                                                     float4* __restrict__ posmap,
+                                                     unsigned long long* __restrict__ shellbits,
+                                                     unsigned long long* __restrict__ counters) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= lv.W * lv.H) return;
+    unsigned long long key = keys[pix];
+    if (key == ~0ull) {
+        if (posmap) posmap[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    int j = (int)(uint32_t)(key & 0xffffffffull);
+    int px = pix % lv.W, py = pix / lv.W;
+    LightSphere o = light_sphere(g, lv, boards[j]);
+    float fx, fy, fz, sc;
+    light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc);
+    float dist = o.s * sc;
+    float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
+    if (posmap) posmap[pix] = make_float4(wx, wy, wz, 1.0f);
+    if (counters) atomicAdd(&counters[2], 1ull);
+    // stepSize * normalize(vec3(+-1,+-1,+-1)): 1/sqrt(3) rounded once to FP32
+    const float k = 1.0f / sqrtf(3.0f);
+    const float off = g.stepSize * k;
+    uint32_t w, b;
+    if (voxel_word_bit(g, wx, wy, wz, &w, &b)) atomicOr(&shellbits[w], 1ull << b);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        float ox = (t & 4) ? -off : off;
+        float oy = (t & 2) ? -off : off;
+        float oz = (t & 1) ? -off : off;
+        if (voxel_word_bit(g, wx + ox, wy + oy, wz + oz, &w, &b)) atomicOr(&shellbits[w], 1ull << b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// level 0 from the occupancy bitmaps; value noise density modulation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lattice_hash(uint32_t seed, int ix, int iy, int iz) {
+    uint32_t h = seed;
+    h ^= (uint32_t)ix * 0x8da6b343u;
+    h ^= (uint32_t)iy * 0xd8163841u;
+    h ^= (uint32_t)iz * 0xcb1ab31fu;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+__device__ __forceinline__ float lattice_value(uint32_t seed, int ix, int iy, int iz) {
+    return (float)(lattice_hash(seed, ix, iy, iz) >> 8) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ float smoothw(float f) { return (f * f) * (3.0f - 2.0f * f); }
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+__device__ float value_noise(uint32_t seed, float px, float py, float pz) {
+    float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    float wx = smoothw(px - fx), wy = smoothw(py - fy), wz = smoothw(pz - fz);
+    float c000 = lattice_value(seed, ix, iy, iz), c100 = lattice_value(seed, ix + 1, iy, iz);
+    float c010 = lattice_value(seed, ix, iy + 1, iz), c110 = lattice_value(seed, ix + 1, iy + 1, iz);
+    float c001 = lattice_value(seed, ix, iy, iz + 1), c101 = lattice_value(seed, ix + 1, iy, iz + 1);
+    float c011 = lattice_value(seed, ix, iy + 1, iz + 1), c111 = lattice_value(seed, ix + 1, iy + 1, iz + 1);
+    float x00 = lerpf(c000, c100, wx), x10 = lerpf(c010, c110, wx);
+    float x01 = lerpf(c001, c101, wx), x11 = lerpf(c011, c111, wx);
+    float y0 = lerpf(x00, x10, wy), y1 = lerpf(x01, x11, wy);
+    return lerpf(y0, y1, wz);
+}
+
+__device__ float fbm_voxel(const VolumeDesc& vd, int N, int x, int y, int z) {
+    float fN = (float)N;
+    float u = ((float)x + 0.5f) / fN, w = ((float)y + 0.5f) / fN, t = ((float)z + 0.5f) / fN;
+    float freq = vd.noise_base_freq, amp = 1.0f, sum = 0.0f, norm = 0.0f;
+    for (int o = 0; o < vd.noise_octaves; ++o) {
+        uint32_t seed = vd.noise_seed + (uint32_t)o * 0x9e3779b9u;
+        sum = sum + amp * value_noise(seed, u * freq, w * freq, t * freq);
+        norm = norm + amp;
+        freq = freq * vd.noise_lacunarity;
+        amp = amp * vd.noise_gain;
+    }
+    return sum / norm;
+}
+
+// one wavefront per level-0 brick; lane = Morton index inside the brick
+__global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const unsigned long long* __restrict__ shellbits,
+                                                const unsigned long long* __restrict__ fillbits, void* __restrict__ levels,
+                                                uint32_t nbricks) {
+    uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
+    uint32_t m = threadIdx.x & 63u;
+    if (brick >= nbricks) return;
+    unsigned long long sh = shellbits[brick];
+    unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
+    uint32_t nb = g.nb0;
+    uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
+    uint32_t lx, ly, lz;
+    demorton6(m, &lx, &ly, &lz);
+    int x = (int)(bx * 4u + lx), y = (int)(by * 4u + ly), z = (int)(bz * 4u + lz);
+    bool inside = x < g.N && y < g.N && z < g.N;
+    bool shell = inside && ((sh >> m) & 1ull);
+    bool fill = inside && ((fl >> m) & 1ull);
+    float r, gg, b, a;
+    if (vd.mode == 0) {
+        r = gg = b = a = shell ? 1.0f : 0.0f;
+    } else {
+        if (shell) { r = gg = b = a = 1.0f; }
+        else if (fill) { r = gg = b = 0.0f; a = 1.0f; }
+        else { r = gg = b = a = 0.0f; }
+        if (vd.noise_octaves > 0 && a != 0.0f) a = a * fbm_voxel(vd, g.N, x, y, z);
+    }
+    uint64_t idx = vd.lv[0].base + (uint64_t)brick * 64u + m;
+    if (vd.format == 0) {
+        ((uint8_t*)levels)[idx] = r != 0.0f ? (uint8_t)255 : (uint8_t)0;
+    } else if (vd.format == 1) {
+        ((float4*)levels)[idx] = make_float4(r, gg, b, a);
+    } else {
+        uint2 v;
+        v.x = f2h(r) | (f2h(gg) << 16);
+        v.y = f2h(b) | (f2h(a) << 16);
+        ((uint2*)levels)[idx] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mip level k from level k-1 (glGenerateMipmap, src/Shaders/VoxelizeShader.cpp:105)
+// parent = sum of children (z, y, x order) / count; R8: integer round half up
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void child_range(int p, int nparent, int nchild, int* c0, int* c1) {
+    *c0 = min(2 * p, nchild - 1);
+    *c1 = min(2 * p + 1, nchild - 1);
+    if (p == nparent - 1) *c1 = nchild - 1;
+}
+__device__ __forceinline__ uint64_t brick_addr(const LevelDesc& L, uint32_t x, uint32_t y, uint32_t z) {
+    return L.base + (uint64_t)(((z >> 2) * L.nb + (y >> 2)) * L.nb + (x >> 2)) * 64u + morton6(x & 3u, y & 3u, z & 3u);
+}
+
+__global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restrict__ levels, uint32_t ntexels) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntexels) return;
+    const LevelDesc P = vd.lv[k], Cl = vd.lv[k - 1];
+    uint32_t brick = t >> 6, m = t & 63u;
+    uint32_t bx = brick % P.nb, by = (brick / P.nb) % P.nb, bz = brick / (P.nb * P.nb);
+    uint32_t lx, ly, lz;
+    demorton6(m, &lx, &ly, &lz);
+    int x = (int)(bx * 4u + lx), y = (int)(by * 4u + ly), z = (int)(bz * 4u + lz);
+    uint64_t out = P.base + t;
+    int np = (int)P.n, nc = (int)Cl.n;
+    if (x >= np || y >= np || z >= np) {
+        if (vd.format == 0) ((uint8_t*)levels)[out] = 0;
+        else if (vd.format == 1) ((float4*)levels)[out] = make_float4(0.f, 0.f, 0.f, 0.f);
+        else ((uint2*)levels)[out] = make_uint2(0u, 0u);
+        return;
+    }
+    int x0, x1, y0, y1, z0, z1;
+    child_range(x, np, nc, &x0, &x1);
+    child_range(y, np, nc, &y0, &y1);
+    child_range(z, np, nc, &z0, &z1);
+    int cnt = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+    if (vd.format == 0) {
+        uint32_t sum = 0;
+        for (int cz = z0; cz <= z1; ++cz)
+            for (int cy = y0; cy <= y1; ++cy)
+                for (int cx = x0; cx <= x1; ++cx) sum += ((const uint8_t*)levels)[brick_addr(Cl, cx, cy, cz)];
+        ((uint8_t*)levels)[out] = (uint8_t)((sum + (uint32_t)cnt / 2u) / (uint32_t)cnt);
+        return;
+    }
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    for (int cz = z0; cz <= z1; ++cz)
+        for (int cy = y0; cy <= y1; ++cy)
+            for (int cx = x0; cx <= x1; ++cx) {
+                uint64_t a = brick_addr(Cl, cx, cy, cz);
+                float4 v;
+                if (vd.format == 1) {
+                    v = ((const float4*)levels)[a];
+                } else {
+                    uint2 u = ((const uint2*)levels)[a];
+                    v = make_float4(h2f(u.x & 0xffffu), h2f(u.x >> 16), h2f(u.y & 0xffffu), h2f(u.y >> 16));
+                }
+                s0 = s0 + v.x; s1 = s1 + v.y; s2 = s2 + v.z; s3 = s3 + v.w;
+            }
+    float fc = (float)cnt;
+    float4 r = make_float4(s0 / fc, s1 / fc, s2 / fc, s3 / fc);
+    if (vd.format == 1) {
+        ((float4*)levels)[out] = r;
+    } else {
+        uint2 v;
+        v.x = f2h(r.x) | (f2h(r.y) << 16);
+        v.y = f2h(r.z) | (f2h(r.w) << 16);
+        ((uint2*)levels)[out] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// camera pass setup: world/view centres, draw-order key, tile box
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, const float4* __restrict__ boards, int S,
+                                                   float4* __restrict__ sph_world, float4* __restrict__ sph_view,
+                                                   int4* __restrict__ sph_tiles, unsigned long long* __restrict__ keys) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S) return;
+    float4 b = boards[j];
+    float cx = g.vpos[0] + b.x, cy = g.vpos[1] + b.y, cz = g.vpos[2] + b.z, s = b.w;
+    float vx = ((cv.V[0] * cx + cv.V[4] * cy) + cv.V[8] * cz) + cv.V[12];
+    float vy = ((cv.V[1] * cx + cv.V[5] * cy) + cv.V[9] * cz) + cv.V[13];
+    float vz = ((cv.V[2] * cx + cv.V[6] * cy) + cv.V[10] * cz) + cv.V[14];
+    sph_world[j] = make_float4(cx, cy, cz, s);
+    sph_view[j] = make_float4(vx, vy, vz, 0.0f);
+    // glm::distance(position + p, camera) (src/CloudVolume.cpp:69)
+    float ex = cv.eye[0] - cx, ey = cv.eye[1] - cy, ez = cv.eye[2] - cz;
+    float dist = sqrtf((ex * ex + ey * ey) + ez * ez);
+    keys[j] = ((unsigned long long)(~__float_as_uint(dist)) << 32) | (unsigned long long)(uint32_t)j;
+    float dz = -vz;
+    int4 tiles = make_int4(1, 0, 1, 0);  // empty
+    if ((s > 0.0f) && (dz >= cv.znear && dz <= cv.zfar)) {
+        float ex_ = cv.kx * dz, ey_ = cv.ky * dz;
+        float xa = ((vx - s) / ex_ + 1.0f) * 0.5f * (float)cv.W - 0.5f;
+        float xb = ((vx + s) / ex_ + 1.0f) * 0.5f * (float)cv.W - 0.5f;
+        float ya = ((vy - s) / ey_ + 1.0f) * 0.5f * (float)cv.H - 0.5f;
+        float yb = ((vy + s) / ey_ + 1.0f) * 0.5f * (float)cv.H - 0.5f;
+        float fx0 = floorf(fminf(xa, xb)) - 1.0f, fx1 = ceilf(fmaxf(xa, xb)) + 1.0f;
+        float fy0 = floorf(fminf(ya, yb)) - 1.0f, fy1 = ceilf(fmaxf(ya, yb)) + 1.0f;
+        if (fx1 >= 0.0f && fy1 >= 0.0f && fx0 <= (float)(cv.W - 1) && fy0 <= (float)(cv.H - 1)) {
+            int x0 = (int)fmaxf(fx0, 0.0f), x1 = (int)fminf(fx1, (float)(cv.W - 1));
+            int y0 = (int)fmaxf(fy0, 0.0f), y1 = (int)fminf(fy1, (float)(cv.H - 1));
+            tiles = make_int4(x0 / kTile, x1 / kTile, y0 / kTile, y1 / kTile);
+        }
+    }
+    sph_tiles[j] = tiles;
+}
+
+// ---------------------------------------------------------------------------
+// draw order (CloudVolume::sortBoards, src/CloudVolume.cpp:65-82)
+// keys = (~distance bits << 32 | index): ascending = far to near, stable.
+// Equal distances make the reference's selection sort order depend on its
+// swap history, so ties trigger a literal replay (k_sort_select).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S, int npad,
+                                                   int32_t* __restrict__ order, int* __restrict__ tie_flag) {
+    extern __shared__ unsigned long long sk[];
+    __shared__ int tie;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) tie = 0;
+    for (int i = tid; i < npad; i += nt) sk[i] = i < S ? keys_in[i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= npad; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < npad; i += nt) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    unsigned long long a = sk[i], b = sk[ixj];
+                    bool up = (i & k) == 0;
+                    if ((a > b) == up) { sk[i] = b; sk[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < S; i += nt) {
+        order[i] = (int32_t)(uint32_t)(sk[i] & 0xffffffffull);
+        if (i + 1 < S && (sk[i] >> 32) == (sk[i + 1] >> 32)) tie = 1;
+    }
+    __syncthreads();
+    if (tid == 0) *tie_flag = tie;
+}
+
+// global-memory bitonic stage for more billboards than one LDS holds
+__global__ void k_bitonic_global(unsigned long long* __restrict__ keys, int npad, int j, int k) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    int ixj = i ^ j;
+    if (ixj > i) {
+        unsigned long long a = keys[i], b = keys[ixj];
+        bool up = (i & k) == 0;
+        if ((a > b) == up) { keys[i] = b; keys[ixj] = a; }
+    }
+}
+__global__ void k_pad_keys(const unsigned long long* __restrict__ in, unsigned long long* __restrict__ out, int S,
+                           int npad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npad) out[i] = i < S ? in[i] : ~0ull;
+}
+__global__ void k_sort_finish(const unsigned long long* __restrict__ sk, int S, int32_t* __restrict__ order,
+                              int* __restrict__ tie_flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    order[i] = (int32_t)(uint32_t)(sk[i] & 0xffffffffull);
+    if (i + 1 < S && (sk[i] >> 32) == (sk[i + 1] >> 32)) atomicOr(tie_flag, 1);
+}
+
+// Literal replay of the reference's selection sort, run only when the fast
+// sort found equal distances: for i: m = first position of the maximum in
+// [i, S); swap(i, m).  One workgroup; dist/idx live in global scratch.
+__global__ void __launch_bounds__(1024) k_sort_select(const int* __restrict__ tie_flag, const float4* __restrict__ sph_world,
+                                                      CameraView cv, int S, float* __restrict__ dist,
+                                                      int32_t* __restrict__ idx, int32_t* __restrict__ order) {
+    if (*tie_flag == 0) return;
+    __shared__ float wv[16];
+    __shared__ int wp[16];
+    __shared__ int best_pos;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int j = tid; j < S; j += nt) {
+        float4 c = sph_world[j];
+        float ex = cv.eye[0] - c.x, ey = cv.eye[1] - c.y, ez = cv.eye[2] - c.z;
+        dist[j] = sqrtf((ex * ex + ey * ey) + ez * ez);
+        idx[j] = j;
+    }
+    __syncthreads();
+    for (int i = 0; i < S; ++i) {
+        float bv = -INFINITY;
+        int bp = 0x7fffffff;
+        for (int j = i + tid; j < S; j += nt) {
+            float v = dist[j];
+            if (bp == 0x7fffffff || v > bv) { bv = v; bp = j; }  // first maximum in the thread's slice
+        }
+        // wave reduce: larger value wins, equal value -> smaller position
+        for (int off = 32; off > 0; off >>= 1) {
+            float ov = __shfl_xor(bv, off, 64);
+            int op = __shfl_xor(bp, off, 64);
+            bool take = (op != 0x7fffffff) && (bp == 0x7fffffff || ov > bv || (ov == bv && op < bp));
+            if (take) { bv = ov; bp = op; }
+        }
+        if ((tid & 63) == 0) { wv[tid >> 6] = bv; wp[tid >> 6] = bp; }
+        __syncthreads();
+        if (tid == 0) {
+            float v = wv[0];
+            int p = wp[0];
+            for (int w = 1; w < nt / 64; ++w) {
+                bool take = (wp[w] != 0x7fffffff) && (p == 0x7fffffff || wv[w] > v || (wv[w] == v && wp[w] < p));
+                if (take) { v = wv[w]; p = wp[w]; }
+            }
+            best_pos = p;
+            if (p != i && p != 0x7fffffff) {
+                float td = dist[i]; dist[i] = dist[p]; dist[p] = td;
+                int ti = idx[i]; idx[i] = idx[p]; idx[p] = ti;
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = tid; j < S; j += nt) order[j] = idx[j];
+}
+
+// ---------------------------------------------------------------------------
+// binning: set bit `rank` in every tile a billboard's box overlaps
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles, const int32_t* __restrict__ order, int S,
+                                             int tiles_x, int words, unsigned long long* __restrict__ masks) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S) return;
+    int4 t = sph_tiles[order[r]];
+    if (t.x > t.y || t.z > t.w) return;
+    unsigned long long bit = 1ull << (r & 63);
+    int w = r >> 6;
+    for (int ty = t.z; ty <= t.w; ++ty)
+        for (int tx = t.x; tx <= t.y; ++tx) atomicOr(&masks[((size_t)ty * tiles_x + tx) * words + w], bit);
+}
+
+// ---------------------------------------------------------------------------
+// cone trace (res/conetrace_frag.glsl:64-79, 176-200) + compositing
+// ---------------------------------------------------------------------------
+template <int FMT>
+__device__ __forceinline__ float4 fetch4(const void* __restrict__ vol, uint64_t i) {
+    if (FMT == 1) return ((const float4*)vol)[i];
+    uint2 u = ((const uint2*)vol)[i];
+    return make_float4(h2f(u.x & 0xffffu), h2f(u.x >> 16), h2f(u.y & 0xffffu), h2f(u.y >> 16));
+}
+template <int FMT>
+__device__ __forceinline__ float fetch1(const void* __restrict__ vol, uint64_t i) {
+    if (FMT == 0) return (float)((const uint8_t*)vol)[i] / 255.0f;
+    if (FMT == 1) return ((const float*)vol)[i * 4];
+    return h2f(((const unsigned short*)vol)[i * 4]);
+}
+
+// GL trilinear filtering of one level with CLAMP_TO_EDGE, lerps x, y, z
+template <int FMT, int NCH>
+__device__ __forceinline__ float4 trilinear(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
+                                            float sz) {
+    const float fn = (float)L.n;
+    const int nm1 = (int)L.n - 1;
+    float ux = fminf(fmaxf(sx, -1.0f), 2.0f) * fn - 0.5f;
+    float uy = fminf(fmaxf(sy, -1.0f), 2.0f) * fn - 0.5f;
+    float uz = fminf(fmaxf(sz, -1.0f), 2.0f) * fn - 0.5f;
+    float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
+    float ax = ux - flx, ay = uy - fly, az = uz - flz;
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    uint32_t x0 = (uint32_t)min(max(ix, 0), nm1), x1 = (uint32_t)min(max(ix + 1, 0), nm1);
+    uint32_t y0 = (uint32_t)min(max(iy, 0), nm1), y1 = (uint32_t)min(max(iy + 1, 0), nm1);
+    uint32_t z0 = (uint32_t)min(max(iz, 0), nm1), z1 = (uint32_t)min(max(iz + 1, 0), nm1);
+    const uint32_t nb = L.nb;
+    uint32_t X0 = ((x0 >> 2) << 6) | spread2(x0 & 3u), X1 = ((x1 >> 2) << 6) | spread2(x1 & 3u);
+    uint32_t Y0 = (((y0 >> 2) * nb) << 6) | (spread2(y0 & 3u) << 1);
+    uint32_t Y1 = (((y1 >> 2) * nb) << 6) | (spread2(y1 & 3u) << 1);
+    uint32_t Z0 = (((z0 >> 2) * nb * nb) << 6) | (spread2(z0 & 3u) << 2);
+    uint32_t Z1 = (((z1 >> 2) * nb * nb) << 6) | (spread2(z1 & 3u) << 2);
+    const uint64_t base = L.base;
+    if (NCH == 1) {
+        float t000 = fetch1<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch1<FMT>(vol, base + (X1 + Y0 + Z0));
+        float t010 = fetch1<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch1<FMT>(vol, base + (X1 + Y1 + Z0));
+        float t001 = fetch1<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch1<FMT>(vol, base + (X1 + Y0 + Z1));
+        float t011 = fetch1<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch1<FMT>(vol, base + (X1 + Y1 + Z1));
+        float c00 = t000 * bx + t100 * ax, c10 = t010 * bx + t110 * ax;
+        float c01 = t001 * bx + t101 * ax, c11 = t011 * bx + t111 * ax;
+        float c0 = c00 * by + c10 * ay, c1 = c01 * by + c11 * ay;
+        return make_float4(c0 * bz + c1 * az, 0.0f, 0.0f, 0.0f);
+    }
+    float4 t000 = fetch4<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch4<FMT>(vol, base + (X1 + Y0 + Z0));
+    float4 t010 = fetch4<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch4<FMT>(vol, base + (X1 + Y1 + Z0));
+    float4 t001 = fetch4<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch4<FMT>(vol, base + (X1 + Y0 + Z1));
+    float4 t011 = fetch4<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch4<FMT>(vol, base + (X1 + Y1 + Z1));
+    float4 o;
+#define CV_TRI(ch)                                                                 \
+    {                                                                              \
+        float c00 = t000.ch * bx + t100.ch * ax, c10 = t010.ch * bx + t110.ch * ax; \
+        float c01 = t001.ch * bx + t101.ch * ax, c11 = t011.ch * bx + t111.ch * ax; \
+        float c0 = c00 * by + c10 * ay, c1 = c01 * by + c11 * ay;                  \
+        o.ch = c0 * bz + c1 * az;                                                  \
+    }
+    CV_TRI(x) CV_TRI(y) CV_TRI(z) CV_TRI(w)
+#undef CV_TRI
+    return o;
+}
+
+// textureLod: lambda -> level / fraction (GL 4.4 8.14.3), (1-f)*tau1 + f*tau2
+template <int FMT, int NCH>
+__device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const VolumeDesc& vd, float lam, float sx,
+                                             float sy, float sz, bool* two) {
+    int lv;
+    float f;
+    if (lam <= 0.0f) { lv = 0; f = 0.0f; }
+    else if (lam >= (float)(vd.L - 1)) { lv = vd.L - 1; f = 0.0f; }
+    else { float fl = floorf(lam); lv = (int)fl; f = lam - fl; }
+    float4 o = trilinear<FMT, NCH>(vol, vd.lv[lv], sx, sy, sz);
+    *two = f != 0.0f;
+    if (f != 0.0f) {
+        float4 o2 = trilinear<FMT, NCH>(vol, vd.lv[lv + 1], sx, sy, sz);
+        float g = 1.0f - f;
+        o.x = o.x * g + o2.x * f;
+        if (NCH == 4) {
+            o.y = o.y * g + o2.y * f;
+            o.z = o.z * g + o2.z * f;
+            o.w = o.w * g + o2.w * f;
+        }
+    }
+    return o;
+}
+
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ __forceinline__ uint32_t q8(float v) { return (uint32_t)(int)floorf(clamp01(v) * 255.0f + 0.5f); }
+
+// CONE: 0 weighted R-sum, 1 front-to-back.  COMP: 0 GL blend, 1 front-to-back.
+template <int FMT, int CONE, int COMP>
+__global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
+                                             const void* __restrict__ vol, const float4* __restrict__ sph_world,
+                                             const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
+                                             const unsigned long long* __restrict__ masks, int words,
+                                             void* __restrict__ image, unsigned long long* __restrict__ counters) {
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
+    const int px = tx * kTile + (lane & 7), py = ty * kTile + (lane >> 3);
+    const bool valid = px < cv.W && py < cv.H;
+    const float rx = pixel_ndc(px, cv.W) * cv.kx;
+    const float ry = pixel_ndc(py, cv.H) * cv.ky;
+    const float fN = (float)g.N;
+
+    float C0, C1, C2, C3;
+    if (COMP == 0) {
+        C0 = cd.clear[0]; C1 = cd.clear[1]; C2 = cd.clear[2]; C3 = cd.clear[3];
+        if (cd.quantize) {
+            C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
+            C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
+        }
+    } else {
+        C0 = C1 = C2 = C3 = 0.0f;
+    }
+    bool done = !valid;
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0;
+    const unsigned long long* tm = masks + (size_t)tile * words;
+
+    for (int wi = 0; wi < words; ++wi) {
+        const int w = COMP == 1 ? (words - 1 - wi) : wi;
+        unsigned long long bits = tm[w];
+        while (bits) {
+            int b;
+            if (COMP == 1) b = 63 - __clzll((long long)bits);
+            else b = __ffsll((long long)bits) - 1;
+            bits &= ~(1ull << b);
+            const int r = (w << 6) | b;
+            const int j = order[r];
+            const float4 cw = sph_world[j];
+            const float4 vv = sph_view[j];
+            if (!done && valid) {
+                const float s = cw.w;
+                const float dz = -vv.z;
+                const float du = rx * dz - vv.x;
+                const float dv = ry * dz - vv.y;
+                const float sc = sphere_contrib(du, dv, s);
+                if (!(sc < 0.01f)) {
+                    ++nfrag;
+                    // fragPos, then pos = fragPos + normalize(fragNor) * radius * sc
+                    float fx = (cw.x + du * cv.S[0]) + dv * cv.U[0];
+                    float fy = (cw.y + du * cv.S[1]) + dv * cv.U[1];
+                    float fz = (cw.z + du * cv.S[2]) + dv * cv.U[2];
+                    float qx = fx + (cv.B[0] * s) * sc;
+                    float qy = fy + (cv.B[1] * s) * sc;
+                    float qz = fz + (cv.B[2] * s) * sc;
+                    // calculateVoxelLerp(pos) / voxelDim
+                    float p0x = (fN * ((qx - g.lo[0]) / g.range[0])) / fN;
+                    float p0y = (fN * ((qy - g.lo[1]) / g.range[1])) / fN;
+                    float p0z = (fN * ((qz - g.lo[2]) / g.range[2])) / fN;
+                    float dx, dy, dzz;
+                    if (cd.directional) {
+                        dx = cd.ldir[0]; dy = cd.ldir[1]; dzz = cd.ldir[2];
+                    } else {
+                        float lx = cd.lpos[0] - qx, ly = cd.lpos[1] - qy, lz = cd.lpos[2] - qz;
+                        float inv = 1.0f / sqrtf((lx * lx + ly * ly) + lz * lz);
+                        dx = (lx * inv) / fN; dy = (ly * inv) / fN; dzz = (lz * inv) / fN;
+                    }
+                    float s0, s1, s2, s3;
+                    if (CONE == 0) {
+                        float color = 0.0f;
+                        for (int i = 0; i < cd.steps; ++i) {
+                            const float h = cd.h[i];
+                            bool two;
+                            float4 sm = sample_lod<FMT, 1>(vol, vd, cd.lam[i], p0x + h * dx, p0y + h * dy, p0z + h * dzz, &two);
+                            ntwo += two ? 1u : 0u;
+                            color = color + (sm.x * (float)(i + 1)) / cd.den;
+                        }
+                        nsamp += (uint32_t)cd.steps;
+                        s0 = s1 = s2 = s3 = clamp01(color);
+                    } else {
+                        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, a = 0.0f;
+                        float k0 = 0.0f, k1 = 0.0f, k2 = 0.0f, ka = 0.0f;
+                        for (int i = 0; i < cd.steps; ++i) {
+                            const float h = cd.h[i];
+                            float sx = p0x + h * dx, sy = p0y + h * dy, sz = p0z + h * dzz;
+                            if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) break;
+                            bool two;
+                            float4 sm = sample_lod<FMT, 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
+                            ntwo += two ? 1u : 0u;
+                            float al;
+                            if (cd.cone_mode == 1) {
+                                al = sm.w;
+                            } else {
+                                float r = h * cd.tanHalf;
+                                float kk = r * cd.kscale;
+                                al = 1.0f - cv_exp2_neg(-(kk * sm.w));
+                            }
+                            float wgt = (1.0f - a) * al;
+                            if (cd.compensated) {
+                                float y, t;
+                                y = wgt * sm.x - k0; t = c0 + y; k0 = (t - c0) - y; c0 = t;
+                                y = wgt * sm.y - k1; t = c1 + y; k1 = (t - c1) - y; c1 = t;
+                                y = wgt * sm.z - k2; t = c2 + y; k2 = (t - c2) - y; c2 = t;
+                                y = wgt - ka; t = a + y; ka = (t - a) - y; a = t;
+                            } else {
+                                c0 = c0 + wgt * sm.x;
+                                c1 = c1 + wgt * sm.y;
+                                c2 = c2 + wgt * sm.z;
+                                a = a + wgt;
+                            }
+                            ++nsamp;
+                            if (a >= cd.alpha_stop) break;
+                        }
+                        s0 = clamp01(c0); s1 = clamp01(c1); s2 = clamp01(c2); s3 = clamp01(a);
+                    }
+                    if (COMP == 0) {
+                        // dst = src * src.a + dst * (1 - src.a)  (src/main.cpp:94-95)
+                        const float al = s3, om = 1.0f - al;
+                        C0 = s0 * al + C0 * om;
+                        C1 = s1 * al + C1 * om;
+                        C2 = s2 * al + C2 * om;
+                        C3 = s3 * al + C3 * om;
+                        if (cd.quantize) {
+                            C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
+                            C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
+                        }
+                    } else {
+                        const float wgt = (1.0f - C3) * s3;
+                        C0 = C0 + wgt * s0;
+                        C1 = C1 + wgt * s1;
+                        C2 = C2 + wgt * s2;
+                        C3 = C3 + wgt;
+                        if (C3 >= cd.alpha_stop) done = true;
+                    }
+                }
+            }
+            if (COMP == 1 && __all(done)) goto finished;
+        }
+    }
+finished:
+    if (valid) {
+        char* row = (char*)image + (size_t)py * cd.pitch;
+        if (cd.pixfmt == 0) {
+            ((float4*)row)[px] = make_float4(C0, C1, C2, C3);
+        } else {
+            uint32_t v = q8(C0) | (q8(C1) << 8) | (q8(C2) << 16) | (q8(C3) << 24);
+            ((uint32_t*)row)[px] = v;
+        }
+    }
+    if (counters) {
+        unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
+        if (lane == 0) {
+            atomicAdd(&counters[3], a);
+            atomicAdd(&counters[4], bsum);
+            atomicAdd(&counters[5], c);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// occupancy count/hash, state readback, level de-bricking
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_occupancy(GridDesc g, const unsigned long long* __restrict__ shellbits,
+                                                   const unsigned long long* __restrict__ fillbits, uint32_t nbricks,
+                                                   unsigned long long* __restrict__ out2) {
+    uint32_t brick = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0, hash = 0;
+    if (brick < nbricks) {
+        unsigned long long sh = shellbits[brick];
+        unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
+        unsigned long long any = sh | fl;
+        uint32_t nb = g.nb0;
+        uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
+        while (any) {
+            int m = __ffsll((long long)any) - 1;
+            any &= ~(1ull << m);
+            uint32_t lx, ly, lz;
+            demorton6((uint32_t)m, &lx, &ly, &lz);
+            unsigned long long x = bx * 4u + lx, y = by * 4u + ly, z = bz * 4u + lz;
+            unsigned long long st = ((sh >> m) & 1ull) ? 2ull : 1ull;
+            unsigned long long lin = x + (unsigned long long)g.N * (y + (unsigned long long)g.N * z);
+            unsigned long long zz = lin * 4ull + st;
+            zz = (zz ^ (zz >> 30)) * 0xbf58476d1ce4e5b9ull;
+            zz = (zz ^ (zz >> 27)) * 0x94d049bb133111ebull;
+            zz ^= zz >> 31;
+            hash += zz;
+            ++cnt;
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    hash = wave_sum_u64(hash);
+    if ((threadIdx.x & 63) == 0 && cnt) {
+        atomicAdd(&out2[0], cnt);
+        atomicAdd(&out2[1], hash);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_read_state(GridDesc g, const unsigned long long* __restrict__ shellbits,
+                                                    const unsigned long long* __restrict__ fillbits,
+                                                    uint8_t* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t N = (uint64_t)g.N;
+    if (i >= N * N * N) return;
+    uint32_t x = (uint32_t)(i % N), y = (uint32_t)((i / N) % N), z = (uint32_t)(i / (N * N));
+    uint32_t w = ((z >> 2) * g.nb0 + (y >> 2)) * g.nb0 + (x >> 2);
+    uint32_t m = morton6(x & 3u, y & 3u, z & 3u);
+    uint8_t st = 0;
+    if ((shellbits[w] >> m) & 1ull) st = 2;
+    else if (fillbits && ((fillbits[w] >> m) & 1ull)) st = 1;
+    out[i] = st;
+}
+
+__global__ void __launch_bounds__(256) k_debrick(LevelDesc L, int tb, const uint8_t* __restrict__ levels,
+                                                 uint8_t* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t n = L.n;
+    if (i >= n * n * n) return;
+    uint32_t x = (uint32_t)(i % n), y = (uint32_t)((i / n) % n), z = (uint32_t)(i / (n * n));
+    uint64_t a = brick_addr(L, x, y, z);
+    const uint8_t* src = levels + a * (uint64_t)tb;
+    uint8_t* dst = out + i * (uint64_t)tb;
+    for (int b = 0; b < tb; ++b) dst[b] = src[b];
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+size_t texel_bytes(int format) { return format == 0 ? 1 : (format == 1 ? 16 : 8); }
+
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s) {
+    if (n == 0) return;
+    unsigned nb = blocks_for(n, 256);
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, s, p, n, value);
+}
+
+void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
+                         unsigned long long* keys, unsigned long long* fillbits, unsigned long long* counters,
+                         hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_light_raster, dim3(S), dim3(256), 0, s, g, lv, mode == 1 ? 1 : 0, boards, keys, fillbits,
+                       counters);
+}
+
+void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards, const unsigned long long* keys,
+                        float4* posmap, unsigned long long* shellbits, unsigned long long* counters, hipStream_t s) {
+    uint64_t n = (uint64_t)lv.W * lv.H;
+    hipLaunchKernelGGL(k_light_splat, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, lv, boards, keys, posmap, shellbits,
+                       counters);
+}
+
+void launch_level0(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
+                   const unsigned long long* fillbits, void* levels, hipStream_t s) {
+    uint32_t nbricks = g.nb0 * g.nb0 * g.nb0;
+    hipLaunchKernelGGL(k_level0, dim3(blocks_for(nbricks, 4)), dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
+                       nbricks);
+}
+
+void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s) {
+    uint32_t nb = vd.lv[k].nb;
+    uint32_t ntex = nb * nb * nb * 64u;
+    hipLaunchKernelGGL(k_mip, dim3(blocks_for(ntex, 256)), dim3(256), 0, s, vd, k, levels, ntex);
+}
+
+void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boards, int S, float4* sph_world,
+                      float4* sph_view, int4* sph_tiles, unsigned long long* keys, hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_cam_setup, dim3(blocks_for(S, 256)), dim3(256), 0, s, g, cv, boards, S, sph_world, sph_view,
+                       sph_tiles, keys);
+}
+
+// returns the padded key count (the scratch `keys` must hold 2 * npad when
+// S exceeds the LDS sort)
+int launch_sort(unsigned long long* keys, int S, const float4* sph_world, const CameraView& cv, int32_t* order,
+                int* tie_flag, hipStream_t s) {
+    if (S <= 0) return 0;
+    int npad = 2;
+    while (npad < S) npad <<= 1;
+    if (npad <= kLdsSortMax) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipFuncSetAttribute((const void*)k_sort_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kLdsSortMax * (int)sizeof(unsigned long long));
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(k_sort_lds, dim3(1), dim3(1024), (size_t)npad * sizeof(unsigned long long), s, keys, S, npad,
+                           order, tie_flag);
+    } else {
+        unsigned long long* pk = keys + npad;  // second half of the scratch
+        hipLaunchKernelGGL(k_pad_keys, dim3(blocks_for(npad, 256)), dim3(256), 0, s, keys, pk, S, npad);
+        for (int k = 2; k <= npad; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1)
+                hipLaunchKernelGGL(k_bitonic_global, dim3(blocks_for(npad, 256)), dim3(256), 0, s, pk, npad, j, k);
+        launch_fill_u64((uint64_t*)tie_flag, 1, 0, s);  // d_tie holds 64 bytes
+        hipLaunchKernelGGL(k_sort_finish, dim3(blocks_for(S, 256)), dim3(256), 0, s, pk, S, order, tie_flag);
+    }
+    // scratch for the selection-sort replay: reuse the key buffer past npad*2
+    float* dist = (float*)(keys + 2 * (size_t)npad);
+    int32_t* idx = (int32_t*)(dist + npad);
+    hipLaunchKernelGGL(k_sort_select, dim3(1), dim3(1024), 0, s, tie_flag, sph_world, cv, S, dist, idx, order);
+    return npad;
+}
+
+void launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
+                unsigned long long* masks, hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_bin, dim3(blocks_for(S, 256)), dim3(256), 0, s, sph_tiles, order, S, tiles_x, words, masks);
+}
+
+template <int FMT, int CONE, int COMP>
+static void launch_cone_t(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
+                          const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
+                          const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                          hipStream_t s) {
+    unsigned ntiles = (unsigned)(cv.tiles_x * cv.tiles_y);
+    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP>), dim3(ntiles), dim3(64), 0, s, g, vd, cv, cd, levels, sph_world,
+                       sph_view, order, masks, words, image, counters);
+}
+
+template <int FMT>
+static void launch_cone_f(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
+                          const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
+                          const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                          hipStream_t s) {
+    const bool weighted = cd.cone_mode == 0;
+    if (weighted && cd.composite == 0)
+        launch_cone_t<FMT, 0, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+    else if (weighted)
+        launch_cone_t<FMT, 0, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+    else if (cd.composite == 0)
+        launch_cone_t<FMT, 1, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+    else
+        launch_cone_t<FMT, 1, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+}
+
+void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd, const void* levels,
+                 const float4* sph_world, const float4* sph_view, const int32_t* order,
+                 const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                 hipStream_t s) {
+    if (vd.format == 0)
+        launch_cone_f<0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+    else if (vd.format == 1)
+        launch_cone_f<1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+    else
+        launch_cone_f<2>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+}
+
+void launch_occupancy(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,
+                      unsigned long long* out2, hipStream_t s) {
+    uint32_t nbricks = g.nb0 * g.nb0 * g.nb0;
+    hipLaunchKernelGGL(k_occupancy, dim3(blocks_for(nbricks, 256)), dim3(256), 0, s, g, shellbits, fillbits, nbricks,
+                       out2);
+}
+
+void launch_read_state(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,
+                       uint8_t* out, hipStream_t s) {
+    uint64_t n = (uint64_t)g.N * g.N * g.N;
+    hipLaunchKernelGGL(k_read_state, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, shellbits, fillbits, out);
+}
+
+void launch_debrick(const VolumeDesc& vd, int k, const void* levels, void* out, hipStream_t s) {
+    uint64_t n = (uint64_t)vd.lv[k].n;
+    n = n * n * n;
+    hipLaunchKernelGGL(k_debrick, dim3(blocks_for(n, 256)), dim3(256), 0, s, vd.lv[k], (int)texel_bytes(vd.format),
+                       (const uint8_t*)levels, (uint8_t*)out);
+}
+
+}  // namespace cv

@@ -1,0 +1,665 @@
+// Host runtime behind the C ABI (include/cloudvox.h).
+//
+// Replaces the GL state the reference's render passes drive:
+//   VoxelizeShader::voxelize   src/Shaders/VoxelizeShader.cpp:18-134
+//   ConeTraceShader::coneTrace src/Shaders/ConeTraceShader.cpp:15-98
+//   CloudVolume (storage, billboards, sortBoards) src/CloudVolume.cpp:7-164
+//   Sun::update                src/Sun.hpp:26-43
+// Host-side matrix math follows glm 0.9.8 (lookAtRH, normalize =
+// v * (1 / sqrt(dot)), dot = (x*x + y*y) + z*z) and is compiled with
+// -ffp-contract=off so the view parameters are reproducible bit for bit.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cloudvox.h"
+#include "cv_kernels.h"
+
+using namespace cv;
+
+struct cr_context_s {
+    int device = 0;
+    std::string err;
+    int profiling = 0;
+    cr_stats stats{};
+    hipEvent_t ev[10] = {};
+    bool events = false;
+};
+
+struct cr_volume_s {
+    cr_context ctx = nullptr;
+    int N = 0, L = 0, format = 0, mode = 0;
+    float blo[3] = {}, bhi[3] = {}, pos[3] = {};
+    VolumeDesc vd{};
+    GridDesc g{};
+    void* d_levels = nullptr;
+    size_t levels_bytes = 0;
+    unsigned long long* d_shell = nullptr;
+    unsigned long long* d_fill = nullptr;
+    uint32_t nbricks0 = 0;
+    int S = 0;
+    float max_radius = 0.0f;
+    float4* d_boards = nullptr;
+    int boards_cap = 0;
+    // pass 1
+    int lw = 0, lh = 0;
+    unsigned long long* d_keys = nullptr;
+    float4* d_posmap = nullptr;
+    size_t light_cap = 0;
+    // camera pass
+    float4* d_sph_world = nullptr;
+    float4* d_sph_view = nullptr;
+    int4* d_sph_tiles = nullptr;
+    unsigned long long* d_sort = nullptr;
+    int32_t* d_order = nullptr;
+    int* d_tie = nullptr;
+    int sph_cap = 0;
+    size_t sort_cap = 0;
+    unsigned long long* d_masks = nullptr;
+    size_t masks_cap = 0;
+    unsigned long long* d_counters = nullptr;
+    uint8_t* d_scratch = nullptr;
+    size_t scratch_cap = 0;
+    bool voxelized = false;
+    int sorted_count = -1;
+};
+
+static thread_local std::string g_last_error;
+
+namespace {
+
+int fail(cr_context ctx, int code, const char* fmt, const char* a = nullptr) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), fmt, a ? a : "");
+    if (ctx) ctx->err = buf;
+    g_last_error = buf;
+    return code;
+}
+
+#define CR_HIP(ctx, call)                                                                             \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) return fail((ctx), CR_EDEVICE, #call ": %s", hipGetErrorString(e_));     \
+    } while (0)
+
+#define CR_HIP_ALLOC(ctx, call)                                                                       \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ == hipErrorOutOfMemory) return fail((ctx), CR_ENOMEM, #call ": out of device memory"); \
+        if (e_ != hipSuccess) return fail((ctx), CR_EDEVICE, #call ": %s", hipGetErrorString(e_));     \
+    } while (0)
+
+// ---- glm 0.9.8 float math ----
+struct F3 { float x, y, z; };
+inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+inline float dot3(F3 a, F3 b) {
+    float x = a.x * b.x, y = a.y * b.y, z = a.z * b.z;
+    return (x + y) + z;
+}
+inline float len3(F3 a) { return std::sqrt(dot3(a, a)); }
+inline F3 norm3(F3 a) { return a * (1.0f / std::sqrt(dot3(a, a))); }
+inline F3 cross3(F3 a, F3 b) { return f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y); }
+inline bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+void look_at(F3 eye, F3 center, F3 up, float M[16]) {
+    F3 f = norm3(center - eye);
+    F3 s = norm3(cross3(f, up));
+    F3 u = cross3(s, f);
+    for (int i = 0; i < 16; ++i) M[i] = 0.0f;
+    M[0] = s.x; M[4] = s.y; M[8] = s.z;
+    M[1] = u.x; M[5] = u.y; M[9] = u.z;
+    M[2] = -f.x; M[6] = -f.y; M[10] = -f.z;
+    M[12] = -dot3(s, eye);
+    M[13] = -dot3(u, eye);
+    M[14] = dot3(f, eye);
+    M[15] = 1.0f;
+}
+
+void basis_from_view(const float V[16], float S[3], float U[3], float B[3]) {
+    S[0] = V[0]; S[1] = V[4]; S[2] = V[8];
+    U[0] = V[1]; U[1] = V[5]; U[2] = V[9];
+    F3 b = norm3(f3(V[2], V[6], V[10]));
+    B[0] = b.x; B[1] = b.y; B[2] = b.z;
+}
+
+// Sun::update (src/Sun.hpp:26-43).  Up falls back to +z when the light looks
+// along +-y (lookAt with up = +y is singular there).
+void sun_update(const cr_volume_s* v, const float sun_pos[3], int W, int H, LightView* lv) {
+    F3 mn = f3(v->blo[0], v->blo[1], v->blo[2]);
+    F3 mx = f3(v->bhi[0], v->bhi[1], v->bhi[2]);
+    F3 vpos = f3(v->pos[0], v->pos[1], v->pos[2]);
+    F3 spos = f3(sun_pos[0], sun_pos[1], sun_pos[2]);
+    F3 lookDir = norm3(vpos - spos);
+    float ml = std::max(len3(mn), len3(mx));
+    F3 lookPos = vpos - lookDir * ml;
+    F3 up = f3(0.0f, 1.0f, 0.0f);
+    if (std::fabs(lookDir.y) > 0.999f) up = f3(0.0f, 0.0f, 1.0f);
+    look_at(lookPos, vpos, up, lv->V);
+    float minmin = 2.0f * std::min(mn.x, std::min(mn.y, mn.z));
+    float maxmax = 2.0f * std::max(mx.x, std::max(mx.y, mx.z));
+    F3 nearPlane = lookPos + lookDir * 0.01f;
+    F3 farPlane = lookPos + (lookDir * 2.0f) * ml;
+    F3 dd = farPlane - nearPlane;
+    float clip = len3(dd);
+    basis_from_view(lv->V, lv->S, lv->U, lv->B);
+    lv->np[0] = nearPlane.x; lv->np[1] = nearPlane.y; lv->np[2] = nearPlane.z;
+    lv->clip = clip;
+    // glm::ortho(minmin, maxmax, minmin, maxmax, 0.01, 0.01 + clip)
+    lv->lsx = (maxmax - minmin) * 0.5f;
+    lv->lox = (maxmax + minmin) * 0.5f;
+    lv->lsy = (maxmax - minmin) * 0.5f;
+    lv->loy = (maxmax + minmin) * 0.5f;
+    lv->znear = 0.01f;
+    lv->zfar = 0.01f + clip;
+    lv->W = W;
+    lv->H = H;
+}
+
+int level_dim(int N, int k) { return std::max(1, N >> k); }
+
+template <class T>
+int ensure(cr_context ctx, T** p, size_t need, size_t* cap) {
+    if (need <= *cap && *p) return CR_OK;
+    if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
+    size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e == hipErrorOutOfMemory) { *p = nullptr; return fail(ctx, CR_ENOMEM, "hipMalloc: out of device memory"); }
+    if (e != hipSuccess) { *p = nullptr; return fail(ctx, CR_EDEVICE, "hipMalloc: %s", hipGetErrorString(e)); }
+    *cap = need;
+    return CR_OK;
+}
+
+int set_device(cr_context ctx) {
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail(ctx, CR_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    return CR_OK;
+}
+
+void stage_event(cr_context ctx, int i, hipStream_t s) {
+    if (ctx->profiling && ctx->events) (void)hipEventRecord(ctx->ev[i], s);
+}
+
+float elapsed(cr_context ctx, int a, int b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cr_abi_version(void) { return CR_ABI_VERSION; }
+
+const char* cr_last_error(cr_context ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_last_error.c_str();
+}
+
+int cr_context_create(int device, unsigned flags, cr_context* out) {
+    (void)flags;
+    if (!out) return fail(nullptr, CR_EINVAL, "cr_context_create: out is NULL");
+    *out = nullptr;
+    if (device < 0) return fail(nullptr, CR_EINVAL, "cr_context_create: negative device");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(nullptr, CR_EDEVICE, "cr_context_create: no HIP device (%s)",
+                                               e != hipSuccess ? hipGetErrorString(e) : "count 0");
+    if (device >= n) return fail(nullptr, CR_EINVAL, "cr_context_create: device index out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, CR_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    cr_context ctx = new (std::nothrow) cr_context_s();
+    if (!ctx) return fail(nullptr, CR_ENOMEM, "cr_context_create: host allocation failed");
+    ctx->device = device;
+    *out = ctx;
+    return CR_OK;
+}
+
+int cr_context_destroy(cr_context ctx) {
+    if (!ctx) return CR_EINVAL;
+    if (ctx->events)
+        for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+    delete ctx;
+    return CR_OK;
+}
+
+int cr_context_set_profiling(cr_context ctx, int profiling) {
+    if (!ctx) return fail(nullptr, CR_EINVAL, "cr_context_set_profiling: ctx is NULL");
+    if (profiling && !ctx->events) {
+        int rc = set_device(ctx);
+        if (rc) return rc;
+        for (auto& e : ctx->ev) CR_HIP(ctx, hipEventCreate(&e));
+        ctx->events = true;
+    }
+    ctx->profiling = profiling ? 1 : 0;
+    return CR_OK;
+}
+
+int cr_get_stats(cr_context ctx, cr_stats* out) {
+    if (!ctx || !out) return fail(ctx, CR_EINVAL, "cr_get_stats: NULL argument");
+    *out = ctx->stats;
+    return CR_OK;
+}
+
+int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const float bounds_hi[3],
+                     const float position[3], int levels, int format, int mode, cr_volume* out) {
+    if (!ctx || !out || !bounds_lo || !bounds_hi || !position)
+        return fail(ctx, CR_EINVAL, "cr_volume_create: NULL argument");
+    *out = nullptr;
+    if (dim < 1 || dim > 2048) return fail(ctx, CR_EINVAL, "cr_volume_create: dim must be in [1, 2048]");
+    int maxl = 1;
+    while ((dim >> maxl) >= 1) ++maxl;  // floor(log2(dim)) + 1
+    if (levels < 1 || levels > maxl || levels > kMaxLevels)
+        return fail(ctx, CR_EINVAL, "cr_volume_create: levels must be in [1, log2(dim) + 1]");
+    if (format < CR_FORMAT_R8 || format > CR_FORMAT_RGBA16F) return fail(ctx, CR_EINVAL, "cr_volume_create: bad format");
+    if (mode != CR_MODE_REF && mode != CR_MODE_RGBA) return fail(ctx, CR_EINVAL, "cr_volume_create: bad mode");
+    if (!finite3(bounds_lo) || !finite3(bounds_hi) || !finite3(position))
+        return fail(ctx, CR_EINVAL, "cr_volume_create: non-finite bounds or position");
+    for (int a = 0; a < 3; ++a)
+        if (!(bounds_hi[a] > bounds_lo[a])) return fail(ctx, CR_EINVAL, "cr_volume_create: empty bounds");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    cr_volume v = new (std::nothrow) cr_volume_s();
+    if (!v) return fail(ctx, CR_ENOMEM, "cr_volume_create: host allocation failed");
+    v->ctx = ctx;
+    v->N = dim; v->L = levels; v->format = format; v->mode = mode;
+    for (int a = 0; a < 3; ++a) { v->blo[a] = bounds_lo[a]; v->bhi[a] = bounds_hi[a]; v->pos[a] = position[a]; }
+    // level layout
+    uint64_t base = 0;
+    for (int k = 0; k < levels; ++k) {
+        uint32_t n = (uint32_t)level_dim(dim, k);
+        uint32_t nb = (n + 3u) / 4u;
+        v->vd.lv[k].n = n;
+        v->vd.lv[k].nb = nb;
+        v->vd.lv[k].base = base;
+        base += (uint64_t)nb * nb * nb * 64u;
+    }
+    v->vd.L = levels;
+    v->vd.format = format;
+    v->vd.mode = mode;
+    v->vd.noise_octaves = 0;
+    v->levels_bytes = base * texel_bytes(format);
+    // grid: world bounds = position + bounds (src/Shaders/VoxelizeShader.cpp:124-126);
+    // voxelSize = (bounds.y - bounds.x) / dimension (src/CloudVolume.cpp:50-54)
+    float vs[3];
+    for (int a = 0; a < 3; ++a) {
+        float wlo = position[a] + bounds_lo[a];
+        float whi = position[a] + bounds_hi[a];
+        v->g.lo[a] = wlo;
+        v->g.range[a] = whi - wlo;
+        v->g.vpos[a] = position[a];
+        vs[a] = (bounds_hi[a] - bounds_lo[a]) / (float)dim;
+    }
+    v->g.stepSize = std::min(vs[0], std::min(vs[1], vs[2]));
+    v->g.N = dim;
+    v->g.nb0 = v->vd.lv[0].nb;
+    v->nbricks0 = v->g.nb0 * v->g.nb0 * v->g.nb0;
+    hipError_t e = hipMalloc(&v->d_levels, v->levels_bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_shell, (size_t)v->nbricks0 * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_fill, (size_t)v->nbricks0 * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_counters, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_tie, 64);
+    if (e != hipSuccess) {
+        cr_volume_destroy(v);
+        return fail(ctx, e == hipErrorOutOfMemory ? CR_ENOMEM : CR_EDEVICE, "cr_volume_create: hipMalloc: %s",
+                    hipGetErrorString(e));
+    }
+    // zero the volume so reads before the first voxelize are defined
+    launch_fill_u64((uint64_t*)v->d_levels, v->levels_bytes / 8, 0, nullptr);
+    launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, nullptr);
+    launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, nullptr);
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        cr_volume_destroy(v);
+        return fail(ctx, CR_EDEVICE, "cr_volume_create: %s", hipGetErrorString(e));
+    }
+    *out = v;
+    return CR_OK;
+}
+
+int cr_volume_destroy(cr_volume v) {
+    if (!v) return CR_EINVAL;
+    if (v->ctx) (void)hipSetDevice(v->ctx->device);
+    void* ptrs[] = {v->d_levels, v->d_shell, v->d_fill, v->d_boards, v->d_keys, v->d_posmap, v->d_sph_world,
+                    v->d_sph_view, v->d_sph_tiles, v->d_sort, v->d_order, v->d_tie, v->d_masks, v->d_counters,
+                    v->d_scratch};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete v;
+    return CR_OK;
+}
+
+int cr_volume_set_noise(cr_volume v, int octaves, uint32_t seed, float base_freq, float lacunarity, float gain) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_set_noise: vol is NULL");
+    if (octaves < 0 || octaves > 16) return fail(v->ctx, CR_EINVAL, "cr_volume_set_noise: octaves must be in [0, 16]");
+    if (!std::isfinite(base_freq) || !std::isfinite(lacunarity) || !std::isfinite(gain))
+        return fail(v->ctx, CR_EINVAL, "cr_volume_set_noise: non-finite parameter");
+    v->vd.noise_octaves = octaves;
+    v->vd.noise_seed = seed;
+    v->vd.noise_base_freq = base_freq;
+    v->vd.noise_lacunarity = lacunarity;
+    v->vd.noise_gain = gain;
+    return CR_OK;
+}
+
+int cr_volume_set_billboards(cr_volume v, const float* xyz, const float* scales, int count, float fluffiness) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_set_billboards: vol is NULL");
+    cr_context ctx = v->ctx;
+    if (count < 0) return fail(ctx, CR_EINVAL, "cr_volume_set_billboards: negative count");
+    if (count > 0 && (!xyz || !scales)) return fail(ctx, CR_EINVAL, "cr_volume_set_billboards: NULL array");
+    if (!std::isfinite(fluffiness)) return fail(ctx, CR_EINVAL, "cr_volume_set_billboards: non-finite fluffiness");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<float4> h((size_t)std::max(count, 1));
+    float maxr = 0.0f;
+    for (int j = 0; j < count; ++j) {
+        // uploadBillboards (src/CloudVolume.cpp:153-160): scale * fluffiness
+        float r = fluffiness != 1.0f ? scales[j] * fluffiness : scales[j];
+        if (!finite3(xyz + 3 * j) || !std::isfinite(r))
+            return fail(ctx, CR_EINVAL, "cr_volume_set_billboards: non-finite position or scale");
+        h[j] = make_float4(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], r);
+        maxr = std::max(maxr, r);
+    }
+    size_t cap = (size_t)v->boards_cap;
+    rc = ensure(ctx, &v->d_boards, (size_t)std::max(count, 1), &cap);
+    if (rc) return rc;
+    v->boards_cap = (int)cap;
+    if (count > 0) CR_HIP(ctx, hipMemcpy(v->d_boards, h.data(), (size_t)count * sizeof(float4), hipMemcpyHostToDevice));
+    v->S = count;
+    v->max_radius = maxr;
+    v->sorted_count = -1;
+    return CR_OK;
+}
+
+int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, void* stream) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_voxelize: vol is NULL");
+    cr_context ctx = v->ctx;
+    if (!light) return fail(ctx, CR_EINVAL, "cr_voxelize: light is NULL");
+    if (light_w <= 0 || light_h <= 0 || light_w > 16384 || light_h > 16384)
+        return fail(ctx, CR_EINVAL, "cr_voxelize: light map size must be in [1, 16384]");
+    if (!finite3(light->position)) return fail(ctx, CR_EINVAL, "cr_voxelize: non-finite light position");
+    {
+        float d0 = v->pos[0] - light->position[0], d1 = v->pos[1] - light->position[1],
+              d2 = v->pos[2] - light->position[2];
+        if (d0 == 0.0f && d1 == 0.0f && d2 == 0.0f)
+            return fail(ctx, CR_EINVAL, "cr_voxelize: light position coincides with the volume position");
+    }
+    // the chord fill walks 2 * radius in stepSize increments with a float
+    // counter; bound it so the walk always advances and terminates
+    if (v->mode == CR_MODE_RGBA && (2.0f * v->max_radius) / v->g.stepSize > 1048576.0f)
+        return fail(ctx, CR_EUNSUPPORTED, "cr_voxelize: billboard radius exceeds 2^19 voxels (chord fill)");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    LightView lv;
+    sun_update(v, light->position, light_w, light_h, &lv);
+    size_t npx = (size_t)light_w * light_h;
+    if ((rc = ensure(ctx, &v->d_keys, npx, &v->light_cap))) return rc;
+    static_assert(sizeof(float4) == 16, "float4");
+    {
+        size_t cap = v->d_posmap ? (size_t)v->lw * v->lh : 0;
+        if (!v->d_posmap || cap < npx) {
+            if (v->d_posmap) (void)hipFree(v->d_posmap);
+            v->d_posmap = nullptr;
+            CR_HIP_ALLOC(ctx, hipMalloc((void**)&v->d_posmap, npx * sizeof(float4)));
+        }
+    }
+    v->lw = light_w;
+    v->lh = light_h;
+    unsigned long long* counters = ctx->profiling ? v->d_counters : nullptr;
+    stage_event(ctx, 0, s);
+    if (counters) launch_fill_u64((uint64_t*)counters, 8, 0, s);
+    // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187)
+    launch_fill_u64((uint64_t*)v->d_keys, npx, ~0ull, s);
+    launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, s);
+    launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, s);
+    stage_event(ctx, 1, s);
+    launch_light_raster(v->g, lv, v->mode, v->d_boards, v->S, v->d_keys, v->d_fill, counters, s);
+    stage_event(ctx, 2, s);
+    launch_light_splat(v->g, lv, v->d_boards, v->d_keys, v->d_posmap, v->d_shell, counters, s);
+    stage_event(ctx, 3, s);
+    launch_level0(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
+    for (int k = 1; k < v->L; ++k) launch_mip(v->vd, k, v->d_levels, s);
+    stage_event(ctx, 4, s);
+    CR_HIP(ctx, hipGetLastError());
+    if (ctx->profiling) {
+        CR_HIP(ctx, hipStreamSynchronize(s));
+        unsigned long long c[8];
+        CR_HIP(ctx, hipMemcpy(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+        ctx->stats.light_fragments = (int64_t)c[0];
+        ctx->stats.shell_texels = (int64_t)c[2];
+        ctx->stats.ms_clear = elapsed(ctx, 0, 1);
+        ctx->stats.ms_light_raster = elapsed(ctx, 1, 2);
+        ctx->stats.ms_splat = elapsed(ctx, 2, 3);
+        ctx->stats.ms_mip = elapsed(ctx, 3, 4);
+    }
+    v->voxelized = true;
+    return CR_OK;
+}
+
+int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p, void* d_image,
+                  int W, int H, size_t pitch, int pixfmt, void* stream) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_cone_trace: vol is NULL");
+    cr_context ctx = v->ctx;
+    if (!light || !cam || !p || !d_image) return fail(ctx, CR_EINVAL, "cr_cone_trace: NULL argument");
+    if (W <= 0 || H <= 0 || W > 16384 || H > 16384) return fail(ctx, CR_EINVAL, "cr_cone_trace: image size must be in [1, 16384]");
+    if (pixfmt != CR_PIXEL_RGBA32F && pixfmt != CR_PIXEL_RGBA8) return fail(ctx, CR_EINVAL, "cr_cone_trace: bad pixel format");
+    size_t px_bytes = pixfmt == CR_PIXEL_RGBA32F ? 16 : 4;
+    if (pitch < (size_t)W * px_bytes) return fail(ctx, CR_EINVAL, "cr_cone_trace: pitch smaller than a row");
+    if (pitch % px_bytes != 0 || ((uintptr_t)d_image % px_bytes) != 0)
+        return fail(ctx, CR_EINVAL, "cr_cone_trace: image pointer or pitch misaligned");
+    if (p->steps < 0 || p->steps > kMaxSteps) return fail(ctx, CR_EUNSUPPORTED, "cr_cone_trace: steps must be in [0, 256]");
+    if (p->cone_mode < CR_CONE_WEIGHTED || p->cone_mode > CR_CONE_FTB_DENSITY) return fail(ctx, CR_EINVAL, "cr_cone_trace: bad cone mode");
+    if (p->composite != CR_COMPOSITE_GL_BLEND && p->composite != CR_COMPOSITE_FTB) return fail(ctx, CR_EINVAL, "cr_cone_trace: bad composite");
+    if (!(p->cone_angle > 0.0f && p->cone_angle < 3.14159265f)) return fail(ctx, CR_EINVAL, "cr_cone_trace: cone angle must be in (0, pi)");
+    if (!(p->initial_height > 0.0f) || !std::isfinite(p->initial_height)) return fail(ctx, CR_EINVAL, "cr_cone_trace: initial height must be > 0");
+    if (!std::isfinite(p->lod_offset) || !(p->down_scaling != 0.0f) || !std::isfinite(p->down_scaling))
+        return fail(ctx, CR_EINVAL, "cr_cone_trace: bad lod offset / down scaling");
+    if (!std::isfinite(p->sigma) || p->sigma < 0.0f) return fail(ctx, CR_EINVAL, "cr_cone_trace: sigma must be finite and >= 0");
+    if (!finite3(cam->eye) || !finite3(cam->target) || !finite3(cam->up)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite camera");
+    if (!(cam->fovy > 0.0f && cam->fovy < 3.14159265f)) return fail(ctx, CR_EINVAL, "cr_cone_trace: fovy must be in (0, pi)");
+    if (!(cam->znear > 0.0f) || !(cam->zfar > cam->znear)) return fail(ctx, CR_EINVAL, "cr_cone_trace: need 0 < znear < zfar");
+    if (light->directional && !finite3(light->direction)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light direction");
+    if (!light->directional && !finite3(light->position)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light position");
+    if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_cone_trace: volume not voxelized");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+
+    // camera view (src/Camera.cpp:59-60)
+    CameraView cv{};
+    look_at(f3(cam->eye[0], cam->eye[1], cam->eye[2]), f3(cam->target[0], cam->target[1], cam->target[2]),
+            f3(cam->up[0], cam->up[1], cam->up[2]), cv.V);
+    basis_from_view(cv.V, cv.S, cv.U, cv.B);
+    for (int a = 0; a < 3; ++a) cv.eye[a] = cam->eye[a];
+    float tanHalfFovy = std::tan(cam->fovy / 2.0f);
+    float aspect = (float)W / (float)H;
+    cv.kx = tanHalfFovy * aspect;
+    cv.ky = tanHalfFovy;
+    cv.znear = cam->znear;
+    cv.zfar = cam->zfar;
+    cv.W = W;
+    cv.H = H;
+    cv.tiles_x = (W + kTile - 1) / kTile;
+    cv.tiles_y = (H + kTile - 1) / kTile;
+
+    // cone schedule (res/conetrace_frag.glsl:70-76), identical for every ray
+    ConeDesc cd{};
+    cd.steps = p->steps;
+    float tanHalf = std::tan(p->cone_angle / 2.0f);
+    cd.tanHalf = tanHalf;
+    float h = p->initial_height;
+    for (int i = 0; i < p->steps; ++i) {
+        float r = h * tanHalf;
+        float lod = std::log2(std::max(1.0f, 2.0f * r));
+        cd.h[i] = h;
+        cd.lam[i] = lod + p->lod_offset;
+        h = h + r;
+    }
+    cd.den = (float)p->steps * p->down_scaling;
+    cd.alpha_stop = p->alpha_stop;
+    cd.kscale = (p->sigma * (float)1.4426950408889634) / (float)v->N;
+    cd.cone_mode = p->cone_mode;
+    cd.composite = p->composite;
+    cd.quantize = (p->composite == CR_COMPOSITE_GL_BLEND && p->quantize_rgba8) ? 1 : 0;
+    cd.compensated = p->compensated ? 1 : 0;
+    for (int q = 0; q < 4; ++q) cd.clear[q] = p->clear_color[q];
+    for (int a = 0; a < 3; ++a) cd.lpos[a] = light->position[a];
+    cd.directional = light->directional ? 1 : 0;
+    if (cd.directional) {
+        F3 ld = norm3(f3(light->direction[0], light->direction[1], light->direction[2]));
+        float fN = (float)v->N;
+        cd.ldir[0] = ld.x / fN;
+        cd.ldir[1] = ld.y / fN;
+        cd.ldir[2] = ld.z / fN;
+    }
+    cd.pixfmt = pixfmt;
+    cd.pitch = pitch;
+
+    // buffers
+    int S = v->S;
+    size_t scap = (size_t)v->sph_cap;
+    if (S > 0) {
+        size_t c0 = scap, c1 = scap, c2 = scap, c3 = scap;
+        if ((rc = ensure(ctx, &v->d_sph_world, (size_t)S, &c0))) return rc;
+        if ((rc = ensure(ctx, &v->d_sph_view, (size_t)S, &c1))) return rc;
+        if ((rc = ensure(ctx, &v->d_sph_tiles, (size_t)S, &c2))) return rc;
+        if ((rc = ensure(ctx, &v->d_order, (size_t)S, &c3))) return rc;
+        v->sph_cap = (int)std::min(std::min(c0, c1), std::min(c2, c3));
+        size_t npad = 2;
+        while (npad < (size_t)S) npad <<= 1;
+        // keys (npad) + padded copy (npad) + selection-sort scratch (npad floats + npad ints)
+        if ((rc = ensure(ctx, &v->d_sort, 3 * npad, &v->sort_cap))) return rc;
+    }
+    int words = (S + 63) / 64;
+    size_t ntiles = (size_t)cv.tiles_x * cv.tiles_y;
+    size_t nmask = ntiles * (size_t)std::max(words, 1);
+    if ((rc = ensure(ctx, &v->d_masks, nmask, &v->masks_cap))) return rc;
+
+    unsigned long long* counters = ctx->profiling ? v->d_counters : nullptr;
+    stage_event(ctx, 5, s);
+    if (counters) launch_fill_u64((uint64_t*)counters, 8, 0, s);
+    launch_cam_setup(v->g, cv, v->d_boards, S, v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort, s);
+    stage_event(ctx, 6, s);
+    launch_sort(v->d_sort, S, v->d_sph_world, cv, v->d_order, v->d_tie, s);
+    stage_event(ctx, 7, s);
+    launch_fill_u64((uint64_t*)v->d_masks, nmask, 0, s);
+    launch_bin(v->d_sph_tiles, v->d_order, S, cv.tiles_x, words, v->d_masks, s);
+    stage_event(ctx, 8, s);
+    launch_cone(v->g, v->vd, cv, cd, v->d_levels, v->d_sph_world, v->d_sph_view, v->d_order, v->d_masks, words,
+                d_image, counters, s);
+    stage_event(ctx, 9, s);
+    CR_HIP(ctx, hipGetLastError());
+    v->sorted_count = S;
+    if (ctx->profiling) {
+        CR_HIP(ctx, hipStreamSynchronize(s));
+        unsigned long long c[8];
+        CR_HIP(ctx, hipMemcpy(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+        ctx->stats.fragments = (int64_t)c[3];
+        ctx->stats.samples = (int64_t)c[4];
+        ctx->stats.two_level = (int64_t)c[5];
+        ctx->stats.ms_setup = elapsed(ctx, 5, 6);
+        ctx->stats.ms_sort = elapsed(ctx, 6, 7);
+        ctx->stats.ms_bin = elapsed(ctx, 7, 8);
+        ctx->stats.ms_cone = elapsed(ctx, 8, 9);
+    }
+    return CR_OK;
+}
+
+int cr_volume_occupancy(cr_volume v, uint64_t* count, uint64_t* hash, void* stream) {
+    if (!v || !count || !hash) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_occupancy: NULL argument");
+    cr_context ctx = v->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_fill_u64((uint64_t*)v->d_counters, 2, 0, s);
+    launch_occupancy(v->g, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_counters, s);
+    CR_HIP(ctx, hipGetLastError());
+    unsigned long long c[2];
+    CR_HIP(ctx, hipMemcpyAsync(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    *count = c[0];
+    *hash = c[1];
+    return CR_OK;
+}
+
+static int scratch(cr_volume v, size_t bytes) {
+    size_t cap = v->scratch_cap;
+    int rc = ensure(v->ctx, &v->d_scratch, bytes, &cap);
+    v->scratch_cap = cap;
+    return rc;
+}
+
+int cr_volume_read_level(cr_volume v, int level, void* host_dst, size_t bytes, void* stream) {
+    if (!v || !host_dst) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_read_level: NULL argument");
+    cr_context ctx = v->ctx;
+    if (level < 0 || level >= v->L) return fail(ctx, CR_EINVAL, "cr_volume_read_level: level out of range");
+    uint64_t n = v->vd.lv[level].n;
+    size_t need = (size_t)(n * n * n) * texel_bytes(v->format);
+    if (bytes < need) return fail(ctx, CR_EINVAL, "cr_volume_read_level: destination too small");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = scratch(v, need))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_debrick(v->vd, level, v->d_levels, v->d_scratch, s);
+    CR_HIP(ctx, hipGetLastError());
+    CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_scratch, need, hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+int cr_volume_read_occupancy(cr_volume v, uint8_t* host_dst, size_t bytes, void* stream) {
+    if (!v || !host_dst) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_read_occupancy: NULL argument");
+    cr_context ctx = v->ctx;
+    size_t need = (size_t)v->N * v->N * v->N;
+    if (bytes < need) return fail(ctx, CR_EINVAL, "cr_volume_read_occupancy: destination too small");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = scratch(v, need))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_read_state(v->g, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_scratch, s);
+    CR_HIP(ctx, hipGetLastError());
+    CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_scratch, need, hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+int cr_volume_read_position_map(cr_volume v, float* host_dst, size_t bytes, void* stream) {
+    if (!v || !host_dst) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_read_position_map: NULL argument");
+    cr_context ctx = v->ctx;
+    if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_volume_read_position_map: volume not voxelized");
+    size_t need = (size_t)v->lw * v->lh * sizeof(float4);
+    if (bytes < need) return fail(ctx, CR_EINVAL, "cr_volume_read_position_map: destination too small");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_posmap, need, hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+int cr_volume_read_draw_order(cr_volume v, int32_t* host_dst, int count, void* stream) {
+    if (!v || !host_dst) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_read_draw_order: NULL argument");
+    cr_context ctx = v->ctx;
+    if (v->sorted_count < 0) return fail(ctx, CR_ESTATE, "cr_volume_read_draw_order: no cone trace since the last upload");
+    if (count < v->sorted_count) return fail(ctx, CR_EINVAL, "cr_volume_read_draw_order: destination too small");
+    if (v->sorted_count == 0) return CR_OK;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_order, (size_t)v->sorted_count * 4, hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+/*
+ * cloudvox.h - C ABI of the MI355X (gfx950) billboard-cloud hot path:
+ * billboard-sphere voxelization (two passes) into a 3D volume, the volume's
+ * 3D mip chain, and the per-pixel voxel cone trace toward the sun with
+ * compositing, rendered headless into a caller-owned device image.
+ *
+ * It replaces the GL render passes of Cloud-Renderer:
+ *   VoxelizeShader::voxelize(CloudVolume*)   src/Shaders/VoxelizeShader.hpp:18,
+ *                                            src/Shaders/VoxelizeShader.cpp:18-134
+ *   ConeTraceShader::coneTrace(CloudVolume*) src/Shaders/ConeTraceShader.hpp:12,
+ *                                            src/Shaders/ConeTraceShader.cpp:15-98
+ * and the GL objects of CloudVolume that they touch
+ *   CloudVolume(dim, bounds, position, mips) src/CloudVolume.cpp:7-55
+ *   CloudVolume::uploadBillboards             src/CloudVolume.cpp:139-164
+ *   CloudVolume::sortBoards                   src/CloudVolume.cpp:65-82
+ *   Sun::update                               src/Sun.hpp:26-43
+ *
+ * Conventions
+ *   - Every entry point returns an int status (CR_OK = 0, negative on error)
+ *     and never aborts; cr_last_error() describes the last failure on the
+ *     context.  No C++ exception crosses this boundary.
+ *   - Host arrays passed in are copied before the call returns.
+ *   - All device work is enqueued on the caller's hipStream_t (passed as
+ *     void*, NULL = the null stream) and is complete after that stream is
+ *     synchronised.  Calls on one volume must not overlap.
+ *   - The library owns the volume, its mips, bitmaps and scratch buffers; the
+ *     caller owns the output image.
+ *   - Images are stored bottom row first (GL window coordinates).
+ */
+#ifndef CLOUDVOX_H_
+#define CLOUDVOX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CR_ABI_VERSION 1
+
+/* status codes */
+#define CR_OK 0
+#define CR_EINVAL (-1)       /* invalid argument */
+#define CR_ENOMEM (-2)       /* device or host allocation failed */
+#define CR_EDEVICE (-3)      /* HIP runtime error / no device */
+#define CR_ESTATE (-4)       /* call out of order (e.g. cone trace before voxelize) */
+#define CR_EUNSUPPORTED (-5) /* valid but unsupported combination */
+
+/* volume storage format (the reference stores GL_R8, src/CloudVolume.cpp:18) */
+#define CR_FORMAT_R8 0
+#define CR_FORMAT_RGBA32F 1
+#define CR_FORMAT_RGBA16F 2
+
+/* volume content mode */
+#define CR_MODE_REF 0  /* shipped shaders: R = 1 on the lit shell only */
+#define CR_MODE_RGBA 1 /* paper semantics: chord fill (0,0,0,1) + lit shell (1,1,1,1) */
+
+/* cone accumulation */
+#define CR_CONE_WEIGHTED 0    /* res/conetrace_frag.glsl:64-79: sum s.r * i / (steps * downScaling) */
+#define CR_CONE_FTB 1         /* front-to-back, alpha_i = s.a */
+#define CR_CONE_FTB_DENSITY 2 /* front-to-back, alpha_i = 1 - 2^(-sigma*log2(e)*coneRadius_i/N * s.a) */
+
+/* pixel compositing */
+#define CR_COMPOSITE_GL_BLEND 0 /* src/main.cpp:94-95 blend, back to front, all fragments */
+#define CR_COMPOSITE_FTB 1      /* front to back, early-out at alpha_stop */
+
+/* output pixel format */
+#define CR_PIXEL_RGBA32F 0
+#define CR_PIXEL_RGBA8 1
+
+typedef struct cr_context_s *cr_context;
+typedef struct cr_volume_s *cr_volume;
+
+/* Sun (src/Sun.hpp:12-19, src/main.cpp:37).  position drives the light view
+ * of the voxelization exactly as Sun::update does; the cone direction is
+ * (position - point) for a point light (reference) or `direction` when
+ * directional != 0. */
+typedef struct {
+    float position[3];
+    float direction[3];
+    int directional;
+} cr_light;
+
+/* Camera (src/Camera.cpp:59-60): V = lookAt(eye, target, up),
+ * P = perspective(fovy, W/H, znear, zfar). */
+typedef struct {
+    float eye[3];
+    float target[3];
+    float up[3];
+    float fovy; /* radians */
+    float znear;
+    float zfar;
+} cr_camera;
+
+/* Cone-trace parameters (src/Shaders/ConeTraceShader.hpp:28-32 defaults:
+ * steps 16, angle 0.9, initial height 0.1, lod offset 0, down scaling 1). */
+typedef struct {
+    int steps;
+    float cone_angle;     /* full aperture, radians */
+    float initial_height; /* voxels */
+    float lod_offset;
+    float down_scaling;
+    int cone_mode;        /* CR_CONE_* */
+    float alpha_stop;     /* early-out threshold of the front-to-back modes */
+    float sigma;          /* extinction per unit texture length (CR_CONE_FTB_DENSITY) */
+    int composite;        /* CR_COMPOSITE_* */
+    int quantize_rgba8;   /* GL_BLEND into an RGBA8 framebuffer: round after every blend */
+    int compensated;      /* Kahan-compensated front-to-back accumulation */
+    float clear_color[4]; /* glClearColor (src/main.cpp:112), GL_BLEND composite only */
+} cr_cone_params;
+
+/* Per-call counters (filled when profiling is enabled on the context). */
+typedef struct {
+    int64_t light_fragments; /* covered pass-1 fragments */
+    int64_t shell_texels;    /* position-map texels with a > 0 */
+    int64_t fragments;       /* shaded camera fragments */
+    int64_t samples;         /* cone samples taken */
+    int64_t two_level;       /* samples that blended two mip levels */
+    float ms_clear, ms_light_raster, ms_splat, ms_mip;  /* voxelize stages */
+    float ms_setup, ms_sort, ms_bin, ms_cone;           /* cone-trace stages */
+} cr_stats;
+
+/* ---- context ---------------------------------------------------------- */
+int cr_context_create(int device, unsigned flags, cr_context *out);
+int cr_context_destroy(cr_context ctx);
+const char *cr_last_error(cr_context ctx);
+/* profiling != 0: record hipEvents around every stage and count fragments
+ * and samples (adds a few tiny kernels and a stream synchronisation per call). */
+int cr_context_set_profiling(cr_context ctx, int profiling);
+int cr_get_stats(cr_context ctx, cr_stats *out);
+int cr_abi_version(void);
+
+/* ---- volume (CloudVolume, src/CloudVolume.cpp:7-55) -------------------- */
+/* bounds_lo/hi are (xBounds.x, yBounds.x, zBounds.x) / (.y ...), relative
+ * to position, as in CloudVolume.  levels in [1, log2(dim) + 1]. */
+int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const float bounds_hi[3],
+                     const float position[3], int levels, int format, int mode, cr_volume *out);
+int cr_volume_destroy(cr_volume vol);
+/* Density modulation by fractal value noise (rgba mode), octaves = 0 disables. */
+int cr_volume_set_noise(cr_volume vol, int octaves, uint32_t seed, float base_freq, float lacunarity,
+                        float gain);
+/* CloudVolume::uploadBillboards (src/CloudVolume.cpp:139-164): billboard
+ * centres relative to position (xyz, 3*count floats) and scales; the radius
+ * used is scale * fluffiness.  count = 0 is allowed (nothing is drawn). */
+int cr_volume_set_billboards(cr_volume vol, const float *xyz, const float *scales, int count,
+                             float fluffiness);
+
+/* ---- the two render passes ------------------------------------------- */
+/* VoxelizeShader::voxelize (src/Shaders/VoxelizeShader.cpp:18-118): clear,
+ * pass 1 from the sun's orthographic view into a light_w x light_h position
+ * map with a depth test, pass 2 splat of 9 voxels per texel, mip chain. */
+int cr_voxelize(cr_volume vol, const cr_light *light, int light_w, int light_h, void *stream);
+
+/* ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82):
+ * sorts billboards far to near from the camera (CloudVolume::sortBoards),
+ * shades every camera fragment with one cone toward the sun through the
+ * mip chain and composites into d_image (W x H, row pitch in bytes). */
+int cr_cone_trace(cr_volume vol, const cr_light *light, const cr_camera *camera,
+                  const cr_cone_params *params, void *d_image, int width, int height,
+                  size_t pitch_bytes, int pixel_format, void *stream);
+
+/* ---- readback (synchronise `stream`) ---------------------------------- */
+/* Occupied level-0 voxels and an order-independent 64-bit hash:
+ * sum over occupied voxels of splitmix64(linear_index * 4 + state),
+ * state 2 = lit shell, 1 = chord fill only; linear_index = x + N*(y + N*z).
+ * The reference's figure is VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115). */
+int cr_volume_occupancy(cr_volume vol, uint64_t *count, uint64_t *hash, void *stream);
+/* Level `level` in x-fastest linear order and native element type
+ * (u8 / float / binary16), channels 1 (R8) or 4. */
+int cr_volume_read_level(cr_volume vol, int level, void *host_dst, size_t bytes, void *stream);
+/* level-0 state bytes (0 empty, 1 fill, 2 shell), N^3, x-fastest */
+int cr_volume_read_occupancy(cr_volume vol, uint8_t *host_dst, size_t bytes, void *stream);
+/* pass-1 position map (RGBA32F, light_w x light_h) of the last cr_voxelize */
+int cr_volume_read_position_map(cr_volume vol, float *host_dst, size_t bytes, void *stream);
+/* draw order of the last cr_cone_trace: order[r] = billboard drawn r-th (far to near) */
+int cr_volume_read_draw_order(cr_volume vol, int32_t *host_dst, int count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLOUDVOX_H_ */
