@@ -67,8 +67,9 @@ class Context:
         if profiling:
             self.set_profiling(True)
 
-    def set_profiling(self, on: bool) -> None:
-        _check(lib().cr_context_set_profiling(self.handle, 1 if on else 0), self.handle)
+    def set_profiling(self, level) -> None:
+        """0 off, 1 per-call counters and stage times (synchronises), 2 stage times summed without syncs."""
+        _check(lib().cr_context_set_profiling(self.handle, int(level)), self.handle)
 
     def stats(self) -> dict:
         s = cr_stats()

@@ -60,7 +60,8 @@ class cr_stats(C.Structure):
                 ("samples", C.c_int64), ("two_level", C.c_int64),
                 ("ms_clear", C.c_float), ("ms_light_raster", C.c_float), ("ms_splat", C.c_float),
                 ("ms_mip", C.c_float), ("ms_setup", C.c_float), ("ms_sort", C.c_float),
-                ("ms_bin", C.c_float), ("ms_cone", C.c_float)]
+                ("ms_bin", C.c_float), ("ms_cone", C.c_float), ("n_voxelize", C.c_int64), ("n_cone", C.c_int64),
+                ("chord_stores", C.c_int64)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/cloudvox.h

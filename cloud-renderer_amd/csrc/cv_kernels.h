@@ -12,6 +12,8 @@ constexpr int kMaxLevels = 16;
 constexpr int kMaxSteps = 256;
 constexpr int kTile = 8;           // camera tiles are 8x8 pixels = one wavefront
 constexpr int kLdsSortMax = 16384; // billboards sorted in one workgroup's LDS
+constexpr int kChordLdsWords = 4096; // LDS bitmap (bricks) per pass-1 tile for chord-fill aggregation
+constexpr int kLightTile = 32;       // pass-1 work item: 32 x 32 light pixels of one billboard
 
 // One mip level in the bricked layout: 4x4x4 bricks, texels Morton-ordered
 // inside a brick (bit order x0 y0 z0 x1 y1 z1), bricks x-fastest.
@@ -25,10 +27,14 @@ struct LevelDesc {
 struct GridDesc {
     float lo[3];     // world bounds low corner = position + bounds_lo
     float range[3];  // world bounds extent
+    float inv_range[3];  // 1 / range, used only when range is a power of two
     float vpos[3];   // CloudVolume::position (billboard centres are relative)
     float stepSize;  // min(voxelSize)
+    float invN;      // 1 / N, used only when N is a power of two
     int N;           // voxels per side
     uint32_t nb0;    // level-0 bricks per side
+    int pow2_range;  // every range[a] is a power of two: x / range[a] == x * inv_range[a] exactly
+    int pow2_N;      // N is a power of two: x / N == x * invN exactly
 };
 
 // Light view of pass 1 (Sun::update output + derived billboard basis)
@@ -82,11 +88,22 @@ struct ConeDesc {
     uint64_t pitch;
 };
 
+// pass-1 work list: per-billboard pixel box and tile count, exclusive scan
+// (offsets[S] = total), and (billboard, tile) items
+struct LightScratch {
+    int4* lbox;
+    int* counts;
+    int* offsets;   // S + 1
+    int2* items;
+    size_t items_cap;
+};
+
 // ---- launch wrappers (cv_kernels.hip) ----
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s);
+size_t light_items_bound(int S, float max_radius, const LightView& lv);
 void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
-                         unsigned long long* keys, unsigned long long* fillbits, unsigned long long* counters,
-                         hipStream_t s);
+                         const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
+                         unsigned long long* counters, hipStream_t s);
 void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards,
                         const unsigned long long* keys, float4* posmap, unsigned long long* shellbits,
                         unsigned long long* counters, hipStream_t s);
@@ -95,8 +112,8 @@ void launch_level0(const GridDesc& g, const VolumeDesc& vd, const unsigned long 
 void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s);
 void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boards, int S, float4* sph_world,
                       float4* sph_view, int4* sph_tiles, unsigned long long* keys, hipStream_t s);
-int launch_sort(unsigned long long* keys, int S, const float4* sph_world, const CameraView& cv, int32_t* order,
-                int* tie_flag, hipStream_t s);
+size_t sort_scratch_words(int S);
+int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s);
 void launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
                 unsigned long long* masks, hipStream_t s);
 void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,

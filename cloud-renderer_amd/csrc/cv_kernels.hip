@@ -60,16 +60,43 @@ __device__ __forceinline__ float sphere_contrib(float du, float dv, float s) {
     return sqrtf(fmaxf(0.0f, 1.0f - q * q));
 }
 
+// x / range[a].  When range[a] is a power of two, 1 / range[a] is exact and
+// x * (1 / range[a]) rounds the same real number as x / range[a], so the
+// result is identical bit for bit (subnormals included) at a fraction of the
+// cost of the IEEE division sequence.
+__device__ __forceinline__ float div_range(const GridDesc& g, float x, int a) {
+    return g.pow2_range ? x * g.inv_range[a] : x / g.range[a];
+}
+__device__ __forceinline__ float div_N(const GridDesc& g, float x) {
+    return g.pow2_N ? x * g.invN : x / (float)g.N;
+}
+
 // calculateVoxelIndex with image-store bounds (res/first_voxelize.glsl:23-37)
 __device__ __forceinline__ bool voxel_word_bit(const GridDesc& g, float px, float py, float pz, uint32_t* word,
                                                uint32_t* bit) {
     float fN = (float)g.N;
-    float lx = fN * ((px - g.lo[0]) / g.range[0]);
-    float ly = fN * ((py - g.lo[1]) / g.range[1]);
-    float lz = fN * ((pz - g.lo[2]) / g.range[2]);
+    float lx = fN * div_range(g, px - g.lo[0], 0);
+    float ly = fN * div_range(g, py - g.lo[1], 1);
+    float lz = fN * div_range(g, pz - g.lo[2], 2);
     if (!(lx > -1.0f && lx < fN && ly > -1.0f && ly < fN && lz > -1.0f && lz < fN)) return false;
     uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
     *word = ((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2);
+    *bit = morton6(ix & 3u, iy & 3u, iz & 3u);
+    return true;
+}
+
+// same map, returning the brick coordinates and the bit inside the brick word
+__device__ __forceinline__ bool voxel_brick_bit(const GridDesc& g, float px, float py, float pz, int* bx, int* by,
+                                                int* bz, uint32_t* bit) {
+    float fN = (float)g.N;
+    float lx = fN * div_range(g, px - g.lo[0], 0);
+    float ly = fN * div_range(g, py - g.lo[1], 1);
+    float lz = fN * div_range(g, pz - g.lo[2], 2);
+    if (!(lx > -1.0f && lx < fN && ly > -1.0f && ly < fN && lz > -1.0f && lz < fN)) return false;
+    uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
+    *bx = (int)(ix >> 2);
+    *by = (int)(iy >> 2);
+    *bz = (int)(iz >> 2);
     *bit = morton6(ix & 3u, iy & 3u, iz & 3u);
     return true;
 }
@@ -142,29 +169,129 @@ __device__ __forceinline__ bool light_fragment(const LightView& lv, const LightS
     return true;
 }
 
+// Work decomposition of pass 1: every billboard's pixel box is cut into
+// kLightTile x kLightTile tiles; one workgroup rasterizes one (billboard, tile)
+// item, so large spheres spread over many CUs instead of forming the tail.
+__global__ void __launch_bounds__(256) k_light_setup(GridDesc g, LightView lv, const float4* __restrict__ boards, int S,
+                                                     int4* __restrict__ lbox, int* __restrict__ counts) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S) return;
+    LightSphere o = light_sphere(g, lv, boards[j]);
+    int4 box = make_int4(1, 1, 0, 0);
+    int cnt = 0;
+    if (o.live) {
+        // conservative pixel box of the billboard square
+        float xa = (((o.vx - o.s) - lv.lox) / lv.lsx + 1.0f) * 0.5f * (float)lv.W - 0.5f;
+        float xb = (((o.vx + o.s) - lv.lox) / lv.lsx + 1.0f) * 0.5f * (float)lv.W - 0.5f;
+        float ya = (((o.vy - o.s) - lv.loy) / lv.lsy + 1.0f) * 0.5f * (float)lv.H - 0.5f;
+        float yb = (((o.vy + o.s) - lv.loy) / lv.lsy + 1.0f) * 0.5f * (float)lv.H - 0.5f;
+        float fx0 = floorf(fminf(xa, xb)) - 1.0f, fx1 = ceilf(fmaxf(xa, xb)) + 1.0f;
+        float fy0 = floorf(fminf(ya, yb)) - 1.0f, fy1 = ceilf(fmaxf(ya, yb)) + 1.0f;
+        if (fx1 >= 0.0f && fy1 >= 0.0f && fx0 <= (float)(lv.W - 1) && fy0 <= (float)(lv.H - 1)) {
+            box.x = (int)fmaxf(fx0, 0.0f);
+            box.z = (int)fminf(fx1, (float)(lv.W - 1));
+            box.y = (int)fmaxf(fy0, 0.0f);
+            box.w = (int)fminf(fy1, (float)(lv.H - 1));
+            cnt = ((box.z - box.x) / kLightTile + 1) * ((box.w - box.y) / kLightTile + 1);
+        }
+    }
+    lbox[j] = box;
+    counts[j] = cnt;
+}
+
+// exclusive scan of counts[0..n) into offsets[0..n], one workgroup of 1024
+__global__ void __launch_bounds__(1024) k_scan_counts(const int* __restrict__ counts, int n, int* __restrict__ offsets) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + tid;
+        int v = i < n ? counts[i] : 0;
+        int x = v;  // inclusive wave scan
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        int wpre = 0;
+        for (int w = 0; w < wid; ++w) wpre += wsum[w];
+        int c = carry;
+        if (i < n) offsets[i] = c + wpre + x - v;
+        __syncthreads();
+        if (tid == 1023) carry = c + wpre + x;
+        __syncthreads();
+    }
+    if (tid == 0) offsets[n] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ counts, const int* __restrict__ offsets,
+                                                      int S, int2* __restrict__ items, int cap) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S) return;
+    int c = counts[j], o = offsets[j];
+    for (int t = 0; t < c && o + t < cap; ++t) items[o + t] = make_int2(j, t);
+}
+
 __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, int chord,
                                                       const float4* __restrict__ boards,
+                                                      const int4* __restrict__ lbox, const int* __restrict__ total_ptr,
+                                                      const int2* __restrict__ items, int cap,
                                                       unsigned long long* __restrict__ keys,
                                                       unsigned long long* __restrict__ fillbits,
                                                       unsigned long long* __restrict__ counters) {
-    const int j = blockIdx.x;
-    LightSphere o = light_sphere(g, lv, boards[j]);
-    if (!o.live) return;
-    // conservative pixel box of the billboard square
-    float xa = (((o.vx - o.s) - lv.lox) / lv.lsx + 1.0f) * 0.5f * (float)lv.W - 0.5f;
-    float xb = (((o.vx + o.s) - lv.lox) / lv.lsx + 1.0f) * 0.5f * (float)lv.W - 0.5f;
-    float ya = (((o.vy - o.s) - lv.loy) / lv.lsy + 1.0f) * 0.5f * (float)lv.H - 0.5f;
-    float yb = (((o.vy + o.s) - lv.loy) / lv.lsy + 1.0f) * 0.5f * (float)lv.H - 0.5f;
-    float fx0 = floorf(fminf(xa, xb)) - 1.0f, fx1 = ceilf(fmaxf(xa, xb)) + 1.0f;
-    float fy0 = floorf(fminf(ya, yb)) - 1.0f, fy1 = ceilf(fmaxf(ya, yb)) + 1.0f;
-    if (!(fx1 >= 0.0f && fy1 >= 0.0f && fx0 <= (float)(lv.W - 1) && fy0 <= (float)(lv.H - 1))) return;
-    int x0 = (int)fmaxf(fx0, 0.0f), x1 = (int)fminf(fx1, (float)(lv.W - 1));
-    int y0 = (int)fmaxf(fy0, 0.0f), y1 = (int)fminf(fy1, (float)(lv.H - 1));
-    int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-    int npix = bw * bh;
+    // Chord stores of one tile stay inside the sphere's slab behind that
+    // tile: gather them in an LDS bitmap over the slab's brick box (1-voxel
+    // margin for rounding) and flush non-zero words with one global atomic
+    // each.  Stores falling outside the box (none expected) go straight to HBM.
+    __shared__ unsigned long long lbits[kChordLdsWords];
+    const int total = min(*total_ptr, cap);
     unsigned long long nfrag = 0, nchord = 0;
-    for (int t = threadIdx.x; t < npix; t += blockDim.x) {
-        int px = x0 + t % bw, py = y0 + t / bw;
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int2 it = items[item];
+    const int j = it.x;
+    LightSphere o = light_sphere(g, lv, boards[j]);
+    const int4 box = lbox[j];
+    const int ntx = (box.z - box.x) / kLightTile + 1;
+    const int x0 = box.x + (it.y % ntx) * kLightTile, y0 = box.y + (it.y / ntx) * kLightTile;
+    const int x1 = min(x0 + kLightTile - 1, box.z), y1 = min(y0 + kLightTile - 1, box.w);
+
+    int bx0 = 0, by0 = 0, bz0 = 0, nbx = 0, nby = 0, nbz = 0;
+    bool use_lds = false;
+    if (chord) {
+        // world box of the chords behind the tile: c + du*S + dv*U + t*B,
+        // du/dv over the tile's view-plane rectangle (within the disc), |t| <= s
+        float du0 = fmaxf((pixel_ndc(x0, lv.W) * lv.lsx + lv.lox) - o.vx, -o.s);
+        float du1 = fminf((pixel_ndc(x1, lv.W) * lv.lsx + lv.lox) - o.vx, o.s);
+        float dv0 = fmaxf((pixel_ndc(y0, lv.H) * lv.lsy + lv.loy) - o.vy, -o.s);
+        float dv1 = fminf((pixel_ndc(y1, lv.H) * lv.lsy + lv.loy) - o.vy, o.s);
+        const float fN = (float)g.N;
+        const float c3[3] = {o.cx, o.cy, o.cz};
+        int b0[3], b1[3];
+        for (int a = 0; a < 3; ++a) {
+            float su0 = du0 * lv.S[a], su1 = du1 * lv.S[a], sv0 = dv0 * lv.U[a], sv1 = dv1 * lv.U[a];
+            float tb = o.s * fabsf(lv.B[a]);
+            float wlo = c3[a] + fminf(su0, su1) + fminf(sv0, sv1) - tb;
+            float whi = c3[a] + fmaxf(su0, su1) + fmaxf(sv0, sv1) + tb;
+            float l0 = fN * ((wlo - g.lo[a]) / g.range[a]) - 1.0f;
+            float l1 = fN * ((whi - g.lo[a]) / g.range[a]) + 1.0f;
+            float f0 = fmaxf(floorf(l0 * 0.25f), 0.0f), f1 = fminf(floorf(l1 * 0.25f), (float)(g.nb0 - 1));
+            b0[a] = (int)f0;
+            b1[a] = f1 >= f0 ? (int)f1 : (int)f0 - 1;
+        }
+        bx0 = b0[0]; by0 = b0[1]; bz0 = b0[2];
+        nbx = b1[0] - b0[0] + 1; nby = b1[1] - b0[1] + 1; nbz = b1[2] - b0[2] + 1;
+        use_lds = du0 <= du1 && dv0 <= dv1 && nbx > 0 && nby > 0 && nbz > 0 && nbx * nby * nbz <= kChordLdsWords;
+        if (use_lds)
+            for (int w = threadIdx.x; w < nbx * nby * nbz; w += blockDim.x) lbits[w] = 0ull;
+        __syncthreads();
+    }
+    // 16 x 16 threads sweep the tile
+    const int tx0 = (int)(threadIdx.x & 15u), ty0 = (int)(threadIdx.x >> 4);
+    for (int py = y0 + ty0; py <= y1; py += 16)
+    for (int px = x0 + tx0; px <= x1; px += 16) {
         float fx, fy, fz, sc;
         if (!light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc)) continue;
         ++nfrag;
@@ -174,12 +301,35 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             // for (float i = 0; i < 2 * dist; i += stepSize) store (0,0,0,1)
             float sx = fx - lv.B[0] * dist, sy = fy - lv.B[1] * dist, sz = fz - lv.B[2] * dist;
             float lim = 2.0f * dist;
+            // consecutive samples of one chord mostly share a brick word:
+            // OR them in a register and store once per word change
+            int cbx = -1, cby = -1, cbz = -1;
+            unsigned long long cur_b = 0ull;
+            auto flush = [&](int gx, int gy, int gz, unsigned long long bits) {
+                if (gx < 0) return;
+                if (use_lds) {
+                    int lx = gx - bx0, ly = gy - by0, lz = gz - bz0;
+                    if (lx >= 0 && lx < nbx && ly >= 0 && ly < nby && lz >= 0 && lz < nbz) {
+                        atomicOr(&lbits[(lz * nby + ly) * nbx + lx], bits);
+                        return;
+                    }
+                }
+                atomicOr(&fillbits[((uint32_t)gz * g.nb0 + (uint32_t)gy) * g.nb0 + (uint32_t)gx], bits);
+            };
             for (float i = 0.0f; i < lim; i += g.stepSize) {
                 float qx = sx + lv.B[0] * i, qy = sy + lv.B[1] * i, qz = sz + lv.B[2] * i;
-                uint32_t w, b;
+                int gx, gy, gz;
+                uint32_t b;
                 ++nchord;
-                if (voxel_word_bit(g, qx, qy, qz, &w, &b)) atomicOr(&fillbits[w], 1ull << b);
+                if (!voxel_brick_bit(g, qx, qy, qz, &gx, &gy, &gz, &b)) continue;
+                if (gx != cbx || gy != cby || gz != cbz) {
+                    flush(cbx, cby, cbz, cur_b);
+                    cbx = gx; cby = gy; cbz = gz;
+                    cur_b = 0ull;
+                }
+                cur_b |= 1ull << b;
             }
+            flush(cbx, cby, cbz, cur_b);
         }
         float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
         float ex = wx - lv.np[0], ey = wy - lv.np[1], ez = wz - lv.np[2];
@@ -190,9 +340,27 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             atomicMin(&keys[(size_t)py * lv.W + px], key);
         }
     }
+    if (chord) {
+        __syncthreads();
+        if (use_lds) {
+            for (int w = threadIdx.x; w < nbx * nby * nbz; w += blockDim.x) {
+                unsigned long long bits = lbits[w];
+                if (!bits) continue;
+                int lx = w % nbx, ly = (w / nbx) % nby, lz = w / (nbx * nby);
+                uint32_t gw = ((uint32_t)(lz + bz0) * g.nb0 + (uint32_t)(ly + by0)) * g.nb0 + (uint32_t)(lx + bx0);
+                atomicOr(&fillbits[gw], bits);
+            }
+        }
+        __syncthreads();  // the next item reuses the bitmap
+    }
+    }  // items
     if (counters) {
-        atomicAdd(&counters[0], nfrag);
-        atomicAdd(&counters[1], nchord);
+        nfrag = wave_sum_u64(nfrag);
+        nchord = wave_sum_u64(nchord);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&counters[0], nfrag);
+            atomicAdd(&counters[1], nchord);
+        }
     }
 }
 
@@ -431,10 +599,111 @@ __global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, co
 
 // ---------------------------------------------------------------------------
 // draw order (CloudVolume::sortBoards, src/CloudVolume.cpp:65-82)
-// keys = (~distance bits << 32 | index): ascending = far to near, stable.
-// Equal distances make the reference's selection sort order depend on its
-// swap history, so ties trigger a literal replay (k_sort_select).
+//
+// The reference runs a selection sort: for i, m = first position of the
+// maximum distance in [i, S), swap(i, m).  Keys (~distance bits << 32 | index)
+// sorted ascending give the distinct distances far to near.  Equal distances
+// are frequent in FP32 (thousands of billboards at similar range), and for
+// them the reference's order depends on its swap history: inside a run of
+// equal distances it takes the member with the lowest *current* position.
+// replay_ties() reproduces that exactly in O(S): positions are tracked
+// through every swap (pos: element -> position, at: position -> element)
+// and only the runs of equal distances scan their members.
 // ---------------------------------------------------------------------------
+template <class I, class P>
+__device__ void replay_ties(int S, const I* s_idx, const uint32_t* s_tie, P* pos, P* at,
+                            int32_t* __restrict__ order) {
+    auto place = [&](int k, int sel) {
+        int m = (int)pos[sel];
+        int e = (int)at[k];
+        at[k] = (P)sel;
+        at[m] = (P)e;
+        pos[sel] = (P)k;
+        pos[e] = (P)m;
+        order[k] = sel;
+    };
+    int i = 0;
+    while (i < S) {
+        int j = i + 1;
+        while (j < S && ((s_tie[j >> 5] >> (j & 31)) & 1u)) ++j;
+        if (j == i + 1) {
+            place(i, (int)s_idx[i]);
+            ++i;
+            continue;
+        }
+        for (int k = i; k < j; ++k) {
+            int best = -1, bp = 0x7fffffff;
+            for (int t = i; t < j; ++t) {
+                int e = (int)s_idx[t];
+                int p = (int)pos[e];
+                if (p >= k && p < bp) { bp = p; best = e; }
+            }
+            place(k, best);
+        }
+        i = j;
+    }
+}
+
+// The same replay run by one full wavefront: sorted indices and tie flags are
+// prefetched 64 at a time into lanes, every step reads pos/at once (broadcast)
+// and lane 0 writes; LDS accesses of one wave complete in order, so each step
+// sees the previous step's writes.  Runs of equal distances are scanned by all
+// lanes with a wave min-reduction over current positions.
+__device__ void replay_ties_wave(int S, const uint16_t* s_idx, uint16_t* pos, uint16_t* at,
+                                 int32_t* __restrict__ order) {
+    // s_idx[t]: bits 0-14 element, bit 15 = same distance as entry t-1
+    const int lane = threadIdx.x & 63;
+    auto place = [&](int k, int sel) {
+        int m = (int)pos[sel];
+        int e = (int)at[k];
+        if (lane == 0) {
+            at[k] = (uint16_t)sel;
+            at[m] = (uint16_t)e;
+            pos[sel] = (uint16_t)k;
+            pos[e] = (uint16_t)m;
+            order[k] = sel;
+        }
+    };
+    auto tie_at = [&](int t) -> int { return (int)(s_idx[t] >> 15); };
+    int i = 0;
+    while (i < S) {
+        const int t = i + lane;
+        const int my_idx = t < S ? (int)(s_idx[t] & 0x7fffu) : 0;
+        const int my_next_tie = (t + 1 < S) ? tie_at(t + 1) : 0;
+        const unsigned long long next_tie = __ballot(my_next_tie);
+        const int lim = min(64, S - i);
+        int l = 0;
+        while (l < lim && !((next_tie >> l) & 1ull)) {
+            place(i + l, __builtin_amdgcn_readlane(my_idx, l));
+            ++l;
+        }
+        if (l == lim) {
+            i += lim;
+            continue;
+        }
+        // run of equal distances starting at step gs
+        const int gs = i + l;
+        int ge = gs + 1;
+        while (ge < S && tie_at(ge)) ++ge;
+        for (int k = gs; k < ge; ++k) {
+            int best_p = 0x7fffffff, best_e = -1;
+            for (int q = gs + lane; q < ge; q += 64) {
+                int e = (int)(s_idx[q] & 0x7fffu);
+                int p = (int)pos[e];
+                if (p >= k && p < best_p) { best_p = p; best_e = e; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                int op = __shfl_xor(best_p, off, 64);
+                int oe = __shfl_xor(best_e, off, 64);
+                if (op < best_p) { best_p = op; best_e = oe; }
+            }
+            if (best_e < 0) return;  // unreachable: ge - k members are unplaced, all at positions >= k
+            place(k, best_e);
+        }
+        i = ge;
+    }
+}
+
 __global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S, int npad,
                                                    int32_t* __restrict__ order, int* __restrict__ tie_flag) {
     extern __shared__ unsigned long long sk[];
@@ -456,12 +725,43 @@ __global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __r
             __syncthreads();
         }
     }
+    // Reuse the key storage: idx u16[npad] (bit 15 = same distance as the
+    // previous key) | pos u16[npad] | at u16[npad].  Chunk c of 1024 keys is
+    // read before its indices are written; those writes cover keys
+    // [256 c, 256 c + 256), all of which were read in chunks <= c, and the
+    // previous key of the next chunk (index 1024 (c+1) - 1) is not touched.
+    uint16_t* s_idx = (uint16_t*)sk;
+    uint16_t* s_pos = s_idx + npad;
+    uint16_t* s_at = s_pos + npad;
+    for (int c = 0; c < npad; c += nt) {
+        const int i = c + tid;
+        uint32_t v = 0;
+        if (i < S) {
+            unsigned long long k = sk[i];
+            v = (uint32_t)(k & 0xffffffffull);
+            if (i > 0 && (k >> 32) == (sk[i - 1] >> 32)) {
+                v |= 0x8000u;
+                tie = 1;
+            }
+        }
+        __syncthreads();
+        if (i < S) s_idx[i] = (uint16_t)v;
+        __syncthreads();
+    }
+    if (!tie) {
+        for (int i = tid; i < S; i += nt) order[i] = (int32_t)s_idx[i];
+        if (tid == 0) *tie_flag = 0;
+        return;
+    }
     for (int i = tid; i < S; i += nt) {
-        order[i] = (int32_t)(uint32_t)(sk[i] & 0xffffffffull);
-        if (i + 1 < S && (sk[i] >> 32) == (sk[i + 1] >> 32)) tie = 1;
+        s_pos[i] = (uint16_t)i;
+        s_at[i] = (uint16_t)i;
     }
     __syncthreads();
-    if (tid == 0) *tie_flag = tie;
+    if (tid < 64) {
+        replay_ties_wave(S, s_idx, s_pos, s_at, order);
+        if (tid == 0) *tie_flag = 1;
+    }
 }
 
 // global-memory bitonic stage for more billboards than one LDS holds
@@ -480,64 +780,25 @@ __global__ void k_pad_keys(const unsigned long long* __restrict__ in, unsigned l
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < npad) out[i] = i < S ? in[i] : ~0ull;
 }
+// tie bits / u16 indices of the global path, then the replay in global memory
 __global__ void k_sort_finish(const unsigned long long* __restrict__ sk, int S, int32_t* __restrict__ order,
-                              int* __restrict__ tie_flag) {
+                              uint32_t* __restrict__ g_idx, uint32_t* __restrict__ g_tie, int* __restrict__ tie_flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S) return;
-    order[i] = (int32_t)(uint32_t)(sk[i] & 0xffffffffull);
-    if (i + 1 < S && (sk[i] >> 32) == (sk[i + 1] >> 32)) atomicOr(tie_flag, 1);
+    uint32_t e = (uint32_t)(sk[i] & 0xffffffffull);
+    order[i] = (int32_t)e;
+    g_idx[i] = e;
+    if (i > 0 && (sk[i] >> 32) == (sk[i - 1] >> 32)) {
+        atomicOr(&g_tie[i >> 5], 1u << (i & 31));
+        atomicOr(tie_flag, 1);
+    }
 }
-
-// Literal replay of the reference's selection sort, run only when the fast
-// sort found equal distances: for i: m = first position of the maximum in
-// [i, S); swap(i, m).  One workgroup; dist/idx live in global scratch.
-__global__ void __launch_bounds__(1024) k_sort_select(const int* __restrict__ tie_flag, const float4* __restrict__ sph_world,
-                                                      CameraView cv, int S, float* __restrict__ dist,
-                                                      int32_t* __restrict__ idx, int32_t* __restrict__ order) {
+__global__ void k_sort_replay_global(int S, const uint32_t* __restrict__ g_idx, const uint32_t* __restrict__ g_tie,
+                                     int32_t* __restrict__ pos, int32_t* __restrict__ at, int32_t* __restrict__ order,
+                                     const int* __restrict__ tie_flag) {
     if (*tie_flag == 0) return;
-    __shared__ float wv[16];
-    __shared__ int wp[16];
-    __shared__ int best_pos;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int j = tid; j < S; j += nt) {
-        float4 c = sph_world[j];
-        float ex = cv.eye[0] - c.x, ey = cv.eye[1] - c.y, ez = cv.eye[2] - c.z;
-        dist[j] = sqrtf((ex * ex + ey * ey) + ez * ez);
-        idx[j] = j;
-    }
-    __syncthreads();
-    for (int i = 0; i < S; ++i) {
-        float bv = -INFINITY;
-        int bp = 0x7fffffff;
-        for (int j = i + tid; j < S; j += nt) {
-            float v = dist[j];
-            if (bp == 0x7fffffff || v > bv) { bv = v; bp = j; }  // first maximum in the thread's slice
-        }
-        // wave reduce: larger value wins, equal value -> smaller position
-        for (int off = 32; off > 0; off >>= 1) {
-            float ov = __shfl_xor(bv, off, 64);
-            int op = __shfl_xor(bp, off, 64);
-            bool take = (op != 0x7fffffff) && (bp == 0x7fffffff || ov > bv || (ov == bv && op < bp));
-            if (take) { bv = ov; bp = op; }
-        }
-        if ((tid & 63) == 0) { wv[tid >> 6] = bv; wp[tid >> 6] = bp; }
-        __syncthreads();
-        if (tid == 0) {
-            float v = wv[0];
-            int p = wp[0];
-            for (int w = 1; w < nt / 64; ++w) {
-                bool take = (wp[w] != 0x7fffffff) && (p == 0x7fffffff || wv[w] > v || (wv[w] == v && wp[w] < p));
-                if (take) { v = wv[w]; p = wp[w]; }
-            }
-            best_pos = p;
-            if (p != i && p != 0x7fffffff) {
-                float td = dist[i]; dist[i] = dist[p]; dist[p] = td;
-                int ti = idx[i]; idx[i] = idx[p]; idx[p] = ti;
-            }
-        }
-        __syncthreads();
-    }
-    for (int j = tid; j < S; j += nt) order[j] = idx[j];
+    for (int i = 0; i < S; ++i) { pos[i] = i; at[i] = i; }
+    replay_ties<uint32_t, int32_t>(S, g_idx, g_tie, pos, at, order);
 }
 
 // ---------------------------------------------------------------------------
@@ -706,16 +967,16 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                     float qy = fy + (cv.B[1] * s) * sc;
                     float qz = fz + (cv.B[2] * s) * sc;
                     // calculateVoxelLerp(pos) / voxelDim
-                    float p0x = (fN * ((qx - g.lo[0]) / g.range[0])) / fN;
-                    float p0y = (fN * ((qy - g.lo[1]) / g.range[1])) / fN;
-                    float p0z = (fN * ((qz - g.lo[2]) / g.range[2])) / fN;
+                    float p0x = div_N(g, fN * div_range(g, qx - g.lo[0], 0));
+                    float p0y = div_N(g, fN * div_range(g, qy - g.lo[1], 1));
+                    float p0z = div_N(g, fN * div_range(g, qz - g.lo[2], 2));
                     float dx, dy, dzz;
                     if (cd.directional) {
                         dx = cd.ldir[0]; dy = cd.ldir[1]; dzz = cd.ldir[2];
                     } else {
                         float lx = cd.lpos[0] - qx, ly = cd.lpos[1] - qy, lz = cd.lpos[2] - qz;
                         float inv = 1.0f / sqrtf((lx * lx + ly * ly) + lz * lz);
-                        dx = (lx * inv) / fN; dy = (ly * inv) / fN; dzz = (lz * inv) / fN;
+                        dx = div_N(g, lx * inv); dy = div_N(g, ly * inv); dzz = div_N(g, lz * inv);
                     }
                     float s0, s1, s2, s3;
                     if (CONE == 0) {
@@ -889,11 +1150,27 @@ void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s) {
 }
 
 void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
-                         unsigned long long* keys, unsigned long long* fillbits, unsigned long long* counters,
-                         hipStream_t s) {
+                         const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
+                         unsigned long long* counters, hipStream_t s) {
     if (S <= 0) return;
-    hipLaunchKernelGGL(k_light_raster, dim3(S), dim3(256), 0, s, g, lv, mode == 1 ? 1 : 0, boards, keys, fillbits,
-                       counters);
+    hipLaunchKernelGGL(k_light_setup, dim3(blocks_for(S, 256)), dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, ls.counts, S, ls.offsets);
+    const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
+    hipLaunchKernelGGL(k_light_expand, dim3(blocks_for(S, 256)), dim3(256), 0, s, ls.counts, ls.offsets, S, ls.items,
+                       cap);
+    unsigned grid = (unsigned)std::min<size_t>(ls.items_cap, 4096);
+    hipLaunchKernelGGL(k_light_raster, dim3(grid), dim3(256), 0, s, g, lv, mode == 1 ? 1 : 0, boards, ls.lbox,
+                       ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+}
+
+size_t light_items_bound(int S, float max_radius, const LightView& lv) {
+    // widest pixel box of a billboard of radius max_radius: the box spans
+    // s * W / lsx pixels plus at most 5 from floor/ceil and the 1-pixel margins
+    double wx = 2.0 * max_radius * lv.W / (2.0 * lv.lsx) + 6.0;
+    double wy = 2.0 * max_radius * lv.H / (2.0 * lv.lsy) + 6.0;
+    size_t tx = std::min<size_t>((size_t)(wx / kLightTile) + 2, (size_t)(lv.W / kLightTile) + 1);
+    size_t ty = std::min<size_t>((size_t)(wy / kLightTile) + 2, (size_t)(lv.H / kLightTile) + 1);
+    return std::max<size_t>((size_t)S * tx * ty, 1);
 }
 
 void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards, const unsigned long long* keys,
@@ -923,10 +1200,16 @@ void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boa
                        sph_tiles, keys);
 }
 
-// returns the padded key count (the scratch `keys` must hold 2 * npad when
-// S exceeds the LDS sort)
-int launch_sort(unsigned long long* keys, int S, const float4* sph_world, const CameraView& cv, int32_t* order,
-                int* tie_flag, hipStream_t s) {
+// Scratch `keys` (sort_scratch_words(S) u64 words): [0, npad) the keys written
+// by k_cam_setup; for S beyond one LDS: [npad, 2 npad) padded keys, then
+// idx u32[npad], tie bits u32[npad / 32], pos i32[npad], at i32[npad].
+size_t sort_scratch_words(int S) {
+    size_t npad = 2;
+    while (npad < (size_t)S) npad <<= 1;
+    return 5 * npad;
+}
+
+int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s) {
     if (S <= 0) return 0;
     int npad = 2;
     while (npad < S) npad <<= 1;
@@ -939,19 +1222,21 @@ int launch_sort(unsigned long long* keys, int S, const float4* sph_world, const 
         }
         hipLaunchKernelGGL(k_sort_lds, dim3(1), dim3(1024), (size_t)npad * sizeof(unsigned long long), s, keys, S, npad,
                            order, tie_flag);
-    } else {
-        unsigned long long* pk = keys + npad;  // second half of the scratch
-        hipLaunchKernelGGL(k_pad_keys, dim3(blocks_for(npad, 256)), dim3(256), 0, s, keys, pk, S, npad);
-        for (int k = 2; k <= npad; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1)
-                hipLaunchKernelGGL(k_bitonic_global, dim3(blocks_for(npad, 256)), dim3(256), 0, s, pk, npad, j, k);
-        launch_fill_u64((uint64_t*)tie_flag, 1, 0, s);  // d_tie holds 64 bytes
-        hipLaunchKernelGGL(k_sort_finish, dim3(blocks_for(S, 256)), dim3(256), 0, s, pk, S, order, tie_flag);
+        return npad;
     }
-    // scratch for the selection-sort replay: reuse the key buffer past npad*2
-    float* dist = (float*)(keys + 2 * (size_t)npad);
-    int32_t* idx = (int32_t*)(dist + npad);
-    hipLaunchKernelGGL(k_sort_select, dim3(1), dim3(1024), 0, s, tie_flag, sph_world, cv, S, dist, idx, order);
+    unsigned long long* pk = keys + npad;
+    uint32_t* g_idx = (uint32_t*)(keys + 2 * (size_t)npad);
+    uint32_t* g_tie = g_idx + npad;
+    int32_t* pos = (int32_t*)(g_tie + npad / 32);
+    int32_t* at = pos + npad;
+    hipLaunchKernelGGL(k_pad_keys, dim3(blocks_for(npad, 256)), dim3(256), 0, s, keys, pk, S, npad);
+    for (int k = 2; k <= npad; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1)
+            hipLaunchKernelGGL(k_bitonic_global, dim3(blocks_for(npad, 256)), dim3(256), 0, s, pk, npad, j, k);
+    launch_fill_u64((uint64_t*)tie_flag, 1, 0, s);  // the flag buffer holds 64 bytes
+    launch_fill_u64((uint64_t*)g_tie, (uint64_t)npad / 64, 0, s);
+    hipLaunchKernelGGL(k_sort_finish, dim3(blocks_for(S, 256)), dim3(256), 0, s, pk, S, order, g_idx, g_tie, tie_flag);
+    hipLaunchKernelGGL(k_sort_replay_global, dim3(1), dim3(1), 0, s, S, g_idx, g_tie, pos, at, order, tie_flag);
     return npad;
 }
 

@@ -28,9 +28,12 @@ struct cr_context_s {
     int device = 0;
     std::string err;
     int profiling = 0;
-    cr_stats stats{};
-    hipEvent_t ev[10] = {};
-    bool events = false;
+    cr_stats stats{};  // level 1: last call; level 2: sums since the last cr_get_stats
+    // event pool: level 1 reuses the first 10 events, level 2 takes 5 per call
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    struct Pending { int kind; size_t base; };  // kind 0 voxelize, 1 cone trace
+    std::vector<Pending> pending;
 };
 
 struct cr_volume_s {
@@ -53,6 +56,10 @@ struct cr_volume_s {
     unsigned long long* d_keys = nullptr;
     float4* d_posmap = nullptr;
     size_t light_cap = 0;
+    int4* d_lbox = nullptr;
+    int* d_lcounts = nullptr;  // counts (S) followed by offsets (S + 1)
+    int2* d_litems = nullptr;
+    size_t lbox_cap = 0, lcounts_cap = 0, litems_cap = 0;
     // camera pass
     float4* d_sph_world = nullptr;
     float4* d_sph_view = nullptr;
@@ -185,15 +192,79 @@ int set_device(cr_context ctx) {
     return CR_OK;
 }
 
-void stage_event(cr_context ctx, int i, hipStream_t s) {
-    if (ctx->profiling && ctx->events) (void)hipEventRecord(ctx->ev[i], s);
-}
-
-float elapsed(cr_context ctx, int a, int b) {
+float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return -1.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0f;
     return ms;
 }
+
+// Accumulate the stage times of every pending level-2 call into ctx->stats.
+int drain_pending(cr_context ctx) {
+    for (const auto& p : ctx->pending) {
+        hipEvent_t* e = &ctx->pool[p.base];
+        hipError_t err = hipEventSynchronize(e[4]);
+        if (err != hipSuccess) return fail(ctx, CR_EDEVICE, "hipEventSynchronize: %s", hipGetErrorString(err));
+        float t[4];
+        for (int k = 0; k < 4; ++k) t[k] = elapsed(e[k], e[k + 1]);
+        cr_stats& s = ctx->stats;
+        if (p.kind == 0) {
+            s.ms_clear += t[0]; s.ms_light_raster += t[1]; s.ms_splat += t[2]; s.ms_mip += t[3];
+            ++s.n_voxelize;
+        } else {
+            s.ms_setup += t[0]; s.ms_sort += t[1]; s.ms_bin += t[2]; s.ms_cone += t[3];
+            ++s.n_cone;
+        }
+    }
+    ctx->pending.clear();
+    ctx->pool_used = 0;
+    return CR_OK;
+}
+
+// Stage events of one API call: five events bracketing four stages.
+struct StageEvents {
+    cr_context ctx = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t* e = nullptr;
+    size_t base = 0;
+    int kind = 0;
+
+    int begin(cr_context c, hipStream_t st, int k) {
+        ctx = c; s = st; kind = k; e = nullptr;
+        if (!c->profiling) return CR_OK;
+        if (c->profiling == 1) {
+            base = (size_t)(5 * k);
+            if (c->pool.size() < 10) {
+                while (c->pool.size() < 10) {
+                    hipEvent_t ev;
+                    hipError_t err = hipEventCreate(&ev);
+                    if (err != hipSuccess) return fail(c, CR_EDEVICE, "hipEventCreate: %s", hipGetErrorString(err));
+                    c->pool.push_back(ev);
+                }
+            }
+        } else {
+            if (c->pending.size() >= 1024) {
+                int rc = drain_pending(c);
+                if (rc) return rc;
+            }
+            base = c->pool_used;
+            while (c->pool.size() < base + 5) {
+                hipEvent_t ev;
+                hipError_t err = hipEventCreate(&ev);
+                if (err != hipSuccess) return fail(c, CR_EDEVICE, "hipEventCreate: %s", hipGetErrorString(err));
+                c->pool.push_back(ev);
+            }
+            c->pool_used += 5;
+        }
+        e = &c->pool[base];
+        return CR_OK;
+    }
+    void mark(int i) {
+        if (e) (void)hipEventRecord(e[i], s);
+    }
+    void end() {
+        if (e && ctx->profiling == 2) ctx->pending.push_back({kind, base});
+    }
+};
 
 }  // namespace
 
@@ -227,26 +298,34 @@ int cr_context_create(int device, unsigned flags, cr_context* out) {
 
 int cr_context_destroy(cr_context ctx) {
     if (!ctx) return CR_EINVAL;
-    if (ctx->events)
-        for (auto& e : ctx->ev) (void)hipEventDestroy(e);
+    (void)hipSetDevice(ctx->device);
+    for (auto& e : ctx->pool) (void)hipEventDestroy(e);
     delete ctx;
     return CR_OK;
 }
 
 int cr_context_set_profiling(cr_context ctx, int profiling) {
     if (!ctx) return fail(nullptr, CR_EINVAL, "cr_context_set_profiling: ctx is NULL");
-    if (profiling && !ctx->events) {
-        int rc = set_device(ctx);
-        if (rc) return rc;
-        for (auto& e : ctx->ev) CR_HIP(ctx, hipEventCreate(&e));
-        ctx->events = true;
-    }
-    ctx->profiling = profiling ? 1 : 0;
+    if (profiling < 0 || profiling > 2) return fail(ctx, CR_EINVAL, "cr_context_set_profiling: level must be 0, 1 or 2");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if (ctx->profiling == 2 && (rc = drain_pending(ctx))) return rc;
+    ctx->profiling = profiling;
+    ctx->stats = cr_stats{};
+    ctx->pool_used = 0;
     return CR_OK;
 }
 
 int cr_get_stats(cr_context ctx, cr_stats* out) {
     if (!ctx || !out) return fail(ctx, CR_EINVAL, "cr_get_stats: NULL argument");
+    if (ctx->profiling == 2) {
+        int rc = set_device(ctx);
+        if (rc) return rc;
+        if ((rc = drain_pending(ctx))) return rc;
+        *out = ctx->stats;
+        ctx->stats = cr_stats{};
+        return CR_OK;
+    }
     *out = ctx->stats;
     return CR_OK;
 }
@@ -301,6 +380,15 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
         vs[a] = (bounds_hi[a] - bounds_lo[a]) / (float)dim;
     }
     v->g.stepSize = std::min(vs[0], std::min(vs[1], vs[2]));
+    // exact reciprocals for power-of-two extents (x / 2^k == x * 2^-k in IEEE arithmetic)
+    auto is_pow2 = [](float x) {  // 2^k with 2^-k a normal float
+        int e;
+        return x >= 0x1p-125f && x <= 0x1p125f && std::frexp(x, &e) == 0.5f;
+    };
+    v->g.pow2_range = is_pow2(v->g.range[0]) && is_pow2(v->g.range[1]) && is_pow2(v->g.range[2]) ? 1 : 0;
+    for (int a = 0; a < 3; ++a) v->g.inv_range[a] = 1.0f / v->g.range[a];
+    v->g.pow2_N = is_pow2((float)dim) ? 1 : 0;
+    v->g.invN = 1.0f / (float)dim;
     v->g.N = dim;
     v->g.nb0 = v->vd.lv[0].nb;
     v->nbricks0 = v->g.nb0 * v->g.nb0 * v->g.nb0;
@@ -330,9 +418,9 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
 int cr_volume_destroy(cr_volume v) {
     if (!v) return CR_EINVAL;
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
-    void* ptrs[] = {v->d_levels, v->d_shell, v->d_fill, v->d_boards, v->d_keys, v->d_posmap, v->d_sph_world,
-                    v->d_sph_view, v->d_sph_tiles, v->d_sort, v->d_order, v->d_tie, v->d_masks, v->d_counters,
-                    v->d_scratch};
+    void* ptrs[] = {v->d_levels, v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
+                    v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
+                    v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -416,32 +504,49 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     }
     v->lw = light_w;
     v->lh = light_h;
-    unsigned long long* counters = ctx->profiling ? v->d_counters : nullptr;
-    stage_event(ctx, 0, s);
+    LightScratch ls{};
+    if (v->S > 0) {
+        if ((rc = ensure(ctx, &v->d_lbox, (size_t)v->S, &v->lbox_cap))) return rc;
+        if ((rc = ensure(ctx, &v->d_lcounts, 2 * (size_t)v->S + 1, &v->lcounts_cap))) return rc;
+        size_t nitems = light_items_bound(v->S, v->max_radius, lv);
+        if ((rc = ensure(ctx, &v->d_litems, nitems, &v->litems_cap))) return rc;
+        ls.lbox = v->d_lbox;
+        ls.counts = v->d_lcounts;
+        ls.offsets = v->d_lcounts + v->S;
+        ls.items = v->d_litems;
+        ls.items_cap = nitems;
+    }
+    unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
+    StageEvents ev;
+    if ((rc = ev.begin(ctx, s, 0))) return rc;
+    ev.mark(0);
     if (counters) launch_fill_u64((uint64_t*)counters, 8, 0, s);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187)
     launch_fill_u64((uint64_t*)v->d_keys, npx, ~0ull, s);
     launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, s);
     launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, s);
-    stage_event(ctx, 1, s);
-    launch_light_raster(v->g, lv, v->mode, v->d_boards, v->S, v->d_keys, v->d_fill, counters, s);
-    stage_event(ctx, 2, s);
+    ev.mark(1);
+    launch_light_raster(v->g, lv, v->mode, v->d_boards, v->S, ls, v->d_keys, v->d_fill, counters, s);
+    ev.mark(2);
     launch_light_splat(v->g, lv, v->d_boards, v->d_keys, v->d_posmap, v->d_shell, counters, s);
-    stage_event(ctx, 3, s);
+    ev.mark(3);
     launch_level0(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
     for (int k = 1; k < v->L; ++k) launch_mip(v->vd, k, v->d_levels, s);
-    stage_event(ctx, 4, s);
+    ev.mark(4);
+    ev.end();
     CR_HIP(ctx, hipGetLastError());
-    if (ctx->profiling) {
+    if (ctx->profiling == 1) {
         CR_HIP(ctx, hipStreamSynchronize(s));
         unsigned long long c[8];
         CR_HIP(ctx, hipMemcpy(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost));
         ctx->stats.light_fragments = (int64_t)c[0];
+        ctx->stats.chord_stores = (int64_t)c[1];
         ctx->stats.shell_texels = (int64_t)c[2];
-        ctx->stats.ms_clear = elapsed(ctx, 0, 1);
-        ctx->stats.ms_light_raster = elapsed(ctx, 1, 2);
-        ctx->stats.ms_splat = elapsed(ctx, 2, 3);
-        ctx->stats.ms_mip = elapsed(ctx, 3, 4);
+        ctx->stats.ms_clear = elapsed(ev.e[0], ev.e[1]);
+        ctx->stats.ms_light_raster = elapsed(ev.e[1], ev.e[2]);
+        ctx->stats.ms_splat = elapsed(ev.e[2], ev.e[3]);
+        ctx->stats.ms_mip = elapsed(ev.e[3], ev.e[4]);
+        ctx->stats.n_voxelize = 1;
     }
     v->voxelized = true;
     return CR_OK;
@@ -536,42 +641,43 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
         if ((rc = ensure(ctx, &v->d_sph_tiles, (size_t)S, &c2))) return rc;
         if ((rc = ensure(ctx, &v->d_order, (size_t)S, &c3))) return rc;
         v->sph_cap = (int)std::min(std::min(c0, c1), std::min(c2, c3));
-        size_t npad = 2;
-        while (npad < (size_t)S) npad <<= 1;
-        // keys (npad) + padded copy (npad) + selection-sort scratch (npad floats + npad ints)
-        if ((rc = ensure(ctx, &v->d_sort, 3 * npad, &v->sort_cap))) return rc;
+        if ((rc = ensure(ctx, &v->d_sort, sort_scratch_words(S), &v->sort_cap))) return rc;
     }
     int words = (S + 63) / 64;
     size_t ntiles = (size_t)cv.tiles_x * cv.tiles_y;
     size_t nmask = ntiles * (size_t)std::max(words, 1);
     if ((rc = ensure(ctx, &v->d_masks, nmask, &v->masks_cap))) return rc;
 
-    unsigned long long* counters = ctx->profiling ? v->d_counters : nullptr;
-    stage_event(ctx, 5, s);
+    unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
+    StageEvents ev;
+    if ((rc = ev.begin(ctx, s, 1))) return rc;
+    ev.mark(0);
     if (counters) launch_fill_u64((uint64_t*)counters, 8, 0, s);
     launch_cam_setup(v->g, cv, v->d_boards, S, v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort, s);
-    stage_event(ctx, 6, s);
-    launch_sort(v->d_sort, S, v->d_sph_world, cv, v->d_order, v->d_tie, s);
-    stage_event(ctx, 7, s);
+    ev.mark(1);
+    launch_sort(v->d_sort, S, v->d_order, v->d_tie, s);
+    ev.mark(2);
     launch_fill_u64((uint64_t*)v->d_masks, nmask, 0, s);
     launch_bin(v->d_sph_tiles, v->d_order, S, cv.tiles_x, words, v->d_masks, s);
-    stage_event(ctx, 8, s);
+    ev.mark(3);
     launch_cone(v->g, v->vd, cv, cd, v->d_levels, v->d_sph_world, v->d_sph_view, v->d_order, v->d_masks, words,
                 d_image, counters, s);
-    stage_event(ctx, 9, s);
+    ev.mark(4);
+    ev.end();
     CR_HIP(ctx, hipGetLastError());
     v->sorted_count = S;
-    if (ctx->profiling) {
+    if (ctx->profiling == 1) {
         CR_HIP(ctx, hipStreamSynchronize(s));
         unsigned long long c[8];
         CR_HIP(ctx, hipMemcpy(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost));
         ctx->stats.fragments = (int64_t)c[3];
         ctx->stats.samples = (int64_t)c[4];
         ctx->stats.two_level = (int64_t)c[5];
-        ctx->stats.ms_setup = elapsed(ctx, 5, 6);
-        ctx->stats.ms_sort = elapsed(ctx, 6, 7);
-        ctx->stats.ms_bin = elapsed(ctx, 7, 8);
-        ctx->stats.ms_cone = elapsed(ctx, 8, 9);
+        ctx->stats.ms_setup = elapsed(ev.e[0], ev.e[1]);
+        ctx->stats.ms_sort = elapsed(ev.e[1], ev.e[2]);
+        ctx->stats.ms_bin = elapsed(ev.e[2], ev.e[3]);
+        ctx->stats.ms_cone = elapsed(ev.e[3], ev.e[4]);
+        ctx->stats.n_cone = 1;
     }
     return CR_OK;
 }

@@ -110,7 +110,9 @@ typedef struct {
     float clear_color[4]; /* glClearColor (src/main.cpp:112), GL_BLEND composite only */
 } cr_cone_params;
 
-/* Per-call counters (filled when profiling is enabled on the context). */
+/* Profiling results.  Level 1: counters and stage times of the last call.
+ * Level 2: stage times summed over every call since the last cr_get_stats
+ * (n_voxelize / n_cone calls), counters zero. */
 typedef struct {
     int64_t light_fragments; /* covered pass-1 fragments */
     int64_t shell_texels;    /* position-map texels with a > 0 */
@@ -119,14 +121,18 @@ typedef struct {
     int64_t two_level;       /* samples that blended two mip levels */
     float ms_clear, ms_light_raster, ms_splat, ms_mip;  /* voxelize stages */
     float ms_setup, ms_sort, ms_bin, ms_cone;           /* cone-trace stages */
+    int64_t n_voxelize, n_cone;                         /* calls summed (level 2) */
+    int64_t chord_stores;    /* chord-fill voxel stores of pass 1 (rgba mode) */
 } cr_stats;
 
 /* ---- context ---------------------------------------------------------- */
 int cr_context_create(int device, unsigned flags, cr_context *out);
 int cr_context_destroy(cr_context ctx);
 const char *cr_last_error(cr_context ctx);
-/* profiling != 0: record hipEvents around every stage and count fragments
- * and samples (adds a few tiny kernels and a stream synchronisation per call). */
+/* profiling 0: off.  1: hipEvents around every stage plus fragment/sample
+ * counters, one stream synchronisation per call.  2: hipEvents only, no
+ * synchronisation; stage times accumulate until cr_get_stats, which waits for
+ * the recorded events and resets the sums. */
 int cr_context_set_profiling(cr_context ctx, int profiling);
 int cr_get_stats(cr_context ctx, cr_stats *out);
 int cr_abi_version(void);

@@ -987,9 +987,9 @@ extern "C" {
 // img64: FP64 replay (optional).
 int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
                    const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
-                   const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1,
+                   const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1, int row_step,
                    float* img32, double* img64, cvo_stats* stats) {
-    if (!vol || W <= 0 || H <= 0 || !cp || cp->steps < 0) return -1;
+    if (!vol || W <= 0 || H <= 0 || !cp || cp->steps < 0 || row_step < 1) return -1;
     Grid g = make_grid(vol);
     int N = vol->dim, L = vol->levels, C = channels(vol->format);
     Volume32 v32;
@@ -1065,6 +1065,7 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
 
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : nfrag, nsamp, ntwo)
     for (int py = r0; py < r1; ++py) {
+        if ((py - r0) % row_step != 0) continue;
         float yn = pixel_ndc(py, H);
         float ry = yn * cs.ky;
         double ynd = (2.0 * ((double)py + 0.5)) / (double)H - 1.0;

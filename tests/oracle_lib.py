@@ -76,7 +76,8 @@ def lib():
         L.cvo_noise_voxels.argtypes = [P(cvo_volume), vp, C.c_int64, vp, vp]
         L.cvo_voxel_index.argtypes = [P(cvo_volume), vp, C.c_int64, vp]
         L.cvo_cone_trace.argtypes = [P(cvo_volume), vp, vp, C.c_int, vp, vp, vp, P(cvo_camera), P(cvo_light),
-                                     P(cvo_cone), C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, P(cvo_stats)]
+                                     P(cvo_cone), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
+                                     P(cvo_stats)]
         L.cvo_cone_trace.restype = C.c_int
         L.cvo_step_table.argtypes = [P(cvo_cone), C.c_int, C.c_int, vp, vp, vp, vp]
         L.cvo_sample.argtypes = [P(cvo_volume), vp, vp, C.c_float, vp, C.c_int64, vp, vp]
@@ -214,7 +215,7 @@ def sort_boards(scene, eye):
     return order
 
 
-def cone_trace(scene, nat, f64, order, cam, row0=0, row1=None, want_f64=True):
+def cone_trace(scene, nat, f64, order, cam, row0=0, row1=None, want_f64=True, row_step=1):
     W, H = scene.W, scene.H
     row1 = H if row1 is None else row1
     vol = volume_of(scene)
@@ -228,7 +229,7 @@ def cone_trace(scene, nat, f64, order, cam, row0=0, row1=None, want_f64=True):
     cp = cone_of(scene.cone)
     rc = lib().cvo_cone_trace(C.byref(vol), ptr(nat), ptr(f64 if img64 is not None else None), scene.S, ptr(xyz),
                               ptr(r), ptr(order), C.byref(k), C.byref(L), C.byref(cp), W, H, int(row0), int(row1),
-                              ptr(img32), ptr(img64), C.byref(st))
+                              int(row_step), ptr(img32), ptr(img64), C.byref(st))
     assert rc == 0
     return img32, img64, st
 

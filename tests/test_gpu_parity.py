@@ -37,12 +37,36 @@ def _scenes(cv):
         "small32_rect": replace(base, W=200, H=120, light_w=130, light_h=210),
         "npot24": replace(base, N=24, levels=5),
         "npot96_plain": replace(sc.scaled(sc.plain256(1), 320, 180), N=96, levels=7),
+        "ties": tie_scene(sc),
+        "ties_ref": sc.reference_mode(tie_scene(sc)),
     }
     return cases
 
 
+def tie_scene(sc):
+    """64 billboards in groups of four related by 90-degree turns about the
+    camera axis (plus exact duplicates): their FP32 camera distances are equal
+    bit for bit, which exercises the selection-sort tie replay."""
+    base = sc.small32(11)
+    rng = np.random.default_rng(11)
+    pts, rad = [], []
+    for g in range(14):
+        x, y, z = rng.uniform(-0.5, 0.5, 3).astype(np.float32)
+        r = np.float32(rng.uniform(0.08, 0.2))
+        for (a, b) in ((x, y), (-y, x), (-x, -y), (y, -x)):
+            pts.append((a, b, z))
+            rad.append(r)
+    for k in range(8):  # exact duplicates
+        pts.append(pts[k * 3])
+        rad.append(rad[k * 3])
+    perm = rng.permutation(len(pts))
+    xyz = np.asarray(pts, np.float32)[perm]
+    r = np.asarray(rad, np.float32)[perm]
+    return replace(base, xyz=xyz, scales=r)
+
+
 CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_rgba32f", "small32_ref_float_blend",
-         "small32_fp16_noise", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain"]
+         "small32_fp16_noise", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
 
 
 @pytest.mark.parametrize("case", CASES)
