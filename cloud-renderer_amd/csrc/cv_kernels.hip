@@ -37,6 +37,11 @@ __device__ __forceinline__ void demorton6(uint32_t m, uint32_t* x, uint32_t* y, 
     *z = ((m >> 2) & 1u) | ((m >> 4) & 2u);
 }
 
+// texel (x, y, z) of a level in the bricked layout
+__device__ __forceinline__ uint64_t brick_addr(const LevelDesc& L, uint32_t x, uint32_t y, uint32_t z) {
+    return L.base + (uint64_t)(((z >> 2) * L.nb + (y >> 2)) * L.nb + (x >> 2)) * 64u + morton6(x & 3u, y & 3u, z & 3u);
+}
+
 __device__ __forceinline__ float h2f(uint32_t bits) {
     unsigned short s = (unsigned short)bits;
     _Float16 h;
@@ -391,14 +396,33 @@ __global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, c
     // stepSize * normalize(vec3(+-1,+-1,+-1)): 1/sqrt(3) rounded once to FP32
     const float k = 1.0f / sqrtf(3.0f);
     const float off = g.stepSize * k;
-    uint32_t w, b;
-    if (voxel_word_bit(g, wx, wy, wz, &w, &b)) atomicOr(&shellbits[w], 1ull << b);
+    // the 9 stores lie within 2 voxels per axis: merge those sharing a brick
+    // word in registers before the atomics (one atomic per distinct word)
+    uint32_t ws[9];
+    unsigned long long bs[9];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        float ox = (t & 4) ? -off : off;
-        float oy = (t & 2) ? -off : off;
-        float oz = (t & 1) ? -off : off;
-        if (voxel_word_bit(g, wx + ox, wy + oy, wz + oz, &w, &b)) atomicOr(&shellbits[w], 1ull << b);
+    for (int t = 0; t < 9; ++t) {
+        float px = wx, py = wy, pz = wz;
+        if (t > 0) {
+            const int u = t - 1;
+            px = wx + ((u & 4) ? -off : off);
+            py = wy + ((u & 2) ? -off : off);
+            pz = wz + ((u & 1) ? -off : off);
+        }
+        uint32_t w, b;
+        bool ok = voxel_word_bit(g, px, py, pz, &w, &b);
+        ws[t] = ok ? w : 0xffffffffu;
+        bs[t] = ok ? (1ull << b) : 0ull;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        bool first = ws[t] != 0xffffffffu;
+#pragma unroll
+        for (int u = 0; u < t; ++u) first = first && ws[u] != ws[t];
+        unsigned long long bits = bs[t];
+#pragma unroll
+        for (int u = t + 1; u < 9; ++u) bits |= ws[u] == ws[t] ? bs[u] : 0ull;
+        if (first) atomicOr(&shellbits[ws[t]], bits);
     }
 }
 
@@ -491,6 +515,136 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
 }
 
 // ---------------------------------------------------------------------------
+// fused level 0 + mips 1..4 for one 16^3 region of level 0 (N % 16 == 0)
+//
+// Level-0 texel values come from the occupancy words (plus noise); the 2x2x2
+// children of a level-1 texel are the 8 consecutive Morton texels of one
+// level-0 brick (Morton bit order x0 y0 z0 is the z, y, x summation order of
+// the mip rule), so a thread owning 8 consecutive texels forms its level-1
+// texel in registers.  Levels 2..4 are reduced from LDS.  Every level's value
+// is rounded to the storage format before it feeds the next level, exactly
+// like a level-by-level build.
+// ---------------------------------------------------------------------------
+template <int FMT>
+__device__ __forceinline__ float4 level0_value(const GridDesc& g, const VolumeDesc& vd, bool shell, bool fill, int x,
+                                               int y, int z) {
+    float r, gg, b, a;
+    if (vd.mode == 0) {
+        r = gg = b = a = shell ? 1.0f : 0.0f;
+    } else {
+        if (shell) { r = gg = b = a = 1.0f; }
+        else if (fill) { r = gg = b = 0.0f; a = 1.0f; }
+        else { r = gg = b = a = 0.0f; }
+        if (FMT != 0 && vd.noise_octaves > 0 && a != 0.0f) a = a * fbm_voxel(vd, g.N, x, y, z);
+    }
+    if (FMT == 0) return make_float4(r != 0.0f ? 255.0f : 0.0f, 0.0f, 0.0f, 0.0f);
+    if (FMT == 2) return make_float4(h2f(f2h(r)), h2f(f2h(gg)), h2f(f2h(b)), h2f(f2h(a)));
+    return make_float4(r, gg, b, a);
+}
+
+// parent of 8 children summed in order; R8 sums hold integers
+template <int FMT>
+__device__ __forceinline__ float4 mip_parent(float4 s) {
+    if (FMT == 0) {
+        uint32_t v = ((uint32_t)s.x + 4u) / 8u;
+        return make_float4((float)v, 0.0f, 0.0f, 0.0f);
+    }
+    float4 r = make_float4(s.x / 8.0f, s.y / 8.0f, s.z / 8.0f, s.w / 8.0f);
+    if (FMT == 2) r = make_float4(h2f(f2h(r.x)), h2f(f2h(r.y)), h2f(f2h(r.z)), h2f(f2h(r.w)));
+    return r;
+}
+
+template <int FMT>
+__device__ __forceinline__ void store_texel(void* levels, uint64_t idx, float4 v) {
+    if (FMT == 0) {
+        ((uint8_t*)levels)[idx] = (uint8_t)(uint32_t)v.x;
+    } else if (FMT == 1) {
+        ((float4*)levels)[idx] = v;
+    } else {
+        uint2 u;
+        u.x = f2h(v.x) | (f2h(v.y) << 16);
+        u.y = f2h(v.z) | (f2h(v.w) << 16);
+        ((uint2*)levels)[idx] = u;
+    }
+}
+
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
+                                                     const unsigned long long* __restrict__ shellbits,
+                                                     const unsigned long long* __restrict__ fillbits,
+                                                     void* __restrict__ levels, int nlev, uint32_t rpa) {
+    __shared__ float4 l1[512];
+    __shared__ float4 l2[64];
+    __shared__ float4 l3[8];
+    const uint32_t r = blockIdx.x;
+    const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
+    const uint32_t nb0 = g.nb0;
+    for (int rep = 0; rep < 2; ++rep) {
+        const uint32_t grp = threadIdx.x + rep * 256u;
+        const uint32_t lb = grp >> 3, q = grp & 7u;
+        const uint32_t lbx = lb & 3u, lby = (lb >> 2) & 3u, lbz = lb >> 4;
+        const uint32_t gbx = 4u * rx + lbx, gby = 4u * ry + lby, gbz = 4u * rz + lbz;
+        const uint32_t brick = (gbz * nb0 + gby) * nb0 + gbx;
+        const unsigned long long sh = shellbits[brick];
+        const unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
+        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t m = q * 8u + c;
+            uint32_t tx, ty, tz;
+            demorton6(m, &tx, &ty, &tz);
+            float4 v = level0_value<FMT>(g, vd, ((sh >> m) & 1ull) != 0, ((fl >> m) & 1ull) != 0,
+                                         (int)(gbx * 4u + tx), (int)(gby * 4u + ty), (int)(gbz * 4u + tz));
+            store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)brick * 64u + m, v);
+            acc = add4(acc, v);
+        }
+        if (nlev > 1) {
+            float4 p = mip_parent<FMT>(acc);
+            const uint32_t x1 = 2u * lbx + (q & 1u), y1 = 2u * lby + ((q >> 1) & 1u), z1 = 2u * lbz + (q >> 2);
+            store_texel<FMT>(levels, brick_addr(vd.lv[1], 8u * rx + x1, 8u * ry + y1, 8u * rz + z1), p);
+            l1[(z1 * 8u + y1) * 8u + x1] = p;
+        }
+    }
+    if (nlev <= 2) return;
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t < 64) {
+        uint32_t x, y, z;
+        demorton6(t, &x, &y, &z);
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t dz = 0; dz < 2; ++dz)
+            for (uint32_t dy = 0; dy < 2; ++dy)
+                for (uint32_t dx = 0; dx < 2; ++dx) s = add4(s, l1[((2u * z + dz) * 8u + 2u * y + dy) * 8u + 2u * x + dx]);
+        float4 p = mip_parent<FMT>(s);
+        store_texel<FMT>(levels, brick_addr(vd.lv[2], 4u * rx + x, 4u * ry + y, 4u * rz + z), p);
+        l2[(z * 4u + y) * 4u + x] = p;
+    }
+    if (nlev <= 3) return;
+    __syncthreads();
+    if (t < 8) {
+        uint32_t x = t & 1u, y = (t >> 1) & 1u, z = t >> 2;
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t dz = 0; dz < 2; ++dz)
+            for (uint32_t dy = 0; dy < 2; ++dy)
+                for (uint32_t dx = 0; dx < 2; ++dx) s = add4(s, l2[((2u * z + dz) * 4u + 2u * y + dy) * 4u + 2u * x + dx]);
+        float4 p = mip_parent<FMT>(s);
+        store_texel<FMT>(levels, brick_addr(vd.lv[3], 2u * rx + x, 2u * ry + y, 2u * rz + z), p);
+        l3[(z * 2u + y) * 2u + x] = p;
+    }
+    if (nlev <= 4) return;
+    __syncthreads();
+    if (t == 0) {
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int c = 0; c < 8; ++c) s = add4(s, l3[c]);
+        store_texel<FMT>(levels, brick_addr(vd.lv[4], rx, ry, rz), mip_parent<FMT>(s));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // mip level k from level k-1 (glGenerateMipmap, src/Shaders/VoxelizeShader.cpp:105)
 // parent = sum of children (z, y, x order) / count; R8: integer round half up
 // ---------------------------------------------------------------------------
@@ -498,9 +652,6 @@ __device__ __forceinline__ void child_range(int p, int nparent, int nchild, int*
     *c0 = min(2 * p, nchild - 1);
     *c1 = min(2 * p + 1, nchild - 1);
     if (p == nparent - 1) *c1 = nchild - 1;
-}
-__device__ __forceinline__ uint64_t brick_addr(const LevelDesc& L, uint32_t x, uint32_t y, uint32_t z) {
-    return L.base + (uint64_t)(((z >> 2) * L.nb + (y >> 2)) * L.nb + (x >> 2)) * 64u + morton6(x & 3u, y & 3u, z & 3u);
 }
 
 __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restrict__ levels, uint32_t ntexels) {
@@ -644,63 +795,91 @@ __device__ void replay_ties(int S, const I* s_idx, const uint32_t* s_tie, P* pos
     }
 }
 
-// The same replay run by one full wavefront: sorted indices and tie flags are
-// prefetched 64 at a time into lanes, every step reads pos/at once (broadcast)
-// and lane 0 writes; LDS accesses of one wave complete in order, so each step
-// sees the previous step's writes.  Runs of equal distances are scanned by all
-// lanes with a wave min-reduction over current positions.
-__device__ void replay_ties_wave(int S, const uint16_t* s_idx, uint16_t* pos, uint16_t* at,
-                                 int32_t* __restrict__ order) {
-    // s_idx[t]: bits 0-14 element, bit 15 = same distance as entry t-1
-    const int lane = threadIdx.x & 63;
-    auto place = [&](int k, int sel) {
-        int m = (int)pos[sel];
-        int e = (int)at[k];
-        if (lane == 0) {
-            at[k] = (uint16_t)sel;
-            at[m] = (uint16_t)e;
-            pos[sel] = (uint16_t)k;
-            pos[e] = (uint16_t)m;
-            order[k] = sel;
-        }
-    };
-    auto tie_at = [&](int t) -> int { return (int)(s_idx[t] >> 15); };
-    int i = 0;
-    while (i < S) {
-        const int t = i + lane;
-        const int my_idx = t < S ? (int)(s_idx[t] & 0x7fffu) : 0;
-        const int my_next_tie = (t + 1 < S) ? tie_at(t + 1) : 0;
-        const unsigned long long next_tie = __ballot(my_next_tie);
-        const int lim = min(64, S - i);
-        int l = 0;
-        while (l < lim && !((next_tie >> l) & 1ull)) {
-            place(i + l, __builtin_amdgcn_readlane(my_idx, l));
-            ++l;
-        }
-        if (l == lim) {
-            i += lim;
+// Parallel form of the same replay for one workgroup.  Let D[k] be the
+// position the element selected at step k held just before its swap.  An
+// unplaced element y sits at position y until step y; whenever the scan
+// reaches its position p it is displaced to D[p].  Hence its position at step
+// k is find(y, k): x = y; while (x < k) x = D[x].  Steps outside runs of equal
+// distances select sorted[k] whatever the positions, so D[k] = find(sorted[k], k)
+// for a whole segment of such steps is computed in parallel rounds (a chain
+// only waits on smaller steps of the same segment).  A run of equal distances
+// is resolved step by step: the unplaced member with the smallest find() wins.
+// s_idx[t]: bits 0-14 element, bit 15 = same distance as entry t-1;
+// s_D: positions, 0xFFFF = not yet known; placed: one bit per element.
+__device__ void replay_ties_block(int S, const uint16_t* s_idx, uint16_t* s_D, uint32_t* placed,
+                                  int32_t* __restrict__ order) {
+    __shared__ int seg_end;
+    __shared__ int red_p[16], red_e[16];
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wid = tid >> 6;
+    auto tie_at = [&](int t) -> bool { return (s_idx[t] >> 15) != 0; };
+    auto elem = [&](int t) -> int { return (int)(s_idx[t] & 0x7fffu); };
+    int k0 = 0;
+    while (k0 < S) {
+        if (k0 + 1 < S && tie_at(k0 + 1)) {
+            // run of equal distances [k0, ge)
+            int ge = k0 + 1;
+            while (ge < S && tie_at(ge)) ++ge;
+            for (int k = k0; k < ge; ++k) {
+                int best_p = 0x7fffffff, best_e = -1;
+                for (int t = k0 + tid; t < ge; t += nt) {
+                    int y = elem(t);
+                    if ((placed[y >> 5] >> (y & 31)) & 1u) continue;
+                    int x = y;
+                    while (x < k) x = (int)s_D[x];
+                    if (x < best_p) { best_p = x; best_e = y; }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    int op = __shfl_xor(best_p, off, 64);
+                    int oe = __shfl_xor(best_e, off, 64);
+                    if (op < best_p) { best_p = op; best_e = oe; }
+                }
+                if (lane == 0) { red_p[wid] = best_p; red_e[wid] = best_e; }
+                __syncthreads();
+                if (tid == 0) {
+                    int p = red_p[0], e = red_e[0];
+                    for (int w = 1; w < nt / 64; ++w)
+                        if (red_p[w] < p) { p = red_p[w]; e = red_e[w]; }
+                    if (e >= 0) {  // always: ge - k members are unplaced
+                        s_D[k] = (uint16_t)p;
+                        order[k] = e;
+                        placed[e >> 5] |= 1u << (e & 31);
+                    }
+                }
+                __syncthreads();
+            }
+            k0 = ge;
             continue;
         }
-        // run of equal distances starting at step gs
-        const int gs = i + l;
-        int ge = gs + 1;
-        while (ge < S && tie_at(ge)) ++ge;
-        for (int k = gs; k < ge; ++k) {
-            int best_p = 0x7fffffff, best_e = -1;
-            for (int q = gs + lane; q < ge; q += 64) {
-                int e = (int)(s_idx[q] & 0x7fffu);
-                int p = (int)pos[e];
-                if (p >= k && p < best_p) { best_p = p; best_e = e; }
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                int op = __shfl_xor(best_p, off, 64);
-                int oe = __shfl_xor(best_e, off, 64);
-                if (op < best_p) { best_p = op; best_e = oe; }
-            }
-            if (best_e < 0) return;  // unreachable: ge - k members are unplaced, all at positions >= k
-            place(k, best_e);
+        // segment [k0, kend): steps outside runs of equal distances
+        if (tid == 0) seg_end = min(k0 + nt, S);
+        __syncthreads();
+        {
+            int k = k0 + tid;
+            if (k < min(k0 + nt, S) && (tie_at(k) || (k + 1 < S && tie_at(k + 1)))) atomicMin(&seg_end, k);
         }
-        i = ge;
+        __syncthreads();
+        const int kend = seg_end;
+        const int k = k0 + tid;
+        bool done = k >= kend;
+        int x = done ? 0 : elem(k);
+        while (true) {
+            if (!done) {
+                while (true) {
+                    while (x < k0) x = (int)s_D[x];
+                    if (x >= k) {
+                        s_D[k] = (uint16_t)x;
+                        order[k] = elem(k);
+                        done = true;
+                        break;
+                    }
+                    int dx = (int)s_D[x];  // x in [k0, k): known once its own chain finished
+                    if (dx == 0xFFFF) break;
+                    x = dx;
+                }
+            }
+            if (__syncthreads_count(done ? 0 : 1) == 0) break;
+        }
+        k0 = kend;
     }
 }
 
@@ -753,15 +932,14 @@ __global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __r
         if (tid == 0) *tie_flag = 0;
         return;
     }
-    for (int i = tid; i < S; i += nt) {
-        s_pos[i] = (uint16_t)i;
-        s_at[i] = (uint16_t)i;
-    }
+    __shared__ uint32_t placed[kLdsSortMax / 32];
+    uint16_t* s_D = s_pos;
+    (void)s_at;
+    for (int i = tid; i < S; i += nt) s_D[i] = 0xFFFFu;
+    for (int w = tid; w < kLdsSortMax / 32; w += nt) placed[w] = 0u;
     __syncthreads();
-    if (tid < 64) {
-        replay_ties_wave(S, s_idx, s_pos, s_at, order);
-        if (tid == 0) *tie_flag = 1;
-    }
+    replay_ties_block(S, s_idx, s_D, placed, order);
+    if (tid == 0) *tie_flag = 1;
 }
 
 // global-memory bitonic stage for more billboards than one LDS holds
@@ -1180,11 +1358,28 @@ void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* bo
                        counters);
 }
 
-void launch_level0(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
-                   const unsigned long long* fillbits, void* levels, hipStream_t s) {
-    uint32_t nbricks = g.nb0 * g.nb0 * g.nb0;
-    hipLaunchKernelGGL(k_level0, dim3(blocks_for(nbricks, 4)), dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
-                       nbricks);
+// level 0 and the whole mip chain; levels 0..4 in one fused pass when the
+// side is a multiple of 16, the remaining levels one launch each
+void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
+                         const unsigned long long* fillbits, void* levels, hipStream_t s) {
+    int first_generic = 1;
+    if (g.N % 16 == 0) {
+        const uint32_t rpa = (uint32_t)g.N / 16u;
+        const int nlev = vd.L < 5 ? vd.L : 5;
+        const dim3 grid(rpa * rpa * rpa);
+        if (vd.format == 0)
+            hipLaunchKernelGGL(k_build_fused<0>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
+        else if (vd.format == 1)
+            hipLaunchKernelGGL(k_build_fused<1>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
+        else
+            hipLaunchKernelGGL(k_build_fused<2>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
+        first_generic = nlev;
+    } else {
+        uint32_t nbricks = g.nb0 * g.nb0 * g.nb0;
+        hipLaunchKernelGGL(k_level0, dim3(blocks_for(nbricks, 4)), dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
+                           nbricks);
+    }
+    for (int k = first_generic; k < vd.L; ++k) launch_mip(vd, k, levels, s);
 }
 
 void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s) {

@@ -530,8 +530,7 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     ev.mark(2);
     launch_light_splat(v->g, lv, v->d_boards, v->d_keys, v->d_posmap, v->d_shell, counters, s);
     ev.mark(3);
-    launch_level0(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
-    for (int k = 1; k < v->L; ++k) launch_mip(v->vd, k, v->d_levels, s);
+    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
     ev.mark(4);
     ev.end();
     CR_HIP(ctx, hipGetLastError());
