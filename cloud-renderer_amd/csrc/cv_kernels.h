@@ -14,6 +14,7 @@ constexpr int kTile = 8;           // camera tiles are 8x8 pixels = one wavefron
 constexpr int kLdsSortMax = 16384; // billboards sorted in one workgroup's LDS
 constexpr int kChordLdsWords = 4096; // LDS bitmap (bricks) per pass-1 tile for chord-fill aggregation
 constexpr int kLightTile = 32;       // pass-1 work item: 32 x 32 light pixels of one billboard
+constexpr unsigned kConeWaves = 6144; // persistent cone-trace wavefronts (256 CUs x 24)
 
 // One mip level in the bricked layout: 4x4x4 bricks, texels Morton-ordered
 // inside a brick (bit order x0 y0 z0 x1 y1 z1), bricks x-fastest.
@@ -114,11 +115,17 @@ void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boa
                       float4* sph_view, int4* sph_tiles, unsigned long long* keys, hipStream_t s);
 size_t sort_scratch_words(int S);
 int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s);
-void launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
-                unsigned long long* masks, hipStream_t s);
+// camera tiles in the order the cone waves take them (heaviest first)
+struct TileQueue {
+    unsigned int* counter;
+    const int* order;
+};
+size_t tile_scratch_ints(int ntiles);
+TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
+                     unsigned long long* masks, int* scratch, hipStream_t s);
 void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
                  const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
-                 const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                 const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
                  hipStream_t s);
 void launch_occupancy(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,
                       unsigned long long* out2, hipStream_t s);

@@ -55,6 +55,10 @@ __device__ __forceinline__ uint32_t f2h(float f) {
     return s;
 }
 
+// filter lerp(a, b, t) = fma(t, b - a, a): a fused multiply-add is correctly
+// rounded, so the host restatement reproduces it exactly
+__device__ __forceinline__ float lerpF(float a, float b, float t) { return fmaf(t, b - a, a); }
+
 // pixel centre -> NDC: (2 * (p + 0.5)) / n - 1
 __device__ __forceinline__ float pixel_ndc(int p, int n) { return (2.0f * ((float)p + 0.5f)) / (float)n - 1.0f; }
 
@@ -112,13 +116,13 @@ __device__ __forceinline__ float cv_exp2_neg(float x) {
     float n = floorf(x + 0.5f);
     float f = x - n;
     float z = f * 0.693147180559945309f;
-    float p = 1.0f / 720.0f;
-    p = p * z + 1.0f / 120.0f;
-    p = p * z + 1.0f / 24.0f;
-    p = p * z + 1.0f / 6.0f;
-    p = p * z + 0.5f;
-    p = p * z + 1.0f;
-    p = p * z + 1.0f;
+    float p = 1.0f / 720.0f;  // Horner with fused multiply-adds
+    p = fmaf(p, z, 1.0f / 120.0f);
+    p = fmaf(p, z, 1.0f / 24.0f);
+    p = fmaf(p, z, 1.0f / 6.0f);
+    p = fmaf(p, z, 0.5f);
+    p = fmaf(p, z, 1.0f);
+    p = fmaf(p, z, 1.0f);
     int ni = (int)n;
     float scale = __uint_as_float((uint32_t)(ni + 127) << 23);
     return p * scale;
@@ -983,7 +987,8 @@ __global__ void k_sort_replay_global(int S, const uint32_t* __restrict__ g_idx, 
 // binning: set bit `rank` in every tile a billboard's box overlaps
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles, const int32_t* __restrict__ order, int S,
-                                             int tiles_x, int words, unsigned long long* __restrict__ masks) {
+                                             int tiles_x, int words, unsigned long long* __restrict__ masks,
+                                             int* __restrict__ tile_count) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= S) return;
     int4 t = sph_tiles[order[r]];
@@ -991,7 +996,48 @@ __global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles,
     unsigned long long bit = 1ull << (r & 63);
     int w = r >> 6;
     for (int ty = t.z; ty <= t.w; ++ty)
-        for (int tx = t.x; tx <= t.y; ++tx) atomicOr(&masks[((size_t)ty * tiles_x + tx) * words + w], bit);
+        for (int tx = t.x; tx <= t.y; ++tx) {
+            atomicOr(&masks[((size_t)ty * tiles_x + tx) * words + w], bit);
+            atomicAdd(&tile_count[ty * tiles_x + tx], 1);
+        }
+}
+
+// Longest job first: tiles are bucketed by log2(number of binned billboards)
+// and handed to the persistent cone waves heaviest bucket first, so the
+// expensive tiles of clump centres do not start last and form the tail.
+__global__ void __launch_bounds__(256) k_tile_bucket(const int* __restrict__ tile_count, int ntiles,
+                                                     int* __restrict__ bucket_of, int* __restrict__ hist) {
+    __shared__ int lh[32];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < 32) lh[threadIdx.x] = 0;
+    __syncthreads();
+    if (t < ntiles) {
+        int c = tile_count[t];
+        int key = 31 - (c == 0 ? 0 : 32 - __clz(c));  // heavy tiles get small keys
+        bucket_of[t] = key;
+        atomicAdd(&lh[key], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 32 && lh[threadIdx.x] > 0) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
+}
+// counting-sort scatter: ranks inside a workgroup from LDS atomics, one global
+// atomic per (workgroup, bucket) to reserve the range
+__global__ void __launch_bounds__(256) k_tile_scatter(const int* __restrict__ bucket_of, int ntiles,
+                                                      const int* __restrict__ hist, int* __restrict__ cursor,
+                                                      int* __restrict__ tile_order) {
+    __shared__ int lh[32], lbase[32];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < 32) lh[threadIdx.x] = 0;
+    __syncthreads();
+    const int key = t < ntiles ? bucket_of[t] : -1;
+    const int lrank = key >= 0 ? atomicAdd(&lh[key], 1) : 0;
+    __syncthreads();
+    if (threadIdx.x < 32 && lh[threadIdx.x] > 0) lbase[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], lh[threadIdx.x]);
+    __syncthreads();
+    if (key < 0) return;
+    int base = 0;
+    for (int k = 0; k < key; ++k) base += hist[k];
+    tile_order[base + lbase[key] + lrank] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,12 +1062,11 @@ __device__ __forceinline__ float4 trilinear(const void* __restrict__ vol, const 
                                             float sz) {
     const float fn = (float)L.n;
     const int nm1 = (int)L.n - 1;
-    float ux = fminf(fmaxf(sx, -1.0f), 2.0f) * fn - 0.5f;
-    float uy = fminf(fmaxf(sy, -1.0f), 2.0f) * fn - 0.5f;
-    float uz = fminf(fmaxf(sz, -1.0f), 2.0f) * fn - 0.5f;
+    float ux = fmaf(fminf(fmaxf(sx, -1.0f), 2.0f), fn, -0.5f);
+    float uy = fmaf(fminf(fmaxf(sy, -1.0f), 2.0f), fn, -0.5f);
+    float uz = fmaf(fminf(fmaxf(sz, -1.0f), 2.0f), fn, -0.5f);
     float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
     float ax = ux - flx, ay = uy - fly, az = uz - flz;
-    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
     int ix = (int)flx, iy = (int)fly, iz = (int)flz;
     uint32_t x0 = (uint32_t)min(max(ix, 0), nm1), x1 = (uint32_t)min(max(ix + 1, 0), nm1);
     uint32_t y0 = (uint32_t)min(max(iy, 0), nm1), y1 = (uint32_t)min(max(iy + 1, 0), nm1);
@@ -1038,22 +1083,22 @@ __device__ __forceinline__ float4 trilinear(const void* __restrict__ vol, const 
         float t010 = fetch1<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch1<FMT>(vol, base + (X1 + Y1 + Z0));
         float t001 = fetch1<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch1<FMT>(vol, base + (X1 + Y0 + Z1));
         float t011 = fetch1<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch1<FMT>(vol, base + (X1 + Y1 + Z1));
-        float c00 = t000 * bx + t100 * ax, c10 = t010 * bx + t110 * ax;
-        float c01 = t001 * bx + t101 * ax, c11 = t011 * bx + t111 * ax;
-        float c0 = c00 * by + c10 * ay, c1 = c01 * by + c11 * ay;
-        return make_float4(c0 * bz + c1 * az, 0.0f, 0.0f, 0.0f);
+        float c00 = lerpF(t000, t100, ax), c10 = lerpF(t010, t110, ax);
+        float c01 = lerpF(t001, t101, ax), c11 = lerpF(t011, t111, ax);
+        float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);
+        return make_float4(lerpF(c0, c1, az), 0.0f, 0.0f, 0.0f);
     }
     float4 t000 = fetch4<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch4<FMT>(vol, base + (X1 + Y0 + Z0));
     float4 t010 = fetch4<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch4<FMT>(vol, base + (X1 + Y1 + Z0));
     float4 t001 = fetch4<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch4<FMT>(vol, base + (X1 + Y0 + Z1));
     float4 t011 = fetch4<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch4<FMT>(vol, base + (X1 + Y1 + Z1));
     float4 o;
-#define CV_TRI(ch)                                                                 \
-    {                                                                              \
-        float c00 = t000.ch * bx + t100.ch * ax, c10 = t010.ch * bx + t110.ch * ax; \
-        float c01 = t001.ch * bx + t101.ch * ax, c11 = t011.ch * bx + t111.ch * ax; \
-        float c0 = c00 * by + c10 * ay, c1 = c01 * by + c11 * ay;                  \
-        o.ch = c0 * bz + c1 * az;                                                  \
+#define CV_TRI(ch)                                                                     \
+    {                                                                                  \
+        float c00 = lerpF(t000.ch, t100.ch, ax), c10 = lerpF(t010.ch, t110.ch, ax);     \
+        float c01 = lerpF(t001.ch, t101.ch, ax), c11 = lerpF(t011.ch, t111.ch, ax);     \
+        float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);                      \
+        o.ch = lerpF(c0, c1, az);                                                      \
     }
     CV_TRI(x) CV_TRI(y) CV_TRI(z) CV_TRI(w)
 #undef CV_TRI
@@ -1073,12 +1118,11 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
     *two = f != 0.0f;
     if (f != 0.0f) {
         float4 o2 = trilinear<FMT, NCH>(vol, vd.lv[lv + 1], sx, sy, sz);
-        float g = 1.0f - f;
-        o.x = o.x * g + o2.x * f;
+        o.x = lerpF(o.x, o2.x, f);
         if (NCH == 4) {
-            o.y = o.y * g + o2.y * f;
-            o.z = o.z * g + o2.z * f;
-            o.w = o.w * g + o2.w * f;
+            o.y = lerpF(o.y, o2.y, f);
+            o.z = lerpF(o.z, o2.z, f);
+            o.w = lerpF(o.w, o2.w, f);
         }
     }
     return o;
@@ -1093,9 +1137,16 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                                              const void* __restrict__ vol, const float4* __restrict__ sph_world,
                                              const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
                                              const unsigned long long* __restrict__ masks, int words,
-                                             void* __restrict__ image, unsigned long long* __restrict__ counters) {
-    const int tile = blockIdx.x;
+                                             void* __restrict__ image, unsigned long long* __restrict__ counters,
+                                             unsigned int* __restrict__ tile_counter,
+                                             const int* __restrict__ tile_order) {
+    // One wavefront per 8x8 tile; tiles are dispatched heaviest first
+    // (tile_order), so the costly clump-centre tiles do not form the tail.
+    (void)tile_counter;
     const int lane = threadIdx.x;
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0;
+    {
+    const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
     const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
     const int px = tx * kTile + (lane & 7), py = ty * kTile + (lane >> 3);
     const bool valid = px < cv.W && py < cv.H;
@@ -1114,7 +1165,6 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
         C0 = C1 = C2 = C3 = 0.0f;
     }
     bool done = !valid;
-    uint32_t nfrag = 0, nsamp = 0, ntwo = 0;
     const unsigned long long* tm = masks + (size_t)tile * words;
 
     for (int wi = 0; wi < words; ++wi) {
@@ -1162,7 +1212,7 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                         for (int i = 0; i < cd.steps; ++i) {
                             const float h = cd.h[i];
                             bool two;
-                            float4 sm = sample_lod<FMT, 1>(vol, vd, cd.lam[i], p0x + h * dx, p0y + h * dy, p0z + h * dzz, &two);
+                            float4 sm = sample_lod<FMT, 1>(vol, vd, cd.lam[i], fmaf(h, dx, p0x), fmaf(h, dy, p0y), fmaf(h, dzz, p0z), &two);
                             ntwo += two ? 1u : 0u;
                             color = color + (sm.x * (float)(i + 1)) / cd.den;
                         }
@@ -1173,7 +1223,7 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                         float k0 = 0.0f, k1 = 0.0f, k2 = 0.0f, ka = 0.0f;
                         for (int i = 0; i < cd.steps; ++i) {
                             const float h = cd.h[i];
-                            float sx = p0x + h * dx, sy = p0y + h * dy, sz = p0z + h * dzz;
+                            float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dzz, p0z);
                             if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) break;
                             bool two;
                             float4 sm = sample_lod<FMT, 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
@@ -1189,14 +1239,14 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                             float wgt = (1.0f - a) * al;
                             if (cd.compensated) {
                                 float y, t;
-                                y = wgt * sm.x - k0; t = c0 + y; k0 = (t - c0) - y; c0 = t;
-                                y = wgt * sm.y - k1; t = c1 + y; k1 = (t - c1) - y; c1 = t;
-                                y = wgt * sm.z - k2; t = c2 + y; k2 = (t - c2) - y; c2 = t;
+                                y = fmaf(wgt, sm.x, -k0); t = c0 + y; k0 = (t - c0) - y; c0 = t;
+                                y = fmaf(wgt, sm.y, -k1); t = c1 + y; k1 = (t - c1) - y; c1 = t;
+                                y = fmaf(wgt, sm.z, -k2); t = c2 + y; k2 = (t - c2) - y; c2 = t;
                                 y = wgt - ka; t = a + y; ka = (t - a) - y; a = t;
                             } else {
-                                c0 = c0 + wgt * sm.x;
-                                c1 = c1 + wgt * sm.y;
-                                c2 = c2 + wgt * sm.z;
+                                c0 = fmaf(wgt, sm.x, c0);
+                                c1 = fmaf(wgt, sm.y, c1);
+                                c2 = fmaf(wgt, sm.z, c2);
                                 a = a + wgt;
                             }
                             ++nsamp;
@@ -1217,9 +1267,9 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                         }
                     } else {
                         const float wgt = (1.0f - C3) * s3;
-                        C0 = C0 + wgt * s0;
-                        C1 = C1 + wgt * s1;
-                        C2 = C2 + wgt * s2;
+                        C0 = fmaf(wgt, s0, C0);
+                        C1 = fmaf(wgt, s1, C1);
+                        C2 = fmaf(wgt, s2, C2);
                         C3 = C3 + wgt;
                         if (C3 >= cd.alpha_stop) done = true;
                     }
@@ -1238,6 +1288,7 @@ finished:
             ((uint32_t*)row)[px] = v;
         }
     }
+    }  // tiles
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
         if (lane == 0) {
@@ -1435,48 +1486,69 @@ int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, 
     return npad;
 }
 
-void launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
-                unsigned long long* masks, hipStream_t s) {
-    if (S <= 0) return;
-    hipLaunchKernelGGL(k_bin, dim3(blocks_for(S, 256)), dim3(256), 0, s, sph_tiles, order, S, tiles_x, words, masks);
+size_t tile_scratch_ints(int ntiles) { return 66 + 3 * (size_t)ntiles + 2; }
+
+// Binning plus the heavy-first tile order.  tile scratch (ints): hist[32] |
+// cursor[32] | cone tile counter (2) | tile_count[ntiles] | bucket_of | tile_order
+TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
+                     unsigned long long* masks, int* scratch, hipStream_t s) {
+    const int ntiles = tiles_x * tiles_y;
+    int* hist = scratch;
+    int* cursor = scratch + 32;
+    unsigned int* counter = (unsigned int*)(scratch + 64);
+    int* tile_count = scratch + 66;
+    int* bucket_of = tile_count + ntiles;
+    int* tile_order = bucket_of + ntiles;
+    launch_fill_u64((uint64_t*)scratch, (uint64_t)(66 + ntiles + 1) / 2, 0, s);
+    if (S > 0)
+        hipLaunchKernelGGL(k_bin, dim3(blocks_for(S, 256)), dim3(256), 0, s, sph_tiles, order, S, tiles_x, words, masks,
+                           tile_count);
+    hipLaunchKernelGGL(k_tile_bucket, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tile_count, ntiles, bucket_of,
+                       hist);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, bucket_of, ntiles, hist, cursor,
+                       tile_order);
+    TileQueue q;
+    q.counter = counter;
+    q.order = tile_order;
+    return q;
 }
 
 template <int FMT, int CONE, int COMP>
 static void launch_cone_t(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
                           const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
-                          const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                          const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
                           hipStream_t s) {
     unsigned ntiles = (unsigned)(cv.tiles_x * cv.tiles_y);
     hipLaunchKernelGGL((k_cone<FMT, CONE, COMP>), dim3(ntiles), dim3(64), 0, s, g, vd, cv, cd, levels, sph_world,
-                       sph_view, order, masks, words, image, counters);
+                       sph_view, order, masks, words, image, counters, tile_counter, tile_order);
 }
 
 template <int FMT>
 static void launch_cone_f(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
                           const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
-                          const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                          const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
                           hipStream_t s) {
     const bool weighted = cd.cone_mode == 0;
     if (weighted && cd.composite == 0)
-        launch_cone_t<FMT, 0, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_t<FMT, 0, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
     else if (weighted)
-        launch_cone_t<FMT, 0, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_t<FMT, 0, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
     else if (cd.composite == 0)
-        launch_cone_t<FMT, 1, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_t<FMT, 1, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
     else
-        launch_cone_t<FMT, 1, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_t<FMT, 1, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
 }
 
 void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd, const void* levels,
                  const float4* sph_world, const float4* sph_view, const int32_t* order,
-                 const unsigned long long* masks, int words, void* image, unsigned long long* counters,
+                 const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
                  hipStream_t s) {
     if (vd.format == 0)
-        launch_cone_f<0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_f<0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
     else if (vd.format == 1)
-        launch_cone_f<1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_f<1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
     else
-        launch_cone_f<2>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, s);
+        launch_cone_f<2>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
 }
 
 void launch_occupancy(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,

@@ -67,6 +67,8 @@ struct cr_volume_s {
     unsigned long long* d_sort = nullptr;
     int32_t* d_order = nullptr;
     int* d_tie = nullptr;
+    int* d_tiles = nullptr;  // binning / tile-queue scratch
+    size_t tiles_cap = 0;
     int sph_cap = 0;
     size_t sort_cap = 0;
     unsigned long long* d_masks = nullptr;
@@ -420,7 +422,7 @@ int cr_volume_destroy(cr_volume v) {
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
     void* ptrs[] = {v->d_levels, v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
-                    v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch};
+                    v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -646,6 +648,7 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
     size_t ntiles = (size_t)cv.tiles_x * cv.tiles_y;
     size_t nmask = ntiles * (size_t)std::max(words, 1);
     if ((rc = ensure(ctx, &v->d_masks, nmask, &v->masks_cap))) return rc;
+    if ((rc = ensure(ctx, &v->d_tiles, tile_scratch_ints((int)ntiles), &v->tiles_cap))) return rc;
 
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
     StageEvents ev;
@@ -657,10 +660,10 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
     launch_sort(v->d_sort, S, v->d_order, v->d_tie, s);
     ev.mark(2);
     launch_fill_u64((uint64_t*)v->d_masks, nmask, 0, s);
-    launch_bin(v->d_sph_tiles, v->d_order, S, cv.tiles_x, words, v->d_masks, s);
+    TileQueue tq = launch_bin(v->d_sph_tiles, v->d_order, S, cv.tiles_x, cv.tiles_y, words, v->d_masks, v->d_tiles, s);
     ev.mark(3);
     launch_cone(v->g, v->vd, cv, cd, v->d_levels, v->d_sph_world, v->d_sph_view, v->d_order, v->d_masks, words,
-                d_image, counters, s);
+                d_image, counters, tq.counter, tq.order, s);
     ev.mark(4);
     ev.end();
     CR_HIP(ctx, hipGetLastError());

@@ -309,13 +309,13 @@ inline float cv_exp2_neg(float x) {
     float n = std::floor(x + 0.5f);
     float f = x - n;
     float z = f * 0.693147180559945309f;
-    float p = 1.0f / 720.0f;
-    p = p * z + 1.0f / 120.0f;
-    p = p * z + 1.0f / 24.0f;
-    p = p * z + 1.0f / 6.0f;
-    p = p * z + 0.5f;
-    p = p * z + 1.0f;
-    p = p * z + 1.0f;
+    float p = 1.0f / 720.0f;  // Horner with fused multiply-adds
+    p = std::fma(p, z, 1.0f / 120.0f);
+    p = std::fma(p, z, 1.0f / 24.0f);
+    p = std::fma(p, z, 1.0f / 6.0f);
+    p = std::fma(p, z, 0.5f);
+    p = std::fma(p, z, 1.0f);
+    p = std::fma(p, z, 1.0f);
     int ni = (int)n;
     float scale = bitsf((uint32_t)(ni + 127) << 23);
     return p * scale;
@@ -759,23 +759,26 @@ template <class T> inline T clampT(T v, T lo, T hi) { return std::fmin(std::fmax
 
 // GL 4.4 8.14.2 trilinear filtering of one level with CLAMP_TO_EDGE:
 //   u = s * w - 1/2, i0 = floor(u), alpha = frac(u), i1 = i0 + 1, both clamped;
-//   evaluated as lerps along x, then y, then z: a*(1-t) + b*t.
+//   evaluated as lerps along x, then y, then z, each lerp(a, b, t) =
+//   fma(t, b - a, a) (GL leaves the filter arithmetic to the implementation;
+//   a fused multiply-add is correctly rounded on every IEEE target).
 // Coordinates are first clamped to [-1, 2] (outside [0,1] every tap is a
 // border texel already; the clamp keeps the arithmetic finite).
 // Pinned by: test_oracle_sampler_centres, test_oracle_sampler_affine,
 // test_oracle_sampler_clamp.
+template <class T> inline T lerpF(T a, T b, T t) { return std::fma(t, b - a, a); }
+
 template <class T, class VOL>
 void trilinear(const VOL& vol, int k, T sx, T sy, T sz, int nch, const int* chs, T* out) {
     int n = level_dim(vol.N, k);
     T fn = (T)n;
-    T u[3] = {clampT(sx, (T)-1, (T)2) * fn - (T)0.5, clampT(sy, (T)-1, (T)2) * fn - (T)0.5,
-              clampT(sz, (T)-1, (T)2) * fn - (T)0.5};
+    T u[3] = {std::fma(clampT(sx, (T)-1, (T)2), fn, (T)-0.5), std::fma(clampT(sy, (T)-1, (T)2), fn, (T)-0.5),
+              std::fma(clampT(sz, (T)-1, (T)2), fn, (T)-0.5)};
     int i0[3], i1[3];
-    T a[3], b[3];
+    T a[3];
     for (int ax = 0; ax < 3; ++ax) {
         T fl = std::floor(u[ax]);
         a[ax] = u[ax] - fl;
-        b[ax] = (T)1 - a[ax];
         int i = (int)fl;
         i0[ax] = std::min(std::max(i, 0), n - 1);
         i1[ax] = std::min(std::max(i + 1, 0), n - 1);
@@ -786,13 +789,13 @@ void trilinear(const VOL& vol, int k, T sx, T sy, T sz, int nch, const int* chs,
         T t010 = (T)vol.texel(k, i0[0], i1[1], i0[2], ch), t110 = (T)vol.texel(k, i1[0], i1[1], i0[2], ch);
         T t001 = (T)vol.texel(k, i0[0], i0[1], i1[2], ch), t101 = (T)vol.texel(k, i1[0], i0[1], i1[2], ch);
         T t011 = (T)vol.texel(k, i0[0], i1[1], i1[2], ch), t111 = (T)vol.texel(k, i1[0], i1[1], i1[2], ch);
-        T c00 = t000 * b[0] + t100 * a[0];
-        T c10 = t010 * b[0] + t110 * a[0];
-        T c01 = t001 * b[0] + t101 * a[0];
-        T c11 = t011 * b[0] + t111 * a[0];
-        T c0 = c00 * b[1] + c10 * a[1];
-        T c1 = c01 * b[1] + c11 * a[1];
-        out[q] = c0 * b[2] + c1 * a[2];
+        T c00 = lerpF(t000, t100, a[0]);
+        T c10 = lerpF(t010, t110, a[0]);
+        T c01 = lerpF(t001, t101, a[0]);
+        T c11 = lerpF(t011, t111, a[0]);
+        T c0 = lerpF(c00, c10, a[1]);
+        T c1 = lerpF(c01, c11, a[1]);
+        out[q] = lerpF(c0, c1, a[2]);
     }
 }
 
@@ -833,7 +836,8 @@ template <class T> StepTab<T> make_steps(const cvo_cone* cp, int L, int N) {
     return t;
 }
 
-// textureLod(volume, p, lod) (GL 4.4 8.14-8.15): (1 - f) * tau1 + f * tau2
+// textureLod(volume, p, lod) (GL 4.4 8.14-8.15): (1 - f) * tau1 + f * tau2,
+// evaluated as lerp(tau1, tau2, f)
 template <class T, class VOL>
 void sample_lod(const VOL& vol, const StepTab<T>& st, int i, T sx, T sy, T sz, int nch, const int* chs, T* out) {
     int lv = st.lvl[i];
@@ -842,8 +846,7 @@ void sample_lod(const VOL& vol, const StepTab<T>& st, int i, T sx, T sy, T sz, i
     if (f != (T)0) {
         T o2[4];
         trilinear<T>(vol, lv + 1, sx, sy, sz, nch, chs, o2);
-        T g = (T)1 - f;
-        for (int q = 0; q < nch; ++q) out[q] = out[q] * g + o2[q] * f;
+        for (int q = 0; q < nch; ++q) out[q] = lerpF(out[q], o2[q], f);
     }
 }
 
@@ -880,7 +883,7 @@ ConeResult<T> trace_cone(const VOL& vol, const StepTab<T>& st, const cvo_cone* c
         for (int i = 1; i <= n; ++i) {
             T h = st.h[i - 1];
             T s;
-            sample_lod<T>(vol, st, i - 1, p.x + h * d.x, p.y + h * d.y, p.z + h * d.z, 1, chR, &s);
+            sample_lod<T>(vol, st, i - 1, std::fma(h, d.x, p.x), std::fma(h, d.y, p.y), std::fma(h, d.z, p.z), 1, chR, &s);
             color = color + (s * (T)i) / den;
         }
         res.nsamp = n;
@@ -895,7 +898,7 @@ ConeResult<T> trace_cone(const VOL& vol, const StepTab<T>& st, const cvo_cone* c
     for (int i = 1; i <= cp->steps; ++i) {
         if (replay >= 0 && n >= replay) break;
         T h = st.h[i - 1];
-        T sx = p.x + h * d.x, sy = p.y + h * d.y, sz = p.z + h * d.z;
+        T sx = std::fma(h, d.x, p.x), sy = std::fma(h, d.y, p.y), sz = std::fma(h, d.z, p.z);
         if (replay < 0 && !(sx >= (T)0 && sx <= (T)1 && sy >= (T)0 && sy <= (T)1 && sz >= (T)0 && sz <= (T)1))
             break;
         T s[4];
@@ -906,7 +909,7 @@ ConeResult<T> trace_cone(const VOL& vol, const StepTab<T>& st, const cvo_cone* c
         T w = ((T)1 - a) * al;
         if (cp->compensated) {
             for (int q = 0; q < 3; ++q) {
-                T y = w * s[q] - cc[q];
+                T y = std::fma(w, s[q], -cc[q]);
                 T t = c[q] + y;
                 cc[q] = (t - c[q]) - y;
                 c[q] = t;
@@ -916,7 +919,7 @@ ConeResult<T> trace_cone(const VOL& vol, const StepTab<T>& st, const cvo_cone* c
             ca = (t - a) - y;
             a = t;
         } else {
-            for (int q = 0; q < 3; ++q) c[q] = c[q] + w * s[q];
+            for (int q = 0; q < 3; ++q) c[q] = std::fma(w, s[q], c[q]);
             a = a + w;
         }
         ++n;
@@ -1160,12 +1163,12 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
                 } else {
                     float A = C32[4 * pix + 3];
                     float w = (1.0f - A) * src[3];
-                    for (int q = 0; q < 3; ++q) C32[4 * pix + q] = C32[4 * pix + q] + w * src[q];
+                    for (int q = 0; q < 3; ++q) C32[4 * pix + q] = std::fma(w, src[q], C32[4 * pix + q]);
                     C32[4 * pix + 3] = A + w;
                     if (dbl) {
                         double Ad = C64[4 * pix + 3];
                         double wd = (1.0 - Ad) * srcd[3];
-                        for (int q = 0; q < 3; ++q) C64[4 * pix + q] += wd * srcd[q];
+                        for (int q = 0; q < 3; ++q) C64[4 * pix + q] = std::fma(wd, srcd[q], C64[4 * pix + q]);
                         C64[4 * pix + 3] = Ad + wd;
                     }
                     if (C32[4 * pix + 3] >= cp->alpha_stop) done[pix] = 1;
