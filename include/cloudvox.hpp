@@ -1,0 +1,208 @@
+// cloudvox.hpp - header-only C++ facade over the C ABI (cloudvox.h) that
+// mirrors the host objects of Cloud-Renderer's render path, so that code
+// written against the reference reads the same:
+//
+//   cr::CloudVolume      CloudVolume (src/CloudVolume.hpp:12-55): position,
+//                        x/y/zBounds, dimension, levels, voxelSize,
+//                        billboards {positions, scales}, fluffiness, update()
+//   cr::Sun              Sun (src/Sun.hpp:12-19): position (+ directional sun)
+//   cr::Camera           Camera (src/Camera.hpp:25-45): eye, lookAt, P/V inputs
+//   cr::VoxelizeShader   VoxelizeShader::voxelize(CloudVolume*)
+//                        (src/Shaders/VoxelizeShader.hpp:18)
+//   cr::ConeTraceShader  ConeTraceShader::coneTrace(CloudVolume*) with the
+//                        vct* parameters (src/Shaders/ConeTraceShader.hpp:12-32)
+//
+// Errors throw cr::Error (C++ only; the C ABI itself never throws).
+#pragma once
+
+#include <array>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cloudvox.h"
+
+namespace cr {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, cr_context ctx) {
+    if (rc != CR_OK) throw Error(rc, std::string("cloudvox: ") + cr_last_error(ctx));
+}
+
+using vec2 = std::array<float, 2>;
+using vec3 = std::array<float, 3>;
+
+class Context {
+  public:
+    explicit Context(int device = 0) { check(cr_context_create(device, 0, &h_), nullptr); }
+    ~Context() {
+        if (h_) cr_context_destroy(h_);
+    }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    cr_context handle() const { return h_; }
+    void setProfiling(int level) { check(cr_context_set_profiling(h_, level), h_); }
+    cr_stats stats() const {
+        cr_stats s;
+        check(cr_get_stats(h_, &s), h_);
+        return s;
+    }
+
+  private:
+    cr_context h_ = nullptr;
+};
+
+class CloudVolume {
+  public:
+    struct Billboards {
+        std::vector<vec3> positions;
+        std::vector<float> scales;
+        int count() const { return (int)positions.size(); }
+    };
+
+    // CloudVolume(int dim, glm::vec2 bounds, glm::vec3 position, int mips)
+    CloudVolume(Context& ctx, int dim, vec2 bounds, vec3 pos, int mips, int format = CR_FORMAT_RGBA32F,
+                int mode = CR_MODE_RGBA)
+        : position(pos), xBounds(bounds), yBounds(bounds), zBounds(bounds), dimension(dim), levels(mips),
+          ctx_(&ctx) {
+        float lo[3] = {bounds[0], bounds[0], bounds[0]};
+        float hi[3] = {bounds[1], bounds[1], bounds[1]};
+        check(cr_volume_create(ctx.handle(), dim, lo, hi, pos.data(), mips, format, mode, &h_), ctx.handle());
+        for (int a = 0; a < 3; ++a) voxelSize[a] = (bounds[1] - bounds[0]) / (float)dim;
+    }
+    ~CloudVolume() {
+        if (h_) cr_volume_destroy(h_);
+    }
+    CloudVolume(const CloudVolume&) = delete;
+    CloudVolume& operator=(const CloudVolume&) = delete;
+
+    void addCloudBoard(const vec3& p, float scale) {
+        billboards.positions.push_back(p);
+        billboards.scales.push_back(scale);
+    }
+    // CloudVolume::update -> uploadBillboards (src/CloudVolume.cpp:84-93, 139-164)
+    void update() {
+        std::vector<float> xyz;
+        xyz.reserve(3 * billboards.positions.size());
+        for (const vec3& p : billboards.positions) xyz.insert(xyz.end(), p.begin(), p.end());
+        check(cr_volume_set_billboards(h_, xyz.data(), billboards.scales.data(), billboards.count(), fluffiness),
+              ctx_->handle());
+    }
+    void setNoise(int octaves, uint32_t seed, float baseFreq = 4.0f, float lacunarity = 2.0f, float gain = 0.5f) {
+        check(cr_volume_set_noise(h_, octaves, seed, baseFreq, lacunarity, gain), ctx_->handle());
+    }
+    // VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115) plus an order-free hash
+    std::pair<uint64_t, uint64_t> occupancy(void* stream = nullptr) const {
+        uint64_t c = 0, h = 0;
+        check(cr_volume_occupancy(h_, &c, &h, stream), ctx_->handle());
+        return {c, h};
+    }
+
+    cr_volume handle() const { return h_; }
+    cr_context context() const { return ctx_->handle(); }
+
+    vec3 position;
+    vec2 xBounds, yBounds, zBounds;
+    int dimension;
+    int levels;
+    vec3 voxelSize{};
+    Billboards billboards;
+    float fluffiness = 1.0f;
+
+  private:
+    Context* ctx_;
+    cr_volume h_ = nullptr;
+};
+
+struct Sun {
+    vec3 position{5.0f, 20.0f, -5.0f};  // src/main.cpp:37
+    vec3 direction{0.0f, 1.0f, 0.0f};
+    bool directional = false;
+    cr_light c() const {
+        cr_light l{};
+        for (int a = 0; a < 3; ++a) {
+            l.position[a] = position[a];
+            l.direction[a] = direction[a];
+        }
+        l.directional = directional ? 1 : 0;
+        return l;
+    }
+};
+
+struct Camera {
+    vec3 position{0.0f, 0.0f, 3.0f};
+    vec3 lookAt{0.0f, 0.0f, 0.0f};
+    vec3 up{0.0f, 1.0f, 0.0f};
+    float fovy = 1.0471976f;
+    float znear = 0.01f, zfar = 2500.0f;  // src/Camera.cpp:59
+    cr_camera c() const {
+        cr_camera k{};
+        for (int a = 0; a < 3; ++a) {
+            k.eye[a] = position[a];
+            k.target[a] = lookAt[a];
+            k.up[a] = up[a];
+        }
+        k.fovy = fovy;
+        k.znear = znear;
+        k.zfar = zfar;
+        return k;
+    }
+};
+
+class VoxelizeShader {
+  public:
+    VoxelizeShader(int width, int height) : width_(width), height_(height) {}
+    // VoxelizeShader::voxelize (src/Shaders/VoxelizeShader.cpp:18-31)
+    void voxelize(CloudVolume* volume, const Sun& sun, void* stream = nullptr) const {
+        cr_light l = sun.c();
+        check(cr_voxelize(volume->handle(), &l, width_, height_, stream), volume->context());
+    }
+
+  private:
+    int width_, height_;
+};
+
+class ConeTraceShader {
+  public:
+    // src/Shaders/ConeTraceShader.hpp:28-32
+    int vctSteps = 16;
+    float vctConeAngle = 0.9f;
+    float vctConeInitialHeight = 0.1f;
+    float vctLodOffset = 0.0f;
+    float vctDownScaling = 1.0f;
+    int coneMode = CR_CONE_WEIGHTED;
+    float alphaStop = 0.99f;
+    float sigma = 24.0f;
+    int composite = CR_COMPOSITE_GL_BLEND;
+    bool quantizeRGBA8 = true;
+    bool compensated = false;
+    std::array<float, 4> clearColor{0.2f, 0.3f, 0.5f, 1.0f};  // src/main.cpp:112
+
+    // ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82) into a device image
+    void coneTrace(CloudVolume* volume, const Sun& sun, const Camera& camera, void* d_image, int width, int height,
+                   size_t pitch, int pixelFormat = CR_PIXEL_RGBA32F, void* stream = nullptr) const {
+        cr_cone_params p{};
+        p.steps = vctSteps;
+        p.cone_angle = vctConeAngle;
+        p.initial_height = vctConeInitialHeight;
+        p.lod_offset = vctLodOffset;
+        p.down_scaling = vctDownScaling;
+        p.cone_mode = coneMode;
+        p.alpha_stop = alphaStop;
+        p.sigma = sigma;
+        p.composite = composite;
+        p.quantize_rgba8 = quantizeRGBA8 ? 1 : 0;
+        p.compensated = compensated ? 1 : 0;
+        for (int q = 0; q < 4; ++q) p.clear_color[q] = clearColor[q];
+        cr_light l = sun.c();
+        cr_camera k = camera.c();
+        check(cr_cone_trace(volume->handle(), &l, &k, &p, d_image, width, height, pitch, pixelFormat, stream),
+              volume->context());
+    }
+};
+
+}  // namespace cr

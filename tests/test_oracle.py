@@ -1,0 +1,568 @@
+"""Pins of the CPU oracle (oracle/cv_oracle.cpp) that do not restate its code.
+
+The reference has no tests, fixtures or golden vectors, so each oracle function
+is checked against what the mathematics fixes: closed forms, brute force on
+tiny inputs, invariants and textbook routines (numpy's IEEE binary16,
+math.exp2, a literal selection sort).  Each check is chosen so that a plausible
+mistake (dropped term, wrong sign or index, transposed operand) fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+F = np.float32
+
+
+def vol(N=32, L=6, fmt=1, mode=1, lo=(-1, -1, -1), hi=(1, 1, 1), pos=(0, 0, 0), noise=None):
+    return O.make_volume(N, L, fmt, mode, lo, hi, pos, noise)
+
+
+# ---------------------------------------------------------------------------
+# index map (res/first_voxelize.glsl:23-37)
+# ---------------------------------------------------------------------------
+def test_oracle_index_map():
+    v = vol(N=32)
+    vs = 2.0 / 32
+    pts = np.array([
+        [-1, -1, -1],                       # lo -> voxel 0
+        [1, 0, 0],                          # x = hi -> N, dropped
+        [-1.01, -1, -1],                    # lerp -0.16 truncates to 0 (kept)
+        [-1 - 1.2 * vs, -1, -1],            # lerp -1.2 -> dropped
+        [-1 + 5 * vs + 1e-4, -1 + 7.5 * vs, -1 + 2.9 * vs],   # -> (5, 7, 2)
+        [0.999, 0.999, 0.999],              # -> (31, 31, 31)
+    ], dtype=F)
+    got = O.voxel_index(v, pts)
+    N = 32
+    assert got[0] == 0
+    assert got[1] == -1
+    assert got[2] == 0
+    assert got[3] == -1
+    assert got[4] == 5 + N * (7 + N * 2)
+    assert got[5] == 31 + N * (31 + N * 31)
+    # world bounds are position + bounds (src/Shaders/VoxelizeShader.cpp:124-126)
+    v2 = vol(N=32, pos=(10, 0, 0))
+    assert O.voxel_index(v2, np.array([[9.0 + 5 * vs + 1e-4, -1, -1]], F))[0] == 5
+    assert O.voxel_index(v2, np.array([[0.0, 0.0, 0.0]], F))[0] == -1
+
+
+# ---------------------------------------------------------------------------
+# binary16 conversions
+# ---------------------------------------------------------------------------
+def test_oracle_half_conversion():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([
+        rng.standard_normal(20000).astype(F) * F(100),
+        rng.uniform(-1, 1, 20000).astype(F) * F(1e-4),          # subnormal halves
+        np.array([0.0, -0.0, 65504, 65519.9, 65520, 1e6, -1e6, np.inf, -np.inf, 2 ** -24, 2 ** -25,
+                  3 * 2 ** -26, 2 ** -14, 1 + 2 ** -11, 1 + 3 * 2 ** -11, 0.1, 1 / 3], F),
+    ])
+    # exact ties: halfway between two halves (round to even)
+    h = np.arange(0x0001, 0x7bff, 37, dtype=np.uint16).view(np.float16).astype(F)
+    nxt = (np.arange(0x0001, 0x7bff, 37, dtype=np.uint16) + 1).view(np.float16).astype(F)
+    ties = ((h.astype(np.float64) + nxt.astype(np.float64)) / 2).astype(F)
+    x = np.concatenate([x, ties])
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).view(np.uint16)
+    got = O.f2h(x)
+    assert np.array_equal(got, want)
+    allh = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    back = O.h2f(allh)
+    ref = allh.view(np.float16).astype(F)
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(back), nan)
+    assert np.array_equal(back[~nan].view(np.uint32), ref[~nan].view(np.uint32))
+
+
+def test_oracle_exp2():
+    x = np.concatenate([np.linspace(-130, 0, 20001), -np.logspace(-8, 2, 2000)]).astype(F)
+    got = O.exp2_neg(x).astype(np.float64)
+    want = np.exp2(x.astype(np.float64))
+    want[x <= -126] = 0.0
+    ok = want > 0
+    rel = np.abs(got[ok] - want[ok]) / want[ok]
+    assert rel.max() < 4e-7
+    assert np.all(got[~ok] == 0)
+    assert O.exp2_neg(np.array([0.0], F))[0] == 1.0
+
+
+# ---------------------------------------------------------------------------
+# light view (src/Sun.hpp:26-43)
+# ---------------------------------------------------------------------------
+def _mat(lv):
+    return np.array(lv.V[:], np.float64).reshape(4, 4).T  # column-major -> row-major
+
+
+@pytest.mark.parametrize("sun", [(10, 0, 0), (3, 4, -2), (0, 10, 0), (0.5, -7, 0.3)])
+def test_oracle_sun_update(sun):
+    v = vol(N=32)
+    lv = O.sun_update(v, sun)
+    M = _mat(lv)
+    R = M[:3, :3]
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-6)
+    assert abs(np.linalg.det(R) - 1) < 1e-6
+    d = -np.asarray(sun, np.float64) / np.linalg.norm(sun)      # lookDir toward the volume
+    look_pos = -d * math.sqrt(3)
+    # the eye maps to the origin and the volume centre lies straight ahead at sqrt(3)
+    assert np.allclose(M @ np.append(look_pos, 1), [0, 0, 0, 1], atol=1e-5)
+    assert np.allclose(M @ np.array([0, 0, 0, 1]), [0, 0, -math.sqrt(3), 1], atol=1e-5)
+    assert np.allclose(np.array(lv.near_plane[:]), look_pos + d * 0.01, atol=1e-6)
+    assert abs(lv.clip - (2 * math.sqrt(3) - 0.01)) < 1e-5
+    assert list(lv.ortho) == [-2, 2, -2, 2]
+    assert abs(lv.zfar - (0.01 + lv.clip)) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# pass 1: analytic hemisphere under an axis-aligned sun
+# ---------------------------------------------------------------------------
+class _Scene:
+    def __init__(self, xyz, r, sun_pos, W=64, H=48, N=32, mode=1, fmt=1):
+        self.xyz = np.asarray(xyz, F).reshape(-1, 3)
+        self.scales = np.asarray(r, F).reshape(-1)
+        self.fluffiness = 1.0
+        self.N, self.levels, self.fmt, self.mode = N, 6, fmt, mode
+        self.bounds_lo, self.bounds_hi, self.position = (-1, -1, -1), (1, 1, 1), (0, 0, 0)
+        self.noise = None
+        self.light_w, self.light_h = W, H
+        self._sun = np.asarray(sun_pos, F)
+        self.sun_dir = self._sun / np.linalg.norm(self._sun)
+        self.directional = 1
+
+    @property
+    def S(self):
+        return self.xyz.shape[0]
+
+    @property
+    def sun_pos(self):
+        return self._sun
+
+
+def test_oracle_light_hemisphere():
+    r = 0.5
+    sc = _Scene([[0.1, -0.2, 0.0]], [r], (0, 0, 10), W=96, H=80)
+    res = O.voxelize(sc)
+    W, H = sc.light_w, sc.light_h
+    # light looks down -z with view x = world x, view y = world y (lookAt with up = +y)
+    px = (np.arange(W) + 0.5) * 2 / W - 1
+    py = (np.arange(H) + 0.5) * 2 / H - 1
+    X, Y = np.meshgrid(px * 2, py * 2)        # ortho window [-2, 2]^2
+    dx, dy = X - 0.1, Y + 0.2
+    q = np.sqrt(dx * dx + dy * dy) / r
+    sc64 = np.sqrt(np.maximum(0, 1 - q * q))
+    covered = sc64 >= 0.01
+    margin = np.abs(sc64 - 0.01) < 1e-5
+    got = res.winner >= 0
+    assert np.array_equal(got[~margin], covered[~margin])
+    assert got.sum() > 0.9 * math.pi * (r / 4 * W) * (r / 4 * H)
+    # the position map holds the hemisphere point facing the sun: z = +sqrt(r^2 - d^2)
+    pm = res.posmap[got]
+    zt = np.sqrt(np.maximum(0, r * r - dx[got] ** 2 - dy[got] ** 2))
+    assert np.allclose(pm[:, 0], X[got], atol=2e-6)
+    assert np.allclose(pm[:, 1], Y[got], atol=2e-6)
+    assert np.allclose(pm[:, 2], zt, atol=2e-5)
+    assert np.all(res.posmap[got][:, 3] == 1) and np.all(res.posmap[~got][:, 3] == 0)
+
+
+def test_oracle_depth_ties():
+    # identical spheres: the first drawn wins every tie (GL_LESS)
+    sc = _Scene([[0, 0, 0], [0, 0, 0]], [0.4, 0.4], (0, 0, 10))
+    res = O.voxelize(sc)
+    assert set(np.unique(res.winner)) == {-1, 0}
+    # the sphere nearer to the sun wins where both cover, in either draw order
+    for order in ([0, 1], [1, 0]):
+        xyz = np.array([[0, 0, 0.3], [0, 0, -0.3]], F)[order]
+        sc = _Scene(xyz, [0.3, 0.3], (0, 0, 10))
+        res = O.voxelize(sc)
+        near = order.index(0)
+        assert set(np.unique(res.winner)) == {-1, near}
+    # concentric: the larger sphere's cap is nearer the sun everywhere
+    sc = _Scene([[0, 0, 0], [0, 0, 0]], [0.2, 0.4], (0, 0, 10))
+    res = O.voxelize(sc)
+    assert set(np.unique(res.winner)) == {-1, 1}
+
+
+# ---------------------------------------------------------------------------
+# pass 2 and chord fill: brute force over the fragments
+# ---------------------------------------------------------------------------
+def _brute_index(p, N=32):
+    """calculateVoxelIndex in float32 for world bounds [-1, 1]^3; -1 when dropped."""
+    p = p.astype(F)
+    lerp = F(N) * ((p - F(-1)) / F(2))
+    ok = np.all((lerp > -1) & (lerp < N), axis=-1)
+    ijk = np.trunc(lerp).astype(np.int64)
+    lin = ijk[..., 0] + N * (ijk[..., 1] + N * ijk[..., 2])
+    return np.where(ok, lin, -1)
+
+
+def test_oracle_splat_bruteforce():
+    rng = np.random.default_rng(3)
+    xyz = rng.uniform(-0.5, 0.5, (6, 3))
+    sc = _Scene(xyz, rng.uniform(0.1, 0.25, 6), (3, 7, 2), W=80, H=64, mode=0)
+    res = O.voxelize(sc)
+    step = F(2.0 / 32)
+    k = F(1.0) / np.sqrt(F(3.0))
+    wp = res.posmap[res.winner >= 0][:, :3].astype(F)
+    want = set()
+    offs = [(0, 0, 0)] + [(sx, sy, sz) for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)]
+    for o in offs:
+        d = np.array(o, F) * (step * k)
+        idx = _brute_index(wp + d)
+        want |= set(idx[idx >= 0].tolist())
+    got = set(np.flatnonzero(res.shell.reshape(-1)).tolist())
+    assert got == want
+    assert res.fill.sum() == 0  # reference mode: no chord fill
+
+
+def test_oracle_chord_bruteforce():
+    rng = np.random.default_rng(5)
+    xyz = rng.uniform(-0.4, 0.4, (3, 3))
+    r = rng.uniform(0.15, 0.3, 3)
+    sun = (1.0, 2.0, 0.5)
+    sc = _Scene(xyz, r, sun, W=48, H=40, N=16, mode=1)
+    res = O.voxelize(sc)
+    v = vol(N=16)
+    lv = O.sun_update(v, sc.sun_pos)
+    V = np.array(lv.V[:], F).reshape(4, 4)            # V[col][row]
+    S_ax = np.array([V[0, 0], V[1, 0], V[2, 0]], F)
+    U_ax = np.array([V[0, 1], V[1, 1], V[2, 1]], F)
+    B = np.array([V[0, 2], V[1, 2], V[2, 2]], F)
+    B = (B * (F(1) / np.sqrt(np.sum(B * B, dtype=F)))).astype(F)
+    W, H = sc.light_w, sc.light_h
+    want = set()
+    step = F(2.0 / 16)
+    for j in range(3):
+        c = xyz[j].astype(F)
+        s = F(r[j])
+        cv = np.array([((V[0, a] * c[0] + V[1, a] * c[1]) + V[2, a] * c[2]) + V[3, a] for a in range(3)], F)
+        for py_ in range(H):
+            for px_ in range(W):
+                xn = (F(2) * (F(px_) + F(0.5))) / F(W) - F(1)
+                yn = (F(2) * (F(py_) + F(0.5))) / F(H) - F(1)
+                du = (xn * F(2) + F(0)) - cv[0]
+                dv = (yn * F(2) + F(0)) - cv[1]
+                q = np.sqrt(du * du + dv * dv) / s
+                scn = np.sqrt(max(F(0), F(1) - q * q))
+                if scn < F(0.01):
+                    continue
+                fp = (c + du * S_ax) + dv * U_ax
+                dist = s * scn
+                start = fp - B * dist
+                i = F(0)
+                while i < F(2) * dist:
+                    idx = _brute_index((start + B * i)[None, :], N=16)[0]
+                    if idx >= 0:
+                        want.add(int(idx))
+                    i = F(i + step)
+    got = set(np.flatnonzero(res.fill.reshape(-1)).tolist())
+    assert got == want
+    assert len(got) > 50
+
+
+# ---------------------------------------------------------------------------
+# mip chain (glGenerateMipmap, src/Shaders/VoxelizeShader.cpp:105)
+# ---------------------------------------------------------------------------
+class _VS:
+    def __init__(self, N, L, fmt=1, mode=1, noise=None):
+        self.N, self.levels, self.fmt, self.mode = N, L, fmt, mode
+        self.bounds_lo, self.bounds_hi, self.position = (-1, -1, -1), (1, 1, 1), (0, 0, 0)
+        self.noise = noise
+
+
+def test_oracle_mips_constant():
+    N = 16
+    shell = np.ones((N, N, N), np.uint8)
+    _, _, nl, fl = O.build_levels(_VS(N, 5), shell, np.zeros_like(shell))
+    for lvl in nl:
+        assert np.all(lvl == 1.0)
+
+
+def test_oracle_mips_single_voxel():
+    N = 16
+    shell = np.zeros((N, N, N), np.uint8)
+    shell[5, 6, 7] = 1          # z, y, x
+    _, _, nl, _ = O.build_levels(_VS(N, 5), shell, np.zeros_like(shell))
+    assert nl[1][2, 3, 3, 0] == 1 / 8 and np.count_nonzero(nl[1][..., 0]) == 1
+    assert nl[2][1, 1, 1, 0] == 1 / 64 and np.count_nonzero(nl[2][..., 0]) == 1
+    assert nl[4][0, 0, 0, 0] == 1 / 4096
+
+
+def test_oracle_mips_mean_and_modes():
+    rng = np.random.default_rng(7)
+    N = 32
+    shell = (rng.random((N, N, N)) < 0.2).astype(np.uint8)
+    fill = (rng.random((N, N, N)) < 0.3).astype(np.uint8)
+    _, f64, nl, fl = O.build_levels(_VS(N, 6), shell, fill)
+    a0 = nl[0][..., 3].mean()
+    for lvl in nl:
+        assert abs(lvl[..., 3].mean() - a0) < 1e-6     # mean preserved (power-of-two N)
+    # rgba semantics: shell (1,1,1,1), fill only (0,0,0,1)
+    st = np.where(shell != 0, 2, np.where(fill != 0, 1, 0))
+    assert np.all(nl[0][st == 2] == 1)
+    assert np.all(nl[0][st == 1] == [0, 0, 0, 1])
+    assert np.all(nl[0][st == 0] == 0)
+    # level k is the 2x2x2 box mean of level k-1
+    for k in range(1, 6):
+        prev = fl[k - 1]
+        n = prev.shape[0] // 2
+        box = prev.reshape(n, 2, n, 2, n, 2, 4).mean(axis=(1, 3, 5))
+        assert np.allclose(fl[k], box, atol=1e-12)
+        assert np.allclose(nl[k], box, atol=1e-6)
+
+
+def test_oracle_mips_npot_and_r8():
+    rng = np.random.default_rng(9)
+    N = 6       # 6 -> 3 -> 1: the last step takes all three children per axis
+    shell = (rng.random((N, N, N)) < 0.5).astype(np.uint8)
+    _, f64, nl, fl = O.build_levels(_VS(N, 3), shell, np.zeros_like(shell))
+    l1 = shell.reshape(3, 2, 3, 2, 3, 2).mean(axis=(1, 3, 5))
+    assert np.allclose(fl[1][..., 0], l1)
+    assert abs(fl[2][0, 0, 0, 0] - l1.mean()) < 1e-12
+    # R8 (the reference's GL_R8): 255 on the shell, integer means rounded half up
+    nat, _, nl8, _ = O.build_levels(_VS(8, 4, fmt=0, mode=0), np.pad(shell, 1)[:8, :8, :8], np.zeros((8, 8, 8), np.uint8))
+    s8 = np.pad(shell, 1)[:8, :8, :8].astype(np.int64) * 255
+    assert np.array_equal(nl8[0], s8)
+    sums = s8.reshape(4, 2, 4, 2, 4, 2).sum(axis=(1, 3, 5))
+    assert np.array_equal(nl8[1], (sums + 4) // 8)
+
+
+def test_oracle_fp16_levels_round_once_per_level():
+    rng = np.random.default_rng(11)
+    N = 16
+    shell = (rng.random((N, N, N)) < 0.3).astype(np.uint8)
+    fill = (rng.random((N, N, N)) < 0.3).astype(np.uint8)
+    n32 = O.build_levels(_VS(N, 5, fmt=1, mode=1, noise=_noise()), shell, fill)[2]
+    n16 = O.build_levels(_VS(N, 5, fmt=2, mode=1, noise=_noise()), shell, fill)[2]
+    # level 0 is the FP32 value rounded to nearest binary16
+    assert np.array_equal(n16[0], n32[0].astype(np.float16).view(np.uint16))
+    # level k = round16(box mean of the rounded level k-1)
+    for k in range(1, 5):
+        prev = n16[k - 1].view(np.float16).astype(np.float32)
+        n = prev.shape[0] // 2
+        box = prev.reshape(n, 2, n, 2, n, 2, 4).astype(np.float64).sum(axis=(1, 3, 5)) / 8
+        assert np.array_equal(n16[k], box.astype(np.float32).astype(np.float16).view(np.uint16))
+
+
+def _noise(octaves=4, seed=5, base=4.0):
+    class N_:
+        pass
+    n = N_()
+    n.octaves, n.seed, n.base_freq, n.lacunarity, n.gain = octaves, seed, base, 2.0, 0.5
+    return n
+
+
+def _fmix32(h):
+    h = np.uint32(h)
+    h ^= h >> np.uint32(16)
+    h = np.uint32((int(h) * 0x85EBCA6B) & 0xFFFFFFFF)
+    h ^= h >> np.uint32(13)
+    h = np.uint32((int(h) * 0xC2B2AE35) & 0xFFFFFFFF)
+    h ^= h >> np.uint32(16)
+    return int(h)
+
+
+def test_oracle_value_noise_lattice():
+    # one octave with the lattice on voxel centres: (x + 0.5) / 8 * 16 = 2x + 1
+    v = vol(N=8, noise=_noise(octaves=1, seed=123, base=16.0))
+    pts = np.array([[x, y, z] for x in range(8) for y in range(0, 8, 3) for z in (0, 7)], np.int32)
+    n32, n64 = O.noise_voxels(v, pts)
+    for (x, y, z), a, b in zip(pts.tolist(), n32, n64):
+        h = 123
+        h ^= ((2 * x + 1) * 0x8DA6B343) & 0xFFFFFFFF
+        h ^= ((2 * y + 1) * 0xD8163841) & 0xFFFFFFFF
+        h ^= ((2 * z + 1) * 0xCB1AB31F) & 0xFFFFFFFF
+        want = (_fmix32(h) >> 8) / 16777216.0
+        assert a == np.float32(want) and b == want
+    # 4 octaves: values in [0, 1), FP32 agrees with FP64
+    v4 = vol(N=32, noise=_noise())
+    pts = np.random.default_rng(0).integers(0, 32, (500, 3)).astype(np.int32)
+    n32, n64 = O.noise_voxels(v4, pts)
+    assert np.all((n64 >= 0) & (n64 < 1)) and np.abs(n32 - n64).max() < 1e-6
+    assert 0.3 < n64.mean() < 0.7
+
+
+# ---------------------------------------------------------------------------
+# sampler (textureLod, GL 4.4 8.14)
+# ---------------------------------------------------------------------------
+def _levels_from(l0):
+    """linear RGBA32F levels with exact box means, for the sampler tests"""
+    lv = [l0]
+    while lv[-1].shape[0] > 1:
+        p = lv[-1]
+        n = p.shape[0] // 2
+        lv.append(p.reshape(n, 2, n, 2, n, 2, 4).mean(axis=(1, 3, 5)))
+    return lv
+
+
+def test_oracle_sampler_centres_affine_clamp():
+    N = 16
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    l0 = np.stack([0.25 + 0.01 * x, 0.5 - 0.02 * y, 0.1 + 0.03 * z, 0.3 + 0.01 * x + 0.02 * y], -1)
+    lv = _levels_from(l0)
+    v = vol(N=N, L=len(lv))
+    nat = np.concatenate([a.reshape(-1) for a in lv]).astype(F)
+    f64 = np.concatenate([a.reshape(-1) for a in lv]).astype(np.float64)
+    # texel centres return the texel
+    idx = np.array([[0, 0, 0], [3, 7, 11], [15, 15, 15], [8, 2, 5]])
+    pts = (idx + 0.5) / N
+    o32, o64 = O.sample(v, nat, f64, 0.0, pts)
+    assert np.allclose(o64, l0[idx[:, 2], idx[:, 1], idx[:, 0]], atol=1e-12)
+    # exact on an affine field inside the volume (level 0)
+    rng = np.random.default_rng(1)
+    p = rng.uniform(0.5 / N, 1 - 0.5 / N, (200, 3)).astype(F)  # the sampler takes FP32 coordinates
+    o32, o64 = O.sample(v, nat, f64, -3.0, p)
+    u = p.astype(np.float64) * N - 0.5
+    want = np.stack([0.25 + 0.01 * u[:, 0], 0.5 - 0.02 * u[:, 1], 0.1 + 0.03 * u[:, 2],
+                     0.3 + 0.01 * u[:, 0] + 0.02 * u[:, 1]], -1)
+    assert np.abs(o64 - want).max() < 1e-12 and np.abs(o32 - want).max() < 1e-5
+    # clamp to edge: beyond the border the edge texel is returned
+    o32, o64 = O.sample(v, nat, f64, 0.0, np.array([[-0.3, 0.5 / N, 1.7]]))
+    assert np.allclose(o64[0], l0[N - 1, 0, 0], atol=1e-12)
+    # lambda >= L-1: top level only; lambda = 1.5: halfway between levels 1 and 2
+    o32, o64 = O.sample(v, nat, f64, 9.0, np.array([[0.3, 0.6, 0.1]]))
+    assert np.allclose(o64[0], lv[-1][0, 0, 0], atol=1e-12)
+    c = np.array([[0.5, 0.5, 0.5]])
+    a, _ = O.sample(v, nat, f64, 1.0, c)
+    b, _ = O.sample(v, nat, f64, 2.0, c)
+    m, _ = O.sample(v, nat, f64, 1.5, c)
+    assert np.allclose(m, (a + b) / 2, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# cone (res/conetrace_frag.glsl:64-79) and the front-to-back extension
+# ---------------------------------------------------------------------------
+def _cone(steps=16, angle=0.9, h0=0.1, mode=0, alpha_stop=0.99, sigma=24.0, ds=1.0, comp=0):
+    k = O.cvo_cone()
+    k.steps, k.cone_angle, k.initial_height, k.lod_offset, k.down_scaling = steps, angle, h0, 0.0, ds
+    k.cone_mode, k.alpha_stop, k.sigma, k.composite, k.quantize_rgba8, k.compensated = mode, alpha_stop, sigma, 1, 0, comp
+    k.clear_color = O.F4(0, 0, 0, 0)
+    return k
+
+
+def _const_levels(N, L, rgba):
+    dims = [max(1, N >> k) for k in range(L)]
+    nat = np.concatenate([np.tile(np.asarray(rgba, F), d ** 3) for d in dims]).astype(F)
+    return nat, nat.astype(np.float64)
+
+
+def test_oracle_cone_constant_weighted():
+    N, L = 16, 5
+    nat, f64 = _const_levels(N, L, (0.6, 0.6, 0.6, 0.6))
+    v = vol(N=N, L=L)
+    for steps, ds in ((16, 1.0), (5, 2.0), (64, 0.5)):
+        o32, o64, n = O.trace_one(v, nat, f64, _cone(steps=steps, ds=ds), (0.5, 0.5, 0.5), (0.01, 0.0, 0.0))
+        assert n == steps
+        v06 = float(F(0.6))                                           # the stored texel value
+        assert abs(o64[0] - v06 * (steps + 1) / (2 * ds)) < 1e-9     # sum of v * i / (steps * ds)
+        assert abs(o32[0] - o64[0]) < 1e-4
+
+
+def test_oracle_cone_ftb_constant_alpha():
+    N, L = 16, 5
+    for alpha in (0.3, 0.05):
+        nat, f64 = _const_levels(N, L, (0.2, 0.4, 0.8, alpha))
+        v = vol(N=N, L=L)
+        o32, o64, n = O.trace_one(v, nat, f64, _cone(steps=200, mode=1, h0=0.001, angle=1e-4), (0.5, 0.5, 0.5),
+                                  (1e-6, 0, 0))
+        a32 = float(F(alpha))                                         # stored texel values
+        rgb = np.array([F(0.2), F(0.4), F(0.8)], np.float64)
+        n_expect = math.ceil(math.log(0.01) / math.log(1 - a32))
+        assert n == n_expect
+        A = 1 - (1 - a32) ** n
+        assert abs(o64[3] - A) < 1e-12
+        assert np.allclose(o64[:3], rgb * A, atol=1e-12)
+
+
+def test_oracle_cone_density_closed_form():
+    N, L = 16, 5
+    nat, f64 = _const_levels(N, L, (1.0, 1.0, 1.0, 1.0))
+    v = vol(N=N, L=L)
+    cone = _cone(steps=20, mode=2, sigma=3.0, alpha_stop=2.0, angle=0.5)
+    h, r, lvl, fr = O.step_table(cone, L, N)
+    # stays inside: start at the centre, tiny direction
+    o32, o64, n = O.trace_one(v, nat, f64, cone, (0.5, 0.5, 0.5), (1e-7, 0, 0))
+    assert n == 20
+    tan_half = math.tan(float(F(0.5)) / 2)      # parameters arrive as FP32
+    hh, tau = float(F(0.1)), 0.0
+    for i in range(20):
+        rr = hh * tan_half
+        tau += 3.0 * rr / N
+        hh += rr
+    assert abs(o64[3] - (1 - math.exp(-tau))) < 1e-9
+    assert abs(o32[3] - o64[3]) < 1e-5
+
+
+def test_oracle_cone_empty_and_exit():
+    N, L = 16, 5
+    nat, f64 = _const_levels(N, L, (0, 0, 0, 0))
+    v = vol(N=N, L=L)
+    o32, o64, n = O.trace_one(v, nat, f64, _cone(mode=1, steps=50), (0.5, 0.5, 0.5), (0.1, 0, 0))
+    assert np.all(o64 == 0)
+    # front-to-back stops before the first sample outside [0,1]^3
+    cone = _cone(mode=1, steps=50, h0=1.0, angle=0.5)
+    h, _, _, _ = O.step_table(cone, L, N)
+    o32, o64, n = O.trace_one(v, nat, f64, cone, (0.5, 0.5, 0.5), (1.0 / N, 0, 0))
+    inside = [0.5 + hh / N <= 1.0 for hh in h]
+    assert n == inside.index(False)
+
+
+def test_oracle_step_table():
+    cone = _cone(steps=40, angle=math.radians(20), h0=0.1)
+    h, r, lvl, fr = O.step_table(cone, 9, 256)
+    t = math.tan(math.radians(10))
+    for i in range(40):
+        assert abs(h[i] - 0.1 * (1 + t) ** i) <= 1e-5 * h[i]
+        lam = math.log2(max(1.0, 2 * r[i]))
+        if lam <= 0:
+            assert lvl[i] == 0 and fr[i] == 0
+        elif lam >= 8:
+            assert lvl[i] == 8 and fr[i] == 0
+        else:
+            assert lvl[i] == math.floor(lam) and abs(fr[i] - (lam - math.floor(lam))) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# sortBoards (src/CloudVolume.cpp:65-82)
+# ---------------------------------------------------------------------------
+def _literal_selection(d):
+    d = list(d)
+    idx = list(range(len(d)))
+    for i in range(len(d)):
+        m = i
+        for j in range(i + 1, len(d)):
+            if d[m] < d[j]:
+                m = j
+        d[i], d[m] = d[m], d[i]
+        idx[i], idx[m] = idx[m], idx[i]
+    return idx
+
+
+class _SortScene:
+    def __init__(self, xyz):
+        self.xyz = np.asarray(xyz, F)
+        self.position = (0, 0, 0)
+
+    @property
+    def S(self):
+        return self.xyz.shape[0]
+
+
+def test_oracle_sort_distinct_and_ties():
+    rng = np.random.default_rng(4)
+    xyz = rng.uniform(-1, 1, (300, 3)).astype(F)
+    eye = (0.3, 0.1, 3.0)
+    order = O.sort_boards(_SortScene(xyz), eye)
+    d = np.sqrt(np.sum((np.asarray(eye, np.float64) - xyz.astype(np.float64)) ** 2, axis=1))
+    assert np.array_equal(order, np.argsort(-d, kind="stable"))
+    # ties: four-fold rotations about the camera axis give equal FP32 distances
+    base = rng.uniform(-0.5, 0.5, (12, 3)).astype(F)
+    pts = []
+    for x, y, z in base:
+        pts += [(x, y, z), (-y, x, z), (-x, -y, z), (y, -x, z)]
+    pts = np.array(pts, F)[rng.permutation(48)]
+    order = O.sort_boards(_SortScene(pts), (0, 0, 3))
+    d32 = np.zeros(48, F)
+    O.lib().cvo_board_distances(48, O.ptr(np.ascontiguousarray(pts)), O.f3((0, 0, 0)), O.f3((0, 0, 3)), O.ptr(d32))
+    assert len(np.unique(d32)) < 48
+    assert list(order) == _literal_selection(d32.tolist())
