@@ -151,12 +151,17 @@ def cpu_baseline(cv, scene, budget_s: float):
 
 
 def load_traffic(name: str):
-    p = REPO / "profiles" / f"{name}_traffic.json"
-    if p.exists():
-        try:
-            return json.loads(p.read_text())
-        except Exception:
-            return None
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    (summary under profiles/, a line `traffic_bytes_per_launch: <int>`)."""
+    p = REPO / "profiles" / f"{name}_traffic.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            try:
+                return {"bytes_per_launch": int(line.split(":", 1)[1].strip())}
+            except ValueError:
+                return None
     return None
 
 
@@ -169,6 +174,8 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true", help="skip the reference-semantics frame")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -185,6 +192,14 @@ def main() -> int:
         return 2
     cv = _load_pkg()
     ctx = cv.Context(0)
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
     scene = cv.scenes.plain256(args.seed)
     log(f"[bench] plain256: N={scene.N} S={scene.S} {scene.W}x{scene.H} steps={args.steps} warmup={args.warmup}")
     r = run_gpu(cv, scene, args.steps, args.warmup, ctx, "plain256")
