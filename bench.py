@@ -174,6 +174,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle time for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true", help="skip the reference-semantics frame")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the 512^3 RGBA16F workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -246,6 +247,19 @@ def main() -> int:
                            "frame_ms": round(rr["secs"] * 1e3 / max(4, args.steps // 4), 4),
                            "stage_ms": {k: round(v, 4) for k, v in rr["stage_ms"].items()},
                            "counters_per_frame": {k: int(v) for k, v in rr["counters"].items()}}
+    if not args.no_fp16:
+        f16 = cv.scenes.fp16_512(args.seed)
+        k = max(4, args.steps // 4)
+        log(f"[bench] fp16_512: N={f16.N} S={f16.S} {f16.W}x{f16.H} steps={k}")
+        rf = run_gpu(cv, f16, k, 2, ctx, "fp16_512")
+        out["fp16_512"] = {"desc": "512^3 RGBA16F volume with 4-octave value-noise density, 16384 clumped spheres, "
+                                   "1920x1080, FP32 filtering and compensated accumulation",
+                           "mrays_per_s": round(f16.W * f16.H * k / rf["secs"] / 1e6, 2),
+                           "frame_ms": round(rf["secs"] * 1e3 / k, 4),
+                           "cone_mrays_per_s": round(f16.W * f16.H / (rf["stage_ms"]["cone"] * 1e-3) / 1e6, 2),
+                           "stage_ms": {kk: round(v, 4) for kk, v in rf["stage_ms"].items()},
+                           "counters_per_frame": {kk: int(v) for kk, v in rf["counters"].items()},
+                           "cone_gather_gbs": round(rf["bytes"]["cone"] / (rf["stage_ms"]["cone"] * 1e-3) / 1e9, 1)}
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle FP32 pass)")
         out["cpu_baseline"] = cpu_baseline(cv, scene, args.cpu_budget)
