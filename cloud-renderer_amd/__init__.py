@@ -30,8 +30,8 @@ from ._abi import (CR_COMPOSITE_FTB, CR_COMPOSITE_GL_BLEND, CR_CONE_FTB, CR_CONE
                    CR_MODE_RGBA, CR_PIXEL_RGBA8, CR_PIXEL_RGBA32F, CloudVoxError, cr_camera, cr_cone_params,
                    cr_light, cr_stats)
 
-__all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "CloudVoxError",
-           "lib", "scenes"]
+__all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "FarFieldMLP",
+           "CloudVoxError", "lib", "scenes"]
 
 
 def lib():
@@ -286,9 +286,8 @@ class ConeTraceShader:
                               1 if self.quantizeRGBA8 else 0, 1 if self.compensated else 0,
                               _abi.F4(*[float(c) for c in self.clearColor]))
 
-    def coneTrace(self, volume: CloudVolume, sun: Sun, camera: Camera, image, width: int, height: int,
-                  pixel_format: int = CR_PIXEL_RGBA32F, pitch: Optional[int] = None, stream=None) -> None:
-        """Render into `image`: a torch CUDA tensor (H, W, 4) or a raw device pointer (int)."""
+    @staticmethod
+    def _image_args(image, width, pixel_format, pitch, stream):
         if hasattr(image, "data_ptr"):
             ptr = image.data_ptr()
             if pitch is None:
@@ -300,12 +299,148 @@ class ConeTraceShader:
             ptr = int(image)
             if pitch is None:
                 pitch = width * (16 if pixel_format == CR_PIXEL_RGBA32F else 4)
+        return ptr, pitch, stream
+
+    def coneTrace(self, volume: CloudVolume, sun: Sun, camera: Camera, image, width: int, height: int,
+                  pixel_format: int = CR_PIXEL_RGBA32F, pitch: Optional[int] = None, stream=None) -> None:
+        """Render into `image`: a torch CUDA tensor (H, W, 4) or a raw device pointer (int)."""
+        ptr, pitch, stream = self._image_args(image, width, pixel_format, pitch, stream)
         light = sun.to_c()
         cam = camera.to_c()
         p = self.params()
         _check(lib().cr_cone_trace(volume.handle, C.byref(light), C.byref(cam), C.byref(p), C.c_void_p(ptr),
                                    int(width), int(height), C.c_size_t(pitch), int(pixel_format),
                                    _stream_handle(stream)), volume.ctx.handle)
+
+    def coneTraceLearned(self, volume: CloudVolume, sun: Sun, camera: Camera, mlp: "FarFieldMLP", image,
+                         width: int, height: int, k: int = 32, err_bound: float = 2e-3,
+                         pixel_format: int = CR_PIXEL_RGBA32F, pitch: Optional[int] = None, stream=None) -> None:
+        """coneTrace with the far field after k samples predicted by `mlp`;
+        cones whose predicted error times remaining transmittance exceeds
+        err_bound fall back to the plain march (err_bound < 0: all do)."""
+        ptr, pitch, stream = self._image_args(image, width, pixel_format, pitch, stream)
+        light = sun.to_c()
+        cam = camera.to_c()
+        p = self.params()
+        _check(lib().cr_cone_trace_learned(volume.handle, C.byref(light), C.byref(cam), C.byref(p), mlp.handle,
+                                           int(k), float(err_bound), C.c_void_p(ptr), int(width), int(height),
+                                           C.c_size_t(pitch), int(pixel_format), _stream_handle(stream)),
+               volume.ctx.handle)
+
+    def segments(self, volume: CloudVolume, sun: Sun, p0, k: int = 32, stream=None):
+        """Training rows of the far-field MLP from the plain march: p0 is a
+        (n, 3) float32 CUDA tensor of cone start points in texture space.
+        Returns (features (n, 17), targets (n, 5), valid (n,) int32) tensors."""
+        import torch
+        n = int(p0.shape[0])
+        dev = p0.device
+        feat = torch.empty((n, _abi.CR_MLP_INPUTS), dtype=torch.float32, device=dev)
+        tgt = torch.empty((n, _abi.CR_MLP_OUTPUTS), dtype=torch.float32, device=dev)
+        valid = torch.empty((n,), dtype=torch.int32, device=dev)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev)
+        light = sun.to_c()
+        p = self.params()
+        _check(lib().cr_volume_segments(volume.handle, C.byref(light), C.byref(p), int(k),
+                                        C.c_void_p(p0.contiguous().data_ptr()), n, C.c_void_p(feat.data_ptr()),
+                                        C.c_void_p(tgt.data_ptr()), C.c_void_p(valid.data_ptr()),
+                                        _stream_handle(stream)), volume.ctx.handle)
+        return feat, tgt, valid
+
+
+class FarFieldMLP:
+    """The learned far field (cr_mlp): 17 -> 64 x 4 (GELU) -> 5, BF16 weights on
+    the matrix cores, FP32 master parameters trained with Adam by the
+    library's own forward/backward kernels."""
+
+    PARAMS = _abi.CR_MLP_PARAMS
+
+    def __init__(self, context: Optional[Context] = None, params: Optional[np.ndarray] = None):
+        self.ctx = context or default_context()
+        h = C.c_void_p()
+        _check(lib().cr_mlp_create(self.ctx.handle, C.byref(h)), self.ctx.handle)
+        self.handle = h
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def init_params(seed: int = 0) -> np.ndarray:
+        """He-style initialisation (normal, std sqrt(2 / fan_in)), zero biases."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        dims = [(17, 64), (64, 64), (64, 64), (64, 64), (64, 5)]
+        out = []
+        for fin, fout in dims:
+            out.append(rng.normal(0.0, math.sqrt(2.0 / fin), size=(fout, fin)).astype(np.float32).ravel())
+            out.append(np.zeros(fout, np.float32))
+        p = np.concatenate(out)
+        assert p.size == FarFieldMLP.PARAMS
+        return p
+
+    def set_params(self, params: np.ndarray) -> None:
+        a = np.ascontiguousarray(params, np.float32).ravel()
+        _check(lib().cr_mlp_set_params(self.handle, _abi.fptr(a), int(a.size)), self.ctx.handle)
+
+    def get_params(self) -> np.ndarray:
+        a = np.empty(self.PARAMS, np.float32)
+        _check(lib().cr_mlp_get_params(self.handle, _abi.fptr(a), int(a.size)), self.ctx.handle)
+        return a
+
+    def forward(self, x, stream=None):
+        import torch
+        x = x.contiguous()
+        n = int(x.shape[0])
+        y = torch.empty((n, _abi.CR_MLP_OUTPUTS), dtype=torch.float32, device=x.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        _check(lib().cr_mlp_forward(self.handle, C.c_void_p(x.data_ptr()), n, C.c_void_p(y.data_ptr()),
+                                    _stream_handle(stream)), self.ctx.handle)
+        return y
+
+    def gradient(self, x, t, stream=None):
+        import torch
+        x, t = x.contiguous(), t.contiguous()
+        g = np.empty(self.PARAMS, np.float32)
+        loss = C.c_float()
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        _check(lib().cr_mlp_gradient(self.handle, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                     int(x.shape[0]), _abi.fptr(g), C.byref(loss), _stream_handle(stream)),
+               self.ctx.handle)
+        return g, float(loss.value)
+
+    def train_step(self, x, t, lr: float, stream=None) -> None:
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        _check(lib().cr_mlp_train_step(self.handle, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                       int(x.shape[0]), float(lr), _stream_handle(stream)), self.ctx.handle)
+
+    def last_loss(self, stream=None) -> float:
+        import torch
+        loss = C.c_float()
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        _check(lib().cr_mlp_last_loss(self.handle, C.byref(loss), _stream_handle(stream)), self.ctx.handle)
+        return float(loss.value)
+
+    def save(self, path) -> None:
+        np.savez_compressed(path, params=self.get_params())
+
+    @staticmethod
+    def load_params(path) -> np.ndarray:
+        with np.load(path, allow_pickle=False) as z:
+            return np.asarray(z["params"], np.float32)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().cr_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 from . import scenes  # noqa: E402

@@ -61,7 +61,13 @@ class cr_stats(C.Structure):
                 ("ms_clear", C.c_float), ("ms_light_raster", C.c_float), ("ms_splat", C.c_float),
                 ("ms_mip", C.c_float), ("ms_setup", C.c_float), ("ms_sort", C.c_float),
                 ("ms_bin", C.c_float), ("ms_cone", C.c_float), ("n_voxelize", C.c_int64), ("n_cone", C.c_int64),
-                ("chord_stores", C.c_int64)]
+                ("chord_stores", C.c_int64), ("mlp_rows", C.c_int64), ("mlp_fallbacks", C.c_int64),
+                ("mlp_evals", C.c_int64)]
+
+
+CR_MLP_INPUTS = 17
+CR_MLP_OUTPUTS = 5
+CR_MLP_PARAMS = 13957
 
 
 # name -> (restype, argtypes); every symbol declared in include/cloudvox.h
@@ -86,6 +92,20 @@ SIGNATURES = {
     "cr_volume_read_occupancy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "cr_volume_read_position_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "cr_volume_read_draw_order": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "cr_mlp_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "cr_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "cr_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "cr_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "cr_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "cr_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float), C.c_void_p]),
+    "cr_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_void_p]),
+    "cr_mlp_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
+    "cr_volume_segments": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_cone_params), C.c_int,
+                                     C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cr_cone_trace_learned": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_camera),
+                                        C.POINTER(cr_cone_params), C.c_void_p, C.c_int, C.c_float, C.c_void_p,
+                                        C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_void_p]),
 }
 
 _lib = None

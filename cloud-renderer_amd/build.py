@@ -21,7 +21,7 @@ LIB_PATH = PKG_DIR / "libcloudvox.so"
 ORACLE_DIR = REPO_DIR / "oracle"
 ORACLE_LIB = ORACLE_DIR / "libcv_oracle.so"
 
-HIP_SOURCES = ["cv_kernels.hip", "cv_runtime.hip"]
+HIP_SOURCES = ["cv_kernels.hip", "cv_mlp.hip", "cv_runtime.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

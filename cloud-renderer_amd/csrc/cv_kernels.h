@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cv_mlp.h"
+
 namespace cv {
 
 constexpr int kMaxLevels = 16;
@@ -87,6 +89,12 @@ struct ConeDesc {
     int directional;
     int pixfmt;
     uint64_t pitch;
+    // learned far field (LEARNED kernels and segment generation)
+    int learned_k;         // samples marched before the far field is predicted
+    float err_bound;       // fall back when (1 - a_k) * predicted error > bound; < 0: always
+    float feat_h[4];       // feature sampling heights (voxels) for LODs 3..6
+    int feat_lvl[4];       // their levels, clamped to L - 1
+    float sun[3];          // sun direction feature
 };
 
 // pass-1 work list: per-billboard pixel box and tile count, exclusive scan
@@ -123,10 +131,27 @@ struct TileQueue {
 size_t tile_scratch_ints(int ntiles);
 TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
                      unsigned long long* masks, int* scratch, hipStream_t s);
-void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
-                 const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
-                 const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
-                 hipStream_t s);
+// arguments of one cone-trace launch
+struct ConeLaunch {
+    GridDesc g;
+    VolumeDesc vd;
+    CameraView cv;
+    ConeDesc cd;
+    const void* levels;
+    const float4* sph_world;
+    const float4* sph_view;
+    const int32_t* order;
+    const unsigned long long* masks;
+    int words;
+    void* image;
+    unsigned long long* counters;  // 9 counters or null
+    const int* tile_order;
+    MlpDev mlp;
+    int learned;
+};
+void launch_cone(const ConeLaunch& a, hipStream_t s);
+void launch_segments(const GridDesc& g, const VolumeDesc& vd, const ConeDesc& cd, const void* levels,
+                     const float* p0, int n, float* feat, float* tgt, int* valid, hipStream_t s);
 void launch_occupancy(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,
                       unsigned long long* out2, hipStream_t s);
 void launch_read_state(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,

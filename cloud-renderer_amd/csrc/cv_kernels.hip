@@ -19,6 +19,7 @@
 //     fragments are visited in draw order without a per-tile sort.
 
 #include "cv_kernels.h"
+#include "cv_mlp.h"
 
 #include <hip/hip_runtime.h>
 
@@ -1131,20 +1132,105 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 __device__ __forceinline__ uint32_t q8(float v) { return (uint32_t)(int)floorf(clamp01(v) * 255.0f + 0.5f); }
 
+// Front-to-back cone state (modes 1/2) and march over steps [i0, i1):
+//   stop before a sample outside [0,1]^3 (returns 1), accumulate
+//   w = (1 - a) * alpha_i;  c += w * s.rgb;  a += w  (Kahan-compensated when
+//   requested), stop once a >= alpha_stop (returns 2); returns 0 after i1.
+struct FtbState {
+    float c0, c1, c2, a;
+    float k0, k1, k2, ka;
+};
+
+template <int FMT>
+__device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
+                                         float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
+                                         int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
+    for (int i = i0; i < i1; ++i) {
+        const float h = cd.h[i];
+        float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dz, p0z);
+        if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) return 1;
+        bool two;
+        float4 sm = sample_lod<FMT, 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
+        ntwo += two ? 1u : 0u;
+        float al;
+        if (cd.cone_mode == 1) {
+            al = sm.w;
+        } else {
+            float r = h * cd.tanHalf;
+            float kk = r * cd.kscale;
+            al = 1.0f - cv_exp2_neg(-(kk * sm.w));
+        }
+        float wgt = (1.0f - st.a) * al;
+        if (cd.compensated) {
+            float y, t;
+            y = fmaf(wgt, sm.x, -st.k0); t = st.c0 + y; st.k0 = (t - st.c0) - y; st.c0 = t;
+            y = fmaf(wgt, sm.y, -st.k1); t = st.c1 + y; st.k1 = (t - st.c1) - y; st.c1 = t;
+            y = fmaf(wgt, sm.z, -st.k2); t = st.c2 + y; st.k2 = (t - st.c2) - y; st.c2 = t;
+            y = wgt - st.ka; t = st.a + y; st.ka = (t - st.a) - y; st.a = t;
+        } else {
+            st.c0 = fmaf(wgt, sm.x, st.c0);
+            st.c1 = fmaf(wgt, sm.y, st.c1);
+            st.c2 = fmaf(wgt, sm.z, st.c2);
+            st.a = st.a + wgt;
+        }
+        ++nsamp;
+        if (st.a >= cd.alpha_stop) return 2;
+    }
+    return 0;
+}
+
+// Far-field features after k samples: last sample position (2x - 1), cone
+// direction (d * N), sun direction, and (density, light) at LODs 3..6 at
+// distance max(h_k, feat_h[j]) along the cone, (0, 0) outside [0,1]^3.
+template <int FMT>
+__device__ __forceinline__ void farfield_features(const VolumeDesc& vd, const void* __restrict__ vol,
+                                                  const ConeDesc& cd, float px, float py, float pz, float dx,
+                                                  float dy, float dz, float fN, float f[kMlpIn]) {
+    const float hk = cd.h[cd.learned_k - 1];
+    const float xx = fmaf(hk, dx, px), xy = fmaf(hk, dy, py), xz = fmaf(hk, dz, pz);
+    f[0] = 2.0f * xx - 1.0f;
+    f[1] = 2.0f * xy - 1.0f;
+    f[2] = 2.0f * xz - 1.0f;
+    f[3] = dx * fN;
+    f[4] = dy * fN;
+    f[5] = dz * fN;
+    f[6] = cd.sun[0];
+    f[7] = cd.sun[1];
+    f[8] = cd.sun[2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float hj = fmaxf(hk, cd.feat_h[j]);
+        const float sx = fmaf(hj, dx, px), sy = fmaf(hj, dy, py), sz = fmaf(hj, dz, pz);
+        float dens = 0.0f, light = 0.0f;
+        if (sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f) {
+            float4 t = trilinear<FMT, 4>(vol, vd.lv[cd.feat_lvl[j]], sx, sy, sz);
+            dens = t.w;
+            light = t.x;
+        }
+        f[9 + 2 * j] = dens;
+        f[10 + 2 * j] = light;
+        __builtin_amdgcn_sched_barrier(0);  // one footprint in flight at a time (bounds registers)
+    }
+}
+
 // CONE: 0 weighted R-sum, 1 front-to-back.  COMP: 0 GL blend, 1 front-to-back.
-template <int FMT, int CONE, int COMP>
+// LEARNED (front-to-back only): after cd.learned_k samples the far field of
+// every still-running cone is predicted by the MLP (one 64-row batch per
+// wave on the matrix cores); a cone whose predicted error, scaled by its
+// remaining transmittance, exceeds cd.err_bound (or every cone when
+// err_bound < 0) falls back to the plain march from step k.
+template <int FMT, int CONE, int COMP, int LEARNED>
 __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
                                              const void* __restrict__ vol, const float4* __restrict__ sph_world,
                                              const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
                                              const unsigned long long* __restrict__ masks, int words,
                                              void* __restrict__ image, unsigned long long* __restrict__ counters,
-                                             unsigned int* __restrict__ tile_counter,
-                                             const int* __restrict__ tile_order) {
+                                             const int* __restrict__ tile_order, MlpDev mlp) {
     // One wavefront per 8x8 tile; tiles are dispatched heaviest first
     // (tile_order), so the costly clump-centre tiles do not form the tail.
-    (void)tile_counter;
+    __shared__ __align__(16) uint16_t xs[LEARNED ? 64 * kMlpInPad : 8];
     const int lane = threadIdx.x;
-    uint32_t nfrag = 0, nsamp = 0, ntwo = 0;
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
     {
     const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
     const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
@@ -1153,6 +1239,7 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
     const float rx = pixel_ndc(px, cv.W) * cv.kx;
     const float ry = pixel_ndc(py, cv.H) * cv.ky;
     const float fN = (float)g.N;
+    const int kk_steps = LEARNED ? min(cd.learned_k, cd.steps) : cd.steps;
 
     float C0, C1, C2, C3;
     if (COMP == 0) {
@@ -1179,6 +1266,10 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
             const int j = order[r];
             const float4 cw = sph_world[j];
             const float4 vv = sph_view[j];
+            bool frag = false, alive = false;
+            float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, dx = 0.0f, dy = 0.0f, dzz = 0.0f;
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+            FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             if (!done && valid) {
                 const float s = cw.w;
                 const float dz = -vv.z;
@@ -1186,6 +1277,7 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                 const float dv = ry * dz - vv.y;
                 const float sc = sphere_contrib(du, dv, s);
                 if (!(sc < 0.01f)) {
+                    frag = true;
                     ++nfrag;
                     // fragPos, then pos = fragPos + normalize(fragNor) * radius * sc
                     float fx = (cw.x + du * cv.S[0]) + dv * cv.U[0];
@@ -1195,10 +1287,9 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                     float qy = fy + (cv.B[1] * s) * sc;
                     float qz = fz + (cv.B[2] * s) * sc;
                     // calculateVoxelLerp(pos) / voxelDim
-                    float p0x = div_N(g, fN * div_range(g, qx - g.lo[0], 0));
-                    float p0y = div_N(g, fN * div_range(g, qy - g.lo[1], 1));
-                    float p0z = div_N(g, fN * div_range(g, qz - g.lo[2], 2));
-                    float dx, dy, dzz;
+                    p0x = div_N(g, fN * div_range(g, qx - g.lo[0], 0));
+                    p0y = div_N(g, fN * div_range(g, qy - g.lo[1], 1));
+                    p0z = div_N(g, fN * div_range(g, qz - g.lo[2], 2));
                     if (cd.directional) {
                         dx = cd.ldir[0]; dy = cd.ldir[1]; dzz = cd.ldir[2];
                     } else {
@@ -1206,7 +1297,6 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                         float inv = 1.0f / sqrtf((lx * lx + ly * ly) + lz * lz);
                         dx = div_N(g, lx * inv); dy = div_N(g, ly * inv); dzz = div_N(g, lz * inv);
                     }
-                    float s0, s1, s2, s3;
                     if (CONE == 0) {
                         float color = 0.0f;
                         for (int i = 0; i < cd.steps; ++i) {
@@ -1219,60 +1309,69 @@ __global__ void __launch_bounds__(64) k_cone(GridDesc g, VolumeDesc vd, CameraVi
                         nsamp += (uint32_t)cd.steps;
                         s0 = s1 = s2 = s3 = clamp01(color);
                     } else {
-                        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, a = 0.0f;
-                        float k0 = 0.0f, k1 = 0.0f, k2 = 0.0f, ka = 0.0f;
-                        for (int i = 0; i < cd.steps; ++i) {
-                            const float h = cd.h[i];
-                            float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dzz, p0z);
-                            if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) break;
-                            bool two;
-                            float4 sm = sample_lod<FMT, 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
-                            ntwo += two ? 1u : 0u;
-                            float al;
-                            if (cd.cone_mode == 1) {
-                                al = sm.w;
-                            } else {
-                                float r = h * cd.tanHalf;
-                                float kk = r * cd.kscale;
-                                al = 1.0f - cv_exp2_neg(-(kk * sm.w));
-                            }
-                            float wgt = (1.0f - a) * al;
-                            if (cd.compensated) {
-                                float y, t;
-                                y = fmaf(wgt, sm.x, -k0); t = c0 + y; k0 = (t - c0) - y; c0 = t;
-                                y = fmaf(wgt, sm.y, -k1); t = c1 + y; k1 = (t - c1) - y; c1 = t;
-                                y = fmaf(wgt, sm.z, -k2); t = c2 + y; k2 = (t - c2) - y; c2 = t;
-                                y = wgt - ka; t = a + y; ka = (t - a) - y; a = t;
-                            } else {
-                                c0 = fmaf(wgt, sm.x, c0);
-                                c1 = fmaf(wgt, sm.y, c1);
-                                c2 = fmaf(wgt, sm.z, c2);
-                                a = a + wgt;
-                            }
-                            ++nsamp;
-                            if (a >= cd.alpha_stop) break;
-                        }
-                        s0 = clamp01(c0); s1 = clamp01(c1); s2 = clamp01(c2); s3 = clamp01(a);
+                        int rs = march_ftb<FMT>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, 0, kk_steps, st, nsamp, ntwo);
+                        alive = LEARNED && rs == 0 && kk_steps < cd.steps;
                     }
-                    if (COMP == 0) {
-                        // dst = src * src.a + dst * (1 - src.a)  (src/main.cpp:94-95)
-                        const float al = s3, om = 1.0f - al;
-                        C0 = s0 * al + C0 * om;
-                        C1 = s1 * al + C1 * om;
-                        C2 = s2 * al + C2 * om;
-                        C3 = s3 * al + C3 * om;
-                        if (cd.quantize) {
-                            C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
-                            C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
-                        }
+                }
+            }
+            if (LEARNED && CONE != 0) {
+                // wave-uniform: all lanes take part in the batched MLP
+                if (__any(alive)) {
+                    float f[kMlpIn];
+                    if (alive) {
+                        farfield_features<FMT>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, fN, f);
+                        nsamp += 4;
                     } else {
-                        const float wgt = (1.0f - C3) * s3;
-                        C0 = fmaf(wgt, s0, C0);
-                        C1 = fmaf(wgt, s1, C1);
-                        C2 = fmaf(wgt, s2, C2);
-                        C3 = C3 + wgt;
-                        if (C3 >= cd.alpha_stop) done = true;
+#pragma unroll
+                        for (int q = 0; q < kMlpIn; ++q) f[q] = 0.0f;
                     }
+                    uint32_t* xrow = (uint32_t*)(xs + lane * kMlpInPad);
+#pragma unroll
+                    for (int q = 0; q < kMlpInPad / 2; ++q) {
+                        float lo = 2 * q < kMlpIn ? f[2 * q] : 0.0f;
+                        float hi = 2 * q + 1 < kMlpIn ? f[2 * q + 1] : 0.0f;
+                        xrow[q] = pack_bf2(lo, hi);
+                    }
+                    __syncthreads();
+                    float y[kMlpOut];
+                    mlp_forward_wave(mlp, xs, lane, y);
+                    __syncthreads();
+                    ++nevals;
+                    if (alive) {
+                        ++nrows;
+                        const float rem = 1.0f - st.a;
+                        if (cd.err_bound < 0.0f || !(rem * y[4] <= cd.err_bound)) {
+                            ++nfall;
+                            march_ftb<FMT>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, kk_steps, cd.steps, st, nsamp, ntwo);
+                        } else {
+                            st.c0 = fmaf(rem, clamp01(y[0]), st.c0);
+                            st.c1 = fmaf(rem, clamp01(y[1]), st.c1);
+                            st.c2 = fmaf(rem, clamp01(y[2]), st.c2);
+                            st.a = st.a + rem * clamp01(y[3]);
+                        }
+                    }
+                }
+            }
+            if (frag) {
+                if (CONE != 0) { s0 = clamp01(st.c0); s1 = clamp01(st.c1); s2 = clamp01(st.c2); s3 = clamp01(st.a); }
+                if (COMP == 0) {
+                    // dst = src * src.a + dst * (1 - src.a)  (src/main.cpp:94-95)
+                    const float al = s3, om = 1.0f - al;
+                    C0 = s0 * al + C0 * om;
+                    C1 = s1 * al + C1 * om;
+                    C2 = s2 * al + C2 * om;
+                    C3 = s3 * al + C3 * om;
+                    if (cd.quantize) {
+                        C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
+                        C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
+                    }
+                } else {
+                    const float wgt = (1.0f - C3) * s3;
+                    C0 = fmaf(wgt, s0, C0);
+                    C1 = fmaf(wgt, s1, C1);
+                    C2 = fmaf(wgt, s2, C2);
+                    C3 = C3 + wgt;
+                    if (C3 >= cd.alpha_stop) done = true;
                 }
             }
             if (COMP == 1 && __all(done)) goto finished;
@@ -1291,11 +1390,59 @@ finished:
     }  // tiles
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
+        unsigned long long e0 = 0, e1 = 0;
+        if (LEARNED) { e0 = wave_sum_u64(nrows); e1 = wave_sum_u64(nfall); }
         if (lane == 0) {
             atomicAdd(&counters[3], a);
             atomicAdd(&counters[4], bsum);
             atomicAdd(&counters[5], c);
+            if (LEARNED) {
+                atomicAdd(&counters[6], e0);
+                atomicAdd(&counters[7], e1);
+                atomicAdd(&counters[8], (unsigned long long)nevals);
+            }
         }
+    }
+}
+
+// Segment samples for training the far-field MLP: one thread per start point
+// p0 (texture space), directional light.  Rows that are still running after
+// k samples get the features, and as targets the rest of the plain march in
+// units of the remaining transmittance: ((c - c_k) / (1 - a_k),
+// (a - a_k) / (1 - a_k), 1 - a_k).
+template <int FMT>
+__global__ void __launch_bounds__(256) k_segments(GridDesc g, VolumeDesc vd, ConeDesc cd, const void* __restrict__ vol,
+                                                  const float* __restrict__ p0, int n, float* __restrict__ feat,
+                                                  float* __restrict__ tgt, int* __restrict__ valid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float px = p0[3 * i], py = p0[3 * i + 1], pz = p0[3 * i + 2];
+    const float dx = cd.ldir[0], dy = cd.ldir[1], dz = cd.ldir[2];
+    const int kk_steps = cd.learned_k;
+    FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t ns = 0, nt = 0;
+    int rs = march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, 0, min(kk_steps, cd.steps), st, ns, nt);
+    const bool rec = rs == 0 && kk_steps < cd.steps;
+    float f[kMlpIn];
+    float ck0 = 0.0f, ck1 = 0.0f, ck2 = 0.0f, ak = 0.0f;
+    if (rec) {
+        farfield_features<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, (float)g.N, f);
+        ck0 = st.c0; ck1 = st.c1; ck2 = st.c2; ak = st.a;
+        march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, kk_steps, cd.steps, st, ns, nt);
+    }
+    valid[i] = rec ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < kMlpIn; ++q) feat[(size_t)i * kMlpIn + q] = rec ? f[q] : 0.0f;
+    float* t = tgt + (size_t)i * kMlpOut;
+    if (rec) {
+        const float om = 1.0f - ak;
+        t[0] = (st.c0 - ck0) / om;
+        t[1] = (st.c1 - ck1) / om;
+        t[2] = (st.c2 - ck2) / om;
+        t[3] = (st.a - ak) / om;
+        t[4] = om;
+    } else {
+        t[0] = t[1] = t[2] = t[3] = t[4] = 0.0f;
     }
 }
 
@@ -1513,42 +1660,49 @@ TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int til
     return q;
 }
 
-template <int FMT, int CONE, int COMP>
-static void launch_cone_t(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
-                          const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
-                          const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
-                          hipStream_t s) {
-    unsigned ntiles = (unsigned)(cv.tiles_x * cv.tiles_y);
-    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP>), dim3(ntiles), dim3(64), 0, s, g, vd, cv, cd, levels, sph_world,
-                       sph_view, order, masks, words, image, counters, tile_counter, tile_order);
+template <int FMT, int CONE, int COMP, int LEARNED>
+static void launch_cone_t(const ConeLaunch& a, hipStream_t s) {
+    unsigned ntiles = (unsigned)(a.cv.tiles_x * a.cv.tiles_y);
+    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
+                       a.levels, a.sph_world, a.sph_view, a.order, a.masks, a.words, a.image, a.counters,
+                       a.tile_order, a.mlp);
 }
 
+// R8 volumes hold one channel: only the weighted R-sum cone reads them (the
+// runtime rejects the front-to-back modes for R8 before launching).
 template <int FMT>
-static void launch_cone_f(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd,
-                          const void* levels, const float4* sph_world, const float4* sph_view, const int32_t* order,
-                          const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
-                          hipStream_t s) {
-    const bool weighted = cd.cone_mode == 0;
-    if (weighted && cd.composite == 0)
-        launch_cone_t<FMT, 0, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
-    else if (weighted)
-        launch_cone_t<FMT, 0, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
-    else if (cd.composite == 0)
-        launch_cone_t<FMT, 1, 0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
-    else
-        launch_cone_t<FMT, 1, 1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
+static void launch_cone_f(const ConeLaunch& a, hipStream_t s) {
+    const bool weighted = a.cd.cone_mode == 0;
+    if constexpr (FMT == 0) {
+        if (a.cd.composite == 0) launch_cone_t<0, 0, 0, 0>(a, s);
+        else launch_cone_t<0, 0, 1, 0>(a, s);
+    } else {
+        if (a.learned) {
+            launch_cone_t<FMT, 1, 1, 1>(a, s);
+            return;
+        }
+        if (weighted && a.cd.composite == 0) launch_cone_t<FMT, 0, 0, 0>(a, s);
+        else if (weighted) launch_cone_t<FMT, 0, 1, 0>(a, s);
+        else if (a.cd.composite == 0) launch_cone_t<FMT, 1, 0, 0>(a, s);
+        else launch_cone_t<FMT, 1, 1, 0>(a, s);
+    }
 }
 
-void launch_cone(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv, const ConeDesc& cd, const void* levels,
-                 const float4* sph_world, const float4* sph_view, const int32_t* order,
-                 const unsigned long long* masks, int words, void* image, unsigned long long* counters, unsigned int* tile_counter, const int* tile_order,
-                 hipStream_t s) {
-    if (vd.format == 0)
-        launch_cone_f<0>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
-    else if (vd.format == 1)
-        launch_cone_f<1>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
+void launch_cone(const ConeLaunch& a, hipStream_t s) {
+    if (a.vd.format == 0) launch_cone_f<0>(a, s);
+    else if (a.vd.format == 1) launch_cone_f<1>(a, s);
+    else launch_cone_f<2>(a, s);
+}
+
+void launch_segments(const GridDesc& g, const VolumeDesc& vd, const ConeDesc& cd, const void* levels,
+                     const float* p0, int n, float* feat, float* tgt, int* valid, hipStream_t s) {
+    if (n <= 0) return;
+    if (vd.format == 1)
+        hipLaunchKernelGGL(k_segments<1>, dim3(blocks_for((uint64_t)n, 256)), dim3(256), 0, s, g, vd, cd, levels, p0,
+                           n, feat, tgt, valid);
     else
-        launch_cone_f<2>(g, vd, cv, cd, levels, sph_world, sph_view, order, masks, words, image, counters, tile_counter, tile_order, s);
+        hipLaunchKernelGGL(k_segments<2>, dim3(blocks_for((uint64_t)n, 256)), dim3(256), 0, s, g, vd, cd, levels, p0,
+                           n, feat, tgt, valid);
 }
 
 void launch_occupancy(const GridDesc& g, const unsigned long long* shellbits, const unsigned long long* fillbits,

@@ -21,6 +21,7 @@
 
 #include "../../include/cloudvox.h"
 #include "cv_kernels.h"
+#include "cv_mlp_launch.h"
 
 using namespace cv;
 
@@ -79,6 +80,33 @@ struct cr_volume_s {
     bool voxelized = false;
     int sorted_count = -1;
 };
+
+// far-field MLP: FP32 master parameters + Adam moments, bf16 packed copies for
+// the matrix cores, per-workgroup gradient partials of the last training step
+struct cr_mlp_s {
+    cr_context ctx = nullptr;
+    float* d_master = nullptr;
+    float* d_m1 = nullptr;
+    float* d_m2 = nullptr;
+    uint16_t* d_fwd = nullptr;
+    uint16_t* d_bwd = nullptr;
+    float* d_bias = nullptr;
+    float* d_grad = nullptr;
+    float* d_loss = nullptr;
+    float* d_partial = nullptr;
+    float* d_loss_part = nullptr;
+    size_t partial_cap = 0;  // workgroups
+    size_t loss_cap = 0;
+    int64_t step = 0;
+};
+
+static inline MlpDev mlp_dev(cr_mlp m) {
+    MlpDev d;
+    d.fwd = m->d_fwd;
+    d.bwd = m->d_bwd;
+    d.bias = m->d_bias;
+    return d;
+}
 
 static thread_local std::string g_last_error;
 
@@ -397,7 +425,7 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     hipError_t e = hipMalloc(&v->d_levels, v->levels_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_shell, (size_t)v->nbricks0 * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_fill, (size_t)v->nbricks0 * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&v->d_counters, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_counters, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_tie, 64);
     if (e != hipSuccess) {
         cr_volume_destroy(v);
@@ -522,7 +550,7 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     StageEvents ev;
     if ((rc = ev.begin(ctx, s, 0))) return rc;
     ev.mark(0);
-    if (counters) launch_fill_u64((uint64_t*)counters, 8, 0, s);
+    if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187)
     launch_fill_u64((uint64_t*)v->d_keys, npx, ~0ull, s);
     launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, s);
@@ -553,8 +581,76 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     return CR_OK;
 }
 
-int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p, void* d_image,
-                  int W, int H, size_t pitch, int pixfmt, void* stream) {
+// far-field options of cr_cone_trace_learned (null for the plain trace)
+struct LearnedOpts {
+    cr_mlp mlp;
+    int k;
+    float err_bound;
+};
+
+static void fill_cone_desc(cr_volume v, const cr_light* light, const cr_cone_params* p, ConeDesc& cd) {
+    cd.steps = p->steps;
+    float tanHalf = std::tan(p->cone_angle / 2.0f);
+    cd.tanHalf = tanHalf;
+    float h = p->initial_height;
+    for (int i = 0; i < p->steps; ++i) {
+        float r = h * tanHalf;
+        float lod = std::log2(std::max(1.0f, 2.0f * r));
+        cd.h[i] = h;
+        cd.lam[i] = lod + p->lod_offset;
+        h = h + r;
+    }
+    cd.den = (float)p->steps * p->down_scaling;
+    cd.alpha_stop = p->alpha_stop;
+    cd.kscale = (p->sigma * (float)1.4426950408889634) / (float)v->N;
+    cd.cone_mode = p->cone_mode;
+    cd.composite = p->composite;
+    cd.quantize = (p->composite == CR_COMPOSITE_GL_BLEND && p->quantize_rgba8) ? 1 : 0;
+    cd.compensated = p->compensated ? 1 : 0;
+    for (int q = 0; q < 4; ++q) cd.clear[q] = p->clear_color[q];
+    for (int a = 0; a < 3; ++a) cd.lpos[a] = light->position[a];
+    cd.directional = light->directional ? 1 : 0;
+    F3 sun;
+    if (cd.directional) {
+        F3 ld = norm3(f3(light->direction[0], light->direction[1], light->direction[2]));
+        float fN = (float)v->N;
+        cd.ldir[0] = ld.x / fN;
+        cd.ldir[1] = ld.y / fN;
+        cd.ldir[2] = ld.z / fN;
+        sun = ld;
+    } else {
+        F3 centre = f3(v->g.lo[0] + v->g.range[0] * 0.5f, v->g.lo[1] + v->g.range[1] * 0.5f,
+                       v->g.lo[2] + v->g.range[2] * 0.5f);
+        sun = norm3(f3(light->position[0], light->position[1], light->position[2]) - centre);
+    }
+    cd.sun[0] = sun.x; cd.sun[1] = sun.y; cd.sun[2] = sun.z;
+    // far-field feature heights: the cone radius is 2^(lod-1) voxels (lambda =
+    // lod) at height 2^(lod-1) / tan(angle / 2); LODs 3..6
+    for (int j = 0; j < 4; ++j) {
+        int lod = 3 + j;
+        cd.feat_h[j] = (float)(1 << (lod - 1)) / tanHalf;
+        cd.feat_lvl[j] = std::min(lod, v->L - 1);
+    }
+    cd.learned_k = 0;
+    cd.err_bound = 0.0f;
+}
+
+static int cone_params_check(cr_context ctx, cr_volume v, const cr_cone_params* p, const char* who) {
+    if (p->steps < 0 || p->steps > kMaxSteps) return fail(ctx, CR_EUNSUPPORTED, "%s: steps must be in [0, 256]", who);
+    if (p->cone_mode < CR_CONE_WEIGHTED || p->cone_mode > CR_CONE_FTB_DENSITY) return fail(ctx, CR_EINVAL, "%s: bad cone mode", who);
+    if (p->composite != CR_COMPOSITE_GL_BLEND && p->composite != CR_COMPOSITE_FTB) return fail(ctx, CR_EINVAL, "%s: bad composite", who);
+    if (!(p->cone_angle > 0.0f && p->cone_angle < 3.14159265f)) return fail(ctx, CR_EINVAL, "%s: cone angle must be in (0, pi)", who);
+    if (!(p->initial_height > 0.0f) || !std::isfinite(p->initial_height)) return fail(ctx, CR_EINVAL, "%s: initial height must be > 0", who);
+    if (!std::isfinite(p->lod_offset) || !(p->down_scaling != 0.0f) || !std::isfinite(p->down_scaling))
+        return fail(ctx, CR_EINVAL, "%s: bad lod offset / down scaling", who);
+    if (!std::isfinite(p->sigma) || p->sigma < 0.0f) return fail(ctx, CR_EINVAL, "%s: sigma must be finite and >= 0", who);
+    if (v->format == CR_FORMAT_R8 && p->cone_mode != CR_CONE_WEIGHTED)
+        return fail(ctx, CR_EUNSUPPORTED, "%s: R8 volumes hold one channel; use the weighted cone", who);
+    return CR_OK;
+}
+
+static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
+                           const LearnedOpts* lo, void* d_image, int W, int H, size_t pitch, int pixfmt, void* stream) {
     if (!v) return fail(nullptr, CR_EINVAL, "cr_cone_trace: vol is NULL");
     cr_context ctx = v->ctx;
     if (!light || !cam || !p || !d_image) return fail(ctx, CR_EINVAL, "cr_cone_trace: NULL argument");
@@ -564,21 +660,22 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
     if (pitch < (size_t)W * px_bytes) return fail(ctx, CR_EINVAL, "cr_cone_trace: pitch smaller than a row");
     if (pitch % px_bytes != 0 || ((uintptr_t)d_image % px_bytes) != 0)
         return fail(ctx, CR_EINVAL, "cr_cone_trace: image pointer or pitch misaligned");
-    if (p->steps < 0 || p->steps > kMaxSteps) return fail(ctx, CR_EUNSUPPORTED, "cr_cone_trace: steps must be in [0, 256]");
-    if (p->cone_mode < CR_CONE_WEIGHTED || p->cone_mode > CR_CONE_FTB_DENSITY) return fail(ctx, CR_EINVAL, "cr_cone_trace: bad cone mode");
-    if (p->composite != CR_COMPOSITE_GL_BLEND && p->composite != CR_COMPOSITE_FTB) return fail(ctx, CR_EINVAL, "cr_cone_trace: bad composite");
-    if (!(p->cone_angle > 0.0f && p->cone_angle < 3.14159265f)) return fail(ctx, CR_EINVAL, "cr_cone_trace: cone angle must be in (0, pi)");
-    if (!(p->initial_height > 0.0f) || !std::isfinite(p->initial_height)) return fail(ctx, CR_EINVAL, "cr_cone_trace: initial height must be > 0");
-    if (!std::isfinite(p->lod_offset) || !(p->down_scaling != 0.0f) || !std::isfinite(p->down_scaling))
-        return fail(ctx, CR_EINVAL, "cr_cone_trace: bad lod offset / down scaling");
-    if (!std::isfinite(p->sigma) || p->sigma < 0.0f) return fail(ctx, CR_EINVAL, "cr_cone_trace: sigma must be finite and >= 0");
+    int rc = cone_params_check(ctx, v, p, "cr_cone_trace");
+    if (rc) return rc;
+    if (lo) {
+        if (!lo->mlp) return fail(ctx, CR_EINVAL, "cr_cone_trace_learned: mlp is NULL");
+        if (p->cone_mode == CR_CONE_WEIGHTED || p->composite != CR_COMPOSITE_FTB)
+            return fail(ctx, CR_EUNSUPPORTED, "cr_cone_trace_learned: needs a front-to-back cone and composite");
+        if (lo->k < 1 || lo->k > kMaxSteps) return fail(ctx, CR_EINVAL, "cr_cone_trace_learned: k must be in [1, 256]");
+        if (std::isnan(lo->err_bound)) return fail(ctx, CR_EINVAL, "cr_cone_trace_learned: error bound is NaN");
+    }
     if (!finite3(cam->eye) || !finite3(cam->target) || !finite3(cam->up)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite camera");
     if (!(cam->fovy > 0.0f && cam->fovy < 3.14159265f)) return fail(ctx, CR_EINVAL, "cr_cone_trace: fovy must be in (0, pi)");
     if (!(cam->znear > 0.0f) || !(cam->zfar > cam->znear)) return fail(ctx, CR_EINVAL, "cr_cone_trace: need 0 < znear < zfar");
     if (light->directional && !finite3(light->direction)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light direction");
     if (!light->directional && !finite3(light->position)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light position");
     if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_cone_trace: volume not voxelized");
-    int rc = set_device(ctx);
+    rc = set_device(ctx);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
 
@@ -601,36 +698,13 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
 
     // cone schedule (res/conetrace_frag.glsl:70-76), identical for every ray
     ConeDesc cd{};
-    cd.steps = p->steps;
-    float tanHalf = std::tan(p->cone_angle / 2.0f);
-    cd.tanHalf = tanHalf;
-    float h = p->initial_height;
-    for (int i = 0; i < p->steps; ++i) {
-        float r = h * tanHalf;
-        float lod = std::log2(std::max(1.0f, 2.0f * r));
-        cd.h[i] = h;
-        cd.lam[i] = lod + p->lod_offset;
-        h = h + r;
-    }
-    cd.den = (float)p->steps * p->down_scaling;
-    cd.alpha_stop = p->alpha_stop;
-    cd.kscale = (p->sigma * (float)1.4426950408889634) / (float)v->N;
-    cd.cone_mode = p->cone_mode;
-    cd.composite = p->composite;
-    cd.quantize = (p->composite == CR_COMPOSITE_GL_BLEND && p->quantize_rgba8) ? 1 : 0;
-    cd.compensated = p->compensated ? 1 : 0;
-    for (int q = 0; q < 4; ++q) cd.clear[q] = p->clear_color[q];
-    for (int a = 0; a < 3; ++a) cd.lpos[a] = light->position[a];
-    cd.directional = light->directional ? 1 : 0;
-    if (cd.directional) {
-        F3 ld = norm3(f3(light->direction[0], light->direction[1], light->direction[2]));
-        float fN = (float)v->N;
-        cd.ldir[0] = ld.x / fN;
-        cd.ldir[1] = ld.y / fN;
-        cd.ldir[2] = ld.z / fN;
-    }
+    fill_cone_desc(v, light, p, cd);
     cd.pixfmt = pixfmt;
     cd.pitch = pitch;
+    if (lo) {
+        cd.learned_k = lo->k;
+        cd.err_bound = lo->err_bound;
+    }
 
     // buffers
     int S = v->S;
@@ -654,7 +728,7 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
     StageEvents ev;
     if ((rc = ev.begin(ctx, s, 1))) return rc;
     ev.mark(0);
-    if (counters) launch_fill_u64((uint64_t*)counters, 8, 0, s);
+    if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
     launch_cam_setup(v->g, cv, v->d_boards, S, v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort, s);
     ev.mark(1);
     launch_sort(v->d_sort, S, v->d_order, v->d_tie, s);
@@ -662,16 +736,34 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
     launch_fill_u64((uint64_t*)v->d_masks, nmask, 0, s);
     TileQueue tq = launch_bin(v->d_sph_tiles, v->d_order, S, cv.tiles_x, cv.tiles_y, words, v->d_masks, v->d_tiles, s);
     ev.mark(3);
-    launch_cone(v->g, v->vd, cv, cd, v->d_levels, v->d_sph_world, v->d_sph_view, v->d_order, v->d_masks, words,
-                d_image, counters, tq.counter, tq.order, s);
+    ConeLaunch cl{};
+    cl.g = v->g;
+    cl.vd = v->vd;
+    cl.cv = cv;
+    cl.cd = cd;
+    cl.levels = v->d_levels;
+    cl.sph_world = v->d_sph_world;
+    cl.sph_view = v->d_sph_view;
+    cl.order = v->d_order;
+    cl.masks = v->d_masks;
+    cl.words = words;
+    cl.image = d_image;
+    cl.counters = counters;
+    cl.tile_order = tq.order;
+    cl.learned = lo ? 1 : 0;
+    if (lo) cl.mlp = mlp_dev(lo->mlp);
+    launch_cone(cl, s);
     ev.mark(4);
     ev.end();
     CR_HIP(ctx, hipGetLastError());
     v->sorted_count = S;
     if (ctx->profiling == 1) {
         CR_HIP(ctx, hipStreamSynchronize(s));
-        unsigned long long c[8];
+        unsigned long long c[9];
         CR_HIP(ctx, hipMemcpy(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost));
+        ctx->stats.mlp_rows = lo ? (int64_t)c[6] : 0;
+        ctx->stats.mlp_fallbacks = lo ? (int64_t)c[7] : 0;
+        ctx->stats.mlp_evals = lo ? (int64_t)c[8] : 0;
         ctx->stats.fragments = (int64_t)c[3];
         ctx->stats.samples = (int64_t)c[4];
         ctx->stats.two_level = (int64_t)c[5];
@@ -681,6 +773,41 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
         ctx->stats.ms_cone = elapsed(ev.e[3], ev.e[4]);
         ctx->stats.n_cone = 1;
     }
+    return CR_OK;
+}
+
+int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p, void* d_image,
+                  int W, int H, size_t pitch, int pixfmt, void* stream) {
+    return cone_trace_impl(v, light, cam, p, nullptr, d_image, W, H, pitch, pixfmt, stream);
+}
+
+int cr_cone_trace_learned(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
+                          cr_mlp mlp, int k, float err_bound, void* d_image, int W, int H, size_t pitch, int pixfmt,
+                          void* stream) {
+    LearnedOpts lo{mlp, k, err_bound};
+    return cone_trace_impl(v, light, cam, p, &lo, d_image, W, H, pitch, pixfmt, stream);
+}
+
+int cr_volume_segments(cr_volume v, const cr_light* light, const cr_cone_params* p, int k, const float* d_p0, int n,
+                       float* d_features, float* d_targets, int32_t* d_valid, void* stream) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_segments: vol is NULL");
+    cr_context ctx = v->ctx;
+    if (!light || !p || (n > 0 && (!d_p0 || !d_features || !d_targets || !d_valid)))
+        return fail(ctx, CR_EINVAL, "cr_volume_segments: NULL argument");
+    if (n < 0) return fail(ctx, CR_EINVAL, "cr_volume_segments: n must be >= 0");
+    int rc = cone_params_check(ctx, v, p, "cr_volume_segments");
+    if (rc) return rc;
+    if (p->cone_mode == CR_CONE_WEIGHTED) return fail(ctx, CR_EUNSUPPORTED, "cr_volume_segments: needs a front-to-back cone");
+    if (!light->directional || !finite3(light->direction))
+        return fail(ctx, CR_EUNSUPPORTED, "cr_volume_segments: needs a finite directional light");
+    if (k < 1 || k > kMaxSteps) return fail(ctx, CR_EINVAL, "cr_volume_segments: k must be in [1, 256]");
+    if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_volume_segments: volume not voxelized");
+    if ((rc = set_device(ctx))) return rc;
+    ConeDesc cd{};
+    fill_cone_desc(v, light, p, cd);
+    cd.learned_k = k;
+    launch_segments(v->g, v->vd, cd, v->d_levels, d_p0, n, d_features, d_targets, (int*)d_valid, (hipStream_t)stream);
+    CR_HIP(ctx, hipGetLastError());
     return CR_OK;
 }
 
@@ -766,6 +893,159 @@ int cr_volume_read_draw_order(cr_volume v, int32_t* host_dst, int count, void* s
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_order, (size_t)v->sorted_count * 4, hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// far-field MLP
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int cr_mlp_create(cr_context ctx, cr_mlp* out) {
+    if (!ctx || !out) return fail(ctx, CR_EINVAL, "cr_mlp_create: NULL argument");
+    *out = nullptr;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    cr_mlp m = new (std::nothrow) cr_mlp_s();
+    if (!m) return fail(ctx, CR_ENOMEM, "cr_mlp_create: host allocation failed");
+    m->ctx = ctx;
+    hipError_t e = hipMalloc((void**)&m->d_master, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_m1, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_m2, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_grad, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_fwd, kMlpFwdBlocks * 512 * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_bwd, kMlpBwdBlocks * 512 * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_bias, kMlpBiasPad * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->d_loss, sizeof(float));
+    if (e == hipSuccess) e = hipMemset(m->d_master, 0, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(m->d_m1, 0, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(m->d_m2, 0, kMlpParamsPad * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(m->d_loss, 0, sizeof(float));
+    if (e != hipSuccess) {
+        cr_mlp_destroy(m);
+        return fail(ctx, e == hipErrorOutOfMemory ? CR_ENOMEM : CR_EDEVICE, "cr_mlp_create: %s", hipGetErrorString(e));
+    }
+    launch_mlp_pack(m->d_master, m->d_fwd, m->d_bwd, m->d_bias, nullptr);
+    CR_HIP(ctx, hipGetLastError());
+    CR_HIP(ctx, hipDeviceSynchronize());
+    *out = m;
+    return CR_OK;
+}
+
+int cr_mlp_destroy(cr_mlp m) {
+    if (!m) return CR_OK;
+    if (m->ctx) (void)hipSetDevice(m->ctx->device);
+    void* ptrs[] = {m->d_master, m->d_m1, m->d_m2, m->d_grad, m->d_fwd, m->d_bwd,
+                    m->d_bias, m->d_loss, m->d_partial, m->d_loss_part};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    delete m;
+    return CR_OK;
+}
+
+int cr_mlp_set_params(cr_mlp m, const float* host, int count) {
+    if (!m || !host) return fail(m ? m->ctx : nullptr, CR_EINVAL, "cr_mlp_set_params: NULL argument");
+    cr_context ctx = m->ctx;
+    if (count != kMlpParams) return fail(ctx, CR_EINVAL, "cr_mlp_set_params: count must be CR_MLP_PARAMS");
+    for (int i = 0; i < count; ++i)
+        if (!std::isfinite(host[i])) return fail(ctx, CR_EINVAL, "cr_mlp_set_params: non-finite parameter");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    CR_HIP(ctx, hipMemcpy(m->d_master, host, (size_t)count * sizeof(float), hipMemcpyHostToDevice));
+    CR_HIP(ctx, hipMemset(m->d_m1, 0, kMlpParamsPad * sizeof(float)));
+    CR_HIP(ctx, hipMemset(m->d_m2, 0, kMlpParamsPad * sizeof(float)));
+    m->step = 0;
+    launch_mlp_pack(m->d_master, m->d_fwd, m->d_bwd, m->d_bias, nullptr);
+    CR_HIP(ctx, hipGetLastError());
+    CR_HIP(ctx, hipDeviceSynchronize());
+    return CR_OK;
+}
+
+int cr_mlp_get_params(cr_mlp m, float* host, int count) {
+    if (!m || !host) return fail(m ? m->ctx : nullptr, CR_EINVAL, "cr_mlp_get_params: NULL argument");
+    cr_context ctx = m->ctx;
+    if (count != kMlpParams) return fail(ctx, CR_EINVAL, "cr_mlp_get_params: count must be CR_MLP_PARAMS");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    CR_HIP(ctx, hipDeviceSynchronize());
+    CR_HIP(ctx, hipMemcpy(host, m->d_master, (size_t)count * sizeof(float), hipMemcpyDeviceToHost));
+    return CR_OK;
+}
+
+int cr_mlp_forward(cr_mlp m, const float* d_x, int n, float* d_y, void* stream) {
+    if (!m) return fail(nullptr, CR_EINVAL, "cr_mlp_forward: mlp is NULL");
+    cr_context ctx = m->ctx;
+    if (n < 0 || (n > 0 && (!d_x || !d_y))) return fail(ctx, CR_EINVAL, "cr_mlp_forward: bad arguments");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    launch_mlp_forward(mlp_dev(m), d_x, n, d_y, (hipStream_t)stream);
+    CR_HIP(ctx, hipGetLastError());
+    return CR_OK;
+}
+
+// forward + backward over n rows; gradient partials per workgroup, reduced
+// in a fixed order (deterministic), optionally followed by an Adam step
+static int mlp_step(cr_mlp m, const float* d_x, const float* d_t, int n, float lr, int apply, hipStream_t s) {
+    cr_context ctx = m->ctx;
+    const int nwg = mlp_train_groups(n);
+    int rc = ensure(ctx, &m->d_partial, (size_t)nwg * kMlpParamsPad, &m->partial_cap);
+    if (rc) return rc;
+    if ((rc = ensure(ctx, &m->d_loss_part, (size_t)nwg, &m->loss_cap))) return rc;
+    launch_mlp_train(mlp_dev(m), d_x, d_t, n, m->d_partial, m->d_loss_part, s);
+    AdamStep st{};
+    st.apply = apply;
+    st.lr = lr;
+    st.beta1 = 0.9f;
+    st.beta2 = 0.999f;
+    st.eps = 1e-8f;
+    if (apply) {
+        m->step += 1;
+        st.bc1 = (float)(1.0 - std::pow(0.9, (double)m->step));
+        st.bc2 = (float)(1.0 - std::pow(0.999, (double)m->step));
+    } else {
+        st.bc1 = st.bc2 = 1.0f;
+    }
+    launch_mlp_adam(m->d_partial, nwg, m->d_loss_part, m->d_master, m->d_m1, m->d_m2, m->d_grad, m->d_loss, st, s);
+    if (apply) launch_mlp_pack(m->d_master, m->d_fwd, m->d_bwd, m->d_bias, s);
+    CR_HIP(ctx, hipGetLastError());
+    return CR_OK;
+}
+
+int cr_mlp_gradient(cr_mlp m, const float* d_x, const float* d_t, int n, float* host_grad, float* host_loss,
+                    void* stream) {
+    if (!m) return fail(nullptr, CR_EINVAL, "cr_mlp_gradient: mlp is NULL");
+    cr_context ctx = m->ctx;
+    if (n < 1 || !d_x || !d_t || !host_grad || !host_loss) return fail(ctx, CR_EINVAL, "cr_mlp_gradient: bad arguments");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = mlp_step(m, d_x, d_t, n, 0.0f, 0, s))) return rc;
+    CR_HIP(ctx, hipMemcpyAsync(host_grad, m->d_grad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipMemcpyAsync(host_loss, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+int cr_mlp_train_step(cr_mlp m, const float* d_x, const float* d_t, int n, float lr, void* stream) {
+    if (!m) return fail(nullptr, CR_EINVAL, "cr_mlp_train_step: mlp is NULL");
+    cr_context ctx = m->ctx;
+    if (n < 1 || !d_x || !d_t) return fail(ctx, CR_EINVAL, "cr_mlp_train_step: bad arguments");
+    if (!(lr >= 0.0f) || !std::isfinite(lr)) return fail(ctx, CR_EINVAL, "cr_mlp_train_step: bad learning rate");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    return mlp_step(m, d_x, d_t, n, lr, 1, (hipStream_t)stream);
+}
+
+int cr_mlp_last_loss(cr_mlp m, float* loss, void* stream) {
+    if (!m || !loss) return fail(m ? m->ctx : nullptr, CR_EINVAL, "cr_mlp_last_loss: NULL argument");
+    cr_context ctx = m->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    CR_HIP(ctx, hipMemcpyAsync(loss, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, s));
     CR_HIP(ctx, hipStreamSynchronize(s));
     return CR_OK;
 }

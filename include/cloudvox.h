@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define CR_ABI_VERSION 1
+#define CR_ABI_VERSION 2
 
 /* status codes */
 #define CR_OK 0
@@ -71,6 +71,7 @@ extern "C" {
 
 typedef struct cr_context_s *cr_context;
 typedef struct cr_volume_s *cr_volume;
+typedef struct cr_mlp_s *cr_mlp;
 
 /* Sun (src/Sun.hpp:12-19, src/main.cpp:37).  position drives the light view
  * of the voxelization exactly as Sun::update does; the cone direction is
@@ -123,6 +124,9 @@ typedef struct {
     float ms_setup, ms_sort, ms_bin, ms_cone;           /* cone-trace stages */
     int64_t n_voxelize, n_cone;                         /* calls summed (level 2) */
     int64_t chord_stores;    /* chord-fill voxel stores of pass 1 (rgba mode) */
+    int64_t mlp_rows;        /* cones whose far field the MLP predicted (learned trace) */
+    int64_t mlp_fallbacks;   /* of those, cones that fell back to the plain march */
+    int64_t mlp_evals;       /* batched MLP evaluations (one per wave and billboard) */
 } cr_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -181,6 +185,56 @@ int cr_volume_read_occupancy(cr_volume vol, uint8_t *host_dst, size_t bytes, voi
 int cr_volume_read_position_map(cr_volume vol, float *host_dst, size_t bytes, void *stream);
 /* draw order of the last cr_cone_trace: order[r] = billboard drawn r-th (far to near) */
 int cr_volume_read_draw_order(cr_volume vol, int32_t *host_dst, int count, void *stream);
+
+/* ---- learned far field ------------------------------------------------ */
+/* No counterpart in the reference (it marches every cone to its end,
+ * res/conetrace_frag.glsl:64-79).  A small MLP, 17 -> 64 -> 64 -> 64 -> 64 -> 5
+ * with GELU (tanh form), BF16 weights and FP32 accumulation on the matrix
+ * cores, predicts the far field of a front-to-back cone after its first k
+ * samples.  Inputs (17): last sample position (2x - 1, texture space), cone
+ * direction, sun direction, (density, light) at LODs 3..6 sampled along the
+ * cone where its radius matches the LOD.  Outputs (5): far-field (r, g, b, a)
+ * in units of the remaining transmittance, and a predicted error. */
+#define CR_MLP_INPUTS 17
+#define CR_MLP_OUTPUTS 5
+#define CR_MLP_PARAMS 13957 /* W0[64][17] b0[64] W1[64][64] b1 W2 b2 W3 b3 W4[5][64] b4[5] */
+
+int cr_mlp_create(cr_context ctx, cr_mlp *out); /* parameters zero */
+int cr_mlp_destroy(cr_mlp mlp);
+/* FP32 master parameters (host arrays of CR_MLP_PARAMS floats); setting
+ * them resets the optimiser state.  Both synchronise the device. */
+int cr_mlp_set_params(cr_mlp mlp, const float *host_params, int count);
+int cr_mlp_get_params(cr_mlp mlp, float *host_params_out, int count);
+/* d_y[n][5] = MLP(d_x[n][17]) (device arrays) */
+int cr_mlp_forward(cr_mlp mlp, const float *d_x, int n, float *d_y, void *stream);
+/* Training rows: d_x[n][17] features, d_t[n][5] targets (r, g, b, a,
+ * remaining transmittance w).  Loss (mean over rows):
+ *   w^2 sum_c (y_c - t_c)^2 + (y_4 - e)^2,  e = max_c |clamp(y_c, 0, 1) - t_c| (held constant).
+ * cr_mlp_gradient: gradient (CR_MLP_PARAMS floats) and loss into host memory,
+ * synchronous, parameters unchanged.  cr_mlp_train_step: one Adam step
+ * (beta 0.9 / 0.999, eps 1e-8) with learning rate lr, asynchronous;
+ * cr_mlp_last_loss reads the loss of the last step (synchronous). */
+int cr_mlp_gradient(cr_mlp mlp, const float *d_x, const float *d_t, int n, float *host_grad, float *host_loss,
+                    void *stream);
+int cr_mlp_train_step(cr_mlp mlp, const float *d_x, const float *d_t, int n, float lr, void *stream);
+int cr_mlp_last_loss(cr_mlp mlp, float *loss, void *stream);
+
+/* Training rows from the plain march: for start points d_p0[n][3] (texture
+ * space) the front-to-back cone toward the directional light is marched as
+ * cr_cone_trace marches it; cones still running after k samples get
+ * d_valid = 1, their features, and targets ((c - c_k) / (1 - a_k),
+ * (a - a_k) / (1 - a_k), 1 - a_k) from the rest of the march. */
+int cr_volume_segments(cr_volume vol, const cr_light *light, const cr_cone_params *params, int k,
+                       const float *d_p0, int n, float *d_features, float *d_targets, int32_t *d_valid,
+                       void *stream);
+/* cr_cone_trace with the far field of every cone still running after k
+ * samples predicted by the MLP; a cone whose predicted error times its
+ * remaining transmittance exceeds err_bound falls back to the plain march
+ * (err_bound < 0: every cone falls back, giving cr_cone_trace's image).
+ * Front-to-back cone and composite, RGBA32F / RGBA16F volumes. */
+int cr_cone_trace_learned(cr_volume vol, const cr_light *light, const cr_camera *camera,
+                          const cr_cone_params *params, cr_mlp mlp, int k, float err_bound, void *d_image,
+                          int width, int height, size_t pitch_bytes, int pixel_format, void *stream);
 
 #ifdef __cplusplus
 }

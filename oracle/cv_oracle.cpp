@@ -1247,3 +1247,267 @@ int cvo_num_threads(void) {
 }
 
 }  // extern "C"
+
+// ========================================================================
+// learned far field: segment features / targets and the MLP
+// ========================================================================
+// No reference code exists for this stage (it is defined by this project);
+// the functions below restate the HIP kernels' definitions so that the GPU
+// path can be checked, and are pinned by closed forms (features on constant
+// volumes), the plain cone march (targets) and finite differences (MLP
+// gradients).
+namespace {
+
+// tanHalf and the feature sampling distances: the cone radius equals
+// 2^(lod-1) voxels (lambda = lod) at height 2^(lod-1) / tan(angle / 2);
+// LODs 3..6, clamped to the last level.
+struct FarField {
+    int k;
+    float feat_h[4];
+    int feat_lvl[4];
+};
+
+FarField make_farfield(const cvo_cone* cp, int L, int k) {
+    FarField f;
+    f.k = k;
+    float tanHalf = std::tan(cp->cone_angle / 2.0f);
+    for (int j = 0; j < 4; ++j) {
+        int lod = 3 + j;
+        f.feat_h[j] = (float)(1 << (lod - 1)) / tanHalf;
+        f.feat_lvl[j] = std::min(lod, L - 1);
+    }
+    return f;
+}
+
+// 17 features after k samples: last sample position (2x - 1), cone direction
+// (d * N), sun direction, then (density, light) at LODs 3..6 at distance
+// max(h_k, feat_h[j]) along the cone ((0, 0) outside [0,1]^3).
+void farfield_features(const Volume32& v, const FarField& ff, const StepTab<float>& st, const float sun[3],
+                       Vec3T<float> p, Vec3T<float> d, float fN, float* f) {
+    float hk = st.h[ff.k - 1];
+    float xx = std::fma(hk, d.x, p.x), xy = std::fma(hk, d.y, p.y), xz = std::fma(hk, d.z, p.z);
+    f[0] = 2.0f * xx - 1.0f;
+    f[1] = 2.0f * xy - 1.0f;
+    f[2] = 2.0f * xz - 1.0f;
+    f[3] = d.x * fN;
+    f[4] = d.y * fN;
+    f[5] = d.z * fN;
+    f[6] = sun[0];
+    f[7] = sun[1];
+    f[8] = sun[2];
+    static const int chs[2] = {3, 0};
+    for (int j = 0; j < 4; ++j) {
+        float hj = std::max(hk, ff.feat_h[j]);
+        float sx = std::fma(hj, d.x, p.x), sy = std::fma(hj, d.y, p.y), sz = std::fma(hj, d.z, p.z);
+        float o[2] = {0.0f, 0.0f};
+        if (sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)
+            trilinear<float>(v, ff.feat_lvl[j], sx, sy, sz, 2, chs, o);
+        f[9 + 2 * j] = o[0];
+        f[10 + 2 * j] = o[1];
+    }
+}
+
+// ---- MLP 17 -> 64 -> 64 -> 64 -> 64 -> 5 (flat FP32 parameters:
+// W0 [64][17], b0, W1 [64][64], b1, W2, b2, W3, b3, W4 [5][64], b4)
+constexpr int kIn = 17, kHid = 64, kOut = 5;
+inline int lin(int l) { return l == 0 ? kIn : kHid; }
+inline int lout(int l) { return l == 4 ? kOut : kHid; }
+inline int woff(int l) {
+    int o = 0;
+    for (int i = 0; i < l; ++i) o += lin(i) * lout(i) + lout(i);
+    return o;
+}
+inline int boff(int l) { return woff(l) + lin(l) * lout(l); }
+constexpr int kParams = 13957;
+
+// bf16 round to nearest even (the GPU's conversion; NaN kept quiet)
+inline float bf16r(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) u = ((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u)) << 16;
+    else u = ((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) << 16;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+inline double rnd(double x, int emulate) { return emulate ? (double)bf16r((float)x) : x; }
+
+// GELU, tanh form: x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3))
+constexpr double kC1 = 1.5957691216057308, kC2 = 1.5957691216057308 * 0.044715;
+inline double gelu_d(double x) { return x / (1.0 + std::exp(-x * (kC1 + kC2 * x * x))); }
+inline double gelu_grad_d(double x) {
+    double sg = 1.0 / (1.0 + std::exp(-x * (kC1 + kC2 * x * x)));
+    return sg + x * sg * (1.0 - sg) * (kC1 + 3.0 * kC2 * x * x);
+}
+
+struct MlpTrace {
+    double H[5][kHid];  // layer inputs (rounded as the GPU rounds them)
+    double Z[4][kHid];  // hidden pre-activations
+    double y[kOut];
+};
+
+// emulate: 1 = round the matrix-core operands (inputs, weights, activations)
+// to bf16 as the GPU does; 0 = pure FP64 network.
+void mlp_fwd(const float* P, const float* x, int emulate, MlpTrace& t) {
+    for (int k = 0; k < kIn; ++k) t.H[0][k] = rnd((double)x[k], emulate);
+    for (int l = 0; l < 5; ++l) {
+        const float* W = P + woff(l);
+        const float* b = P + boff(l);
+        for (int o = 0; o < lout(l); ++o) {
+            double z = (double)b[o];
+            for (int k = 0; k < lin(l); ++k) z += rnd((double)W[o * lin(l) + k], emulate) * t.H[l][k];
+            if (l < 4) {
+                t.Z[l][o] = z;
+                t.H[l + 1][o] = rnd(gelu_d(z), emulate);
+            } else {
+                t.y[o] = z;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Segment samples of the learned far field.  For each start point p0 (texture
+// space) the front-to-back cone (modes 1/2, directional light) is marched as
+// in cvo_cone_trace; if it is still running after k samples (not exited, not
+// saturated, k < steps) the 17 features are recorded with the state (c_k, a_k),
+// the march continues to its end (c, a), and the targets are the far field in
+// units of the remaining transmittance:
+//     t_rgb = (c - c_k) / (1 - a_k),  t_a = (a - a_k) / (1 - a_k),  t_4 = 1 - a_k.
+// valid[i] = 1 for recorded rows (0 otherwise, features/targets zero).
+int cvo_segments(const cvo_volume* vol, const void* levels_native, const cvo_cone* cp, const cvo_light* light, int k,
+                 const float* p0, int64_t n, float* feat, float* tgt, int32_t* valid) {
+    if (!vol || !cp || cp->cone_mode == 0 || k < 1 || !light || !light->directional) return -1;
+    int N = vol->dim, L = vol->levels, C = channels(vol->format);
+    Volume32 v32;
+    v32.N = N; v32.L = L; v32.C = C; v32.format = vol->format; v32.data = levels_native;
+    v32.off = level_offsets(N, L, C);
+    StepTab<float> st = make_steps<float>(cp, L, N);
+    FarField ff = make_farfield(cp, L, k);
+    V3 ld = normalize(v3(light->direction[0], light->direction[1], light->direction[2]));
+    float fN = (float)N;
+    Vec3T<float> d = {ld.x / fN, ld.y / fN, ld.z / fN};
+    float sun[3] = {ld.x, ld.y, ld.z};
+    static const int chRGBA[4] = {0, 1, 2, 3};
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        Vec3T<float> p = {p0[3 * i], p0[3 * i + 1], p0[3 * i + 2]};
+        float c[3] = {0, 0, 0}, a = 0, cc[3] = {0, 0, 0}, ca = 0;
+        float ck[3] = {0, 0, 0}, ak = 0;
+        bool rec = false;
+        float f[17];
+        int nsmp = 0;
+        for (int s = 1; s <= cp->steps; ++s) {
+            float h = st.h[s - 1];
+            float sx = std::fma(h, d.x, p.x), sy = std::fma(h, d.y, p.y), sz = std::fma(h, d.z, p.z);
+            if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) break;
+            float sm[4];
+            sample_lod<float>(v32, st, s - 1, sx, sy, sz, 4, chRGBA, sm);
+            float al = cp->cone_mode == 1 ? sm[3] : 1.0f - cv_exp2_neg(-(st.kk[s - 1] * sm[3]));
+            float w = (1.0f - a) * al;
+            if (cp->compensated) {
+                for (int q = 0; q < 3; ++q) {
+                    float y = std::fma(w, sm[q], -cc[q]);
+                    float t = c[q] + y;
+                    cc[q] = (t - c[q]) - y;
+                    c[q] = t;
+                }
+                float y = w - ca;
+                float t = a + y;
+                ca = (t - a) - y;
+                a = t;
+            } else {
+                for (int q = 0; q < 3; ++q) c[q] = std::fma(w, sm[q], c[q]);
+                a = a + w;
+            }
+            ++nsmp;
+            if (a >= cp->alpha_stop) break;
+            if (nsmp == k && k < cp->steps) {
+                rec = true;
+                ck[0] = c[0]; ck[1] = c[1]; ck[2] = c[2]; ak = a;
+                farfield_features(v32, ff, st, sun, p, d, fN, f);
+            }
+        }
+        valid[i] = rec ? 1 : 0;
+        for (int q = 0; q < 17; ++q) feat[17 * i + q] = rec ? f[q] : 0.0f;
+        if (rec) {
+            float om = 1.0f - ak;
+            tgt[5 * i + 0] = (c[0] - ck[0]) / om;
+            tgt[5 * i + 1] = (c[1] - ck[1]) / om;
+            tgt[5 * i + 2] = (c[2] - ck[2]) / om;
+            tgt[5 * i + 3] = (a - ak) / om;
+            tgt[5 * i + 4] = om;
+        } else {
+            for (int q = 0; q < 5; ++q) tgt[5 * i + q] = 0.0f;
+        }
+    }
+    return 0;
+}
+
+int cvo_mlp_param_count(void) { return kParams; }
+
+// Batched MLP inference: X [n][17] -> Y [n][5].
+int cvo_mlp_forward(const float* params, const float* X, int64_t n, int emulate, float* Y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        MlpTrace t;
+        mlp_fwd(params, X + 17 * i, emulate, t);
+        for (int q = 0; q < kOut; ++q) Y[kOut * i + q] = (float)t.y[q];
+    }
+    return 0;
+}
+
+// Loss and its gradient over a batch (mean over the n rows):
+//     w^2 sum_c (y_c - t_c)^2 + (y_4 - e)^2,  w = t_4,
+//     e = max_c |clamp(y_c, 0, 1) - t_c| (a constant; e_fixed[i] if given).
+// With emulate = 1 the backward rounds its matrix-core operands (output
+// gradients, hidden gradients) to bf16 where the GPU does; bias gradients use
+// the unrounded values.
+int cvo_mlp_grad(const float* params, const float* X, const float* T, int64_t n, int emulate, const float* e_fixed,
+                 double* grad, double* loss) {
+    std::vector<double> g(kParams, 0.0);
+    double lsum = 0.0;
+    const double inv_n = 1.0 / (double)n;
+    for (int64_t i = 0; i < n; ++i) {
+        MlpTrace t;
+        mlp_fwd(params, X + 17 * i, emulate, t);
+        const float* tt = T + 5 * i;
+        double w = tt[4];
+        double e = 0.0;
+        for (int q = 0; q < 4; ++q) e = std::max(e, std::fabs(std::min(std::max(t.y[q], 0.0), 1.0) - (double)tt[q]));
+        if (e_fixed) e = e_fixed[i];
+        double G[kHid], Gr[kHid];
+        for (int q = 0; q < 4; ++q) {
+            double dq = t.y[q] - (double)tt[q];
+            G[q] = 2.0 * w * w * dq * inv_n;
+            lsum += w * w * dq * dq;
+        }
+        G[4] = 2.0 * (t.y[4] - e) * inv_n;
+        lsum += (t.y[4] - e) * (t.y[4] - e);
+        for (int l = 4; l >= 0; --l) {
+            const float* W = params + woff(l);
+            for (int o = 0; o < lout(l); ++o) Gr[o] = rnd(G[o], emulate);
+            for (int o = 0; o < lout(l); ++o) {
+                g[boff(l) + o] += G[o];
+                for (int k = 0; k < lin(l); ++k) g[woff(l) + o * lin(l) + k] += Gr[o] * t.H[l][k];
+            }
+            if (l > 0) {
+                double Gn[kHid];
+                for (int k = 0; k < lin(l); ++k) {
+                    double dh = 0.0;
+                    for (int o = 0; o < lout(l); ++o) dh += rnd((double)W[o * lin(l) + k], emulate) * Gr[o];
+                    Gn[k] = dh * gelu_grad_d(t.Z[l - 1][k]);
+                }
+                for (int k = 0; k < lin(l); ++k) G[k] = Gn[k];
+            }
+        }
+    }
+    for (int p = 0; p < kParams; ++p) grad[p] = g[p];
+    *loss = lsum * inv_n;
+    return 0;
+}
+
+}  // extern "C"

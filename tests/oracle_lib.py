@@ -84,6 +84,13 @@ def lib():
         L.cvo_trace_one.argtypes = [P(cvo_volume), vp, vp, P(cvo_cone), P(C.c_float), P(C.c_float), vp, vp,
                                     P(C.c_int32)]
         L.cvo_num_threads.restype = C.c_int
+        L.cvo_segments.argtypes = [P(cvo_volume), vp, P(cvo_cone), P(cvo_light), C.c_int, vp, C.c_int64, vp, vp, vp]
+        L.cvo_segments.restype = C.c_int
+        L.cvo_mlp_param_count.restype = C.c_int
+        L.cvo_mlp_forward.argtypes = [vp, vp, C.c_int64, C.c_int, vp]
+        L.cvo_mlp_forward.restype = C.c_int
+        L.cvo_mlp_grad.argtypes = [vp, vp, vp, C.c_int64, C.c_int, vp, vp, P(C.c_double)]
+        L.cvo_mlp_grad.restype = C.c_int
         _lib = L
     return _lib
 
@@ -302,3 +309,38 @@ def noise_voxels(vol: cvo_volume, xyz_int: np.ndarray):
     o64 = np.zeros(xyz_int.shape[0], np.float64)
     lib().cvo_noise_voxels(C.byref(vol), ptr(xyz_int), xyz_int.shape[0], ptr(o32), ptr(o64))
     return o32, o64
+
+
+# ---- learned far field ------------------------------------------------------
+
+def segments(vol: cvo_volume, nat, cone: cvo_cone, light: cvo_light, k: int, p0):
+    """Features (n, 17), targets (n, 5), valid (n,) of cones started at p0 (n, 3)."""
+    p0 = np.ascontiguousarray(p0, np.float32).reshape(-1, 3)
+    n = p0.shape[0]
+    feat = np.zeros((n, 17), np.float32)
+    tgt = np.zeros((n, 5), np.float32)
+    valid = np.zeros(n, np.int32)
+    rc = lib().cvo_segments(C.byref(vol), ptr(nat), C.byref(cone), C.byref(light), int(k), ptr(p0), n, ptr(feat),
+                            ptr(tgt), ptr(valid))
+    if rc != 0:
+        raise ValueError("cvo_segments rejected its arguments")
+    return feat, tgt, valid
+
+
+def mlp_forward(params, x, emulate=1):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 17)
+    y = np.zeros((x.shape[0], 5), np.float32)
+    lib().cvo_mlp_forward(ptr(params), ptr(x), x.shape[0], int(emulate), ptr(y))
+    return y
+
+
+def mlp_grad(params, x, t, emulate=1, e_fixed=None):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, 17)
+    t = np.ascontiguousarray(t, np.float32).reshape(-1, 5)
+    ef = None if e_fixed is None else np.ascontiguousarray(e_fixed, np.float32)
+    g = np.zeros(lib().cvo_mlp_param_count(), np.float64)
+    loss = C.c_double()
+    lib().cvo_mlp_grad(ptr(params), ptr(x), ptr(t), x.shape[0], int(emulate), ptr(ef), ptr(g), C.byref(loss))
+    return g, float(loss.value)

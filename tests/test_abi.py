@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol(cv):
 
 
 def test_abi_version(cv):
-    assert cv.lib().cr_abi_version() == 1
+    assert cv.lib().cr_abi_version() == 2
 
 
 def test_struct_layouts_match_header(cv):
@@ -38,7 +38,7 @@ def test_struct_layouts_match_header(cv):
     assert C.sizeof(_abi.cr_light) == 7 * 4
     assert C.sizeof(_abi.cr_camera) == 12 * 4
     assert C.sizeof(_abi.cr_cone_params) == 15 * 4
-    assert C.sizeof(_abi.cr_stats) == 5 * 8 + 8 * 4 + 3 * 8
+    assert C.sizeof(_abi.cr_stats) == 5 * 8 + 8 * 4 + 6 * 8
 
 
 def _has_gpu():

@@ -1,0 +1,203 @@
+"""Learned far field on the GPU, through the C ABI.
+
+  * MLP inference (matrix cores, bf16) against the oracle network that rounds
+    at the same points: close, and closer than to the pure FP64 network;
+  * training gradients against the oracle's backward: relative L2 error and
+    cosine bounds; one Adam step matches the closed-form update;
+  * training on a synthetic regression reduces the loss;
+  * segment rows (features, targets, valid) bit-exact against the oracle,
+    RGBA32F and RGBA16F volumes;
+  * the learned cone trace with every cone falling back equals cr_cone_trace
+    bit for bit; with a zero network and no fallback it equals the plain trace
+    truncated after k samples bit for bit.
+"""
+import math
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import farfield_starts
+import oracle_lib as O
+from gpu_common import log
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    dims = [(17, 64), (64, 64), (64, 64), (64, 64), (64, 5)]
+    out = []
+    for fin, fout in dims:
+        out.append((rng.normal(0, math.sqrt(2.0 / fin), (fout, fin)) * scale).astype(np.float32).ravel())
+        out.append((rng.normal(0, 0.1, fout) * scale).astype(np.float32))
+    return np.concatenate(out)
+
+
+def _rows(seed, n):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 17)).astype(np.float32)
+    t = rng.uniform(0, 1, (n, 5)).astype(np.float32)
+    return x, t
+
+
+def test_mlp_forward_matches_oracle(cv, gpu_ctx):
+    import torch
+    log("[farfield] MLP forward, 1000 rows (15 full waves + a partial one)")
+    p = _params(1)
+    x, _ = _rows(2, 1000)
+    mlp = cv.FarFieldMLP(gpu_ctx, p)
+    y = mlp.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    mlp.close()
+    y1 = O.mlp_forward(p, x, emulate=1)
+    y0 = O.mlp_forward(p, x, emulate=0)
+    scale = float(np.abs(y1).max())
+    e1 = float(np.abs(y - y1).max())
+    e0 = float(np.abs(y - y0).max())
+    m1 = float(np.abs(y - y1).mean())
+    log(f"[farfield] forward: max |gpu - bf16 oracle| {e1:.2e}, mean {m1:.2e}, max |gpu - fp64| {e0:.2e}, "
+        f"scale {scale:.2f}")
+    assert e1 <= 1e-2 * scale
+    assert m1 <= 5e-4 * scale
+    assert float(np.abs(y - y1).mean()) < 0.5 * float(np.abs(y - y0).mean())
+
+
+def test_mlp_gradient_matches_oracle(cv, gpu_ctx):
+    import torch
+    log("[farfield] MLP gradient, 700 rows")
+    p = _params(3, 0.8)
+    x, t = _rows(4, 700)
+    mlp = cv.FarFieldMLP(gpu_ctx, p)
+    g, loss = mlp.gradient(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+    mlp.close()
+    go, lo = O.mlp_grad(p, x, t, emulate=1)
+    rel = float(np.linalg.norm(g - go) / np.linalg.norm(go))
+    cos = float(np.dot(g, go) / (np.linalg.norm(g) * np.linalg.norm(go)))
+    log(f"[farfield] gradient: rel L2 {rel:.2e}, cosine {cos:.8f}, loss {loss:.6f} vs {lo:.6f}")
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert rel <= 1e-2
+    assert cos >= 0.9999
+    # every block, including the biases, individually
+    offs = [0, 1088, 1152, 5248, 5312, 9408, 9472, 13568, 13632, 13952, 13957]
+    for b in range(10):
+        a, o = g[offs[b]:offs[b + 1]], go[offs[b]:offs[b + 1]]
+        assert np.linalg.norm(a - o) <= 2e-2 * np.linalg.norm(o) + 1e-9, b
+
+
+def test_mlp_adam_step(cv, gpu_ctx):
+    import torch
+    log("[farfield] one Adam step")
+    p = _params(5, 0.5)
+    x, t = _rows(6, 256)
+    mlp = cv.FarFieldMLP(gpu_ctx, p)
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    g, _ = mlp.gradient(xd, td)
+    lr = 1e-3
+    mlp.train_step(xd, td, lr)
+    q = mlp.get_params()
+    mlp.close()
+    # first step: m = 0.1 g, v = 0.001 g^2; mhat = g, vhat = g^2 -> p - lr * g / (|g| + eps)
+    want = p - lr * g / (np.abs(g) + 1e-8)
+    assert np.allclose(q, want, rtol=1e-5, atol=1e-6)
+
+
+def test_mlp_training_reduces_loss(cv, gpu_ctx):
+    import torch
+    log("[farfield] 300 Adam steps on a synthetic regression")
+    rng = np.random.default_rng(7)
+    n = 8192
+    x = rng.uniform(-1, 1, (n, 17)).astype(np.float32)
+    t = np.zeros((n, 5), np.float32)
+    s = 0.5 + 0.5 * np.sin(2.0 * x[:, 0] + x[:, 1] * x[:, 2])
+    t[:, 0] = t[:, 1] = t[:, 2] = 0.8 * s
+    t[:, 3] = s
+    t[:, 4] = 0.5 + 0.5 * x[:, 3] ** 2
+    mlp = cv.FarFieldMLP(gpu_ctx, cv.FarFieldMLP.init_params(0))
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    _, l0 = mlp.gradient(xd, td)
+    for i in range(300):
+        idx = torch.from_numpy(rng.integers(0, n, 1024)).cuda()
+        mlp.train_step(xd[idx].contiguous(), td[idx].contiguous(), 2e-3)
+    _, l1 = mlp.gradient(xd, td)
+    mlp.close()
+    log(f"[farfield] loss {l0:.4e} -> {l1:.4e}")
+    assert l1 < 0.1 * l0
+
+
+def _seg_scene(cv, fmt):
+    sc = cv.scenes.small32(1)
+    sc = replace(sc, N=64, levels=7, fmt=fmt, cone=replace(sc.cone, steps=96, cone_angle=math.radians(20.0),
+                                                             compensated=1 if fmt == 2 else 0))
+    if fmt == 2:
+        sc = replace(sc, noise=cv.scenes.Noise(seed=5))
+    return sc
+
+
+@pytest.mark.parametrize("fmt,k", [(1, 24), (1, 33), (2, 24)])
+def test_segments_match_oracle(cv, gpu_ctx, fmt, k):
+    import torch
+    sc = _seg_scene(cv, fmt)
+    log(f"[farfield] segments fmt={fmt} k={k}")
+    p0 = farfield_starts.starts(cv, sc, 2000, 13)
+    vol, sun, cam, vox, tr = cv.scene_objects(sc, 0, context=gpu_ctx)
+    try:
+        vox.voxelize(vol, sun)
+        f, t, v = tr.segments(vol, sun, torch.from_numpy(p0).cuda(), k)
+        torch.cuda.synchronize()
+        f, t, v = f.cpu().numpy(), t.cpu().numpy(), v.cpu().numpy()
+    finally:
+        vol.close()
+    vr = O.voxelize(sc)
+    nat, _, _, _ = O.build_levels(sc, vr.shell, vr.fill, want_f64=False)
+    fo, to, vo = O.segments(O.volume_of(sc), nat, O.cone_of(sc.cone), O.light_of(sc), k, p0)
+    log(f"[farfield] segments: {int(v.sum())} of {len(v)} rows valid")
+    assert v.sum() > 100
+    assert np.array_equal(v, vo)
+    assert np.array_equal(f.view(np.uint32), fo.view(np.uint32))
+    assert np.array_equal(t.view(np.uint32), to.view(np.uint32))
+
+
+def _render(cv, ctx, scene, fn):
+    import torch
+    vol, sun, cam, vox, tr = cv.scene_objects(scene, 0, context=ctx)
+    try:
+        vox.voxelize(vol, sun)
+        img = torch.zeros((scene.H, scene.W, 4), dtype=torch.float32, device="cuda")
+        fn(vol, sun, cam, tr, img)
+        torch.cuda.synchronize()
+        return img.cpu().numpy()
+    finally:
+        vol.close()
+
+
+def _learned_scene(cv):
+    return cv.scenes.scaled(cv.scenes.plain256(2), 320, 180)
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_learned_all_fallback_equals_plain(cv, gpu_ctx, fmt):
+    sc = _learned_scene(cv)
+    if fmt == 2:
+        sc = replace(sc, fmt=2, cone=replace(sc.cone, compensated=1))
+    log(f"[farfield] learned trace, every cone falls back, fmt={fmt}")
+    mlp = cv.FarFieldMLP(gpu_ctx, _params(8))
+    W, H = sc.W, sc.H
+    plain = _render(cv, gpu_ctx, sc, lambda v, s, c, tr, img: tr.coneTrace(v, s, c, img, W, H))
+    learned = _render(cv, gpu_ctx, sc, lambda v, s, c, tr, img: tr.coneTraceLearned(v, s, c, mlp, img, W, H, k=32,
+                                                                                     err_bound=-1.0))
+    mlp.close()
+    assert np.array_equal(plain.view(np.uint32), learned.view(np.uint32))
+
+
+def test_learned_zero_network_equals_truncated(cv, gpu_ctx):
+    sc = _learned_scene(cv)
+    k = 24
+    log("[farfield] learned trace with a zero network = plain trace truncated after k samples")
+    mlp = cv.FarFieldMLP(gpu_ctx)
+    W, H = sc.W, sc.H
+    trunc_sc = replace(sc, cone=replace(sc.cone, steps=k))
+    trunc = _render(cv, gpu_ctx, trunc_sc, lambda v, s, c, tr, img: tr.coneTrace(v, s, c, img, W, H))
+    learned = _render(cv, gpu_ctx, sc, lambda v, s, c, tr, img: tr.coneTraceLearned(v, s, c, mlp, img, W, H, k=k,
+                                                                                     err_bound=1e30))
+    mlp.close()
+    assert np.array_equal(trunc.view(np.uint32), learned.view(np.uint32))
