@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import importlib.util
 import json
+import math
 import os
 import sys
 import time
@@ -121,6 +122,65 @@ def run_gpu(cv, scene, steps: int, warmup: int, ctx, label: str):
     return {"secs": secs, "stage_ms": stage_ms, "counters": avg, "bytes": bytes_}
 
 
+def learned_entry(cv, ctx, scene, args):
+    """Learned far field (k = 32, committed parameters) on the bench workload:
+    whole frames at every azimuth at two error bounds (2e-3, and no
+    fallback), frame time and cone-stage time, PSNR of the images against the
+    plain trace of the same azimuths, fallback rate."""
+    import torch
+    from cloud_renderer_amd import farfield as ff
+
+    params = cv.FarFieldMLP.load_params(ff.weights_path(32))
+    mlp = cv.FarFieldMLP(ctx, params)
+    W, H = scene.W, scene.H
+    vol, sun, _, vox, tr = cv.scene_objects(scene, 0, context=ctx)
+    cams = [cv.Camera(eye=c.eye, target=c.target, up=c.up, fovy=c.fovy, znear=c.znear, zfar=c.zfar)
+            for c in scene.cameras]
+    stream = torch.cuda.current_stream()
+    plain = torch.empty((len(cams), H, W, 4), dtype=torch.float32, device="cuda")
+    img = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    vox.voxelize(vol, sun, stream=stream)
+    for a, cam in enumerate(cams):
+        tr.coneTrace(vol, sun, cam, plain[a], W, H, stream=stream)
+    res = {"k": 32, "params": "cloud-renderer_amd/weights/farfield_k32.npz"}
+    steps = max(4, args.steps // 2)
+    for name, bound in (("err_bound_2e-3", 2e-3), ("no_fallback", 1e30)):
+        # quality and counters per azimuth
+        ctx.set_profiling(1)
+        se, rows, falls = 0.0, 0, 0
+        for a, cam in enumerate(cams):
+            tr.coneTraceLearned(vol, sun, cam, mlp, img, W, H, k=32, err_bound=bound, stream=stream)
+            st = ctx.stats()
+            rows += st["mlp_rows"]
+            falls += st["mlp_fallbacks"]
+            se += float(((img - plain[a]).double() ** 2).mean())
+        ctx.set_profiling(0)
+        mse = se / len(cams)
+        for i in range(2):
+            vox.voxelize(vol, sun, stream=stream)
+            tr.coneTraceLearned(vol, sun, cams[i % len(cams)], mlp, img, W, H, k=32, err_bound=bound, stream=stream)
+        torch.cuda.synchronize()
+        ctx.set_profiling(2)
+        t0 = time.perf_counter()
+        for i in range(steps):
+            vox.voxelize(vol, sun, stream=stream)
+            tr.coneTraceLearned(vol, sun, cams[i % len(cams)], mlp, img, W, H, k=32, err_bound=bound, stream=stream)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.set_profiling(0)
+        cone_ms = st["ms_cone"] / max(1, st["n_cone"])
+        res[name] = {"err_bound": bound, "mrays_per_s": round(W * H * steps / secs / 1e6, 2),
+                     "frame_ms": round(secs * 1e3 / steps, 4), "cone_ms": round(cone_ms, 4),
+                     "cone_mrays_per_s": round(W * H / (cone_ms * 1e-3) / 1e6, 2),
+                     "psnr_vs_plain_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None,
+                     "fallback_rate": round(falls / max(1, rows), 4), "mlp_rows_per_frame": rows // len(cams)}
+        log(f"[bench] learned {name}: {json.dumps(res[name])}")
+    vol.close()
+    mlp.close()
+    return res
+
+
 def cpu_baseline(cv, scene, budget_s: float):
     """Oracle FP32 pass (test infrastructure, the checker) timed on the host."""
     sys.path.insert(0, str(REPO / "tests"))
@@ -175,6 +235,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true", help="skip the reference-semantics frame")
     ap.add_argument("--no-fp16", action="store_true", help="skip the 512^3 RGBA16F workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned far-field frames")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -260,6 +321,8 @@ def main() -> int:
                            "stage_ms": {kk: round(v, 4) for kk, v in rf["stage_ms"].items()},
                            "counters_per_frame": {kk: int(v) for kk, v in rf["counters"].items()},
                            "cone_gather_gbs": round(rf["bytes"]["cone"] / (rf["stage_ms"]["cone"] * 1e-3) / 1e9, 1)}
+    if not args.no_learned:
+        out["learned"] = learned_entry(cv, ctx, scene, args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle FP32 pass)")
         out["cpu_baseline"] = cpu_baseline(cv, scene, args.cpu_budget)

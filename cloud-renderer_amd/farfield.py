@@ -150,30 +150,42 @@ def training_scenes(cv, count: int, seed0: int = 1000, W: int = 480, H: int = 27
     return out
 
 
-def evaluate(cv, mlp, scene, k: int, err_bound: float, cam_index: int = 0, context=None):
-    """Render `scene` plainly, truncated after k samples, and learned; return
-    the images (CPU numpy) and the learned pass's counters."""
+def evaluate(cv, mlp, scene, k: int, bounds, cam_index: int = 0, context=None, repeats: int = 5):
+    """Render `scene` plainly, truncated after k samples, and learned at every
+    error bound in `bounds`.  Returns a dict with the images (CPU numpy), the
+    cone-stage time of each (median of `repeats` event-timed calls) and the
+    learned passes' counters."""
     import torch
 
     vol, sun, cam, vox, tr = cv.scene_objects(scene, cam_index, context=context)
     ctx = vol.ctx
-    try:
-        vox.voxelize(vol, sun)
-        W, H = scene.W, scene.H
-        plain = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
-        learned = torch.zeros_like(plain)
-        trunc = torch.zeros_like(plain)
-        tr.coneTrace(vol, sun, cam, plain, W, H)
-        steps = tr.vctSteps
-        tr.vctSteps = min(k, steps)
-        tr.coneTrace(vol, sun, cam, trunc, W, H)
-        tr.vctSteps = steps
+    W, H = scene.W, scene.H
+
+    def run(fn):
+        img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
         ctx.set_profiling(1)
-        tr.coneTraceLearned(vol, sun, cam, mlp, learned, W, H, k=k, err_bound=err_bound)
+        fn(img)
         st = ctx.stats()
+        times = []
+        for _ in range(repeats):
+            fn(img)
+            times.append(ctx.stats()["ms_cone"])
         ctx.set_profiling(0)
         torch.cuda.synchronize()
-        return plain.cpu().numpy(), trunc.cpu().numpy(), learned.cpu().numpy(), st
+        st["ms_cone"] = float(np.median(times))
+        return img.cpu().numpy(), st
+
+    try:
+        vox.voxelize(vol, sun)
+        out = {}
+        out["plain"] = run(lambda img: tr.coneTrace(vol, sun, cam, img, W, H))
+        steps = tr.vctSteps
+        tr.vctSteps = min(k, steps)
+        out["truncated"] = run(lambda img: tr.coneTrace(vol, sun, cam, img, W, H))
+        tr.vctSteps = steps
+        for b in bounds:
+            out[b] = run(lambda img, b=b: tr.coneTraceLearned(vol, sun, cam, mlp, img, W, H, k=k, err_bound=b))
+        return out
     finally:
         vol.close()
 

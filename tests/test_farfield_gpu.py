@@ -201,3 +201,47 @@ def test_learned_zero_network_equals_truncated(cv, gpu_ctx):
                                                                                      err_bound=1e30))
     mlp.close()
     assert np.array_equal(trunc.view(np.uint32), learned.view(np.uint32))
+
+
+def _weights(cv):
+    from cloud_renderer_amd import farfield as ff
+    p = ff.weights_path(32)
+    assert p.exists(), f"trained far-field parameters missing: {p} (cloud-renderer_amd/train_farfield.py writes them)"
+    return cv.FarFieldMLP.load_params(p)
+
+
+def test_trained_network_gains_over_truncation(cv, gpu_ctx):
+    """What the trained network adds: against the plain march, the learned
+    trace (no fallback) must beat the trace truncated after k = 32 samples by
+    >= 10 dB and reach >= 40 dB; at the 2e-3 error bound it must stay >= 60 dB.
+    Also scored against the oracle's FP64 plain march on every 8th row."""
+    from cloud_renderer_amd import farfield as ff
+    sc = cv.scenes.scaled(cv.scenes.plain256(0), 480, 270)
+    sc = replace(sc, light_w=1920, light_h=1080)
+    log("[farfield] trained network on a held-out plain256 frame at 480x270")
+    mlp = cv.FarFieldMLP(gpu_ctx, _weights(cv))
+    res = ff.evaluate(cv, mlp, sc, 32, [2e-3, 1e30], context=gpu_ctx, repeats=1)
+    mlp.close()
+    plain = res["plain"][0]
+    trunc = res["truncated"][0]
+    l_spec, st_spec = res[2e-3]
+    l_free, st_free = res[1e30]
+    p_trunc = ff.psnr(trunc, plain)
+    p_free = ff.psnr(l_free, plain)
+    p_spec = ff.psnr(l_spec, plain)
+    # FP64 oracle (plain march) on a row subsample
+    vr = O.voxelize(sc)
+    nat, f64, _, _ = O.build_levels(sc, vr.shell, vr.fill)
+    order = O.sort_boards(sc, sc.cameras[0].eye)
+    _, i64, _ = O.cone_trace(sc, nat, f64, order, sc.cameras[0], row_step=8)
+    rows = np.arange(0, sc.H, 8)
+    p_free64 = ff.psnr(l_free[rows], i64[rows])
+    p_plain64 = ff.psnr(plain[rows], i64[rows])
+    fb = st_spec["mlp_fallbacks"] / max(1, st_spec["mlp_rows"])
+    log(f"[farfield] PSNR vs plain: truncated {p_trunc:.2f} dB, learned {p_free:.2f} dB (no fallback), "
+        f"{p_spec:.2f} dB at bound 2e-3 (fallback rate {fb:.4f}); vs FP64 oracle: learned {p_free64:.2f} dB, "
+        f"plain {p_plain64:.2f} dB")
+    assert p_free >= 40.0
+    assert p_free >= p_trunc + 10.0
+    assert p_spec >= 60.0
+    assert p_free64 >= 40.0
