@@ -181,6 +181,66 @@ def learned_entry(cv, ctx, scene, args):
     return res
 
 
+def batched_entry(cv, ctx, args):
+    """Batched workload: 64 scenes (volume side from {64..256}, 256..8192
+    spheres, square images from {256..1024}), rendered by one hipGraph with a
+    single cone-trace launch over all scenes' tiles; launch shape chosen by
+    the library's autotuner.  Compared with the same scenes rendered one after
+    another through the per-scene entry points."""
+    import torch
+
+    scenes = cv.scenes.batched64(args.seed)
+    entries = []
+    for sc in scenes:
+        vol, sun, cam, vox, tr = cv.scene_objects(sc, 0, context=ctx)
+        img = torch.empty((sc.H, sc.W, 4), dtype=torch.float32, device="cuda")
+        entries.append((vol, sun, cam, tr, img, vox))
+    rays = sum(sc.W * sc.H for sc in scenes)
+    b = cv.Batch(entries, context=ctx)
+    best, tuned_ms = b.autotune(repeats=3)
+    log(f"[bench] batched64: autotuned {best} ({tuned_ms:.3f} ms per run)")
+    stream = torch.cuda.current_stream()
+
+    def sequential():
+        for vol, sun, cam, tr, img, vox in entries:
+            vox.voxelize(vol, sun, stream=stream)
+            tr.coneTrace(vol, sun, cam, img, img.shape[1], img.shape[0], stream=stream)
+
+    sequential()
+    torch.cuda.synchronize()
+    k = max(4, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        sequential()
+    torch.cuda.synchronize()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / k
+    for _ in range(2):
+        b.run(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        b.run(stream)
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / k
+    ctx.set_profiling(1)
+    b.run(stream)
+    st = ctx.stats()
+    ctx.set_profiling(0)
+    b.close()
+    for vol, *_ in entries:
+        vol.close()
+    out = {"desc": "64 scenes, one hipGraph: per-scene voxelize/sort/bin + one cone-trace launch over all tiles",
+           "scenes": len(scenes), "rays_per_run": rays, "ms_per_run": round(batch_ms, 4),
+           "mrays_per_s": round(rays / (batch_ms * 1e-3) / 1e6, 2), "config": best,
+           "autotuned_ms_per_run": round(tuned_ms, 4), "sequential_ms_per_run": round(seq_ms, 4),
+           "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
+           "eager_phase_ms": {"voxelize": round(st["ms_setup"], 4), "sort_bin": round(st["ms_bin"], 4),
+                              "cone": round(st["ms_cone"], 4)},
+           "samples": int(st["samples"]), "fragments": int(st["fragments"])}
+    log(f"[bench] batched64: {json.dumps(out)}")
+    return out
+
+
 def cpu_baseline(cv, scene, budget_s: float):
     """Oracle FP32 pass (test infrastructure, the checker) timed on the host."""
     sys.path.insert(0, str(REPO / "tests"))
@@ -236,6 +296,7 @@ def main() -> int:
     ap.add_argument("--no-ref-mode", action="store_true", help="skip the reference-semantics frame")
     ap.add_argument("--no-fp16", action="store_true", help="skip the 512^3 RGBA16F workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned far-field frames")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -323,6 +384,8 @@ def main() -> int:
                            "cone_gather_gbs": round(rf["bytes"]["cone"] / (rf["stage_ms"]["cone"] * 1e-3) / 1e9, 1)}
     if not args.no_learned:
         out["learned"] = learned_entry(cv, ctx, scene, args)
+    if not args.no_batched:
+        out["batched64"] = batched_entry(cv, ctx, args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle FP32 pass)")
         out["cpu_baseline"] = cpu_baseline(cv, scene, args.cpu_budget)

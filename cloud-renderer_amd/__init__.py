@@ -30,7 +30,7 @@ from ._abi import (CR_COMPOSITE_FTB, CR_COMPOSITE_GL_BLEND, CR_CONE_FTB, CR_CONE
                    CR_MODE_RGBA, CR_PIXEL_RGBA8, CR_PIXEL_RGBA32F, CloudVoxError, cr_camera, cr_cone_params,
                    cr_light, cr_stats)
 
-__all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "FarFieldMLP",
+__all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "FarFieldMLP", "Batch",
            "CloudVoxError", "lib", "scenes"]
 
 
@@ -434,6 +434,70 @@ class FarFieldMLP:
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib().cr_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many scenes rendered by one hipGraph (cr_batch): every scene's voxelize,
+    draw order and binning, then one cone-trace launch over all scenes' tiles.
+
+    entries: list of (volume, sun, camera, tracer, image, voxelizer) with
+    images as torch CUDA tensors (H, W, 4) float32."""
+
+    def __init__(self, entries, context: Optional[Context] = None):
+        self.ctx = context or entries[0][0].ctx
+        items = (_abi.cr_batch_item * len(entries))()
+        self._keep = []
+        for i, (vol, sun, cam, tr, img, vox) in enumerate(entries):
+            it = items[i]
+            it.volume = vol.handle
+            it.light = sun.to_c()
+            it.light_w, it.light_h = int(vox.width), int(vox.height)
+            it.camera = cam.to_c()
+            it.params = tr.params()
+            it.d_image = img.data_ptr()
+            it.height, it.width = int(img.shape[0]), int(img.shape[1])
+            it.pitch_bytes = img.stride(0) * img.element_size()
+            it.pixel_format = CR_PIXEL_RGBA32F if img.element_size() == 4 else CR_PIXEL_RGBA8
+            self._keep.append((vol, img))
+        h = C.c_void_p()
+        _check(lib().cr_batch_create(self.ctx.handle, items, len(entries), C.byref(h)), self.ctx.handle)
+        self.handle = h
+
+    def config(self) -> dict:
+        c = _abi.cr_batch_config()
+        _check(lib().cr_batch_get_config(self.handle, C.byref(c)), self.ctx.handle)
+        return {"tile_w": c.tile_w, "waves_per_group": c.waves_per_group, "groups": c.groups}
+
+    def set_config(self, tile_w: int = 8, waves_per_group: int = 1, groups: int = 0) -> None:
+        c = _abi.cr_batch_config(int(tile_w), int(waves_per_group), int(groups))
+        _check(lib().cr_batch_set_config(self.handle, C.byref(c)), self.ctx.handle)
+
+    def run(self, stream=None) -> None:
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream()
+        _check(lib().cr_batch_run(self.handle, _stream_handle(stream)), self.ctx.handle)
+
+    def autotune(self, repeats: int = 3, stream=None):
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream()
+        best = _abi.cr_batch_config()
+        ms = C.c_float()
+        _check(lib().cr_batch_autotune(self.handle, int(repeats), _stream_handle(stream), C.byref(best),
+                                       C.byref(ms)), self.ctx.handle)
+        return {"tile_w": best.tile_w, "waves_per_group": best.waves_per_group, "groups": best.groups}, float(ms.value)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().cr_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

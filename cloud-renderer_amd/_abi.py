@@ -65,6 +65,16 @@ class cr_stats(C.Structure):
                 ("mlp_evals", C.c_int64)]
 
 
+class cr_batch_item(C.Structure):
+    _fields_ = [("volume", C.c_void_p), ("light", cr_light), ("light_w", C.c_int), ("light_h", C.c_int),
+                ("camera", cr_camera), ("params", cr_cone_params), ("d_image", C.c_void_p), ("width", C.c_int),
+                ("height", C.c_int), ("pitch_bytes", C.c_size_t), ("pixel_format", C.c_int)]
+
+
+class cr_batch_config(C.Structure):
+    _fields_ = [("tile_w", C.c_int), ("waves_per_group", C.c_int), ("groups", C.c_int)]
+
+
 CR_MLP_INPUTS = 17
 CR_MLP_OUTPUTS = 5
 CR_MLP_PARAMS = 13957
@@ -103,6 +113,13 @@ SIGNATURES = {
     "cr_mlp_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
     "cr_volume_segments": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_cone_params), C.c_int,
                                      C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cr_batch_create": (C.c_int, [C.c_void_p, C.POINTER(cr_batch_item), C.c_int, C.POINTER(C.c_void_p)]),
+    "cr_batch_destroy": (C.c_int, [C.c_void_p]),
+    "cr_batch_set_config": (C.c_int, [C.c_void_p, C.POINTER(cr_batch_config)]),
+    "cr_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(cr_batch_config)]),
+    "cr_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cr_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(cr_batch_config),
+                                    C.POINTER(C.c_float)]),
     "cr_cone_trace_learned": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_camera),
                                         C.POINTER(cr_cone_params), C.c_void_p, C.c_int, C.c_float, C.c_void_p,
                                         C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_void_p]),

@@ -12,7 +12,7 @@ namespace cv {
 
 constexpr int kMaxLevels = 16;
 constexpr int kMaxSteps = 256;
-constexpr int kTile = 8;           // camera tiles are 8x8 pixels = one wavefront
+constexpr int kTile = 8;           // default camera tile: 8x8 pixels = one wavefront
 constexpr int kLdsSortMax = 16384; // billboards sorted in one workgroup's LDS
 constexpr int kChordLdsWords = 4096; // LDS bitmap (bricks) per pass-1 tile for chord-fill aggregation
 constexpr int kLightTile = 32;       // pass-1 work item: 32 x 32 light pixels of one billboard
@@ -69,6 +69,8 @@ struct CameraView {
     float znear, zfar;
     int W, H;
     int tiles_x, tiles_y;
+    int tile_w, tile_h;  // camera tile shape, tile_w * tile_h = 64 pixels = one wavefront
+    int tile_shift;      // log2(tile_w)
 };
 
 struct ConeDesc {
@@ -131,6 +133,35 @@ struct TileQueue {
 size_t tile_scratch_ints(int ntiles);
 TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
                      unsigned long long* masks, int* scratch, hipStream_t s);
+// One scene of a batched cone-trace launch (device resident; the batched
+// kernel reads it with wave-uniform loads).
+struct BatchScene {
+    GridDesc g;
+    VolumeDesc vd;
+    CameraView cv;
+    ConeDesc cd;
+    const void* levels;
+    const float4* sph_world;
+    const float4* sph_view;
+    const int32_t* order;
+    const unsigned long long* masks;
+    void* image;
+    int words;
+    int tile0;  // first global tile of the scene
+};
+
+// batched cone trace: work items are global tiles in `order` (heavy first);
+// waves_per_group waves per workgroup, `groups` workgroups striding over the
+// items (groups = 0: one workgroup per waves_per_group items)
+void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total_tiles,
+                       int format, int cone_mode, int composite, int waves_per_group, int groups,
+                       unsigned long long* counters, hipStream_t s);
+// binning into a caller-given tile_count array, and the heavy-first order of
+// an array of tile counts (scratch: 64 + ntiles ints)
+void launch_bin_counts(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
+                       unsigned long long* masks, int* tile_count, hipStream_t s);
+void launch_tile_order(const int* tile_count, int ntiles, int* scratch, int* tile_order, hipStream_t s);
+
 // arguments of one cone-trace launch
 struct ConeLaunch {
     GridDesc g;

@@ -747,7 +747,7 @@ __global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, co
         if (fx1 >= 0.0f && fy1 >= 0.0f && fx0 <= (float)(cv.W - 1) && fy0 <= (float)(cv.H - 1)) {
             int x0 = (int)fmaxf(fx0, 0.0f), x1 = (int)fminf(fx1, (float)(cv.W - 1));
             int y0 = (int)fmaxf(fy0, 0.0f), y1 = (int)fminf(fy1, (float)(cv.H - 1));
-            tiles = make_int4(x0 / kTile, x1 / kTile, y0 / kTile, y1 / kTile);
+            tiles = make_int4(x0 / cv.tile_w, x1 / cv.tile_w, y0 / cv.tile_h, y1 / cv.tile_h);
         }
     }
     sph_tiles[j] = tiles;
@@ -1219,24 +1219,19 @@ __device__ __forceinline__ void farfield_features(const VolumeDesc& vd, const vo
 // wave on the matrix cores); a cone whose predicted error, scaled by its
 // remaining transmittance, exceeds cd.err_bound (or every cone when
 // err_bound < 0) falls back to the plain march from step k.
-// The learned variant asks for 3 waves per SIMD: the inline MLP needs ~100
-// registers beside the march state, and without the bound the kernel gets 2.
+// Shade one camera tile: every billboard binned to it, in rank order (front
+// to back for COMP = 1, back to front for COMP = 0), one pixel per lane.
 template <int FMT, int CONE, int COMP, int LEARNED>
-__global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
-                                             const void* __restrict__ vol, const float4* __restrict__ sph_world,
-                                             const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
-                                             const unsigned long long* __restrict__ masks, int words,
-                                             void* __restrict__ image, unsigned long long* __restrict__ counters,
-                                             const int* __restrict__ tile_order, MlpDev mlp) {
-    // One wavefront per 8x8 tile; tiles are dispatched heaviest first
-    // (tile_order), so the costly clump-centre tiles do not form the tail.
-    __shared__ __align__(16) uint16_t xs[LEARNED ? 64 * kMlpInPad : 8];
-    const int lane = threadIdx.x;
-    uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
-    {
-    const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
+__device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
+                                          const ConeDesc& cd, const void* __restrict__ vol,
+                                          const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
+                                          const int32_t* __restrict__ order,
+                                          const unsigned long long* __restrict__ masks, int words,
+                                          void* __restrict__ image, const MlpDev& mlp, uint16_t* xs, int tile,
+                                          int lane, uint32_t& nfrag, uint32_t& nsamp, uint32_t& ntwo,
+                                          uint32_t& nrows, uint32_t& nfall, uint32_t& nevals) {
     const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
-    const int px = tx * kTile + (lane & 7), py = ty * kTile + (lane >> 3);
+    const int px = tx * cv.tile_w + (lane & (cv.tile_w - 1)), py = ty * cv.tile_h + (lane >> cv.tile_shift);
     const bool valid = px < cv.W && py < cv.H;
     const float rx = pixel_ndc(px, cv.W) * cv.kx;
     const float ry = pixel_ndc(py, cv.H) * cv.ky;
@@ -1376,10 +1371,10 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, Volume
                     if (C3 >= cd.alpha_stop) done = true;
                 }
             }
-            if (COMP == 1 && __all(done)) goto finished;
+            if (COMP == 1 && __all(done)) goto tile_done;
         }
     }
-finished:
+tile_done:
     if (valid) {
         char* row = (char*)image + (size_t)py * cd.pitch;
         if (cd.pixfmt == 0) {
@@ -1389,7 +1384,25 @@ finished:
             ((uint32_t*)row)[px] = v;
         }
     }
-    }  // tiles
+}
+
+// The learned variant asks for 3 waves per SIMD: the inline MLP needs ~100
+// registers beside the march state, and without the bound the kernel gets 2.
+template <int FMT, int CONE, int COMP, int LEARNED>
+__global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
+                                             const void* __restrict__ vol, const float4* __restrict__ sph_world,
+                                             const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
+                                             const unsigned long long* __restrict__ masks, int words,
+                                             void* __restrict__ image, unsigned long long* __restrict__ counters,
+                                             const int* __restrict__ tile_order, MlpDev mlp) {
+    // One wavefront per tile; tiles are dispatched heaviest first
+    // (tile_order), so the costly clump-centre tiles do not form the tail.
+    __shared__ __align__(16) uint16_t xs[LEARNED ? 64 * kMlpInPad : 8];
+    const int lane = threadIdx.x;
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
+    const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
+    cone_tile<FMT, CONE, COMP, LEARNED>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp, xs,
+                                        tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals);
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
         unsigned long long e0 = 0, e1 = 0;
@@ -1403,6 +1416,42 @@ finished:
                 atomicAdd(&counters[7], e1);
                 atomicAdd(&counters[8], (unsigned long long)nevals);
             }
+        }
+    }
+}
+
+// Batched cone trace: global tiles of many scenes (heavy first), each wave
+// finds its scene by binary search over the scenes' first tiles.
+template <int FMT, int CONE, int COMP>
+__global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict__ scenes,
+                                                    const int* __restrict__ tile0, int nscenes,
+                                                    const int* __restrict__ gorder, int total,
+                                                    unsigned long long* __restrict__ counters) {
+    const int lane = threadIdx.x & 63;
+    const int wpg = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
+    MlpDev none{nullptr, nullptr, nullptr};
+    for (int item = blockIdx.x * wpg + wave; item < total; item += gridDim.x * wpg) {
+        const int gt = __builtin_amdgcn_readfirstlane(gorder[item]);
+        int lo = 0, hi = nscenes - 1;
+        while (lo < hi) {  // last scene with tile0 <= gt
+            const int mid = (lo + hi + 1) >> 1;
+            if (tile0[mid] <= gt) lo = mid;
+            else hi = mid - 1;
+        }
+        const int sidx = __builtin_amdgcn_readfirstlane(lo);
+        const BatchScene& B = scenes[sidx];
+        cone_tile<FMT, CONE, COMP, 0>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order, B.masks,
+                                      B.words, B.image, none, nullptr, gt - B.tile0, lane, nfrag, nsamp, ntwo, nrows,
+                                      nfall, nevals);
+    }
+    if (counters) {
+        unsigned long long a = wave_sum_u64(nfrag), b = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
+        if (lane == 0) {
+            atomicAdd(&counters[3], a);
+            atomicAdd(&counters[4], b);
+            atomicAdd(&counters[5], c);
         }
     }
 }
@@ -1637,21 +1686,39 @@ int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, 
 
 size_t tile_scratch_ints(int ntiles) { return 66 + 3 * (size_t)ntiles + 2; }
 
-// Binning plus the heavy-first tile order.  tile scratch (ints): hist[32] |
-// cursor[32] | cone tile counter (2) | tile_count[ntiles] | bucket_of | tile_order
-TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
-                     unsigned long long* masks, int* scratch, hipStream_t s) {
-    const int ntiles = tiles_x * tiles_y;
-    int* hist = scratch;
-    int* cursor = scratch + 32;
-    unsigned int* counter = (unsigned int*)(scratch + 64);
-    int* tile_count = scratch + 66;
-    int* bucket_of = tile_count + ntiles;
-    int* tile_order = bucket_of + ntiles;
-    launch_fill_u64((uint64_t*)scratch, (uint64_t)(66 + ntiles + 1) / 2, 0, s);
+void launch_bin_counts(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
+                       unsigned long long* masks, int* tile_count, hipStream_t s) {
     if (S > 0)
         hipLaunchKernelGGL(k_bin, dim3(blocks_for(S, 256)), dim3(256), 0, s, sph_tiles, order, S, tiles_x, words, masks,
                            tile_count);
+}
+
+// heavy-first order of ntiles tile counts; scratch: hist[32] | cursor[32] | bucket_of[ntiles]
+void launch_tile_order(const int* tile_count, int ntiles, int* scratch, int* tile_order, hipStream_t s) {
+    int* hist = scratch;
+    int* cursor = scratch + 32;
+    int* bucket_of = scratch + 64;
+    launch_fill_u64((uint64_t*)scratch, 32, 0, s);
+    hipLaunchKernelGGL(k_tile_bucket, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tile_count, ntiles, bucket_of,
+                       hist);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, bucket_of, ntiles, hist, cursor,
+                       tile_order);
+}
+
+// Binning plus the heavy-first tile order.  tile scratch (ints): hist[32] |
+// cursor[32] | (2 unused) | tile_count[ntiles] | bucket_of | tile_order
+TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
+                     unsigned long long* masks, int* scratch, hipStream_t s) {
+    const int ntiles = tiles_x * tiles_y;
+    unsigned int* counter = (unsigned int*)(scratch + 64);
+    int* tile_count = scratch + 66;
+    int* tile_order = tile_count + 2 * (size_t)ntiles;
+    launch_fill_u64((uint64_t*)tile_count, (uint64_t)(ntiles + 1) / 2, 0, s);
+    launch_bin_counts(sph_tiles, order, S, tiles_x, words, masks, tile_count, s);
+    int* hist = scratch;
+    int* cursor = scratch + 32;
+    int* bucket_of = tile_count + ntiles;
+    launch_fill_u64((uint64_t*)scratch, 32, 0, s);
     hipLaunchKernelGGL(k_tile_bucket, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tile_count, ntiles, bucket_of,
                        hist);
     hipLaunchKernelGGL(k_tile_scatter, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, bucket_of, ntiles, hist, cursor,
@@ -1660,6 +1727,40 @@ TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int til
     q.counter = counter;
     q.order = tile_order;
     return q;
+}
+
+template <int FMT, int CONE, int COMP>
+static void launch_cone_batch_t(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total,
+                                int wpg, int groups, unsigned long long* counters, hipStream_t s) {
+    const int items_per_group = wpg;
+    unsigned grid = groups > 0 ? (unsigned)groups : (unsigned)((total + items_per_group - 1) / items_per_group);
+    if (grid == 0) return;
+    hipLaunchKernelGGL((k_cone_batch<FMT, CONE, COMP>), dim3(grid), dim3(64 * wpg), 0, s, scenes, tile0, nscenes,
+                       order, total, counters);
+}
+
+void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total_tiles,
+                       int format, int cone_mode, int composite, int waves_per_group, int groups,
+                       unsigned long long* counters, hipStream_t s) {
+    if (total_tiles <= 0) return;
+    const bool w = cone_mode == 0;
+    if (format == 1) {
+        if (w) {
+            if (composite == 0) launch_cone_batch_t<1, 0, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+            else launch_cone_batch_t<1, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        } else {
+            if (composite == 0) launch_cone_batch_t<1, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+            else launch_cone_batch_t<1, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        }
+    } else {
+        if (w) {
+            if (composite == 0) launch_cone_batch_t<2, 0, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+            else launch_cone_batch_t<2, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        } else {
+            if (composite == 0) launch_cone_batch_t<2, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+            else launch_cone_batch_t<2, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        }
+    }
 }
 
 template <int FMT, int CONE, int COMP, int LEARNED>

@@ -79,6 +79,7 @@ struct cr_volume_s {
     size_t scratch_cap = 0;
     bool voxelized = false;
     int sorted_count = -1;
+    uint64_t gen = 0;  // bumped by every change of billboards / noise (invalidates captured batches)
 };
 
 // far-field MLP: FP32 master parameters + Adam moments, bf16 packed copies for
@@ -467,6 +468,7 @@ int cr_volume_set_noise(cr_volume v, int octaves, uint32_t seed, float base_freq
     v->vd.noise_base_freq = base_freq;
     v->vd.noise_lacunarity = lacunarity;
     v->vd.noise_gain = gain;
+    v->gen += 1;
     return CR_OK;
 }
 
@@ -496,10 +498,18 @@ int cr_volume_set_billboards(cr_volume v, const float* xyz, const float* scales,
     v->S = count;
     v->max_radius = maxr;
     v->sorted_count = -1;
+    v->gen += 1;
     return CR_OK;
 }
 
-int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, void* stream) {
+// pass-1/pass-2/mip launches of one voxelize call, prepared on the host
+struct VoxPlan {
+    LightView lv;
+    LightScratch ls;
+    size_t npx;
+};
+
+static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int light_h, VoxPlan& plan) {
     if (!v) return fail(nullptr, CR_EINVAL, "cr_voxelize: vol is NULL");
     cr_context ctx = v->ctx;
     if (!light) return fail(ctx, CR_EINVAL, "cr_voxelize: light is NULL");
@@ -518,10 +528,9 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
         return fail(ctx, CR_EUNSUPPORTED, "cr_voxelize: billboard radius exceeds 2^19 voxels (chord fill)");
     int rc = set_device(ctx);
     if (rc) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    LightView lv;
-    sun_update(v, light->position, light_w, light_h, &lv);
+    sun_update(v, light->position, light_w, light_h, &plan.lv);
     size_t npx = (size_t)light_w * light_h;
+    plan.npx = npx;
     if ((rc = ensure(ctx, &v->d_keys, npx, &v->light_cap))) return rc;
     static_assert(sizeof(float4) == 16, "float4");
     {
@@ -534,34 +543,49 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     }
     v->lw = light_w;
     v->lh = light_h;
-    LightScratch ls{};
+    plan.ls = LightScratch{};
     if (v->S > 0) {
         if ((rc = ensure(ctx, &v->d_lbox, (size_t)v->S, &v->lbox_cap))) return rc;
         if ((rc = ensure(ctx, &v->d_lcounts, 2 * (size_t)v->S + 1, &v->lcounts_cap))) return rc;
-        size_t nitems = light_items_bound(v->S, v->max_radius, lv);
+        size_t nitems = light_items_bound(v->S, v->max_radius, plan.lv);
         if ((rc = ensure(ctx, &v->d_litems, nitems, &v->litems_cap))) return rc;
-        ls.lbox = v->d_lbox;
-        ls.counts = v->d_lcounts;
-        ls.offsets = v->d_lcounts + v->S;
-        ls.items = v->d_litems;
-        ls.items_cap = nitems;
+        plan.ls.lbox = v->d_lbox;
+        plan.ls.counts = v->d_lcounts;
+        plan.ls.offsets = v->d_lcounts + v->S;
+        plan.ls.items = v->d_litems;
+        plan.ls.items_cap = nitems;
     }
+    return CR_OK;
+}
+
+// kernels only (no allocation, no host synchronisation): safe under graph capture
+static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
+                             hipStream_t s) {
+    if (ev) ev->mark(0);
+    if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
+    // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187)
+    launch_fill_u64((uint64_t*)v->d_keys, plan.npx, ~0ull, s);
+    launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, s);
+    launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, s);
+    if (ev) ev->mark(1);
+    launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, v->d_fill, counters, s);
+    if (ev) ev->mark(2);
+    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, v->d_posmap, v->d_shell, counters, s);
+    if (ev) ev->mark(3);
+    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
+    if (ev) ev->mark(4);
+}
+
+int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, void* stream) {
+    VoxPlan plan;
+    int rc = voxelize_prepare(v, light, light_w, light_h, plan);
+    if (rc) return rc;
+    cr_context ctx = v->ctx;
+    hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
     StageEvents ev;
     if ((rc = ev.begin(ctx, s, 0))) return rc;
-    ev.mark(0);
-    if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
-    // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187)
-    launch_fill_u64((uint64_t*)v->d_keys, npx, ~0ull, s);
-    launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, s);
-    launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, s);
-    ev.mark(1);
-    launch_light_raster(v->g, lv, v->mode, v->d_boards, v->S, ls, v->d_keys, v->d_fill, counters, s);
-    ev.mark(2);
-    launch_light_splat(v->g, lv, v->d_boards, v->d_keys, v->d_posmap, v->d_shell, counters, s);
-    ev.mark(3);
-    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
-    ev.mark(4);
+    voxelize_enqueue(v, plan, counters, &ev, s);
     ev.end();
     CR_HIP(ctx, hipGetLastError());
     if (ctx->profiling == 1) {
@@ -649,8 +673,21 @@ static int cone_params_check(cr_context ctx, cr_volume v, const cr_cone_params* 
     return CR_OK;
 }
 
-static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
-                           const LearnedOpts* lo, void* d_image, int W, int H, size_t pitch, int pixfmt, void* stream) {
+// descriptors and buffers of one cone-trace call, prepared on the host
+struct ConePlan {
+    CameraView cv;
+    ConeDesc cd;
+    int S;
+    int words;
+    size_t ntiles;
+    size_t nmask;
+};
+
+// validation, camera view, cone schedule and buffer allocation (tile_w x
+// 64 / tile_w camera tiles); the volume's voxelized state is not checked here
+static int cone_prepare(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
+                        const LearnedOpts* lo, void* d_image, int W, int H, size_t pitch, int pixfmt, int tile_w,
+                        ConePlan& plan) {
     if (!v) return fail(nullptr, CR_EINVAL, "cr_cone_trace: vol is NULL");
     cr_context ctx = v->ctx;
     if (!light || !cam || !p || !d_image) return fail(ctx, CR_EINVAL, "cr_cone_trace: NULL argument");
@@ -660,6 +697,8 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     if (pitch < (size_t)W * px_bytes) return fail(ctx, CR_EINVAL, "cr_cone_trace: pitch smaller than a row");
     if (pitch % px_bytes != 0 || ((uintptr_t)d_image % px_bytes) != 0)
         return fail(ctx, CR_EINVAL, "cr_cone_trace: image pointer or pitch misaligned");
+    if (tile_w != 4 && tile_w != 8 && tile_w != 16 && tile_w != 32)
+        return fail(ctx, CR_EINVAL, "cr_cone_trace: tile width must be 4, 8, 16 or 32");
     int rc = cone_params_check(ctx, v, p, "cr_cone_trace");
     if (rc) return rc;
     if (lo) {
@@ -674,13 +713,12 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     if (!(cam->znear > 0.0f) || !(cam->zfar > cam->znear)) return fail(ctx, CR_EINVAL, "cr_cone_trace: need 0 < znear < zfar");
     if (light->directional && !finite3(light->direction)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light direction");
     if (!light->directional && !finite3(light->position)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light position");
-    if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_cone_trace: volume not voxelized");
     rc = set_device(ctx);
     if (rc) return rc;
-    hipStream_t s = (hipStream_t)stream;
 
     // camera view (src/Camera.cpp:59-60)
-    CameraView cv{};
+    CameraView& cv = plan.cv;
+    cv = CameraView{};
     look_at(f3(cam->eye[0], cam->eye[1], cam->eye[2]), f3(cam->target[0], cam->target[1], cam->target[2]),
             f3(cam->up[0], cam->up[1], cam->up[2]), cv.V);
     basis_from_view(cv.V, cv.S, cv.U, cv.B);
@@ -693,11 +731,15 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     cv.zfar = cam->zfar;
     cv.W = W;
     cv.H = H;
-    cv.tiles_x = (W + kTile - 1) / kTile;
-    cv.tiles_y = (H + kTile - 1) / kTile;
+    cv.tile_w = tile_w;
+    cv.tile_h = 64 / tile_w;
+    cv.tile_shift = tile_w == 4 ? 2 : tile_w == 8 ? 3 : tile_w == 16 ? 4 : 5;
+    cv.tiles_x = (W + cv.tile_w - 1) / cv.tile_w;
+    cv.tiles_y = (H + cv.tile_h - 1) / cv.tile_h;
 
     // cone schedule (res/conetrace_frag.glsl:70-76), identical for every ray
-    ConeDesc cd{};
+    ConeDesc& cd = plan.cd;
+    cd = ConeDesc{};
     fill_cone_desc(v, light, p, cd);
     cd.pixfmt = pixfmt;
     cd.pitch = pitch;
@@ -708,6 +750,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
 
     // buffers
     int S = v->S;
+    plan.S = S;
     size_t scap = (size_t)v->sph_cap;
     if (S > 0) {
         size_t c0 = scap, c1 = scap, c2 = scap, c3 = scap;
@@ -718,35 +761,51 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
         v->sph_cap = (int)std::min(std::min(c0, c1), std::min(c2, c3));
         if ((rc = ensure(ctx, &v->d_sort, sort_scratch_words(S), &v->sort_cap))) return rc;
     }
-    int words = (S + 63) / 64;
-    size_t ntiles = (size_t)cv.tiles_x * cv.tiles_y;
-    size_t nmask = ntiles * (size_t)std::max(words, 1);
-    if ((rc = ensure(ctx, &v->d_masks, nmask, &v->masks_cap))) return rc;
-    if ((rc = ensure(ctx, &v->d_tiles, tile_scratch_ints((int)ntiles), &v->tiles_cap))) return rc;
+    plan.words = (S + 63) / 64;
+    plan.ntiles = (size_t)cv.tiles_x * cv.tiles_y;
+    plan.nmask = plan.ntiles * (size_t)std::max(plan.words, 1);
+    if ((rc = ensure(ctx, &v->d_masks, plan.nmask, &v->masks_cap))) return rc;
+    if ((rc = ensure(ctx, &v->d_tiles, tile_scratch_ints((int)plan.ntiles), &v->tiles_cap))) return rc;
+    return CR_OK;
+}
 
+// camera setup + draw order (kernels only)
+static void cone_enqueue_sort(cr_volume v, const ConePlan& plan, StageEvents* ev, hipStream_t s) {
+    launch_cam_setup(v->g, plan.cv, v->d_boards, plan.S, v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort, s);
+    if (ev) ev->mark(1);
+    launch_sort(v->d_sort, plan.S, v->d_order, v->d_tie, s);
+    if (ev) ev->mark(2);
+    launch_fill_u64((uint64_t*)v->d_masks, plan.nmask, 0, s);
+}
+
+static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
+                           const LearnedOpts* lo, void* d_image, int W, int H, size_t pitch, int pixfmt, void* stream) {
+    ConePlan plan;
+    int rc = cone_prepare(v, light, cam, p, lo, d_image, W, H, pitch, pixfmt, kTile, plan);
+    if (rc) return rc;
+    cr_context ctx = v->ctx;
+    if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_cone_trace: volume not voxelized");
+    hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
     StageEvents ev;
     if ((rc = ev.begin(ctx, s, 1))) return rc;
     ev.mark(0);
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
-    launch_cam_setup(v->g, cv, v->d_boards, S, v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort, s);
-    ev.mark(1);
-    launch_sort(v->d_sort, S, v->d_order, v->d_tie, s);
-    ev.mark(2);
-    launch_fill_u64((uint64_t*)v->d_masks, nmask, 0, s);
-    TileQueue tq = launch_bin(v->d_sph_tiles, v->d_order, S, cv.tiles_x, cv.tiles_y, words, v->d_masks, v->d_tiles, s);
+    cone_enqueue_sort(v, plan, &ev, s);
+    TileQueue tq = launch_bin(v->d_sph_tiles, v->d_order, plan.S, plan.cv.tiles_x, plan.cv.tiles_y, plan.words,
+                              v->d_masks, v->d_tiles, s);
     ev.mark(3);
     ConeLaunch cl{};
     cl.g = v->g;
     cl.vd = v->vd;
-    cl.cv = cv;
-    cl.cd = cd;
+    cl.cv = plan.cv;
+    cl.cd = plan.cd;
     cl.levels = v->d_levels;
     cl.sph_world = v->d_sph_world;
     cl.sph_view = v->d_sph_view;
     cl.order = v->d_order;
     cl.masks = v->d_masks;
-    cl.words = words;
+    cl.words = plan.words;
     cl.image = d_image;
     cl.counters = counters;
     cl.tile_order = tq.order;
@@ -756,7 +815,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     ev.mark(4);
     ev.end();
     CR_HIP(ctx, hipGetLastError());
-    v->sorted_count = S;
+    v->sorted_count = plan.S;
     if (ctx->profiling == 1) {
         CR_HIP(ctx, hipStreamSynchronize(s));
         unsigned long long c[9];
@@ -1047,6 +1106,314 @@ int cr_mlp_last_loss(cr_mlp m, float* loss, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     CR_HIP(ctx, hipMemcpyAsync(loss, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, s));
     CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched scenes: one hipGraph of every scene's voxelize / sort / bin and a
+// single cone-trace launch over all scenes' tiles
+// ---------------------------------------------------------------------------
+struct cr_batch_s {
+    cr_context ctx = nullptr;
+    std::vector<cr_batch_item> items;
+    int format = 0, cone_mode = 0, composite = 0;
+    cr_batch_config cfg{8, 1, 0};
+    BatchScene* d_scenes = nullptr;
+    size_t scenes_cap = 0;
+    int* d_tile0 = nullptr;
+    size_t tile0_cap = 0;
+    int* d_counts = nullptr;
+    size_t counts_cap = 0;
+    int* d_order = nullptr;
+    size_t order_cap = 0;
+    int* d_oscratch = nullptr;
+    size_t oscratch_cap = 0;
+    unsigned long long* d_counters = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool have_graph = false;
+    cr_batch_config graph_cfg{0, 0, 0};
+    std::vector<uint64_t> graph_gen;
+    int total_tiles = 0;
+};
+
+namespace {
+
+bool same_cfg(const cr_batch_config& a, const cr_batch_config& b) {
+    return a.tile_w == b.tile_w && a.waves_per_group == b.waves_per_group && a.groups == b.groups;
+}
+
+void drop_graph(cr_batch b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->have_graph = false;
+}
+
+// host preparation (allocations, descriptor upload), then the kernels
+int batch_prepare(cr_batch b, std::vector<VoxPlan>& vp, std::vector<ConePlan>& cp) {
+    cr_context ctx = b->ctx;
+    const size_t n = b->items.size();
+    vp.resize(n);
+    cp.resize(n);
+    std::vector<BatchScene> hs(n);
+    std::vector<int> tile0(n);
+    int total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const cr_batch_item& it = b->items[i];
+        cr_volume v = it.volume;
+        int rc = voxelize_prepare(v, &it.light, it.light_w, it.light_h, vp[i]);
+        if (rc) return rc;
+        rc = cone_prepare(v, &it.light, &it.camera, &it.params, nullptr, it.d_image, it.width, it.height,
+                          it.pitch_bytes, it.pixel_format, b->cfg.tile_w, cp[i]);
+        if (rc) return rc;
+        if (cp[i].ntiles > (size_t)(1 << 30) - (size_t)total)
+            return fail(ctx, CR_EUNSUPPORTED, "cr_batch: too many tiles");
+        BatchScene& B = hs[i];
+        B.g = v->g;
+        B.vd = v->vd;
+        B.cv = cp[i].cv;
+        B.cd = cp[i].cd;
+        B.levels = v->d_levels;
+        B.sph_world = v->d_sph_world;
+        B.sph_view = v->d_sph_view;
+        B.order = v->d_order;
+        B.masks = v->d_masks;
+        B.image = it.d_image;
+        B.words = cp[i].words;
+        B.tile0 = total;
+        tile0[i] = total;
+        total += (int)cp[i].ntiles;
+    }
+    b->total_tiles = total;
+    int rc;
+    if ((rc = ensure(ctx, &b->d_scenes, n, &b->scenes_cap))) return rc;
+    if ((rc = ensure(ctx, &b->d_tile0, n, &b->tile0_cap))) return rc;
+    if ((rc = ensure(ctx, &b->d_counts, (size_t)total + 2, &b->counts_cap))) return rc;
+    if ((rc = ensure(ctx, &b->d_order, (size_t)total, &b->order_cap))) return rc;
+    if ((rc = ensure(ctx, &b->d_oscratch, 64 + (size_t)total, &b->oscratch_cap))) return rc;
+    if (!b->d_counters) {
+        size_t cap = 0;
+        if ((rc = ensure(ctx, &b->d_counters, 16, &cap))) return rc;
+    }
+    CR_HIP(ctx, hipMemcpy(b->d_scenes, hs.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice));
+    CR_HIP(ctx, hipMemcpy(b->d_tile0, tile0.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    return CR_OK;
+}
+
+// kernels only: voxelize every scene, camera setup / sort / bin every scene,
+// one heavy-first order over all tiles, one batched cone trace
+void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector<ConePlan>& cp,
+                   unsigned long long* counters, hipEvent_t* ev, hipStream_t s) {
+    const size_t n = b->items.size();
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
+    for (size_t i = 0; i < n; ++i) voxelize_enqueue(b->items[i].volume, vp[i], nullptr, nullptr, s);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    launch_fill_u64((uint64_t*)b->d_counts, (uint64_t)(b->total_tiles + 1) / 2, 0, s);
+    int t0 = 0;
+    for (size_t i = 0; i < n; ++i) {
+        cr_volume v = b->items[i].volume;
+        cone_enqueue_sort(v, cp[i], nullptr, s);
+        launch_bin_counts(v->d_sph_tiles, v->d_order, cp[i].S, cp[i].cv.tiles_x, cp[i].words, v->d_masks,
+                          b->d_counts + t0, s);
+        t0 += (int)cp[i].ntiles;
+    }
+    launch_tile_order(b->d_counts, b->total_tiles, b->d_oscratch, b->d_order, s);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    launch_cone_batch(b->d_scenes, b->d_tile0, (int)n, b->d_order, b->total_tiles, b->format, b->cone_mode,
+                      b->composite, b->cfg.waves_per_group, b->cfg.groups, counters, s);
+    if (ev) (void)hipEventRecord(ev[3], s);
+}
+
+bool graph_current(cr_batch b) {
+    if (!b->have_graph || !same_cfg(b->cfg, b->graph_cfg)) return false;
+    for (size_t i = 0; i < b->items.size(); ++i)
+        if (b->items[i].volume->gen != b->graph_gen[i]) return false;
+    return true;
+}
+
+int batch_capture(cr_batch b) {
+    cr_context ctx = b->ctx;
+    drop_graph(b);
+    std::vector<VoxPlan> vp;
+    std::vector<ConePlan> cp;
+    int rc = batch_prepare(b, vp, cp);
+    if (rc) return rc;
+    CR_HIP(ctx, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    batch_enqueue(b, vp, cp, nullptr, nullptr, b->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (e != hipSuccess) return fail(ctx, CR_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        return fail(ctx, CR_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    b->have_graph = true;
+    b->graph_cfg = b->cfg;
+    b->graph_gen.resize(b->items.size());
+    for (size_t i = 0; i < b->items.size(); ++i) b->graph_gen[i] = b->items[i].volume->gen;
+    return CR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cr_batch_create(cr_context ctx, const cr_batch_item* items, int count, cr_batch* out) {
+    if (!ctx || !out || !items) return fail(ctx, CR_EINVAL, "cr_batch_create: NULL argument");
+    *out = nullptr;
+    if (count < 1 || count > 4096) return fail(ctx, CR_EINVAL, "cr_batch_create: count must be in [1, 4096]");
+    for (int i = 0; i < count; ++i) {
+        const cr_batch_item& it = items[i];
+        if (!it.volume || it.volume->ctx != ctx) return fail(ctx, CR_EINVAL, "cr_batch_create: volume of another context or NULL");
+        if (it.volume->format == CR_FORMAT_R8) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: R8 volumes are not batched");
+        if (it.volume->format != items[0].volume->format || it.params.cone_mode != items[0].params.cone_mode ||
+            it.params.composite != items[0].params.composite)
+            return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: scenes must share format, cone mode and composite");
+        for (int j = 0; j < i; ++j)
+            if (items[j].volume == it.volume) return fail(ctx, CR_EINVAL, "cr_batch_create: a volume appears twice");
+    }
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    cr_batch b = new (std::nothrow) cr_batch_s();
+    if (!b) return fail(ctx, CR_ENOMEM, "cr_batch_create: host allocation failed");
+    b->ctx = ctx;
+    b->items.assign(items, items + count);
+    b->format = items[0].volume->format;
+    b->cone_mode = items[0].params.cone_mode;
+    b->composite = items[0].params.composite;
+    hipError_t e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(ctx, CR_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    // validate every scene once (and allocate its buffers)
+    std::vector<VoxPlan> vp;
+    std::vector<ConePlan> cp;
+    if ((rc = batch_prepare(b, vp, cp))) {
+        cr_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return CR_OK;
+}
+
+int cr_batch_destroy(cr_batch b) {
+    if (!b) return CR_OK;
+    (void)hipSetDevice(b->ctx->device);
+    drop_graph(b);
+    void* ptrs[] = {b->d_scenes, b->d_tile0, b->d_counts, b->d_order, b->d_oscratch, b->d_counters};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+    return CR_OK;
+}
+
+int cr_batch_set_config(cr_batch b, const cr_batch_config* cfg) {
+    if (!b || !cfg) return fail(b ? b->ctx : nullptr, CR_EINVAL, "cr_batch_set_config: NULL argument");
+    if (cfg->tile_w != 4 && cfg->tile_w != 8 && cfg->tile_w != 16 && cfg->tile_w != 32)
+        return fail(b->ctx, CR_EINVAL, "cr_batch_set_config: tile_w must be 4, 8, 16 or 32");
+    if (cfg->waves_per_group != 1 && cfg->waves_per_group != 2 && cfg->waves_per_group != 4)
+        return fail(b->ctx, CR_EINVAL, "cr_batch_set_config: waves_per_group must be 1, 2 or 4");
+    if (cfg->groups < 0 || cfg->groups > (1 << 20)) return fail(b->ctx, CR_EINVAL, "cr_batch_set_config: bad group count");
+    b->cfg = *cfg;
+    return CR_OK;
+}
+
+int cr_batch_get_config(cr_batch b, cr_batch_config* cfg) {
+    if (!b || !cfg) return fail(b ? b->ctx : nullptr, CR_EINVAL, "cr_batch_get_config: NULL argument");
+    *cfg = b->cfg;
+    return CR_OK;
+}
+
+int cr_batch_run(cr_batch b, void* stream) {
+    if (!b) return fail(nullptr, CR_EINVAL, "cr_batch_run: batch is NULL");
+    cr_context ctx = b->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (ctx->profiling == 1) {
+        // eager run with counters and phase times
+        std::vector<VoxPlan> vp;
+        std::vector<ConePlan> cp;
+        if ((rc = batch_prepare(b, vp, cp))) return rc;
+        hipEvent_t ev[4];
+        for (int i = 0; i < 4; ++i) CR_HIP(ctx, hipEventCreate(&ev[i]));
+        batch_enqueue(b, vp, cp, b->d_counters, ev, s);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        float t[3] = {0, 0, 0};
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
+        unsigned long long c[9] = {};
+        if (e == hipSuccess) e = hipMemcpy(c, b->d_counters, sizeof(c), hipMemcpyDeviceToHost);
+        for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]);
+        if (e != hipSuccess) return fail(ctx, CR_EDEVICE, "cr_batch_run: %s", hipGetErrorString(e));
+        ctx->stats = cr_stats{};
+        ctx->stats.fragments = (int64_t)c[3];
+        ctx->stats.samples = (int64_t)c[4];
+        ctx->stats.two_level = (int64_t)c[5];
+        ctx->stats.ms_setup = t[0];
+        ctx->stats.ms_bin = t[1];
+        ctx->stats.ms_cone = t[2];
+        ctx->stats.n_cone = 1;
+    } else {
+        if (!graph_current(b) && (rc = batch_capture(b))) return rc;
+        CR_HIP(ctx, hipGraphLaunch(b->exec, s));
+    }
+    for (auto& it : b->items) {
+        it.volume->voxelized = true;
+        it.volume->sorted_count = it.volume->S;
+    }
+    return CR_OK;
+}
+
+int cr_batch_autotune(cr_batch b, int repeats, void* stream, cr_batch_config* best, float* best_ms) {
+    if (!b) return fail(nullptr, CR_EINVAL, "cr_batch_autotune: batch is NULL");
+    cr_context ctx = b->ctx;
+    if (repeats < 1 || repeats > 1000) return fail(ctx, CR_EINVAL, "cr_batch_autotune: repeats must be in [1, 1000]");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    int saved_prof = ctx->profiling;
+    ctx->profiling = 0;
+    hipEvent_t e0, e1;
+    CR_HIP(ctx, hipEventCreate(&e0));
+    CR_HIP(ctx, hipEventCreate(&e1));
+    const int tws[3] = {8, 16, 4};
+    const int wpgs[3] = {1, 2, 4};
+    const int groups[2] = {0, 2048};
+    cr_batch_config bestc = b->cfg;
+    float bestt = 1e30f;
+    for (int a = 0; a < 3 && rc == CR_OK; ++a)
+        for (int w = 0; w < 3 && rc == CR_OK; ++w)
+            for (int g = 0; g < 2 && rc == CR_OK; ++g) {
+                b->cfg = cr_batch_config{tws[a], wpgs[w], groups[g] / wpgs[w]};
+                if ((rc = cr_batch_run(b, s))) break;  // capture + warm-up
+                hipError_t e = hipEventRecord(e0, s);
+                for (int r = 0; r < repeats && e == hipSuccess; ++r) e = hipGraphLaunch(b->exec, s);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) { rc = fail(ctx, CR_EDEVICE, "cr_batch_autotune: %s", hipGetErrorString(e)); break; }
+                ms /= (float)repeats;
+                if (ms < bestt) { bestt = ms; bestc = b->cfg; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->profiling = saved_prof;
+    if (rc) return rc;
+    b->cfg = bestc;
+    if (best) *best = bestc;
+    if (best_ms) *best_ms = bestt;
     return CR_OK;
 }
 

@@ -72,6 +72,7 @@ extern "C" {
 typedef struct cr_context_s *cr_context;
 typedef struct cr_volume_s *cr_volume;
 typedef struct cr_mlp_s *cr_mlp;
+typedef struct cr_batch_s *cr_batch;
 
 /* Sun (src/Sun.hpp:12-19, src/main.cpp:37).  position drives the light view
  * of the voxelization exactly as Sun::update does; the cone direction is
@@ -235,6 +236,50 @@ int cr_volume_segments(cr_volume vol, const cr_light *light, const cr_cone_param
 int cr_cone_trace_learned(cr_volume vol, const cr_light *light, const cr_camera *camera,
                           const cr_cone_params *params, cr_mlp mlp, int k, float err_bound, void *d_image,
                           int width, int height, size_t pitch_bytes, int pixel_format, void *stream);
+
+/* ---- batched scenes ---------------------------------------------------- */
+/* No counterpart in the reference (one scene per frame, src/main.cpp:98-124).
+ * Many independent scenes rendered by one hipGraph: every scene's voxelize
+ * and draw-order / binning passes, then a single cone-trace launch over the
+ * camera tiles of all scenes, heaviest tiles first.  Volumes are created and
+ * given billboards by the caller (distinct volumes, one context, RGBA32F or
+ * RGBA16F, shared cone mode and composite); the graph is captured on the
+ * first run and captured again when a configuration or a volume's billboards
+ * or noise change.  Results equal per-scene cr_voxelize + cr_cone_trace. */
+typedef struct {
+    cr_volume volume;
+    cr_light light;
+    int light_w, light_h;
+    cr_camera camera;
+    cr_cone_params params;
+    void *d_image;
+    int width, height;
+    size_t pitch_bytes;
+    int pixel_format;
+} cr_batch_item;
+
+/* launch shape: camera tiles of tile_w x (64 / tile_w) pixels (one wave per
+ * tile), waves_per_group waves per workgroup, and `groups` persistent
+ * workgroups striding over the tiles (0: one workgroup per waves_per_group
+ * tiles).  Default {8, 1, 0}. */
+typedef struct {
+    int tile_w;          /* 4, 8, 16 or 32 */
+    int waves_per_group; /* 1, 2 or 4 */
+    int groups;          /* 0 or the persistent grid size */
+} cr_batch_config;
+
+int cr_batch_create(cr_context ctx, const cr_batch_item *items, int count, cr_batch *out);
+int cr_batch_destroy(cr_batch batch);
+int cr_batch_set_config(cr_batch batch, const cr_batch_config *config);
+int cr_batch_get_config(cr_batch batch, cr_batch_config *config);
+/* Profiling level 1: eager run with counters; ms_setup = voxelize of all
+ * scenes, ms_bin = camera setup + sort + binning of all scenes, ms_cone = the
+ * batched cone trace.  Otherwise the graph is launched on `stream`. */
+int cr_batch_run(cr_batch batch, void *stream);
+/* Times every launch shape (tile_w 8/16/4 x waves 1/2/4 x one workgroup per
+ * tile or 2048 persistent waves), `repeats` graph launches each, keeps the
+ * fastest and returns it with its time per run. */
+int cr_batch_autotune(cr_batch batch, int repeats, void *stream, cr_batch_config *best, float *best_ms);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,7 @@ def test_struct_layouts_match_header(cv):
     assert C.sizeof(_abi.cr_camera) == 12 * 4
     assert C.sizeof(_abi.cr_cone_params) == 15 * 4
     assert C.sizeof(_abi.cr_stats) == 5 * 8 + 8 * 4 + 6 * 8
+    assert C.sizeof(_abi.cr_batch_config) == 3 * 4
 
 
 def _has_gpu():
