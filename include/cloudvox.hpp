@@ -182,9 +182,7 @@ class ConeTraceShader {
     bool compensated = false;
     std::array<float, 4> clearColor{0.2f, 0.3f, 0.5f, 1.0f};  // src/main.cpp:112
 
-    // ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82) into a device image
-    void coneTrace(CloudVolume* volume, const Sun& sun, const Camera& camera, void* d_image, int width, int height,
-                   size_t pitch, int pixelFormat = CR_PIXEL_RGBA32F, void* stream = nullptr) const {
+    cr_cone_params params() const {
         cr_cone_params p{};
         p.steps = vctSteps;
         p.cone_angle = vctConeAngle;
@@ -198,11 +196,72 @@ class ConeTraceShader {
         p.quantize_rgba8 = quantizeRGBA8 ? 1 : 0;
         p.compensated = compensated ? 1 : 0;
         for (int q = 0; q < 4; ++q) p.clear_color[q] = clearColor[q];
+        return p;
+    }
+
+    // ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82) into a device image
+    void coneTrace(CloudVolume* volume, const Sun& sun, const Camera& camera, void* d_image, int width, int height,
+                   size_t pitch, int pixelFormat = CR_PIXEL_RGBA32F, void* stream = nullptr) const {
+        cr_cone_params p = params();
         cr_light l = sun.c();
         cr_camera k = camera.c();
         check(cr_cone_trace(volume->handle(), &l, &k, &p, d_image, width, height, pitch, pixelFormat, stream),
               volume->context());
     }
+
+    // coneTrace with the far field after k samples predicted by `mlp`
+    void coneTraceLearned(CloudVolume* volume, const Sun& sun, const Camera& camera, cr_mlp mlp, int k,
+                          float errBound, void* d_image, int width, int height, size_t pitch,
+                          int pixelFormat = CR_PIXEL_RGBA32F, void* stream = nullptr) const {
+        cr_cone_params p = params();
+        cr_light l = sun.c();
+        cr_camera c = camera.c();
+        check(cr_cone_trace_learned(volume->handle(), &l, &c, &p, mlp, k, errBound, d_image, width, height, pitch,
+                                    pixelFormat, stream),
+              volume->context());
+    }
+};
+
+// The learned far field (cr_mlp): parameters in the CR_MLP_PARAMS layout.
+class FarFieldMLP {
+  public:
+    explicit FarFieldMLP(Context& ctx) : ctx_(ctx.handle()) { check(cr_mlp_create(ctx_, &h_), ctx_); }
+    FarFieldMLP(const FarFieldMLP&) = delete;
+    FarFieldMLP& operator=(const FarFieldMLP&) = delete;
+    ~FarFieldMLP() { cr_mlp_destroy(h_); }
+    void setParams(const std::vector<float>& p) { check(cr_mlp_set_params(h_, p.data(), (int)p.size()), ctx_); }
+    std::vector<float> params() const {
+        std::vector<float> p(CR_MLP_PARAMS);
+        check(cr_mlp_get_params(h_, p.data(), (int)p.size()), ctx_);
+        return p;
+    }
+    cr_mlp handle() const { return h_; }
+
+  private:
+    cr_context ctx_;
+    cr_mlp h_ = nullptr;
+};
+
+// Many scenes in one hipGraph (cr_batch).
+class Batch {
+  public:
+    Batch(Context& ctx, const std::vector<cr_batch_item>& items) : ctx_(ctx.handle()) {
+        check(cr_batch_create(ctx_, items.data(), (int)items.size(), &h_), ctx_);
+    }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    ~Batch() { cr_batch_destroy(h_); }
+    void run(void* stream = nullptr) { check(cr_batch_run(h_, stream), ctx_); }
+    cr_batch_config autotune(int repeats = 3, void* stream = nullptr) {
+        cr_batch_config best{};
+        float ms = 0.0f;
+        check(cr_batch_autotune(h_, repeats, stream, &best, &ms), ctx_);
+        return best;
+    }
+
+  private:
+    cr_context ctx_;
+    cr_batch h_ = nullptr;
 };
 
 }  // namespace cr

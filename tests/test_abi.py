@@ -95,7 +95,13 @@ def test_python_layer_fails_loudly_without_library(tmp_path, monkeypatch):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
 def test_cpp_facade_compiles():
     hpp = REPO / "include" / "cloudvox.hpp"
-    src = "#include \"cloudvox.hpp\"\nint main() { cr::Context* c = nullptr; (void)c; return 0; }\n"
+    src = ("#include \"cloudvox.hpp\"\n"
+           "void use(cr::Context& c, cr::CloudVolume& v) {\n"
+           "  cr::FarFieldMLP m(c); cr::ConeTraceShader t; cr::Sun s; cr::Camera k;\n"
+           "  t.coneTraceLearned(&v, s, k, m.handle(), 32, 2e-3f, nullptr, 8, 8, 128);\n"
+           "  std::vector<cr_batch_item> items(1); cr::Batch b(c, items); b.run(); (void)b.autotune();\n"
+           "}\n"
+           "int main() { return 0; }\n")
     p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", str(hpp.parent), "-x", "c++",
                         "-"], input=src, text=True, capture_output=True)
     assert p.returncode == 0, p.stderr
