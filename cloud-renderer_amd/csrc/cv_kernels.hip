@@ -7,7 +7,9 @@
 //
 // Memory layout (HBM):
 //   * volume levels: 4x4x4 bricks, Morton order inside a brick (x0 y0 z0 x1 y1 z1),
-//     bricks x-fastest.  A 2x2x2 trilinear footprint is one 128 B line for
+//     bricks x-fastest.  RGBA texels are stored in the channel order
+//     (r, a, g, b): light and density, which grey (rgba-mode) volumes need
+//     alone, are one aligned 8-byte (FP32) or 4-byte (FP16) load.  A 2x2x2 trilinear footprint is one 128 B line for
 //     RGBA32F texels whenever it is brick-aligned, and the 8 corner addresses
 //     are sums of per-axis terms.
 //   * occupancy: one 64-bit word per level-0 brick for the lit shell and one
@@ -510,11 +512,11 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
     if (vd.format == 0) {
         ((uint8_t*)levels)[idx] = r != 0.0f ? (uint8_t)255 : (uint8_t)0;
     } else if (vd.format == 1) {
-        ((float4*)levels)[idx] = make_float4(r, gg, b, a);
+        ((float4*)levels)[idx] = make_float4(r, a, gg, b);
     } else {
         uint2 v;
-        v.x = f2h(r) | (f2h(gg) << 16);
-        v.y = f2h(b) | (f2h(a) << 16);
+        v.x = f2h(r) | (f2h(a) << 16);
+        v.y = f2h(gg) | (f2h(b) << 16);
         ((uint2*)levels)[idx] = v;
     }
 }
@@ -564,11 +566,11 @@ __device__ __forceinline__ void store_texel(void* levels, uint64_t idx, float4 v
     if (FMT == 0) {
         ((uint8_t*)levels)[idx] = (uint8_t)(uint32_t)v.x;
     } else if (FMT == 1) {
-        ((float4*)levels)[idx] = v;
+        ((float4*)levels)[idx] = make_float4(v.x, v.w, v.y, v.z);  // stored (r, a, g, b)
     } else {
         uint2 u;
-        u.x = f2h(v.x) | (f2h(v.y) << 16);
-        u.y = f2h(v.z) | (f2h(v.w) << 16);
+        u.x = f2h(v.x) | (f2h(v.w) << 16);
+        u.y = f2h(v.y) | (f2h(v.z) << 16);
         ((uint2*)levels)[idx] = u;
     }
 }
@@ -689,6 +691,7 @@ __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restr
         ((uint8_t*)levels)[out] = (uint8_t)((sum + (uint32_t)cnt / 2u) / (uint32_t)cnt);
         return;
     }
+    // every stored channel slot is averaged on its own (slot order is irrelevant)
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
     for (int cz = z0; cz <= z1; ++cz)
         for (int cy = y0; cy <= y1; ++cy)
@@ -985,22 +988,27 @@ __global__ void k_sort_replay_global(int S, const uint32_t* __restrict__ g_idx, 
 }
 
 // ---------------------------------------------------------------------------
-// binning: set bit `rank` in every tile a billboard's box overlaps
+// binning: set bit `rank` in every tile a billboard's box overlaps; one
+// wave per billboard, lanes striding over its tile box (a near billboard
+// covers ~1000 tiles, which one thread would walk alone)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles, const int32_t* __restrict__ order, int S,
                                              int tiles_x, int words, unsigned long long* __restrict__ masks,
                                              int* __restrict__ tile_count) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (r >= S) return;
-    int4 t = sph_tiles[order[r]];
+    const int4 t = sph_tiles[order[r]];
     if (t.x > t.y || t.z > t.w) return;
-    unsigned long long bit = 1ull << (r & 63);
-    int w = r >> 6;
-    for (int ty = t.z; ty <= t.w; ++ty)
-        for (int tx = t.x; tx <= t.y; ++tx) {
-            atomicOr(&masks[((size_t)ty * tiles_x + tx) * words + w], bit);
-            atomicAdd(&tile_count[ty * tiles_x + tx], 1);
-        }
+    const int nx = t.y - t.x + 1;
+    const int n = nx * (t.w - t.z + 1);
+    const unsigned long long bit = 1ull << (r & 63);
+    const int w = r >> 6;
+    for (int k = lane; k < n; k += 64) {
+        const int ty = t.z + k / nx, tx = t.x + k % nx;
+        atomicOr(&masks[((size_t)ty * tiles_x + tx) * words + w], bit);
+        atomicAdd(&tile_count[ty * tiles_x + tx], 1);
+    }
 }
 
 // Longest job first: tiles are bucketed by log2(number of binned billboards)
@@ -1044,11 +1052,22 @@ __global__ void __launch_bounds__(256) k_tile_scatter(const int* __restrict__ bu
 // ---------------------------------------------------------------------------
 // cone trace (res/conetrace_frag.glsl:64-79, 176-200) + compositing
 // ---------------------------------------------------------------------------
+// RGBA of one texel (stored as r, a, g, b)
 template <int FMT>
 __device__ __forceinline__ float4 fetch4(const void* __restrict__ vol, uint64_t i) {
-    if (FMT == 1) return ((const float4*)vol)[i];
+    if (FMT == 1) {
+        float4 t = ((const float4*)vol)[i];
+        return make_float4(t.x, t.z, t.w, t.y);
+    }
     uint2 u = ((const uint2*)vol)[i];
-    return make_float4(h2f(u.x & 0xffffu), h2f(u.x >> 16), h2f(u.y & 0xffffu), h2f(u.y >> 16));
+    return make_float4(h2f(u.x & 0xffffu), h2f(u.y & 0xffffu), h2f(u.y >> 16), h2f(u.x >> 16));
+}
+// (r, a) of one RGBA texel
+template <int FMT>
+__device__ __forceinline__ float2 fetch_ra(const void* __restrict__ vol, uint64_t i) {
+    if (FMT == 1) return ((const float2*)vol)[2 * i];
+    uint32_t u = ((const uint32_t*)vol)[2 * i];
+    return make_float2(h2f(u & 0xffffu), h2f(u >> 16));
 }
 template <int FMT>
 __device__ __forceinline__ float fetch1(const void* __restrict__ vol, uint64_t i) {
@@ -1089,6 +1108,24 @@ __device__ __forceinline__ float4 trilinear(const void* __restrict__ vol, const 
         float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);
         return make_float4(lerpF(c0, c1, az), 0.0f, 0.0f, 0.0f);
     }
+    if (NCH == 2) {
+        // grey texels (r = g = b): filter r and a, copy r to g and b
+        float2 t000 = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z0));
+        float2 t010 = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z0));
+        float2 t001 = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z1));
+        float2 t011 = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z1));
+        float2 o2;
+#define CV_TRI2(ch)                                                                    \
+    {                                                                                  \
+        float c00 = lerpF(t000.ch, t100.ch, ax), c10 = lerpF(t010.ch, t110.ch, ax);     \
+        float c01 = lerpF(t001.ch, t101.ch, ax), c11 = lerpF(t011.ch, t111.ch, ax);     \
+        float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);                      \
+        o2.ch = lerpF(c0, c1, az);                                                     \
+    }
+        CV_TRI2(x) CV_TRI2(y)
+#undef CV_TRI2
+        return make_float4(o2.x, o2.x, o2.x, o2.y);
+    }
     float4 t000 = fetch4<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch4<FMT>(vol, base + (X1 + Y0 + Z0));
     float4 t010 = fetch4<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch4<FMT>(vol, base + (X1 + Y1 + Z0));
     float4 t001 = fetch4<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch4<FMT>(vol, base + (X1 + Y0 + Z1));
@@ -1124,6 +1161,10 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
             o.y = lerpF(o.y, o2.y, f);
             o.z = lerpF(o.z, o2.z, f);
             o.w = lerpF(o.w, o2.w, f);
+        } else if (NCH == 2) {
+            o.y = o.x;
+            o.z = o.x;
+            o.w = lerpF(o.w, o2.w, f);
         }
     }
     return o;
@@ -1141,7 +1182,10 @@ struct FtbState {
     float k0, k1, k2, ka;
 };
 
-template <int FMT>
+// GREY: the volume's texels have r = g = b (rgba mode: every store is
+// (1,1,1,1) or (0,0,0,1), mips average channels alike, noise scales only a),
+// so only r and a are filtered -- bit-identical to filtering all four.
+template <int FMT, bool GREY>
 __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
                                          float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
                                          int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
@@ -1150,7 +1194,7 @@ __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __res
         float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dz, p0z);
         if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) return 1;
         bool two;
-        float4 sm = sample_lod<FMT, 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
+        float4 sm = sample_lod<FMT, GREY ? 2 : 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
         ntwo += two ? 1u : 0u;
         float al;
         if (cd.cone_mode == 1) {
@@ -1221,7 +1265,7 @@ __device__ __forceinline__ void farfield_features(const VolumeDesc& vd, const vo
 // err_bound < 0) falls back to the plain march from step k.
 // Shade one camera tile: every billboard binned to it, in rank order (front
 // to back for COMP = 1, back to front for COMP = 0), one pixel per lane.
-template <int FMT, int CONE, int COMP, int LEARNED>
+template <int FMT, int CONE, int COMP, int LEARNED, bool GREY>
 __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
                                           const ConeDesc& cd, const void* __restrict__ vol,
                                           const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
@@ -1306,7 +1350,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                         nsamp += (uint32_t)cd.steps;
                         s0 = s1 = s2 = s3 = clamp01(color);
                     } else {
-                        int rs = march_ftb<FMT>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, 0, kk_steps, st, nsamp, ntwo);
+                        int rs = march_ftb<FMT, GREY>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, 0, kk_steps, st, nsamp, ntwo);
                         alive = LEARNED && rs == 0 && kk_steps < cd.steps;
                     }
                 }
@@ -1339,7 +1383,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                         const float rem = 1.0f - st.a;
                         if (cd.err_bound < 0.0f || !(rem * y[4] <= cd.err_bound)) {
                             ++nfall;
-                            march_ftb<FMT>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, kk_steps, cd.steps, st, nsamp, ntwo);
+                            march_ftb<FMT, GREY>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, kk_steps, cd.steps, st, nsamp, ntwo);
                         } else {
                             st.c0 = fmaf(rem, clamp01(y[0]), st.c0);
                             st.c1 = fmaf(rem, clamp01(y[1]), st.c1);
@@ -1388,7 +1432,7 @@ tile_done:
 
 // The learned variant asks for 3 waves per SIMD: the inline MLP needs ~100
 // registers beside the march state, and without the bound the kernel gets 2.
-template <int FMT, int CONE, int COMP, int LEARNED>
+template <int FMT, int CONE, int COMP, int LEARNED, bool GREY>
 __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
                                              const void* __restrict__ vol, const float4* __restrict__ sph_world,
                                              const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
@@ -1401,7 +1445,7 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, Volume
     const int lane = threadIdx.x;
     uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
     const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
-    cone_tile<FMT, CONE, COMP, LEARNED>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp, xs,
+    cone_tile<FMT, CONE, COMP, LEARNED, GREY>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp, xs,
                                         tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals);
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
@@ -1422,7 +1466,7 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, Volume
 
 // Batched cone trace: global tiles of many scenes (heavy first), each wave
 // finds its scene by binary search over the scenes' first tiles.
-template <int FMT, int CONE, int COMP>
+template <int FMT, int CONE, int COMP, bool GREY>
 __global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict__ scenes,
                                                     const int* __restrict__ tile0, int nscenes,
                                                     const int* __restrict__ gorder, int total,
@@ -1442,7 +1486,7 @@ __global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict
         }
         const int sidx = __builtin_amdgcn_readfirstlane(lo);
         const BatchScene& B = scenes[sidx];
-        cone_tile<FMT, CONE, COMP, 0>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order, B.masks,
+        cone_tile<FMT, CONE, COMP, 0, GREY>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order, B.masks,
                                       B.words, B.image, none, nullptr, gt - B.tile0, lane, nfrag, nsamp, ntwo, nrows,
                                       nfall, nevals);
     }
@@ -1472,14 +1516,14 @@ __global__ void __launch_bounds__(256) k_segments(GridDesc g, VolumeDesc vd, Con
     const int kk_steps = cd.learned_k;
     FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t ns = 0, nt = 0;
-    int rs = march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, 0, min(kk_steps, cd.steps), st, ns, nt);
+    int rs = march_ftb<FMT, false>(vd, vol, cd, px, py, pz, dx, dy, dz, 0, min(kk_steps, cd.steps), st, ns, nt);
     const bool rec = rs == 0 && kk_steps < cd.steps;
     float f[kMlpIn];
     float ck0 = 0.0f, ck1 = 0.0f, ck2 = 0.0f, ak = 0.0f;
     if (rec) {
         farfield_features<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, (float)g.N, f);
         ck0 = st.c0; ck1 = st.c1; ck2 = st.c2; ak = st.a;
-        march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, kk_steps, cd.steps, st, ns, nt);
+        march_ftb<FMT, false>(vd, vol, cd, px, py, pz, dx, dy, dz, kk_steps, cd.steps, st, ns, nt);
     }
     valid[i] = rec ? 1 : 0;
 #pragma unroll
@@ -1557,9 +1601,15 @@ __global__ void __launch_bounds__(256) k_debrick(LevelDesc L, int tb, const uint
     if (i >= n * n * n) return;
     uint32_t x = (uint32_t)(i % n), y = (uint32_t)((i / n) % n), z = (uint32_t)(i / (n * n));
     uint64_t a = brick_addr(L, x, y, z);
-    const uint8_t* src = levels + a * (uint64_t)tb;
-    uint8_t* dst = out + i * (uint64_t)tb;
-    for (int b = 0; b < tb; ++b) dst[b] = src[b];
+    if (tb == 16) {  // stored (r, a, g, b) -> RGBA
+        float4 t = ((const float4*)levels)[a];
+        ((float4*)out)[i] = make_float4(t.x, t.z, t.w, t.y);
+    } else if (tb == 8) {
+        uint2 u = ((const uint2*)levels)[a];
+        ((uint2*)out)[i] = make_uint2((u.x & 0xffffu) | (u.y << 16), (u.y >> 16) | (u.x & 0xffff0000u));
+    } else {
+        out[i] = levels[a];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1689,7 +1739,7 @@ size_t tile_scratch_ints(int ntiles) { return 66 + 3 * (size_t)ntiles + 2; }
 void launch_bin_counts(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
                        unsigned long long* masks, int* tile_count, hipStream_t s) {
     if (S > 0)
-        hipLaunchKernelGGL(k_bin, dim3(blocks_for(S, 256)), dim3(256), 0, s, sph_tiles, order, S, tiles_x, words, masks,
+        hipLaunchKernelGGL(k_bin, dim3(blocks_for(S, 4)), dim3(256), 0, s, sph_tiles, order, S, tiles_x, words, masks,
                            tile_count);
 }
 
@@ -1729,18 +1779,18 @@ TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int til
     return q;
 }
 
-template <int FMT, int CONE, int COMP>
+template <int FMT, int CONE, int COMP, bool GREY = false>
 static void launch_cone_batch_t(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total,
                                 int wpg, int groups, unsigned long long* counters, hipStream_t s) {
     const int items_per_group = wpg;
     unsigned grid = groups > 0 ? (unsigned)groups : (unsigned)((total + items_per_group - 1) / items_per_group);
     if (grid == 0) return;
-    hipLaunchKernelGGL((k_cone_batch<FMT, CONE, COMP>), dim3(grid), dim3(64 * wpg), 0, s, scenes, tile0, nscenes,
+    hipLaunchKernelGGL((k_cone_batch<FMT, CONE, COMP, GREY>), dim3(grid), dim3(64 * wpg), 0, s, scenes, tile0, nscenes,
                        order, total, counters);
 }
 
 void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total_tiles,
-                       int format, int cone_mode, int composite, int waves_per_group, int groups,
+                       int format, int cone_mode, int composite, int grey, int waves_per_group, int groups,
                        unsigned long long* counters, hipStream_t s) {
     if (total_tiles <= 0) return;
     const bool w = cone_mode == 0;
@@ -1750,6 +1800,7 @@ void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, 
             else launch_cone_batch_t<1, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
         } else {
             if (composite == 0) launch_cone_batch_t<1, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+            else if (grey) launch_cone_batch_t<1, 1, 1, true>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
             else launch_cone_batch_t<1, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
         }
     } else {
@@ -1758,35 +1809,40 @@ void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, 
             else launch_cone_batch_t<2, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
         } else {
             if (composite == 0) launch_cone_batch_t<2, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+            else if (grey) launch_cone_batch_t<2, 1, 1, true>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
             else launch_cone_batch_t<2, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
         }
     }
 }
 
-template <int FMT, int CONE, int COMP, int LEARNED>
+template <int FMT, int CONE, int COMP, int LEARNED, bool GREY = false>
 static void launch_cone_t(const ConeLaunch& a, hipStream_t s) {
     unsigned ntiles = (unsigned)(a.cv.tiles_x * a.cv.tiles_y);
-    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
+    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED, GREY>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
                        a.levels, a.sph_world, a.sph_view, a.order, a.masks, a.words, a.image, a.counters,
                        a.tile_order, a.mlp);
 }
 
 // R8 volumes hold one channel: only the weighted R-sum cone reads them (the
-// runtime rejects the front-to-back modes for R8 before launching).
+// runtime rejects the front-to-back modes for R8 before launching).  Volumes
+// in rgba mode are grey (r = g = b): their front-to-back kernels filter r and a.
 template <int FMT>
 static void launch_cone_f(const ConeLaunch& a, hipStream_t s) {
     const bool weighted = a.cd.cone_mode == 0;
+    const bool grey = a.vd.mode == 1;
     if constexpr (FMT == 0) {
         if (a.cd.composite == 0) launch_cone_t<0, 0, 0, 0>(a, s);
         else launch_cone_t<0, 0, 1, 0>(a, s);
     } else {
         if (a.learned) {
-            launch_cone_t<FMT, 1, 1, 1>(a, s);
+            if (grey) launch_cone_t<FMT, 1, 1, 1, true>(a, s);
+            else launch_cone_t<FMT, 1, 1, 1>(a, s);
             return;
         }
         if (weighted && a.cd.composite == 0) launch_cone_t<FMT, 0, 0, 0>(a, s);
         else if (weighted) launch_cone_t<FMT, 0, 1, 0>(a, s);
         else if (a.cd.composite == 0) launch_cone_t<FMT, 1, 0, 0>(a, s);
+        else if (grey) launch_cone_t<FMT, 1, 1, 0, true>(a, s);
         else launch_cone_t<FMT, 1, 1, 0>(a, s);
     }
 }
