@@ -14,7 +14,7 @@ constexpr int kMaxLevels = 16;
 constexpr int kMaxSteps = 256;
 constexpr int kTile = 8;           // default camera tile: 8x8 pixels = one wavefront
 constexpr int kLdsSortMax = 16384; // billboards sorted in one workgroup's LDS
-constexpr int kChordLdsWords = 4096; // LDS bitmap (bricks) per pass-1 tile for chord-fill aggregation
+constexpr int kChordLdsWords = 4096; // largest LDS bitmap (bricks) per pass-1 tile for chord-fill aggregation
 constexpr int kLightTile = 32;       // pass-1 work item: 32 x 32 light pixels of one billboard
 constexpr unsigned kConeWaves = 6144; // persistent cone-trace wavefronts (256 CUs x 24)
 
@@ -38,6 +38,8 @@ struct GridDesc {
     uint32_t nb0;    // level-0 bricks per side
     int pow2_range;  // every range[a] is a power of two: x / range[a] == x * inv_range[a] exactly
     int pow2_N;      // N is a power of two: x / N == x * invN exactly
+    int pow2_step;   // stepSize is a power of two: the chord counter i += stepSize equals k * stepSize
+    float inv_step;  // 1 / stepSize
 };
 
 // Light view of pass 1 (Sun::update output + derived billboard basis)

@@ -98,12 +98,25 @@ __device__ __forceinline__ bool voxel_word_bit(const GridDesc& g, float px, floa
 }
 
 // same map, returning the brick coordinates and the bit inside the brick word
+// P2: the grid's extents are powers of two, x / range == x * (1 / range) exactly
+template <bool P2>
+__device__ __forceinline__ float div_range_t(const GridDesc& g, float x, int a) {
+    return P2 ? x * g.inv_range[a] : x / g.range[a];
+}
+template <bool P2 = false>
 __device__ __forceinline__ bool voxel_brick_bit(const GridDesc& g, float px, float py, float pz, int* bx, int* by,
                                                 int* bz, uint32_t* bit) {
     float fN = (float)g.N;
-    float lx = fN * div_range(g, px - g.lo[0], 0);
-    float ly = fN * div_range(g, py - g.lo[1], 1);
-    float lz = fN * div_range(g, pz - g.lo[2], 2);
+    float lx, ly, lz;
+    if (P2) {
+        lx = fN * div_range_t<true>(g, px - g.lo[0], 0);
+        ly = fN * div_range_t<true>(g, py - g.lo[1], 1);
+        lz = fN * div_range_t<true>(g, pz - g.lo[2], 2);
+    } else {
+        lx = fN * div_range(g, px - g.lo[0], 0);
+        ly = fN * div_range(g, py - g.lo[1], 1);
+        lz = fN * div_range(g, pz - g.lo[2], 2);
+    }
     if (!(lx > -1.0f && lx < fN && ly > -1.0f && ly < fN && lz > -1.0f && lz < fN)) return false;
     uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
     *bx = (int)(ix >> 2);
@@ -247,6 +260,12 @@ __global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ co
     for (int t = 0; t < c && o + t < cap; ++t) items[o + t] = make_int2(j, t);
 }
 
+constexpr int kTilePixels = kLightTile * kLightTile;
+
+// P2: power-of-two extents (multiplications instead of IEEE divisions in
+// the index map, and an exact k * stepSize chord counter); WORDS: size of the
+// LDS chord bitmap (bigger volumes have bigger slabs behind a tile)
+template <bool P2, int WORDS>
 __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, int chord,
                                                       const float4* __restrict__ boards,
                                                       const int4* __restrict__ lbox, const int* __restrict__ total_ptr,
@@ -258,8 +277,18 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     // tile: gather them in an LDS bitmap over the slab's brick box (1-voxel
     // margin for rounding) and flush non-zero words with one global atomic
     // each.  Stores falling outside the box (none expected) go straight to HBM.
-    __shared__ unsigned long long lbits[kChordLdsWords];
+    //
+    // With a power-of-two stepSize the chord loop counter i += stepSize is
+    // exactly k * stepSize: covered fragments are queued in LDS with their
+    // sample counts and the chords are walked from the queue, so no lane
+    // idles on a pixel outside the disc.
+    __shared__ unsigned long long lbits[WORDS];
+    __shared__ float f_sx[kTilePixels], f_sy[kTilePixels], f_sz[kTilePixels];
+    __shared__ int f_n[kTilePixels];
+    __shared__ int nfr_s;
+    const int tid = threadIdx.x;
     const int total = min(*total_ptr, cap);
+    const bool flat = P2 && chord;
     unsigned long long nfrag = 0, nchord = 0;
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
     const int2 it = items[item];
@@ -295,15 +324,28 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         }
         bx0 = b0[0]; by0 = b0[1]; bz0 = b0[2];
         nbx = b1[0] - b0[0] + 1; nby = b1[1] - b0[1] + 1; nbz = b1[2] - b0[2] + 1;
-        use_lds = du0 <= du1 && dv0 <= dv1 && nbx > 0 && nby > 0 && nbz > 0 && nbx * nby * nbz <= kChordLdsWords;
+        use_lds = du0 <= du1 && dv0 <= dv1 && nbx > 0 && nby > 0 && nbz > 0 && nbx * nby * nbz <= WORDS;
         if (use_lds)
-            for (int w = threadIdx.x; w < nbx * nby * nbz; w += blockDim.x) lbits[w] = 0ull;
+            for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) lbits[w] = 0ull;
+        if (tid == 0) nfr_s = 0;
         __syncthreads();
     }
-    // 16 x 16 threads sweep the tile
-    const int tx0 = (int)(threadIdx.x & 15u), ty0 = (int)(threadIdx.x >> 4);
-    for (int py = y0 + ty0; py <= y1; py += 16)
-    for (int px = x0 + tx0; px <= x1; px += 16) {
+    // store one brick word of chord samples (LDS bitmap, or HBM outside it)
+    auto flush = [&](int gx, int gy, int gz, unsigned long long bits) {
+        if (gx < 0) return;
+        if (use_lds) {
+            int lx = gx - bx0, ly = gy - by0, lz = gz - bz0;
+            if (lx >= 0 && lx < nbx && ly >= 0 && ly < nby && lz >= 0 && lz < nbz) {
+                atomicOr(&lbits[(lz * nby + ly) * nbx + lx], bits);
+                return;
+            }
+        }
+        atomicOr(&fillbits[((uint32_t)gz * g.nb0 + (uint32_t)gy) * g.nb0 + (uint32_t)gx], bits);
+    };
+    // pass over the tile's pixels: depth keys; chords queued (flat) or walked
+    const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
+    for (int p = tid; p < tw * th; p += blockDim.x) {
+        const int px = x0 + p % tw, py = y0 + p / tw;
         float fx, fy, fz, sc;
         if (!light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc)) continue;
         ++nfrag;
@@ -313,35 +355,35 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             // for (float i = 0; i < 2 * dist; i += stepSize) store (0,0,0,1)
             float sx = fx - lv.B[0] * dist, sy = fy - lv.B[1] * dist, sz = fz - lv.B[2] * dist;
             float lim = 2.0f * dist;
-            // consecutive samples of one chord mostly share a brick word:
-            // OR them in a register and store once per word change
-            int cbx = -1, cby = -1, cbz = -1;
-            unsigned long long cur_b = 0ull;
-            auto flush = [&](int gx, int gy, int gz, unsigned long long bits) {
-                if (gx < 0) return;
-                if (use_lds) {
-                    int lx = gx - bx0, ly = gy - by0, lz = gz - bz0;
-                    if (lx >= 0 && lx < nbx && ly >= 0 && ly < nby && lz >= 0 && lz < nbz) {
-                        atomicOr(&lbits[(lz * nby + ly) * nbx + lx], bits);
-                        return;
+            if (flat) {
+                // k * stepSize < lim  <=>  k < lim / stepSize (exact: power-of-two scaling)
+                const int n = lim > 0.0f ? (int)ceilf(lim * g.inv_step) : 0;
+                if (n > 0) {
+                    const int slot = atomicAdd(&nfr_s, 1);
+                    f_sx[slot] = sx; f_sy[slot] = sy; f_sz[slot] = sz;
+                    f_n[slot] = n;
+                    nchord += (unsigned long long)n;
+                }
+            } else {
+                // consecutive samples of one chord mostly share a brick word:
+                // OR them in a register and store once per word change
+                int cbx = -1, cby = -1, cbz = -1;
+                unsigned long long cur_b = 0ull;
+                for (float i = 0.0f; i < lim; i += g.stepSize) {
+                    float qx = sx + lv.B[0] * i, qy = sy + lv.B[1] * i, qz = sz + lv.B[2] * i;
+                    int gx, gy, gz;
+                    uint32_t b;
+                    ++nchord;
+                    if (!voxel_brick_bit<P2>(g, qx, qy, qz, &gx, &gy, &gz, &b)) continue;
+                    if (gx != cbx || gy != cby || gz != cbz) {
+                        flush(cbx, cby, cbz, cur_b);
+                        cbx = gx; cby = gy; cbz = gz;
+                        cur_b = 0ull;
                     }
+                    cur_b |= 1ull << b;
                 }
-                atomicOr(&fillbits[((uint32_t)gz * g.nb0 + (uint32_t)gy) * g.nb0 + (uint32_t)gx], bits);
-            };
-            for (float i = 0.0f; i < lim; i += g.stepSize) {
-                float qx = sx + lv.B[0] * i, qy = sy + lv.B[1] * i, qz = sz + lv.B[2] * i;
-                int gx, gy, gz;
-                uint32_t b;
-                ++nchord;
-                if (!voxel_brick_bit(g, qx, qy, qz, &gx, &gy, &gz, &b)) continue;
-                if (gx != cbx || gy != cby || gz != cbz) {
-                    flush(cbx, cby, cbz, cur_b);
-                    cbx = gx; cby = gy; cbz = gz;
-                    cur_b = 0ull;
-                }
-                cur_b |= 1ull << b;
+                flush(cbx, cby, cbz, cur_b);
             }
-            flush(cbx, cby, cbz, cur_b);
         }
         float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
         float ex = wx - lv.np[0], ey = wy - lv.np[1], ez = wz - lv.np[2];
@@ -352,10 +394,37 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             atomicMin(&keys[(size_t)py * lv.W + px], key);
         }
     }
+    if (flat) {
+        __syncthreads();
+        // every lane walks one queued chord: no lane idles on a pixel
+        // outside the disc, and neighbouring queue slots (neighbouring
+        // pixels) have similar chord lengths
+        const int nfr = nfr_s;
+        for (int f = tid; f < nfr; f += blockDim.x) {
+            const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
+            const int n = f_n[f];
+            int cbx = -1, cby = -1, cbz = -1;
+            unsigned long long cur_b = 0ull;
+            for (int k = 0; k < n; ++k) {
+                const float i = (float)k * g.stepSize;
+                float qx = sx + lv.B[0] * i, qy = sy + lv.B[1] * i, qz = sz + lv.B[2] * i;
+                int gx, gy, gz;
+                uint32_t b;
+                if (!voxel_brick_bit<P2>(g, qx, qy, qz, &gx, &gy, &gz, &b)) continue;
+                if (gx != cbx || gy != cby || gz != cbz) {
+                    flush(cbx, cby, cbz, cur_b);
+                    cbx = gx; cby = gy; cbz = gz;
+                    cur_b = 0ull;
+                }
+                cur_b |= 1ull << b;
+            }
+            flush(cbx, cby, cbz, cur_b);
+        }
+    }
     if (chord) {
         __syncthreads();
         if (use_lds) {
-            for (int w = threadIdx.x; w < nbx * nby * nbz; w += blockDim.x) {
+            for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) {
                 unsigned long long bits = lbits[w];
                 if (!bits) continue;
                 int lx = w % nbx, ly = (w / nbx) % nby, lz = w / (nbx * nby);
@@ -363,7 +432,7 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                 atomicOr(&fillbits[gw], bits);
             }
         }
-        __syncthreads();  // the next item reuses the bitmap
+        __syncthreads();  // the next item reuses the bitmap and the queue
     }
     }  // items
     if (counters) {
@@ -1143,6 +1212,60 @@ __device__ __forceinline__ float4 trilinear(const void* __restrict__ vol, const 
     return o;
 }
 
+// Grey sampling split into fetch and filter, so a two-level sample issues
+// the 16 loads of both footprints before waiting for any of them.
+struct GreyTaps {
+    float2 t[8];  // (r, a) at x0y0z0, x1y0z0, x0y1z0, x1y1z0, x0y0z1, x1y0z1, x0y1z1, x1y1z1
+    float ax, ay, az;
+};
+template <int FMT>
+__device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
+                                           float sz, GreyTaps& T) {
+    const float fn = (float)L.n;
+    const int nm1 = (int)L.n - 1;
+    float ux = fmaf(fminf(fmaxf(sx, -1.0f), 2.0f), fn, -0.5f);
+    float uy = fmaf(fminf(fmaxf(sy, -1.0f), 2.0f), fn, -0.5f);
+    float uz = fmaf(fminf(fmaxf(sz, -1.0f), 2.0f), fn, -0.5f);
+    float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
+    T.ax = ux - flx; T.ay = uy - fly; T.az = uz - flz;
+    int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    uint32_t x0 = (uint32_t)min(max(ix, 0), nm1), x1 = (uint32_t)min(max(ix + 1, 0), nm1);
+    uint32_t y0 = (uint32_t)min(max(iy, 0), nm1), y1 = (uint32_t)min(max(iy + 1, 0), nm1);
+    uint32_t z0 = (uint32_t)min(max(iz, 0), nm1), z1 = (uint32_t)min(max(iz + 1, 0), nm1);
+    const uint32_t nb = L.nb;
+    uint32_t X0 = ((x0 >> 2) << 6) | spread2(x0 & 3u), X1 = ((x1 >> 2) << 6) | spread2(x1 & 3u);
+    uint32_t Y0 = (((y0 >> 2) * nb) << 6) | (spread2(y0 & 3u) << 1);
+    uint32_t Y1 = (((y1 >> 2) * nb) << 6) | (spread2(y1 & 3u) << 1);
+    uint32_t Z0 = (((z0 >> 2) * nb * nb) << 6) | (spread2(z0 & 3u) << 2);
+    uint32_t Z1 = (((z1 >> 2) * nb * nb) << 6) | (spread2(z1 & 3u) << 2);
+    const uint64_t base = L.base;
+    T.t[0] = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z0));
+    T.t[1] = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z0));
+    T.t[2] = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z0));
+    T.t[3] = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z0));
+    T.t[4] = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z1));
+    T.t[5] = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z1));
+    T.t[6] = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z1));
+    T.t[7] = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z1));
+}
+// lerps along x, then y, then z (the order of trilinear())
+__device__ __forceinline__ float2 grey_filter(const GreyTaps& T) {
+    float2 o;
+    {
+        float c00 = lerpF(T.t[0].x, T.t[1].x, T.ax), c10 = lerpF(T.t[2].x, T.t[3].x, T.ax);
+        float c01 = lerpF(T.t[4].x, T.t[5].x, T.ax), c11 = lerpF(T.t[6].x, T.t[7].x, T.ax);
+        float c0 = lerpF(c00, c10, T.ay), c1 = lerpF(c01, c11, T.ay);
+        o.x = lerpF(c0, c1, T.az);
+    }
+    {
+        float c00 = lerpF(T.t[0].y, T.t[1].y, T.ax), c10 = lerpF(T.t[2].y, T.t[3].y, T.ax);
+        float c01 = lerpF(T.t[4].y, T.t[5].y, T.ax), c11 = lerpF(T.t[6].y, T.t[7].y, T.ax);
+        float c0 = lerpF(c00, c10, T.ay), c1 = lerpF(c01, c11, T.ay);
+        o.y = lerpF(c0, c1, T.az);
+    }
+    return o;
+}
+
 // textureLod: lambda -> level / fraction (GL 4.4 8.14.3), (1-f)*tau1 + f*tau2
 template <int FMT, int NCH>
 __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const VolumeDesc& vd, float lam, float sx,
@@ -1152,6 +1275,20 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
     if (lam <= 0.0f) { lv = 0; f = 0.0f; }
     else if (lam >= (float)(vd.L - 1)) { lv = vd.L - 1; f = 0.0f; }
     else { float fl = floorf(lam); lv = (int)fl; f = lam - fl; }
+    if (NCH == 2) {
+        const bool tw = f != 0.0f;
+        *two = tw;
+        GreyTaps A, B;
+        grey_fetch<FMT>(vol, vd.lv[lv], sx, sy, sz, A);
+        if (tw) grey_fetch<FMT>(vol, vd.lv[lv + 1], sx, sy, sz, B);  // in flight with A
+        float2 a = grey_filter(A);
+        if (tw) {
+            float2 b = grey_filter(B);
+            a.x = lerpF(a.x, b.x, f);
+            a.y = lerpF(a.y, b.y, f);
+        }
+        return make_float4(a.x, a.x, a.x, a.y);
+    }
     float4 o = trilinear<FMT, NCH>(vol, vd.lv[lv], sx, sy, sz);
     *two = f != 0.0f;
     if (f != 0.0f) {
@@ -1636,8 +1773,17 @@ void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const
     hipLaunchKernelGGL(k_light_expand, dim3(blocks_for(S, 256)), dim3(256), 0, s, ls.counts, ls.offsets, S, ls.items,
                        cap);
     unsigned grid = (unsigned)std::min<size_t>(ls.items_cap, 4096);
-    hipLaunchKernelGGL(k_light_raster, dim3(grid), dim3(256), 0, s, g, lv, mode == 1 ? 1 : 0, boards, ls.lbox,
-                       ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+    const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
+    const int chord = mode == 1 ? 1 : 0;
+    if (p2 && g.nb0 <= 64)
+        hipLaunchKernelGGL((k_light_raster<true, 2048>), dim3(grid), dim3(256), 0, s, g, lv, chord, boards, ls.lbox,
+                           ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+    else if (p2)
+        hipLaunchKernelGGL((k_light_raster<true, 4096>), dim3(grid), dim3(256), 0, s, g, lv, chord, boards, ls.lbox,
+                           ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+    else
+        hipLaunchKernelGGL((k_light_raster<false, 4096>), dim3(grid), dim3(256), 0, s, g, lv, chord, boards, ls.lbox,
+                           ls.offsets + S, ls.items, cap, keys, fillbits, counters);
 }
 
 size_t light_items_bound(int S, float max_radius, const LightView& lv) {

@@ -419,6 +419,8 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     v->g.pow2_range = is_pow2(v->g.range[0]) && is_pow2(v->g.range[1]) && is_pow2(v->g.range[2]) ? 1 : 0;
     for (int a = 0; a < 3; ++a) v->g.inv_range[a] = 1.0f / v->g.range[a];
     v->g.pow2_N = is_pow2((float)dim) ? 1 : 0;
+    v->g.pow2_step = is_pow2(v->g.stepSize) ? 1 : 0;
+    v->g.inv_step = 1.0f / v->g.stepSize;
     v->g.invN = 1.0f / (float)dim;
     v->g.N = dim;
     v->g.nb0 = v->vd.lv[0].nb;
