@@ -448,57 +448,93 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
 // ---------------------------------------------------------------------------
 // depth resolve + pass 2 (res/second_voxelize.glsl:34-51)
 // ---------------------------------------------------------------------------
+// posmap (optional): the pass-1 position map (RGBA32F, 0 where no fragment
+// won); shellbits (optional): the 9-voxel splat of pass 2.  A frame runs the
+// splat alone; the position map is rebuilt from the same keys on request.
+// A workgroup covers 16 x 16 light texels; neighbouring texels hit the same
+// shell words (the light map samples each voxel a few times), so the words
+// are OR-ed in an LDS hash table and flushed with one global atomic each.
+constexpr int kSplatSlots = 512;
+constexpr uint32_t kSplatEmpty = 0xffffffffu;
+
 __global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, const float4* __restrict__ boards,
                                                      const unsigned long long* __restrict__ keys,
                                                      float4* __restrict__ posmap,
                                                      unsigned long long* __restrict__ shellbits,
                                                      unsigned long long* __restrict__ counters) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= lv.W * lv.H) return;
-    unsigned long long key = keys[pix];
-    if (key == ~0ull) {
-        if (posmap) posmap[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
+    __shared__ uint32_t hkey[kSplatSlots];
+    __shared__ unsigned long long hbits[kSplatSlots];
+    const int t = threadIdx.x;
+    if (shellbits) {
+        for (int i = t; i < kSplatSlots; i += 256) { hkey[i] = kSplatEmpty; hbits[i] = 0ull; }
+        __syncthreads();
     }
-    int j = (int)(uint32_t)(key & 0xffffffffull);
-    int px = pix % lv.W, py = pix / lv.W;
-    LightSphere o = light_sphere(g, lv, boards[j]);
-    float fx, fy, fz, sc;
-    light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc);
-    float dist = o.s * sc;
-    float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
-    if (posmap) posmap[pix] = make_float4(wx, wy, wz, 1.0f);
-    if (counters) atomicAdd(&counters[2], 1ull);
-    // stepSize * normalize(vec3(+-1,+-1,+-1)): 1/sqrt(3) rounded once to FP32
-    const float k = 1.0f / sqrtf(3.0f);
-    const float off = g.stepSize * k;
-    // the 9 stores lie within 2 voxels per axis: merge those sharing a brick
-    // word in registers before the atomics (one atomic per distinct word)
-    uint32_t ws[9];
-    unsigned long long bs[9];
+    const int tiles_x = (lv.W + 15) / 16;
+    const int px = (int)(blockIdx.x % tiles_x) * 16 + (t & 15), py = (int)(blockIdx.x / tiles_x) * 16 + (t >> 4);
+    const bool inside = px < lv.W && py < lv.H;
+    const size_t pix = (size_t)py * lv.W + px;
+    const unsigned long long key = inside ? keys[pix] : ~0ull;
+    if (inside && key == ~0ull && posmap) posmap[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (key != ~0ull) {
+        int j = (int)(uint32_t)(key & 0xffffffffull);
+        LightSphere o = light_sphere(g, lv, boards[j]);
+        float fx, fy, fz, sc;
+        light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc);
+        float dist = o.s * sc;
+        float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
+        if (posmap) posmap[pix] = make_float4(wx, wy, wz, 1.0f);
+        if (counters) atomicAdd(&counters[2], 1ull);
+        if (shellbits) {
+            // stepSize * normalize(vec3(+-1,+-1,+-1)): 1/sqrt(3) rounded once to FP32
+            const float k = 1.0f / sqrtf(3.0f);
+            const float off = g.stepSize * k;
+            // the 9 stores lie within 2 voxels per axis: merge those sharing a
+            // brick word in registers first
+            uint32_t ws[9];
+            unsigned long long bs[9];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        float px = wx, py = wy, pz = wz;
-        if (t > 0) {
-            const int u = t - 1;
-            px = wx + ((u & 4) ? -off : off);
-            py = wy + ((u & 2) ? -off : off);
-            pz = wz + ((u & 1) ? -off : off);
+            for (int q = 0; q < 9; ++q) {
+                float qx = wx, qy = wy, qz = wz;
+                if (q > 0) {
+                    const int u = q - 1;
+                    qx = wx + ((u & 4) ? -off : off);
+                    qy = wy + ((u & 2) ? -off : off);
+                    qz = wz + ((u & 1) ? -off : off);
+                }
+                uint32_t w, b;
+                bool ok = voxel_word_bit(g, qx, qy, qz, &w, &b);
+                ws[q] = ok ? w : kSplatEmpty;
+                bs[q] = ok ? (1ull << b) : 0ull;
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                bool first = ws[q] != kSplatEmpty;
+#pragma unroll
+                for (int u = 0; u < q; ++u) first = first && ws[u] != ws[q];
+                unsigned long long bits = bs[q];
+#pragma unroll
+                for (int u = q + 1; u < 9; ++u) bits |= ws[u] == ws[q] ? bs[u] : 0ull;
+                if (!first) continue;
+                // LDS hash insert (linear probing, bounded); overflow goes to HBM
+                uint32_t h = (ws[q] * 2654435761u) >> 23;  // 9 bits
+                bool done = false;
+                for (int probe = 0; probe < 16 && !done; ++probe) {
+                    uint32_t prev = atomicCAS(&hkey[h], kSplatEmpty, ws[q]);
+                    if (prev == kSplatEmpty || prev == ws[q]) {
+                        atomicOr(&hbits[h], bits);
+                        done = true;
+                    } else {
+                        h = (h + 1u) & (uint32_t)(kSplatSlots - 1);
+                    }
+                }
+                if (!done) atomicOr(&shellbits[ws[q]], bits);
+            }
         }
-        uint32_t w, b;
-        bool ok = voxel_word_bit(g, px, py, pz, &w, &b);
-        ws[t] = ok ? w : 0xffffffffu;
-        bs[t] = ok ? (1ull << b) : 0ull;
     }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        bool first = ws[t] != 0xffffffffu;
-#pragma unroll
-        for (int u = 0; u < t; ++u) first = first && ws[u] != ws[t];
-        unsigned long long bits = bs[t];
-#pragma unroll
-        for (int u = t + 1; u < 9; ++u) bits |= ws[u] == ws[t] ? bs[u] : 0ull;
-        if (first) atomicOr(&shellbits[ws[t]], bits);
+    if (shellbits) {
+        __syncthreads();
+        for (int i = t; i < kSplatSlots; i += 256)
+            if (hkey[i] != kSplatEmpty) atomicOr(&shellbits[hkey[i]], hbits[i]);
     }
 }
 
@@ -656,6 +692,12 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
     __shared__ float4 l1[512];
     __shared__ float4 l2[64];
     __shared__ float4 l3[8];
+    // level-0 texels of half a region (32 bricks) staged for coalesced stores:
+    // a thread computes 8 Morton-consecutive texels (one level-1 parent), a
+    // store instruction then writes 64 consecutive texels of one brick
+    // (FP32 only: the FP16 build is bound by the noise evaluation and keeps
+    // its occupancy)
+    __shared__ float4 stage[FMT == 1 ? 2048 : 1];
     const uint32_t r = blockIdx.x;
     const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
     const uint32_t nb0 = g.nb0;
@@ -675,8 +717,20 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
             demorton6(m, &tx, &ty, &tz);
             float4 v = level0_value<FMT>(g, vd, ((sh >> m) & 1ull) != 0, ((fl >> m) & 1ull) != 0,
                                          (int)(gbx * 4u + tx), (int)(gby * 4u + ty), (int)(gbz * 4u + tz));
-            store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)brick * 64u + m, v);
+            if (FMT != 1) store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)brick * 64u + m, v);
+            else stage[threadIdx.x * 8u + c] = v;
             acc = add4(acc, v);
+        }
+        if (FMT == 1) {
+            __syncthreads();
+            // 2048 staged texels = 32 bricks of this half region, 64 per store
+            for (uint32_t i = threadIdx.x; i < 2048u; i += 256u) {
+                const uint32_t sb = rep * 32u + (i >> 6), sm = i & 63u;
+                const uint32_t sbx = 4u * rx + (sb & 3u), sby = 4u * ry + ((sb >> 2) & 3u), sbz = 4u * rz + (sb >> 4);
+                const uint32_t sbrick = (sbz * nb0 + sby) * nb0 + sbx;
+                store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)sbrick * 64u + sm, stage[i]);
+            }
+            __syncthreads();
         }
         if (nlev > 1) {
             float4 p = mip_parent<FMT>(acc);
@@ -1798,8 +1852,8 @@ size_t light_items_bound(int S, float max_radius, const LightView& lv) {
 
 void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards, const unsigned long long* keys,
                         float4* posmap, unsigned long long* shellbits, unsigned long long* counters, hipStream_t s) {
-    uint64_t n = (uint64_t)lv.W * lv.H;
-    hipLaunchKernelGGL(k_light_splat, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, lv, boards, keys, posmap, shellbits,
+    const uint64_t tiles = (uint64_t)((lv.W + 15) / 16) * (uint64_t)((lv.H + 15) / 16);
+    hipLaunchKernelGGL(k_light_splat, dim3((unsigned)tiles), dim3(256), 0, s, g, lv, boards, keys, posmap, shellbits,
                        counters);
 }
 

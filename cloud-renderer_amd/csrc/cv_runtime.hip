@@ -80,6 +80,9 @@ struct cr_volume_s {
     bool voxelized = false;
     int sorted_count = -1;
     uint64_t gen = 0;  // bumped by every change of billboards / noise (invalidates captured batches)
+    uint64_t vox_gen = 0;  // gen at the last voxelize
+    LightView last_lv{};   // light view of the last voxelize
+    size_t posmap_cap = 0;
 };
 
 // far-field MLP: FP32 master parameters + Adam moments, bf16 packed copies for
@@ -534,17 +537,10 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
     size_t npx = (size_t)light_w * light_h;
     plan.npx = npx;
     if ((rc = ensure(ctx, &v->d_keys, npx, &v->light_cap))) return rc;
-    static_assert(sizeof(float4) == 16, "float4");
-    {
-        size_t cap = v->d_posmap ? (size_t)v->lw * v->lh : 0;
-        if (!v->d_posmap || cap < npx) {
-            if (v->d_posmap) (void)hipFree(v->d_posmap);
-            v->d_posmap = nullptr;
-            CR_HIP_ALLOC(ctx, hipMalloc((void**)&v->d_posmap, npx * sizeof(float4)));
-        }
-    }
     v->lw = light_w;
     v->lh = light_h;
+    v->last_lv = plan.lv;  // the position map is rebuilt from the keys on request
+    v->vox_gen = v->gen;
     plan.ls = LightScratch{};
     if (v->S > 0) {
         if ((rc = ensure(ctx, &v->d_lbox, (size_t)v->S, &v->lbox_cap))) return rc;
@@ -572,7 +568,7 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
     if (ev) ev->mark(1);
     launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, v->d_fill, counters, s);
     if (ev) ev->mark(2);
-    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, v->d_posmap, v->d_shell, counters, s);
+    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, v->d_shell, counters, s);
     if (ev) ev->mark(3);
     launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
     if (ev) ev->mark(4);
@@ -934,11 +930,17 @@ int cr_volume_read_position_map(cr_volume v, float* host_dst, size_t bytes, void
     if (!v || !host_dst) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_read_position_map: NULL argument");
     cr_context ctx = v->ctx;
     if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_volume_read_position_map: volume not voxelized");
-    size_t need = (size_t)v->lw * v->lh * sizeof(float4);
+    if (v->gen != v->vox_gen)
+        return fail(ctx, CR_ESTATE, "cr_volume_read_position_map: billboards changed since the last voxelize");
+    size_t npx = (size_t)v->lw * v->lh;
+    size_t need = npx * sizeof(float4);
     if (bytes < need) return fail(ctx, CR_EINVAL, "cr_volume_read_position_map: destination too small");
     int rc = set_device(ctx);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    if ((rc = ensure(ctx, &v->d_posmap, npx, &v->posmap_cap))) return rc;
+    launch_light_splat(v->g, v->last_lv, v->d_boards, v->d_keys, v->d_posmap, nullptr, nullptr, s);
+    CR_HIP(ctx, hipGetLastError());
     CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_posmap, need, hipMemcpyDeviceToHost, s));
     CR_HIP(ctx, hipStreamSynchronize(s));
     return CR_OK;

@@ -182,7 +182,9 @@ int cr_volume_occupancy(cr_volume vol, uint64_t *count, uint64_t *hash, void *st
 int cr_volume_read_level(cr_volume vol, int level, void *host_dst, size_t bytes, void *stream);
 /* level-0 state bytes (0 empty, 1 fill, 2 shell), N^3, x-fastest */
 int cr_volume_read_occupancy(cr_volume vol, uint8_t *host_dst, size_t bytes, void *stream);
-/* pass-1 position map (RGBA32F, light_w x light_h) of the last cr_voxelize */
+/* pass-1 position map (RGBA32F, light_w x light_h) of the last cr_voxelize,
+ * rebuilt from its depth test on request (CR_ESTATE if the billboards
+ * changed since) */
 int cr_volume_read_position_map(cr_volume vol, float *host_dst, size_t bytes, void *stream);
 /* draw order of the last cr_cone_trace: order[r] = billboard drawn r-th (far to near) */
 int cr_volume_read_draw_order(cr_volume vol, int32_t *host_dst, int count, void *stream);
