@@ -400,25 +400,36 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         // outside the disc, and neighbouring queue slots (neighbouring
         // pixels) have similar chord lengths
         const int nfr = nfr_s;
+        const float fN = (float)g.N;
+        const float Bx = lv.B[0], By = lv.B[1], Bz = lv.B[2];
         for (int f = tid; f < nfr; f += blockDim.x) {
             const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
             const int n = f_n[f];
-            int cbx = -1, cby = -1, cbz = -1;
+            // brick word as one key (10 bits per axis; N <= 2048 gives <= 512
+            // bricks per axis), bits of the current word OR-ed in a register
+            uint32_t cur_key = 0xffffffffu;
             unsigned long long cur_b = 0ull;
             for (int k = 0; k < n; ++k) {
                 const float i = (float)k * g.stepSize;
-                float qx = sx + lv.B[0] * i, qy = sy + lv.B[1] * i, qz = sz + lv.B[2] * i;
-                int gx, gy, gz;
-                uint32_t b;
-                if (!voxel_brick_bit<P2>(g, qx, qy, qz, &gx, &gy, &gz, &b)) continue;
-                if (gx != cbx || gy != cby || gz != cbz) {
-                    flush(cbx, cby, cbz, cur_b);
-                    cbx = gx; cby = gy; cbz = gz;
+                const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
+                // voxel_brick_bit with power-of-two extents, branch free
+                const float lx = fN * ((qx - g.lo[0]) * g.inv_range[0]);
+                const float ly = fN * ((qy - g.lo[1]) * g.inv_range[1]);
+                const float lz = fN * ((qz - g.lo[2]) * g.inv_range[2]);
+                const bool in = lx > -1.0f && lx < fN && ly > -1.0f && ly < fN && lz > -1.0f && lz < fN;
+                const uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
+                const uint32_t key = ((iz >> 2) << 20) | ((iy >> 2) << 10) | (ix >> 2);
+                const unsigned long long m = 1ull << morton6(ix & 3u, iy & 3u, iz & 3u);
+                if (in && key != cur_key) {
+                    if (cur_key != 0xffffffffu)
+                        flush((int)(cur_key & 1023u), (int)((cur_key >> 10) & 1023u), (int)(cur_key >> 20), cur_b);
+                    cur_key = key;
                     cur_b = 0ull;
                 }
-                cur_b |= 1ull << b;
+                cur_b |= in ? m : 0ull;
             }
-            flush(cbx, cby, cbz, cur_b);
+            if (cur_key != 0xffffffffu)
+                flush((int)(cur_key & 1023u), (int)((cur_key >> 10) & 1023u), (int)(cur_key >> 20), cur_b);
         }
     }
     if (chord) {
