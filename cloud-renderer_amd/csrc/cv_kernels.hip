@@ -1516,9 +1516,16 @@ __device__ __forceinline__ void blend_frag(const ConeDesc& cd, float s0, float s
     }
 }
 
-// Ranks gathered per pass of the lane-walk cone kernel.
+// Ranks gathered per pass of the cone walk.
 constexpr int kConeChunk = 256;
 
+// CONE: 0 weighted R-sum, 1 front-to-back.  COMP: 0 GL blend, 1 front-to-back.
+// LEARNED (front-to-back only): after cd.learned_k samples the far field of
+// every still-running cone is predicted by the MLP (one 64-row batch per
+// wave on the matrix cores); a cone whose predicted error, scaled by its
+// remaining transmittance, exceeds cd.err_bound (or every cone when
+// err_bound < 0) falls back to the plain march from step k.
+//
 // Shade one camera tile, one pixel per lane, each lane walking only the
 // billboards that cover its own pixel.  The tile's binned billboards are
 // taken in rank order (front to back for COMP = 1, back to front for
@@ -1530,21 +1537,26 @@ constexpr int kConeChunk = 256;
 // over every billboard, so the image is the same bit for bit; what changes is
 // that a march no longer idles the lanes the billboard misses.  All lanes of
 // a round start at step 0, so the LOD of step i (and with it the one- or
-// two-level sampling branch) stays uniform across the wave.
-template <int FMT, int CONE, int COMP, bool GREY>
-__device__ __forceinline__ void cone_tile_lanes(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
-                                                const ConeDesc& cd, const void* __restrict__ vol,
-                                                const float4* __restrict__ sph_world,
-                                                const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
-                                                const unsigned long long* __restrict__ masks, int words,
-                                                void* __restrict__ image, unsigned long long* cover, int* id,
-                                                int tile, int lane, uint32_t& nfrag, uint32_t& nsamp, uint32_t& ntwo) {
+// two-level sampling branch) stays uniform across the wave, and in the
+// learned kernels the lanes reach step k together: their cones form the
+// wave's 64-row MLP batch.
+template <int FMT, int CONE, int COMP, int LEARNED, bool GREY>
+__device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
+                                          const ConeDesc& cd, const void* __restrict__ vol,
+                                          const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
+                                          const int32_t* __restrict__ order,
+                                          const unsigned long long* __restrict__ masks, int words,
+                                          void* __restrict__ image, const MlpDev& mlp, uint16_t* xs,
+                                          unsigned long long* cover, int* id, int tile, int lane, uint32_t& nfrag,
+                                          uint32_t& nsamp, uint32_t& ntwo, uint32_t& nrows, uint32_t& nfall,
+                                          uint32_t& nevals) {
     const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
     const int px = tx * cv.tile_w + (lane & (cv.tile_w - 1)), py = ty * cv.tile_h + (lane >> cv.tile_shift);
     const bool valid = px < cv.W && py < cv.H;
     const float rx = pixel_ndc(px, cv.W) * cv.kx;
     const float ry = pixel_ndc(py, cv.H) * cv.ky;
     const float fN = (float)g.N;
+    const int kk_steps = LEARNED ? min(cd.learned_k, cd.steps) : cd.steps;
 
     float C0, C1, C2, C3;
     if (COMP == 0) {
@@ -1613,6 +1625,10 @@ __device__ __forceinline__ void cone_tile_lanes(const GridDesc& g, const VolumeD
                 }
             }
             if (!__any(have)) break;
+            ConeFrag f = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+            bool alive = false;
             if (have) {
                 const float4 cw = sph_world[j];
                 const float4 vv = sph_view[j];
@@ -1621,8 +1637,7 @@ __device__ __forceinline__ void cone_tile_lanes(const GridDesc& g, const VolumeD
                 const float dv = ry * dz - vv.y;
                 const float sc = sphere_contrib(du, dv, cw.w);
                 ++nfrag;
-                const ConeFrag f = cone_frag(g, cv, cd, cw, du, dv, sc, fN);
-                float s0, s1, s2, s3;
+                f = cone_frag(g, cv, cd, cw, du, dv, sc, fN);
                 if (CONE == 0) {
                     float color = 0.0f;
                     for (int i = 0; i < cd.steps; ++i) {
@@ -1636,145 +1651,27 @@ __device__ __forceinline__ void cone_tile_lanes(const GridDesc& g, const VolumeD
                     nsamp += (uint32_t)cd.steps;
                     s0 = s1 = s2 = s3 = clamp01(color);
                 } else {
-                    FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                    march_ftb<FMT, GREY>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0, cd.steps, st, nsamp,
-                                         ntwo);
-                    s0 = clamp01(st.c0); s1 = clamp01(st.c1); s2 = clamp01(st.c2); s3 = clamp01(st.a);
-                }
-                blend_frag<COMP>(cd, s0, s1, s2, s3, C0, C1, C2, C3, done);
-            }
-        }
-        // the next gather overwrites the list every lane has finished reading
-        __builtin_amdgcn_wave_barrier();
-    }
-    if (valid) {
-        char* row = (char*)image + (size_t)py * cd.pitch;
-        if (cd.pixfmt == 0) {
-            ((float4*)row)[px] = make_float4(C0, C1, C2, C3);
-        } else {
-            uint32_t v = q8(C0) | (q8(C1) << 8) | (q8(C2) << 16) | (q8(C3) << 24);
-            ((uint32_t*)row)[px] = v;
-        }
-    }
-}
-
-// CONE: 0 weighted R-sum, 1 front-to-back.  COMP: 0 GL blend, 1 front-to-back.
-// LEARNED (front-to-back only): after cd.learned_k samples the far field of
-// every still-running cone is predicted by the MLP (one 64-row batch per
-// wave on the matrix cores); a cone whose predicted error, scaled by its
-// remaining transmittance, exceeds cd.err_bound (or every cone when
-// err_bound < 0) falls back to the plain march from step k.
-// Shade one camera tile: every billboard binned to it, in rank order (front
-// to back for COMP = 1, back to front for COMP = 0), one pixel per lane, the
-// wave walking the billboards together (the MLP batch needs the wave's 64
-// rows at once).  Only the learned kernels use this walk; the plain ones use
-// cone_tile_lanes.
-template <int FMT, int CONE, int COMP, int LEARNED, bool GREY>
-__device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
-                                          const ConeDesc& cd, const void* __restrict__ vol,
-                                          const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
-                                          const int32_t* __restrict__ order,
-                                          const unsigned long long* __restrict__ masks, int words,
-                                          void* __restrict__ image, const MlpDev& mlp, uint16_t* xs, int tile,
-                                          int lane, uint32_t& nfrag, uint32_t& nsamp, uint32_t& ntwo,
-                                          uint32_t& nrows, uint32_t& nfall, uint32_t& nevals) {
-    const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
-    const int px = tx * cv.tile_w + (lane & (cv.tile_w - 1)), py = ty * cv.tile_h + (lane >> cv.tile_shift);
-    const bool valid = px < cv.W && py < cv.H;
-    const float rx = pixel_ndc(px, cv.W) * cv.kx;
-    const float ry = pixel_ndc(py, cv.H) * cv.ky;
-    const float fN = (float)g.N;
-    const int kk_steps = LEARNED ? min(cd.learned_k, cd.steps) : cd.steps;
-
-    float C0, C1, C2, C3;
-    if (COMP == 0) {
-        C0 = cd.clear[0]; C1 = cd.clear[1]; C2 = cd.clear[2]; C3 = cd.clear[3];
-        if (cd.quantize) {
-            C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
-            C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
-        }
-    } else {
-        C0 = C1 = C2 = C3 = 0.0f;
-    }
-    bool done = !valid;
-    const unsigned long long* tm = masks + (size_t)tile * words;
-
-    for (int wi = 0; wi < words; ++wi) {
-        const int w = COMP == 1 ? (words - 1 - wi) : wi;
-        unsigned long long bits = tm[w];
-        while (bits) {
-            int b;
-            if (COMP == 1) b = 63 - __clzll((long long)bits);
-            else b = __ffsll((long long)bits) - 1;
-            bits &= ~(1ull << b);
-            const int r = (w << 6) | b;
-            const int j = order[r];
-            const float4 cw = sph_world[j];
-            const float4 vv = sph_view[j];
-            bool frag = false, alive = false;
-            float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, dx = 0.0f, dy = 0.0f, dzz = 0.0f;
-            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-            FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            if (!done && valid) {
-                const float s = cw.w;
-                const float dz = -vv.z;
-                const float du = rx * dz - vv.x;
-                const float dv = ry * dz - vv.y;
-                const float sc = sphere_contrib(du, dv, s);
-                if (!(sc < 0.01f)) {
-                    frag = true;
-                    ++nfrag;
-                    // fragPos, then pos = fragPos + normalize(fragNor) * radius * sc
-                    float fx = (cw.x + du * cv.S[0]) + dv * cv.U[0];
-                    float fy = (cw.y + du * cv.S[1]) + dv * cv.U[1];
-                    float fz = (cw.z + du * cv.S[2]) + dv * cv.U[2];
-                    float qx = fx + (cv.B[0] * s) * sc;
-                    float qy = fy + (cv.B[1] * s) * sc;
-                    float qz = fz + (cv.B[2] * s) * sc;
-                    // calculateVoxelLerp(pos) / voxelDim
-                    p0x = div_N(g, fN * div_range(g, qx - g.lo[0], 0));
-                    p0y = div_N(g, fN * div_range(g, qy - g.lo[1], 1));
-                    p0z = div_N(g, fN * div_range(g, qz - g.lo[2], 2));
-                    if (cd.directional) {
-                        dx = cd.ldir[0]; dy = cd.ldir[1]; dzz = cd.ldir[2];
-                    } else {
-                        float lx = cd.lpos[0] - qx, ly = cd.lpos[1] - qy, lz = cd.lpos[2] - qz;
-                        float inv = 1.0f / sqrtf((lx * lx + ly * ly) + lz * lz);
-                        dx = div_N(g, lx * inv); dy = div_N(g, ly * inv); dzz = div_N(g, lz * inv);
-                    }
-                    if (CONE == 0) {
-                        float color = 0.0f;
-                        for (int i = 0; i < cd.steps; ++i) {
-                            const float h = cd.h[i];
-                            bool two;
-                            float4 sm = sample_lod<FMT, 1>(vol, vd, cd.lam[i], fmaf(h, dx, p0x), fmaf(h, dy, p0y), fmaf(h, dzz, p0z), &two);
-                            ntwo += two ? 1u : 0u;
-                            color = color + (sm.x * (float)(i + 1)) / cd.den;
-                        }
-                        nsamp += (uint32_t)cd.steps;
-                        s0 = s1 = s2 = s3 = clamp01(color);
-                    } else {
-                        int rs = march_ftb<FMT, GREY>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, 0, kk_steps, st, nsamp, ntwo);
-                        alive = LEARNED && rs == 0 && kk_steps < cd.steps;
-                    }
+                    const int rs = march_ftb<FMT, GREY>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0,
+                                                        kk_steps, st, nsamp, ntwo);
+                    alive = LEARNED && rs == 0 && kk_steps < cd.steps;
                 }
             }
             if (LEARNED && CONE != 0) {
                 // wave-uniform: all lanes take part in the batched MLP
                 if (__any(alive)) {
-                    float f[kMlpIn];
+                    float x[kMlpIn];
                     if (alive) {
-                        farfield_features<FMT>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, fN, f);
+                        farfield_features<FMT>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, fN, x);
                         nsamp += 4;
                     } else {
 #pragma unroll
-                        for (int q = 0; q < kMlpIn; ++q) f[q] = 0.0f;
+                        for (int q = 0; q < kMlpIn; ++q) x[q] = 0.0f;
                     }
                     uint32_t* xrow = (uint32_t*)(xs + lane * kMlpInPad);
 #pragma unroll
                     for (int q = 0; q < kMlpInPad / 2; ++q) {
-                        float lo = 2 * q < kMlpIn ? f[2 * q] : 0.0f;
-                        float hi = 2 * q + 1 < kMlpIn ? f[2 * q + 1] : 0.0f;
+                        float lo = 2 * q < kMlpIn ? x[2 * q] : 0.0f;
+                        float hi = 2 * q + 1 < kMlpIn ? x[2 * q + 1] : 0.0f;
                         xrow[q] = pack_bf2(lo, hi);
                     }
                     __syncthreads();
@@ -1787,7 +1684,8 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                         const float rem = 1.0f - st.a;
                         if (cd.err_bound < 0.0f || !(rem * y[4] <= cd.err_bound)) {
                             ++nfall;
-                            march_ftb<FMT, GREY>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dzz, kk_steps, cd.steps, st, nsamp, ntwo);
+                            march_ftb<FMT, GREY>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, kk_steps,
+                                                 cd.steps, st, nsamp, ntwo);
                         } else {
                             st.c0 = fmaf(rem, clamp01(y[0]), st.c0);
                             st.c1 = fmaf(rem, clamp01(y[1]), st.c1);
@@ -1797,32 +1695,14 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                     }
                 }
             }
-            if (frag) {
+            if (have) {
                 if (CONE != 0) { s0 = clamp01(st.c0); s1 = clamp01(st.c1); s2 = clamp01(st.c2); s3 = clamp01(st.a); }
-                if (COMP == 0) {
-                    // dst = src * src.a + dst * (1 - src.a)  (src/main.cpp:94-95)
-                    const float al = s3, om = 1.0f - al;
-                    C0 = s0 * al + C0 * om;
-                    C1 = s1 * al + C1 * om;
-                    C2 = s2 * al + C2 * om;
-                    C3 = s3 * al + C3 * om;
-                    if (cd.quantize) {
-                        C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
-                        C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
-                    }
-                } else {
-                    const float wgt = (1.0f - C3) * s3;
-                    C0 = fmaf(wgt, s0, C0);
-                    C1 = fmaf(wgt, s1, C1);
-                    C2 = fmaf(wgt, s2, C2);
-                    C3 = C3 + wgt;
-                    if (C3 >= cd.alpha_stop) done = true;
-                }
+                blend_frag<COMP>(cd, s0, s1, s2, s3, C0, C1, C2, C3, done);
             }
-            if (COMP == 1 && __all(done)) goto tile_done;
         }
+        // the next gather overwrites the list every lane has finished reading
+        __builtin_amdgcn_wave_barrier();
     }
-tile_done:
     if (valid) {
         char* row = (char*)image + (size_t)py * cd.pitch;
         if (cd.pixfmt == 0) {
@@ -1846,18 +1726,13 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, Volume
     // One wavefront per tile; tiles are dispatched heaviest first
     // (tile_order), so the costly clump-centre tiles do not form the tail.
     __shared__ __align__(16) uint16_t xs[LEARNED ? 64 * kMlpInPad : 8];
-    __shared__ unsigned long long cover[LEARNED ? 1 : kConeChunk];
-    __shared__ int id[LEARNED ? 1 : kConeChunk];
+    __shared__ unsigned long long cover[kConeChunk];
+    __shared__ int id[kConeChunk];
     const int lane = threadIdx.x;
     uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
     const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
-    if constexpr (LEARNED) {
-        cone_tile<FMT, CONE, COMP, LEARNED, GREY>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image,
-                                                  mlp, xs, tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals);
-    } else {
-        cone_tile_lanes<FMT, CONE, COMP, GREY>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image,
-                                               cover, id, tile, lane, nfrag, nsamp, ntwo);
-    }
+    cone_tile<FMT, CONE, COMP, LEARNED, GREY>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp,
+                                              xs, cover, id, tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals);
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
         unsigned long long e0 = 0, e1 = 0;
@@ -1887,7 +1762,8 @@ __global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     __shared__ unsigned long long cover[4][kConeChunk];  // waves per workgroup <= 4
     __shared__ int id[4][kConeChunk];
-    uint32_t nfrag = 0, nsamp = 0, ntwo = 0;
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
+    const MlpDev none{nullptr, nullptr, nullptr};
     for (int item = blockIdx.x * wpg + wave; item < total; item += gridDim.x * wpg) {
         const int gt = __builtin_amdgcn_readfirstlane(gorder[item]);
         int lo = 0, hi = nscenes - 1;
@@ -1898,9 +1774,9 @@ __global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict
         }
         const int sidx = __builtin_amdgcn_readfirstlane(lo);
         const BatchScene& B = scenes[sidx];
-        cone_tile_lanes<FMT, CONE, COMP, GREY>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order,
-                                               B.masks, B.words, B.image, cover[wave], id[wave], gt - B.tile0, lane,
-                                               nfrag, nsamp, ntwo);
+        cone_tile<FMT, CONE, COMP, 0, GREY>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order, B.masks,
+                                            B.words, B.image, none, nullptr, cover[wave], id[wave], gt - B.tile0, lane,
+                                            nfrag, nsamp, ntwo, nrows, nfall, nevals);
     }
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), b = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
