@@ -34,6 +34,9 @@ __device__ __forceinline__ uint32_t spread2(uint32_t v) { return (v & 1u) | ((v 
 __device__ __forceinline__ uint32_t morton6(uint32_t x, uint32_t y, uint32_t z) {
     return spread2(x) | (spread2(y) << 1) | (spread2(z) << 2);
 }
+// bit of (x, y, z) in a fill word: x fastest inside the 4x4x4 brick (the
+// shell words and the texel bricks use Morton order)
+__device__ __forceinline__ uint32_t lin6(uint32_t x, uint32_t y, uint32_t z) { return x | (y << 2) | (z << 4); }
 __device__ __forceinline__ void demorton6(uint32_t m, uint32_t* x, uint32_t* y, uint32_t* z) {
     *x = (m & 1u) | ((m >> 2) & 2u);
     *y = ((m >> 1) & 1u) | ((m >> 3) & 2u);
@@ -97,7 +100,7 @@ __device__ __forceinline__ bool voxel_word_bit(const GridDesc& g, float px, floa
     return true;
 }
 
-// same map, returning the brick coordinates and the bit inside the brick word
+// same map, returning the brick coordinates and the bit inside the fill word
 // P2: the grid's extents are powers of two, x / range == x * (1 / range) exactly
 template <bool P2>
 __device__ __forceinline__ float div_range_t(const GridDesc& g, float x, int a) {
@@ -122,7 +125,7 @@ __device__ __forceinline__ bool voxel_brick_bit(const GridDesc& g, float px, flo
     *bx = (int)(ix >> 2);
     *by = (int)(iy >> 2);
     *bz = (int)(iz >> 2);
-    *bit = morton6(ix & 3u, iy & 3u, iz & 3u);
+    *bit = lin6(ix & 3u, iy & 3u, iz & 3u);
     return true;
 }
 
@@ -402,10 +405,14 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         const int nfr = nfr_s;
         const float fN = (float)g.N;
         const float Bx = lv.B[0], By = lv.B[1], Bz = lv.B[2];
+        // N * ((q - lo) * (1 / range)): both factors are powers of two, so
+        // scaling once by their product rounds identically
+        const float kx = fN * g.inv_range[0], ky = fN * g.inv_range[1], kz = fN * g.inv_range[2];
+        const uint32_t uN = (uint32_t)g.N;
         for (int f = tid; f < nfr; f += blockDim.x) {
             const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
             const int n = f_n[f];
-            // brick word as one key (10 bits per axis; N <= 2048 gives <= 512
+            // brick word as one key (9 bits per axis: N <= 2048 gives <= 512
             // bricks per axis), bits of the current word OR-ed in a register
             uint32_t cur_key = 0xffffffffu;
             unsigned long long cur_b = 0ull;
@@ -413,23 +420,27 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                 const float i = (float)k * g.stepSize;
                 const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
                 // voxel_brick_bit with power-of-two extents, branch free
-                const float lx = fN * ((qx - g.lo[0]) * g.inv_range[0]);
-                const float ly = fN * ((qy - g.lo[1]) * g.inv_range[1]);
-                const float lz = fN * ((qz - g.lo[2]) * g.inv_range[2]);
-                const bool in = lx > -1.0f && lx < fN && ly > -1.0f && ly < fN && lz > -1.0f && lz < fN;
+                // clamped to [-2, N]: -1 < l < N  <=>  0 <= (int)l < N (truncation
+                // toward zero), and with N a power of two the three unsigned
+                // bounds fold into one test of the OR
+                const float lx = __builtin_amdgcn_fmed3f((qx - g.lo[0]) * kx, -2.0f, fN);
+                const float ly = __builtin_amdgcn_fmed3f((qy - g.lo[1]) * ky, -2.0f, fN);
+                const float lz = __builtin_amdgcn_fmed3f((qz - g.lo[2]) * kz, -2.0f, fN);
                 const uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
-                const uint32_t key = ((iz >> 2) << 20) | ((iy >> 2) << 10) | (ix >> 2);
-                const unsigned long long m = 1ull << morton6(ix & 3u, iy & 3u, iz & 3u);
+                const bool in = (ix | iy | iz) < uN;
+                // brick coordinates (< 512) in 9-bit fields
+                const uint32_t key = (ix >> 2) | ((iy << 7) & 0x3fe00u) | ((iz << 16) & 0x7fc0000u);
+                const unsigned long long m = 1ull << ((ix & 3u) | ((iy << 2) & 0xcu) | ((iz << 4) & 0x30u));
                 if (in && key != cur_key) {
                     if (cur_key != 0xffffffffu)
-                        flush((int)(cur_key & 1023u), (int)((cur_key >> 10) & 1023u), (int)(cur_key >> 20), cur_b);
+                        flush((int)(cur_key & 511u), (int)((cur_key >> 9) & 511u), (int)(cur_key >> 18), cur_b);
                     cur_key = key;
                     cur_b = 0ull;
                 }
                 cur_b |= in ? m : 0ull;
             }
             if (cur_key != 0xffffffffu)
-                flush((int)(cur_key & 1023u), (int)((cur_key >> 10) & 1023u), (int)(cur_key >> 20), cur_b);
+                flush((int)(cur_key & 511u), (int)((cur_key >> 9) & 511u), (int)(cur_key >> 18), cur_b);
         }
     }
     if (chord) {
@@ -614,7 +625,7 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
     int x = (int)(bx * 4u + lx), y = (int)(by * 4u + ly), z = (int)(bz * 4u + lz);
     bool inside = x < g.N && y < g.N && z < g.N;
     bool shell = inside && ((sh >> m) & 1ull);
-    bool fill = inside && ((fl >> m) & 1ull);
+    bool fill = inside && ((fl >> lin6(lx, ly, lz)) & 1ull);
     float r, gg, b, a;
     if (vd.mode == 0) {
         r = gg = b = a = shell ? 1.0f : 0.0f;
@@ -726,7 +737,7 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
             const uint32_t m = q * 8u + c;
             uint32_t tx, ty, tz;
             demorton6(m, &tx, &ty, &tz);
-            float4 v = level0_value<FMT>(g, vd, ((sh >> m) & 1ull) != 0, ((fl >> m) & 1ull) != 0,
+            float4 v = level0_value<FMT>(g, vd, ((sh >> m) & 1ull) != 0, ((fl >> lin6(tx, ty, tz)) & 1ull) != 0,
                                          (int)(gbx * 4u + tx), (int)(gby * 4u + ty), (int)(gbz * 4u + tz));
             if (FMT != 1) store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)brick * 64u + m, v);
             else stage[threadIdx.x * 8u + c] = v;
@@ -1840,7 +1851,13 @@ __global__ void __launch_bounds__(256) k_occupancy(GridDesc g, const unsigned lo
     if (brick < nbricks) {
         unsigned long long sh = shellbits[brick];
         unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
-        unsigned long long any = sh | fl;
+        unsigned long long flm = 0;  // fill bits in Morton order
+        while (fl) {
+            const uint32_t l = (uint32_t)(__ffsll((long long)fl) - 1);
+            fl &= fl - 1ull;
+            flm |= 1ull << morton6(l & 3u, (l >> 2) & 3u, l >> 4);
+        }
+        unsigned long long any = sh | flm;
         uint32_t nb = g.nb0;
         uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
         while (any) {
@@ -1878,7 +1895,7 @@ __global__ void __launch_bounds__(256) k_read_state(GridDesc g, const unsigned l
     uint32_t m = morton6(x & 3u, y & 3u, z & 3u);
     uint8_t st = 0;
     if ((shellbits[w] >> m) & 1ull) st = 2;
-    else if (fillbits && ((fillbits[w] >> m) & 1ull)) st = 1;
+    else if (fillbits && ((fillbits[w] >> lin6(x & 3u, y & 3u, z & 3u)) & 1ull)) st = 1;
     out[i] = st;
 }
 
