@@ -581,18 +581,27 @@ __device__ __forceinline__ float lattice_value(uint32_t seed, int ix, int iy, in
 __device__ __forceinline__ float smoothw(float f) { return (f * f) * (3.0f - 2.0f * f); }
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
+// lattice values at the 8 corners of cell (ix, iy, iz), corner k at
+// (ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2))
+__device__ __forceinline__ void cell_corners(uint32_t seed, int ix, int iy, int iz, float c[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = lattice_value(seed, ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
+}
+// smoothstep-weighted trilinear blend of the corners: x, then y, then z
+__device__ __forceinline__ float cell_blend(const float c[8], float wx, float wy, float wz) {
+    float x00 = lerpf(c[0], c[1], wx), x10 = lerpf(c[2], c[3], wx);
+    float x01 = lerpf(c[4], c[5], wx), x11 = lerpf(c[6], c[7], wx);
+    float y0 = lerpf(x00, x10, wy), y1 = lerpf(x01, x11, wy);
+    return lerpf(y0, y1, wz);
+}
+
 __device__ float value_noise(uint32_t seed, float px, float py, float pz) {
     float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
     int ix = (int)fx, iy = (int)fy, iz = (int)fz;
     float wx = smoothw(px - fx), wy = smoothw(py - fy), wz = smoothw(pz - fz);
-    float c000 = lattice_value(seed, ix, iy, iz), c100 = lattice_value(seed, ix + 1, iy, iz);
-    float c010 = lattice_value(seed, ix, iy + 1, iz), c110 = lattice_value(seed, ix + 1, iy + 1, iz);
-    float c001 = lattice_value(seed, ix, iy, iz + 1), c101 = lattice_value(seed, ix + 1, iy, iz + 1);
-    float c011 = lattice_value(seed, ix, iy + 1, iz + 1), c111 = lattice_value(seed, ix + 1, iy + 1, iz + 1);
-    float x00 = lerpf(c000, c100, wx), x10 = lerpf(c010, c110, wx);
-    float x01 = lerpf(c001, c101, wx), x11 = lerpf(c011, c111, wx);
-    float y0 = lerpf(x00, x10, wy), y1 = lerpf(x01, x11, wy);
-    return lerpf(y0, y1, wz);
+    float c[8];
+    cell_corners(seed, ix, iy, iz, c);
+    return cell_blend(c, wx, wy, wz);
 }
 
 __device__ float fbm_voxel(const VolumeDesc& vd, int N, int x, int y, int z) {
@@ -607,6 +616,60 @@ __device__ float fbm_voxel(const VolumeDesc& vd, int N, int x, int y, int z) {
         amp = amp * vd.noise_gain;
     }
     return sum / norm;
+}
+
+// fbm_voxel of the 2x2x2 block at even (x0, y0, z0), voxel v at
+// (x0 + (v & 1), y0 + ((v >> 1) & 1), z0 + (v >> 2)), for the voxels whose bit
+// is set in `need`.  Per octave the corners of the first voxel's cell are
+// hashed once and reused by every voxel of the block in that cell (all of
+// them unless a cell boundary cuts the block); the per-voxel arithmetic is
+// that of fbm_voxel, octave by octave, so the results are identical.
+__device__ void fbm_block8(const VolumeDesc& vd, int N, int x0, int y0, int z0, uint32_t need, float out[8]) {
+    const float fN = (float)N;
+    float u[2], w[2], t[2];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        u[d] = ((float)(x0 + d) + 0.5f) / fN;
+        w[d] = ((float)(y0 + d) + 0.5f) / fN;
+        t[d] = ((float)(z0 + d) + 0.5f) / fN;
+    }
+    float sum[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) sum[v] = 0.0f;
+    float freq = vd.noise_base_freq, amp = 1.0f, norm = 0.0f;
+    for (int o = 0; o < vd.noise_octaves; ++o) {
+        const uint32_t seed = vd.noise_seed + (uint32_t)o * 0x9e3779b9u;
+        float px[2], py[2], pz[2], fx[2], fy[2], fz[2];
+        int ix[2], iy[2], iz[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            px[d] = u[d] * freq; py[d] = w[d] * freq; pz[d] = t[d] * freq;
+            fx[d] = floorf(px[d]); fy[d] = floorf(py[d]); fz[d] = floorf(pz[d]);
+            ix[d] = (int)fx[d]; iy[d] = (int)fy[d]; iz[d] = (int)fz[d];
+        }
+        float c[8];
+        cell_corners(seed, ix[0], iy[0], iz[0], c);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            if (!((need >> v) & 1u)) continue;
+            const int dx = v & 1, dy = (v >> 1) & 1, dz = v >> 2;
+            const float wx = smoothw(px[dx] - fx[dx]), wy = smoothw(py[dy] - fy[dy]), wz = smoothw(pz[dz] - fz[dz]);
+            float val;
+            if (ix[dx] == ix[0] && iy[dy] == iy[0] && iz[dz] == iz[0]) {
+                val = cell_blend(c, wx, wy, wz);
+            } else {
+                float e[8];
+                cell_corners(seed, ix[dx], iy[dy], iz[dz], e);
+                val = cell_blend(e, wx, wy, wz);
+            }
+            sum[v] = sum[v] + amp * val;
+        }
+        norm = norm + amp;
+        freq = freq * vd.noise_lacunarity;
+        amp = amp * vd.noise_gain;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) out[v] = sum[v] / norm;
 }
 
 // one wavefront per level-0 brick; lane = Morton index inside the brick
@@ -659,9 +722,9 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
 // is rounded to the storage format before it feeds the next level, exactly
 // like a level-by-level build.
 // ---------------------------------------------------------------------------
+// noise: the voxel's fbm value (used when the volume has noise and a != 0)
 template <int FMT>
-__device__ __forceinline__ float4 level0_value(const GridDesc& g, const VolumeDesc& vd, bool shell, bool fill, int x,
-                                               int y, int z) {
+__device__ __forceinline__ float4 level0_value(const VolumeDesc& vd, bool shell, bool fill, float noise) {
     float r, gg, b, a;
     if (vd.mode == 0) {
         r = gg = b = a = shell ? 1.0f : 0.0f;
@@ -669,7 +732,7 @@ __device__ __forceinline__ float4 level0_value(const GridDesc& g, const VolumeDe
         if (shell) { r = gg = b = a = 1.0f; }
         else if (fill) { r = gg = b = 0.0f; a = 1.0f; }
         else { r = gg = b = a = 0.0f; }
-        if (FMT != 0 && vd.noise_octaves > 0 && a != 0.0f) a = a * fbm_voxel(vd, g.N, x, y, z);
+        if (FMT != 0 && vd.noise_octaves > 0 && a != 0.0f) a = a * noise;
     }
     if (FMT == 0) return make_float4(r != 0.0f ? 255.0f : 0.0f, 0.0f, 0.0f, 0.0f);
     if (FMT == 2) return make_float4(h2f(f2h(r)), h2f(f2h(gg)), h2f(f2h(b)), h2f(f2h(a)));
@@ -731,14 +794,28 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
         const uint32_t brick = (gbz * nb0 + gby) * nb0 + gbx;
         const unsigned long long sh = shellbits[brick];
         const unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
+        // the thread's 8 texels (Morton q*8 .. q*8+7) are the 2x2x2 block at
+        // brick offset 2 * (q & 1, (q >> 1) & 1, q >> 2)
+        const uint32_t ox = 2u * (q & 1u), oy = 2u * ((q >> 1) & 1u), oz = 2u * (q >> 2);
+        uint32_t occ = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t m = q * 8u + c;
+            const bool on = ((sh >> m) & 1ull) || ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull);
+            occ |= on ? (1u << c) : 0u;
+        }
+        float nz[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) nz[c] = 1.0f;
+        if (FMT != 0 && vd.mode != 0 && vd.noise_octaves > 0 && occ)
+            fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), occ, nz);
         float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (uint32_t c = 0; c < 8; ++c) {
             const uint32_t m = q * 8u + c;
-            uint32_t tx, ty, tz;
-            demorton6(m, &tx, &ty, &tz);
-            float4 v = level0_value<FMT>(g, vd, ((sh >> m) & 1ull) != 0, ((fl >> lin6(tx, ty, tz)) & 1ull) != 0,
-                                         (int)(gbx * 4u + tx), (int)(gby * 4u + ty), (int)(gbz * 4u + tz));
+            float4 v = level0_value<FMT>(vd, ((sh >> m) & 1ull) != 0,
+                                         ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull) != 0,
+                                         nz[c]);
             if (FMT != 1) store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)brick * 64u + m, v);
             else stage[threadIdx.x * 8u + c] = v;
             acc = add4(acc, v);
