@@ -409,9 +409,42 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         // scaling once by their product rounds identically
         const float kx = fN * g.inv_range[0], ky = fN * g.inv_range[1], kz = fN * g.inv_range[2];
         const uint32_t uN = (uint32_t)g.N;
+        const uint32_t ubx = (uint32_t)bx0, uby = (uint32_t)by0, ubz = (uint32_t)bz0;
+        const uint32_t unx = (uint32_t)nbx, uny = (uint32_t)nby, unz = (uint32_t)nbz;
         for (int f = tid; f < nfr; f += blockDim.x) {
             const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
             const int n = f_n[f];
+            if (use_lds) {
+                // the LDS word of the current brick, its bits OR-ed in a
+                // register: a brick change costs one LDS atomic
+                int cur_w = -1;
+                unsigned long long cur_b = 0ull;
+                for (int k = 0; k < n; ++k) {
+                    const float i = (float)k * g.stepSize;
+                    const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
+                    const float lx = __builtin_amdgcn_fmed3f((qx - g.lo[0]) * kx, -2.0f, fN);
+                    const float ly = __builtin_amdgcn_fmed3f((qy - g.lo[1]) * ky, -2.0f, fN);
+                    const float lz = __builtin_amdgcn_fmed3f((qz - g.lo[2]) * kz, -2.0f, fN);
+                    const uint32_t ix = (uint32_t)(int)lx, iy = (uint32_t)(int)ly, iz = (uint32_t)(int)lz;
+                    const bool in = (ix | iy | iz) < uN;
+                    // brick offsets in the slab's box (unsigned: below the box wraps high)
+                    const uint32_t ux = (ix >> 2) - ubx, uy = (iy >> 2) - uby, uz = (iz >> 2) - ubz;
+                    const bool inbox = (ux < unx) & (uy < uny) & (uz < unz);
+                    const int w = (int)(__umul24(__umul24(uz, uny) + uy, unx) + ux);  // used when inbox
+                    const unsigned long long m = 1ull << ((ix & 3u) | ((iy << 2) & 0xcu) | ((iz << 4) & 0x30u));
+                    if (in && !inbox)  // outside the slab's box (not expected): straight to HBM
+                        atomicOr(&fillbits[((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2)], m);
+                    const bool take = in && inbox;
+                    if (take && w != cur_w) {
+                        if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
+                        cur_w = w;
+                        cur_b = 0ull;
+                    }
+                    cur_b |= take ? m : 0ull;
+                }
+                if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
+                continue;
+            }
             // brick word as one key (9 bits per axis: N <= 2048 gives <= 512
             // bricks per axis), bits of the current word OR-ed in a register
             uint32_t cur_key = 0xffffffffu;
