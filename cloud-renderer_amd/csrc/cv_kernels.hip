@@ -1146,26 +1146,102 @@ __device__ void replay_ties_block(int S, const uint16_t* s_idx, uint16_t* s_D, u
     }
 }
 
-__global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S, int npad,
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m, 64), hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Bitonic sort of the wave's 64 * E keys held in registers, key r = lane * E + e
+// (ascending): partners within a lane swap registers, partners in other
+// lanes are exchanged with cross-lane shuffles; no LDS and no barriers.
+template <int E>
+__device__ __forceinline__ void wave_bitonic(unsigned long long (&v)[E], int lane) {
+    constexpr int n = 64 * E;
+#pragma unroll
+    for (int k = 2; k <= n; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < E) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int p = e ^ j;
+                    if (p > e) {
+                        const bool up = ((lane * E + e) & k) == 0;
+                        const unsigned long long a = v[e], b = v[p];
+                        const bool sw = up ? (a > b) : (a < b);
+                        v[e] = sw ? b : a;
+                        v[p] = sw ? a : b;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int r = lane * E + e;
+                    const bool up = (r & k) == 0;
+                    const unsigned long long o = shfl_xor_u64(v[e], j / E);
+                    const bool keep_min = ((r & j) == 0) == up;
+                    const unsigned long long mn = o < v[e] ? o : v[e], mx = o < v[e] ? v[e] : o;
+                    v[e] = keep_min ? mn : mx;
+                }
+            }
+        }
+    }
+}
+
+// Sort of npad = 1024 * E keys (the first S real, the rest ~0) in one
+// workgroup of 1024: each wave bitonic-sorts its 64 * E keys in registers,
+// then four merge rounds pair the 16 sorted runs through LDS (merge path:
+// a binary search gives each thread its E outputs).  The keys are
+// (~distance bits << 32 | index), distinct for distinct billboards, so the
+// ascending order is the draw order up to runs of equal distances.
+template <int E>
+__global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S,
                                                    int32_t* __restrict__ order, int* __restrict__ tie_flag) {
     extern __shared__ unsigned long long sk[];
     __shared__ int tie;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    constexpr int npad = 1024 * E;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     if (tid == 0) tie = 0;
-    for (int i = tid; i < npad; i += nt) sk[i] = i < S ? keys_in[i] : ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= npad; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < npad; i += nt) {
-                int ixj = i ^ j;
-                if (ixj > i) {
-                    unsigned long long a = sk[i], b = sk[ixj];
-                    bool up = (i & k) == 0;
-                    if ((a > b) == up) { sk[i] = b; sk[ixj] = a; }
-                }
-            }
-            __syncthreads();
+    {
+        unsigned long long v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = tid * E + e;
+            v[e] = i < S ? keys_in[i] : ~0ull;
         }
+        wave_bitonic<E>(v, lane);
+#pragma unroll
+        for (int e = 0; e < E; ++e) sk[tid * E + e] = v[e];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int L = 64 * E; L < npad; L <<= 1) {
+        const int pos = tid * E;
+        const int base = pos & ~(2 * L - 1);
+        const int o = pos - base;
+        const unsigned long long* A = sk + base;
+        const unsigned long long* B = A + L;
+        // i = how many of the first o merged keys come from A (ties: A first)
+        int lo = max(0, o - L), hi = min(o, L);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (A[mid] <= B[o - 1 - mid]) lo = mid + 1;
+            else hi = mid;
+        }
+        int ia = lo, ib = o - lo;
+        unsigned long long out[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const unsigned long long a = ia < L ? A[ia] : ~0ull, b = ib < L ? B[ib] : ~0ull;
+            const bool take_a = ib >= L || (ia < L && a <= b);
+            out[e] = take_a ? a : b;
+            ia += take_a ? 1 : 0;
+            ib += take_a ? 0 : 1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) sk[pos + e] = out[e];
+        __syncthreads();
     }
     // Reuse the key storage: idx u16[npad] (bit 15 = same distance as the
     // previous key) | pos u16[npad] | at u16[npad].  Chunk c of 1024 keys is
@@ -2127,20 +2203,37 @@ size_t sort_scratch_words(int S) {
     return 5 * npad;
 }
 
+template <int E>
+static int launch_sort_lds(const unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s) {
+    static_assert(1024 * E <= kLdsSortMax, "one workgroup's LDS");
+    const int bytes = 1024 * E * (int)sizeof(unsigned long long);
+    if (bytes > 64 * 1024) {
+        static bool attr_set = false;  // raise the dynamic LDS limit once per instantiation
+        if (!attr_set) {
+            if (hipFuncSetAttribute((const void*)k_sort_lds<E>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+                hipSuccess)
+                return -1;
+            attr_set = true;
+        }
+    }
+    hipLaunchKernelGGL(k_sort_lds<E>, dim3(1), dim3(1024), (size_t)bytes, s, keys, S, order, tie_flag);
+    return 1024 * E;
+}
+
 int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s) {
     if (S <= 0) return 0;
     int npad = 2;
     while (npad < S) npad <<= 1;
     if (npad <= kLdsSortMax) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipFuncSetAttribute((const void*)k_sort_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kLdsSortMax * (int)sizeof(unsigned long long));
-            attr_set = true;
+        int r;
+        switch (npad < 1024 ? 1024 : npad) {
+            case 1024: r = launch_sort_lds<1>(keys, S, order, tie_flag, s); break;
+            case 2048: r = launch_sort_lds<2>(keys, S, order, tie_flag, s); break;
+            case 4096: r = launch_sort_lds<4>(keys, S, order, tie_flag, s); break;
+            case 8192: r = launch_sort_lds<8>(keys, S, order, tie_flag, s); break;
+            default: r = launch_sort_lds<16>(keys, S, order, tie_flag, s); break;
         }
-        hipLaunchKernelGGL(k_sort_lds, dim3(1), dim3(1024), (size_t)npad * sizeof(unsigned long long), s, keys, S, npad,
-                           order, tie_flag);
-        return npad;
+        if (r > 0) return r;  // else (LDS limit not raised): the global-memory sort below
     }
     unsigned long long* pk = keys + npad;
     uint32_t* g_idx = (uint32_t*)(keys + 2 * (size_t)npad);
