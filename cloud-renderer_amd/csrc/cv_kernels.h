@@ -156,7 +156,7 @@ struct BatchScene {
 // waves_per_group waves per workgroup, `groups` workgroups striding over the
 // items (groups = 0: one workgroup per waves_per_group items)
 void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total_tiles,
-                       int format, int cone_mode, int composite, int grey, int waves_per_group, int groups,
+                       int format, int cone_mode, int composite, int waves_per_group, int groups,
                        unsigned long long* counters, hipStream_t s);
 // binning into a caller-given tile_count array, and the heavy-first order of
 // an array of tile counts (scratch: 64 + ntiles ints)

@@ -705,43 +705,114 @@ __device__ void fbm_block8(const VolumeDesc& vd, int N, int x0, int y0, int z0, 
     for (int v = 0; v < 8; ++v) out[v] = sum[v] / norm;
 }
 
+// ---------------------------------------------------------------------------
+// level 0 and the mip chain
+//
+// RGBA levels hold (r, a, r+, a+) slots (x+ = min(x + 1, n - 1)).  Every
+// writer below fills both halves of its slots, except the fused build at a
+// region's last column: there the second half is written by the next region
+// (from its own first column), the first half by the region itself.
+// ---------------------------------------------------------------------------
+// (r, a) of a level-0 voxel; noise: its fbm value (used when the volume has
+// noise and a != 0); RGBA16F values are rounded to half
+template <int FMT>
+__device__ __forceinline__ float2 level0_ra(const VolumeDesc& vd, bool shell, bool fill, float noise) {
+    float r, a;
+    if (vd.mode == 0) {
+        r = a = shell ? 1.0f : 0.0f;
+    } else {
+        if (shell) { r = a = 1.0f; }
+        else if (fill) { r = 0.0f; a = 1.0f; }
+        else { r = a = 0.0f; }
+        if (vd.noise_octaves > 0 && a != 0.0f) a = a * noise;
+    }
+    if (FMT == 2) return make_float2(h2f(f2h(r)), h2f(f2h(a)));
+    return make_float2(r, a);
+}
+
+// parent of 8 children summed in order (RGBA16F rounded to half)
+template <int FMT>
+__device__ __forceinline__ float2 mip_parent2(float2 s, float cnt) {
+    float2 r = make_float2(s.x / cnt, s.y / cnt);
+    if (FMT == 2) r = make_float2(h2f(f2h(r.x)), h2f(f2h(r.y)));
+    return r;
+}
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+
+// slot stores and loads of the RGBA formats (levels indexed in texels)
+template <int FMT>
+__device__ __forceinline__ void store_pair(void* levels, uint64_t idx, float2 own, float2 nb) {
+    if (FMT == 1) {
+        ((float4*)levels)[idx] = make_float4(own.x, own.y, nb.x, nb.y);
+    } else {
+        ((uint2*)levels)[idx] = make_uint2(f2h(own.x) | (f2h(own.y) << 16), f2h(nb.x) | (f2h(nb.y) << 16));
+    }
+}
+template <int FMT>
+__device__ __forceinline__ void store_half(void* levels, uint64_t idx, int half, float2 v) {
+    if (FMT == 1) ((float2*)levels)[2 * idx + half] = v;
+    else ((uint32_t*)levels)[2 * idx + half] = f2h(v.x) | (f2h(v.y) << 16);
+}
+__device__ __forceinline__ float2 load_ra(const void* levels, int format, uint64_t idx) {
+    if (format == 1) return ((const float2*)levels)[2 * idx];
+    const uint32_t u = ((const uint32_t*)levels)[2 * idx];
+    return make_float2(h2f(u & 0xffffu), h2f(u >> 16));
+}
+__device__ __forceinline__ void store_pair_rt(void* levels, int format, uint64_t idx, float2 own, float2 nb) {
+    if (format == 1) store_pair<1>(levels, idx, own, nb);
+    else store_pair<2>(levels, idx, own, nb);
+}
+
 // one wavefront per level-0 brick; lane = Morton index inside the brick
+// (sides that are not a multiple of 16; the fused build covers the others)
 __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const unsigned long long* __restrict__ shellbits,
                                                 const unsigned long long* __restrict__ fillbits, void* __restrict__ levels,
                                                 uint32_t nbricks) {
-    uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
-    uint32_t m = threadIdx.x & 63u;
-    if (brick >= nbricks) return;
-    unsigned long long sh = shellbits[brick];
-    unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
-    uint32_t nb = g.nb0;
-    uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
+    const uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t m = threadIdx.x & 63u;
+    if (brick >= nbricks) return;  // whole waves
+    const unsigned long long sh = shellbits[brick];
+    const unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
+    const uint32_t nb = g.nb0;
+    const uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
     uint32_t lx, ly, lz;
     demorton6(m, &lx, &ly, &lz);
-    int x = (int)(bx * 4u + lx), y = (int)(by * 4u + ly), z = (int)(bz * 4u + lz);
-    bool inside = x < g.N && y < g.N && z < g.N;
-    bool shell = inside && ((sh >> m) & 1ull);
-    bool fill = inside && ((fl >> lin6(lx, ly, lz)) & 1ull);
-    float r, gg, b, a;
-    if (vd.mode == 0) {
-        r = gg = b = a = shell ? 1.0f : 0.0f;
-    } else {
-        if (shell) { r = gg = b = a = 1.0f; }
-        else if (fill) { r = gg = b = 0.0f; a = 1.0f; }
-        else { r = gg = b = a = 0.0f; }
-        if (vd.noise_octaves > 0 && a != 0.0f) a = a * fbm_voxel(vd, g.N, x, y, z);
-    }
-    uint64_t idx = vd.lv[0].base + (uint64_t)brick * 64u + m;
+    const int x = (int)(bx * 4u + lx), y = (int)(by * 4u + ly), z = (int)(bz * 4u + lz);
+    const bool inside = x < g.N && y < g.N && z < g.N;
+    const bool shell = inside && ((sh >> m) & 1ull);
+    const bool fill = inside && ((fl >> lin6(lx, ly, lz)) & 1ull);
+    const uint64_t idx = vd.lv[0].base + (uint64_t)brick * 64u + m;
     if (vd.format == 0) {
-        ((uint8_t*)levels)[idx] = r != 0.0f ? (uint8_t)255 : (uint8_t)0;
-    } else if (vd.format == 1) {
-        ((float4*)levels)[idx] = make_float4(r, a, gg, b);
-    } else {
-        uint2 v;
-        v.x = f2h(r) | (f2h(a) << 16);
-        v.y = f2h(gg) | (f2h(b) << 16);
-        ((uint2*)levels)[idx] = v;
+        ((uint8_t*)levels)[idx] = shell ? (uint8_t)255 : (uint8_t)0;
+        return;
     }
+    const bool noisy = vd.mode != 0 && vd.noise_octaves > 0;
+    float2 v = vd.format == 1 ? level0_ra<1>(vd, shell, fill, 1.0f) : level0_ra<2>(vd, shell, fill, 1.0f);
+    if (noisy && v.y != 0.0f) {
+        const float nz = fbm_voxel(vd, g.N, x, y, z);
+        v = vd.format == 1 ? level0_ra<1>(vd, shell, fill, nz) : level0_ra<2>(vd, shell, fill, nz);
+    }
+    // the +x neighbour: the lane of (lx + 1) in this brick, the next brick's
+    // first column for lx = 3, the voxel itself at x = N - 1
+    const int src = (int)morton6(min(lx + 1u, 3u), ly, lz);
+    float2 nbv = make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+    if (!inside) {
+        v = nbv = make_float2(0.0f, 0.0f);
+    } else if (x + 1 >= g.N) {
+        nbv = v;
+    } else if (lx == 3u) {
+        const uint32_t nbrick = brick + 1u;
+        const uint32_t mm = morton6(0u, ly, lz);
+        const bool nsh = (shellbits[nbrick] >> mm) & 1ull;
+        const bool nfi = fillbits ? ((fillbits[nbrick] >> lin6(0u, ly, lz)) & 1ull) != 0 : false;
+        float2 w = vd.format == 1 ? level0_ra<1>(vd, nsh, nfi, 1.0f) : level0_ra<2>(vd, nsh, nfi, 1.0f);
+        if (noisy && w.y != 0.0f) {
+            const float nz = fbm_voxel(vd, g.N, x + 1, y, z);
+            w = vd.format == 1 ? level0_ra<1>(vd, nsh, nfi, nz) : level0_ra<2>(vd, nsh, nfi, nz);
+        }
+        nbv = w;
+    }
+    store_pair_rt(levels, vd.format, idx, v, nbv);
 }
 
 // ---------------------------------------------------------------------------
@@ -753,74 +824,29 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
 // the mip rule), so a thread owning 8 consecutive texels forms its level-1
 // texel in registers.  Levels 2..4 are reduced from LDS.  Every level's value
 // is rounded to the storage format before it feeds the next level, exactly
-// like a level-by-level build.
+// like a level-by-level build.  Level 0 is staged in LDS half a region at a
+// time (16 x 16 x 8 texels), which gives the +x neighbours of all but the
+// region's last column; those are this region's first column, which it
+// writes into the previous region's last slots (second halves only).
 // ---------------------------------------------------------------------------
-// noise: the voxel's fbm value (used when the volume has noise and a != 0)
-template <int FMT>
-__device__ __forceinline__ float4 level0_value(const VolumeDesc& vd, bool shell, bool fill, float noise) {
-    float r, gg, b, a;
-    if (vd.mode == 0) {
-        r = gg = b = a = shell ? 1.0f : 0.0f;
-    } else {
-        if (shell) { r = gg = b = a = 1.0f; }
-        else if (fill) { r = gg = b = 0.0f; a = 1.0f; }
-        else { r = gg = b = a = 0.0f; }
-        if (FMT != 0 && vd.noise_octaves > 0 && a != 0.0f) a = a * noise;
-    }
-    if (FMT == 0) return make_float4(r != 0.0f ? 255.0f : 0.0f, 0.0f, 0.0f, 0.0f);
-    if (FMT == 2) return make_float4(h2f(f2h(r)), h2f(f2h(gg)), h2f(f2h(b)), h2f(f2h(a)));
-    return make_float4(r, gg, b, a);
-}
-
-// parent of 8 children summed in order; R8 sums hold integers
-template <int FMT>
-__device__ __forceinline__ float4 mip_parent(float4 s) {
-    if (FMT == 0) {
-        uint32_t v = ((uint32_t)s.x + 4u) / 8u;
-        return make_float4((float)v, 0.0f, 0.0f, 0.0f);
-    }
-    float4 r = make_float4(s.x / 8.0f, s.y / 8.0f, s.z / 8.0f, s.w / 8.0f);
-    if (FMT == 2) r = make_float4(h2f(f2h(r.x)), h2f(f2h(r.y)), h2f(f2h(r.z)), h2f(f2h(r.w)));
-    return r;
-}
-
-template <int FMT>
-__device__ __forceinline__ void store_texel(void* levels, uint64_t idx, float4 v) {
-    if (FMT == 0) {
-        ((uint8_t*)levels)[idx] = (uint8_t)(uint32_t)v.x;
-    } else if (FMT == 1) {
-        ((float4*)levels)[idx] = make_float4(v.x, v.w, v.y, v.z);  // stored (r, a, g, b)
-    } else {
-        uint2 u;
-        u.x = f2h(v.x) | (f2h(v.w) << 16);
-        u.y = f2h(v.y) | (f2h(v.z) << 16);
-        ((uint2*)levels)[idx] = u;
-    }
-}
-
-__device__ __forceinline__ float4 add4(float4 a, float4 b) {
-    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-}
-
 template <int FMT>
 __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
                                                      const unsigned long long* __restrict__ shellbits,
                                                      const unsigned long long* __restrict__ fillbits,
                                                      void* __restrict__ levels, int nlev, uint32_t rpa) {
-    __shared__ float4 l1[512];
-    __shared__ float4 l2[64];
-    __shared__ float4 l3[8];
-    // level-0 texels of half a region (32 bricks) staged for coalesced stores:
-    // a thread computes 8 Morton-consecutive texels (one level-1 parent), a
-    // store instruction then writes 64 consecutive texels of one brick
-    // (FP32 only: the FP16 build is bound by the noise evaluation and keeps
-    // its occupancy)
-    __shared__ float4 stage[FMT == 1 ? 2048 : 1];
+    static_assert(FMT == 1 || FMT == 2, "RGBA formats");
+    __shared__ float2 stage[2048];  // [brick of the half region][Morton]
+    __shared__ float2 l1[512];
+    __shared__ float2 l2[64];
+    __shared__ float2 l3[8];
+    const uint32_t tid = threadIdx.x;
     const uint32_t r = blockIdx.x;
     const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
+    const bool last_x = rx + 1u == rpa;
     const uint32_t nb0 = g.nb0;
-    for (int rep = 0; rep < 2; ++rep) {
-        const uint32_t grp = threadIdx.x + rep * 256u;
+    const bool noisy = vd.mode != 0 && vd.noise_octaves > 0;
+    for (uint32_t rep = 0; rep < 2; ++rep) {
+        const uint32_t grp = tid + rep * 256u;
         const uint32_t lb = grp >> 3, q = grp & 7u;
         const uint32_t lbx = lb & 3u, lby = (lb >> 2) & 3u, lbz = lb >> 4;
         const uint32_t gbx = 4u * rx + lbx, gby = 4u * ry + lby, gbz = 4u * rz + lbz;
@@ -840,69 +866,178 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
         float nz[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) nz[c] = 1.0f;
-        if (FMT != 0 && vd.mode != 0 && vd.noise_octaves > 0 && occ)
+        if (noisy && occ)
             fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), occ, nz);
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
         for (uint32_t c = 0; c < 8; ++c) {
             const uint32_t m = q * 8u + c;
-            float4 v = level0_value<FMT>(vd, ((sh >> m) & 1ull) != 0,
-                                         ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull) != 0,
-                                         nz[c]);
-            if (FMT != 1) store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)brick * 64u + m, v);
-            else stage[threadIdx.x * 8u + c] = v;
-            acc = add4(acc, v);
-        }
-        if (FMT == 1) {
-            __syncthreads();
-            // 2048 staged texels = 32 bricks of this half region, 64 per store
-            for (uint32_t i = threadIdx.x; i < 2048u; i += 256u) {
-                const uint32_t sb = rep * 32u + (i >> 6), sm = i & 63u;
-                const uint32_t sbx = 4u * rx + (sb & 3u), sby = 4u * ry + ((sb >> 2) & 3u), sbz = 4u * rz + (sb >> 4);
-                const uint32_t sbrick = (sbz * nb0 + sby) * nb0 + sbx;
-                store_texel<FMT>(levels, vd.lv[0].base + (uint64_t)sbrick * 64u + sm, stage[i]);
-            }
-            __syncthreads();
+            const float2 v = level0_ra<FMT>(vd, ((sh >> m) & 1ull) != 0,
+                                            ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull) != 0,
+                                            nz[c]);
+            stage[tid * 8u + c] = v;
+            acc = add2(acc, v);
         }
         if (nlev > 1) {
-            float4 p = mip_parent<FMT>(acc);
             const uint32_t x1 = 2u * lbx + (q & 1u), y1 = 2u * lby + ((q >> 1) & 1u), z1 = 2u * lbz + (q >> 2);
-            store_texel<FMT>(levels, brick_addr(vd.lv[1], 8u * rx + x1, 8u * ry + y1, 8u * rz + z1), p);
+            l1[(z1 * 8u + y1) * 8u + x1] = mip_parent2<FMT>(acc, 8.0f);
+        }
+        __syncthreads();
+        if (rx > 0u && tid < 128u) {
+            // first column (x = 0) -> second halves of the previous region's x = 15
+            const uint32_t ey = tid & 15u, ez = tid >> 4;
+            const uint32_t sbl = ((ez >> 2) << 4) | ((ey >> 2) << 2);
+            const uint32_t pbrick = ((4u * rz + 2u * rep + (ez >> 2)) * nb0 + 4u * ry + (ey >> 2)) * nb0 + 4u * rx - 1u;
+            store_half<FMT>(levels, vd.lv[0].base + (uint64_t)pbrick * 64u + morton6(3u, ey & 3u, ez & 3u), 1,
+                            stage[(sbl << 6) | morton6(0u, ey & 3u, ez & 3u)]);
+        }
+        // 2048 staged texels = 32 bricks of this half region, 64 per store
+        for (uint32_t i = tid; i < 2048u; i += 256u) {
+            const uint32_t sbl = i >> 6, sm = i & 63u;
+            const uint32_t sbx = sbl & 3u, sby = (sbl >> 2) & 3u, sbz = sbl >> 4;
+            uint32_t tx, ty, tz;
+            demorton6(sm, &tx, &ty, &tz);
+            const uint32_t sbrick = ((4u * rz + 2u * rep + sbz) * nb0 + 4u * ry + sby) * nb0 + 4u * rx + sbx;
+            const uint64_t idx = vd.lv[0].base + (uint64_t)sbrick * 64u + sm;
+            if (tx < 3u) store_pair<FMT>(levels, idx, stage[i], stage[(i & ~63u) | morton6(tx + 1u, ty, tz)]);
+            else if (sbx < 3u) store_pair<FMT>(levels, idx, stage[i], stage[((sbl + 1u) << 6) | morton6(0u, ty, tz)]);
+            else if (last_x) store_pair<FMT>(levels, idx, stage[i], stage[i]);
+            else store_half<FMT>(levels, idx, 0, stage[i]);
+        }
+        __syncthreads();  // the next half reuses stage
+    }
+    if (nlev <= 1) return;
+    // level 1 (l1 complete): the last column's neighbour is in the next region
+    for (uint32_t i = tid; i < 512u; i += 256u) {
+        const uint32_t x1 = i & 7u, y1 = (i >> 3) & 7u, z1 = i >> 6;
+        const uint64_t idx = brick_addr(vd.lv[1], 8u * rx + x1, 8u * ry + y1, 8u * rz + z1);
+        if (x1 < 7u) store_pair<FMT>(levels, idx, l1[i], l1[i + 1u]);
+        else if (last_x) store_pair<FMT>(levels, idx, l1[i], l1[i]);
+        else store_half<FMT>(levels, idx, 0, l1[i]);
+    }
+    if (rx > 0u && tid < 64u) {
+        const uint32_t y1 = tid & 7u, z1 = tid >> 3;
+        store_half<FMT>(levels, brick_addr(vd.lv[1], 8u * rx - 1u, 8u * ry + y1, 8u * rz + z1), 1,
+                        l1[(z1 * 8u + y1) * 8u]);
+    }
+    if (nlev <= 2) return;
+    if (tid < 64u) {
+        uint32_t x, y, z;
+        demorton6(tid, &x, &y, &z);
+        float2 s = make_float2(0.0f, 0.0f);
+        for (uint32_t dz = 0; dz < 2; ++dz)
+            for (uint32_t dy = 0; dy < 2; ++dy)
+                for (uint32_t dx = 0; dx < 2; ++dx) s = add2(s, l1[((2u * z + dz) * 8u + 2u * y + dy) * 8u + 2u * x + dx]);
+        l2[(z * 4u + y) * 4u + x] = mip_parent2<FMT>(s, 8.0f);
+    }
+    __syncthreads();
+    if (tid < 64u) {
+        uint32_t x, y, z;
+        demorton6(tid, &x, &y, &z);
+        const uint32_t i = (z * 4u + y) * 4u + x;
+        const uint64_t idx = brick_addr(vd.lv[2], 4u * rx + x, 4u * ry + y, 4u * rz + z);
+        if (x < 3u) store_pair<FMT>(levels, idx, l2[i], l2[i + 1u]);
+        else if (last_x) store_pair<FMT>(levels, idx, l2[i], l2[i]);
+        else store_half<FMT>(levels, idx, 0, l2[i]);
+        if (rx > 0u && x == 0u)
+            store_half<FMT>(levels, brick_addr(vd.lv[2], 4u * rx - 1u, 4u * ry + y, 4u * rz + z), 1, l2[i]);
+    }
+    if (nlev <= 3) return;
+    if (tid < 8u) {
+        const uint32_t x = tid & 1u, y = (tid >> 1) & 1u, z = tid >> 2;
+        float2 s = make_float2(0.0f, 0.0f);
+        for (uint32_t dz = 0; dz < 2; ++dz)
+            for (uint32_t dy = 0; dy < 2; ++dy)
+                for (uint32_t dx = 0; dx < 2; ++dx) s = add2(s, l2[((2u * z + dz) * 4u + 2u * y + dy) * 4u + 2u * x + dx]);
+        l3[(z * 2u + y) * 2u + x] = mip_parent2<FMT>(s, 8.0f);
+    }
+    __syncthreads();
+    if (tid < 8u) {
+        const uint32_t x = tid & 1u, y = (tid >> 1) & 1u, z = tid >> 2;
+        const uint32_t i = (z * 2u + y) * 2u + x;
+        const uint64_t idx = brick_addr(vd.lv[3], 2u * rx + x, 2u * ry + y, 2u * rz + z);
+        if (x == 0u) store_pair<FMT>(levels, idx, l3[i], l3[i + 1u]);
+        else if (last_x) store_pair<FMT>(levels, idx, l3[i], l3[i]);
+        else store_half<FMT>(levels, idx, 0, l3[i]);
+        if (rx > 0u && x == 0u)
+            store_half<FMT>(levels, brick_addr(vd.lv[3], 2u * rx - 1u, 2u * ry + y, 2u * rz + z), 1, l3[i]);
+    }
+    if (nlev <= 4) return;
+    if (tid == 0u) {
+        float2 s = make_float2(0.0f, 0.0f);
+        for (int c = 0; c < 8; ++c) s = add2(s, l3[c]);
+        const float2 p = mip_parent2<FMT>(s, 8.0f);
+        const uint64_t idx = brick_addr(vd.lv[4], rx, ry, rz);
+        if (last_x) store_pair<FMT>(levels, idx, p, p);
+        else store_half<FMT>(levels, idx, 0, p);
+        if (rx > 0u) store_half<FMT>(levels, brick_addr(vd.lv[4], rx - 1u, ry, rz), 1, p);
+    }
+}
+
+// R8 variant of the fused build (one byte per texel, no neighbour halves)
+__global__ void __launch_bounds__(256) k_build_fused_r8(GridDesc g, VolumeDesc vd,
+                                                        const unsigned long long* __restrict__ shellbits,
+                                                        void* __restrict__ levels, int nlev, uint32_t rpa) {
+    __shared__ uint32_t l1[512];
+    __shared__ uint32_t l2[64];
+    __shared__ uint32_t l3[8];
+    const uint32_t r = blockIdx.x;
+    const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
+    const uint32_t nb0 = g.nb0;
+    uint8_t* out = (uint8_t*)levels;
+    for (uint32_t rep = 0; rep < 2; ++rep) {
+        const uint32_t grp = threadIdx.x + rep * 256u;
+        const uint32_t lb = grp >> 3, q = grp & 7u;
+        const uint32_t lbx = lb & 3u, lby = (lb >> 2) & 3u, lbz = lb >> 4;
+        const uint32_t brick = ((4u * rz + lbz) * nb0 + 4u * ry + lby) * nb0 + 4u * rx + lbx;
+        const unsigned long long sh = shellbits[brick];
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t m = q * 8u + c;
+            const uint32_t v = ((sh >> m) & 1ull) ? 255u : 0u;
+            out[vd.lv[0].base + (uint64_t)brick * 64u + m] = (uint8_t)v;
+            acc += v;
+        }
+        if (nlev > 1) {
+            const uint32_t p = (acc + 4u) / 8u;
+            const uint32_t x1 = 2u * lbx + (q & 1u), y1 = 2u * lby + ((q >> 1) & 1u), z1 = 2u * lbz + (q >> 2);
+            out[brick_addr(vd.lv[1], 8u * rx + x1, 8u * ry + y1, 8u * rz + z1)] = (uint8_t)p;
             l1[(z1 * 8u + y1) * 8u + x1] = p;
         }
     }
     if (nlev <= 2) return;
     __syncthreads();
     const uint32_t t = threadIdx.x;
-    if (t < 64) {
+    if (t < 64u) {
         uint32_t x, y, z;
         demorton6(t, &x, &y, &z);
-        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t s = 0;
         for (uint32_t dz = 0; dz < 2; ++dz)
             for (uint32_t dy = 0; dy < 2; ++dy)
-                for (uint32_t dx = 0; dx < 2; ++dx) s = add4(s, l1[((2u * z + dz) * 8u + 2u * y + dy) * 8u + 2u * x + dx]);
-        float4 p = mip_parent<FMT>(s);
-        store_texel<FMT>(levels, brick_addr(vd.lv[2], 4u * rx + x, 4u * ry + y, 4u * rz + z), p);
+                for (uint32_t dx = 0; dx < 2; ++dx) s += l1[((2u * z + dz) * 8u + 2u * y + dy) * 8u + 2u * x + dx];
+        const uint32_t p = (s + 4u) / 8u;
+        out[brick_addr(vd.lv[2], 4u * rx + x, 4u * ry + y, 4u * rz + z)] = (uint8_t)p;
         l2[(z * 4u + y) * 4u + x] = p;
     }
     if (nlev <= 3) return;
     __syncthreads();
-    if (t < 8) {
-        uint32_t x = t & 1u, y = (t >> 1) & 1u, z = t >> 2;
-        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t < 8u) {
+        const uint32_t x = t & 1u, y = (t >> 1) & 1u, z = t >> 2;
+        uint32_t s = 0;
         for (uint32_t dz = 0; dz < 2; ++dz)
             for (uint32_t dy = 0; dy < 2; ++dy)
-                for (uint32_t dx = 0; dx < 2; ++dx) s = add4(s, l2[((2u * z + dz) * 4u + 2u * y + dy) * 4u + 2u * x + dx]);
-        float4 p = mip_parent<FMT>(s);
-        store_texel<FMT>(levels, brick_addr(vd.lv[3], 2u * rx + x, 2u * ry + y, 2u * rz + z), p);
+                for (uint32_t dx = 0; dx < 2; ++dx) s += l2[((2u * z + dz) * 4u + 2u * y + dy) * 4u + 2u * x + dx];
+        const uint32_t p = (s + 4u) / 8u;
+        out[brick_addr(vd.lv[3], 2u * rx + x, 2u * ry + y, 2u * rz + z)] = (uint8_t)p;
         l3[(z * 2u + y) * 2u + x] = p;
     }
     if (nlev <= 4) return;
     __syncthreads();
-    if (t == 0) {
-        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        for (int c = 0; c < 8; ++c) s = add4(s, l3[c]);
-        store_texel<FMT>(levels, brick_addr(vd.lv[4], rx, ry, rz), mip_parent<FMT>(s));
+    if (t == 0u) {
+        uint32_t s = 0;
+        for (int c = 0; c < 8; ++c) s += l3[c];
+        out[brick_addr(vd.lv[4], rx, ry, rz)] = (uint8_t)((s + 4u) / 8u);
     }
 }
 
@@ -914,6 +1049,22 @@ __device__ __forceinline__ void child_range(int p, int nparent, int nchild, int*
     *c0 = min(2 * p, nchild - 1);
     *c1 = min(2 * p + 1, nchild - 1);
     if (p == nparent - 1) *c1 = nchild - 1;
+}
+
+// (r, a) of RGBA parent (x, y, z) from the children's (r, a)
+__device__ __forceinline__ float2 mip_parent_at(const VolumeDesc& vd, const LevelDesc& Cl, const void* levels, int x,
+                                                int y, int z, int np) {
+    const int nc = (int)Cl.n;
+    int x0, x1, y0, y1, z0, z1;
+    child_range(x, np, nc, &x0, &x1);
+    child_range(y, np, nc, &y0, &y1);
+    child_range(z, np, nc, &z0, &z1);
+    const int cnt = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+    float2 s = make_float2(0.0f, 0.0f);
+    for (int cz = z0; cz <= z1; ++cz)
+        for (int cy = y0; cy <= y1; ++cy)
+            for (int cx = x0; cx <= x1; ++cx) s = add2(s, load_ra(levels, vd.format, brick_addr(Cl, cx, cy, cz)));
+    return vd.format == 1 ? mip_parent2<1>(s, (float)cnt) : mip_parent2<2>(s, (float)cnt);
 }
 
 __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restrict__ levels, uint32_t ntexels) {
@@ -929,16 +1080,15 @@ __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restr
     int np = (int)P.n, nc = (int)Cl.n;
     if (x >= np || y >= np || z >= np) {
         if (vd.format == 0) ((uint8_t*)levels)[out] = 0;
-        else if (vd.format == 1) ((float4*)levels)[out] = make_float4(0.f, 0.f, 0.f, 0.f);
-        else ((uint2*)levels)[out] = make_uint2(0u, 0u);
+        else store_pair_rt(levels, vd.format, out, make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f));
         return;
     }
-    int x0, x1, y0, y1, z0, z1;
-    child_range(x, np, nc, &x0, &x1);
-    child_range(y, np, nc, &y0, &y1);
-    child_range(z, np, nc, &z0, &z1);
-    int cnt = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
     if (vd.format == 0) {
+        int x0, x1, y0, y1, z0, z1;
+        child_range(x, np, nc, &x0, &x1);
+        child_range(y, np, nc, &y0, &y1);
+        child_range(z, np, nc, &z0, &z1);
+        int cnt = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
         uint32_t sum = 0;
         for (int cz = z0; cz <= z1; ++cz)
             for (int cy = y0; cy <= y1; ++cy)
@@ -946,31 +1096,10 @@ __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restr
         ((uint8_t*)levels)[out] = (uint8_t)((sum + (uint32_t)cnt / 2u) / (uint32_t)cnt);
         return;
     }
-    // every stored channel slot is averaged on its own (slot order is irrelevant)
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-    for (int cz = z0; cz <= z1; ++cz)
-        for (int cy = y0; cy <= y1; ++cy)
-            for (int cx = x0; cx <= x1; ++cx) {
-                uint64_t a = brick_addr(Cl, cx, cy, cz);
-                float4 v;
-                if (vd.format == 1) {
-                    v = ((const float4*)levels)[a];
-                } else {
-                    uint2 u = ((const uint2*)levels)[a];
-                    v = make_float4(h2f(u.x & 0xffffu), h2f(u.x >> 16), h2f(u.y & 0xffffu), h2f(u.y >> 16));
-                }
-                s0 = s0 + v.x; s1 = s1 + v.y; s2 = s2 + v.z; s3 = s3 + v.w;
-            }
-    float fc = (float)cnt;
-    float4 r = make_float4(s0 / fc, s1 / fc, s2 / fc, s3 / fc);
-    if (vd.format == 1) {
-        ((float4*)levels)[out] = r;
-    } else {
-        uint2 v;
-        v.x = f2h(r.x) | (f2h(r.y) << 16);
-        v.y = f2h(r.z) | (f2h(r.w) << 16);
-        ((uint2*)levels)[out] = v;
-    }
+    // the parent and its +x neighbour, each averaged from its own children
+    const float2 own = mip_parent_at(vd, Cl, levels, x, y, z, np);
+    const float2 nb = x + 1 < np ? mip_parent_at(vd, Cl, levels, x + 1, y, z, np) : own;
+    store_pair_rt(levels, vd.format, out, own, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -1383,23 +1512,15 @@ __global__ void __launch_bounds__(256) k_tile_scatter(const int* __restrict__ bu
 // ---------------------------------------------------------------------------
 // cone trace (res/conetrace_frag.glsl:64-79, 176-200) + compositing
 // ---------------------------------------------------------------------------
-// RGBA of one texel (stored as r, a, g, b)
-template <int FMT>
-__device__ __forceinline__ float4 fetch4(const void* __restrict__ vol, uint64_t i) {
-    if (FMT == 1) {
-        float4 t = ((const float4*)vol)[i];
-        return make_float4(t.x, t.z, t.w, t.y);
-    }
-    uint2 u = ((const uint2*)vol)[i];
-    return make_float4(h2f(u.x & 0xffffu), h2f(u.y & 0xffffu), h2f(u.y >> 16), h2f(u.x >> 16));
-}
-// (r, a) of one RGBA texel
-template <int FMT>
-__device__ __forceinline__ float2 fetch_ra(const void* __restrict__ vol, uint64_t i) {
-    if (FMT == 1) return ((const float2*)vol)[2 * i];
-    uint32_t u = ((const uint32_t*)vol)[2 * i];
-    return make_float2(h2f(u & 0xffffu), h2f(u >> 16));
-}
+// Texel layout of the RGBA formats.  Every RGBA volume is grey (r = g = b:
+// the stores are (1,1,1,1), (0,0,0,1) or (s,s,s,s), the mips average the
+// channels alike and noise scales only a), so a texel slot holds (r, a) of
+// its texel and (r, a) of the +x neighbour x+ = min(x + 1, n - 1):
+//   RGBA32F: float4 (r, a, r+, a+);  RGBA16F: halves (r, a, r+, a+).
+// A trilinear footprint then takes one slot per y/z corner (4 loads, not 8);
+// readback rebuilds RGBA as (r, r, r, a).
+
+// r of one texel (R8, or the r of an RGBA slot)
 template <int FMT>
 __device__ __forceinline__ float fetch1(const void* __restrict__ vol, uint64_t i) {
     if (FMT == 0) return (float)((const uint8_t*)vol)[i] / 255.0f;
@@ -1407,75 +1528,50 @@ __device__ __forceinline__ float fetch1(const void* __restrict__ vol, uint64_t i
     return h2f(((const unsigned short*)vol)[i * 4]);
 }
 
-// GL trilinear filtering of one level with CLAMP_TO_EDGE, lerps x, y, z
-template <int FMT, int NCH>
-__device__ __forceinline__ float4 trilinear(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
-                                            float sz) {
+// bricked-layout offsets of the clamped corner coordinates of one axis
+struct AxisTaps {
+    float u;      // texel-space coordinate - 0.5
+    float fl;     // floor(u)
+    int i;        // (int)fl
+    uint32_t c0, c1;  // min(max(i, 0), n - 1), min(max(i + 1, 0), n - 1)
+};
+__device__ __forceinline__ AxisTaps axis_taps(float s, float fn, int nm1) {
+    AxisTaps a;
+    a.u = fmaf(fminf(fmaxf(s, -1.0f), 2.0f), fn, -0.5f);
+    a.fl = floorf(a.u);
+    a.i = (int)a.fl;
+    a.c0 = (uint32_t)min(max(a.i, 0), nm1);
+    a.c1 = (uint32_t)min(max(a.i + 1, 0), nm1);
+    return a;
+}
+
+// GL trilinear filtering of r in one level with CLAMP_TO_EDGE, lerps x, y, z
+template <int FMT>
+__device__ __forceinline__ float trilinear_r(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
+                                             float sz) {
     const float fn = (float)L.n;
     const int nm1 = (int)L.n - 1;
-    float ux = fmaf(fminf(fmaxf(sx, -1.0f), 2.0f), fn, -0.5f);
-    float uy = fmaf(fminf(fmaxf(sy, -1.0f), 2.0f), fn, -0.5f);
-    float uz = fmaf(fminf(fmaxf(sz, -1.0f), 2.0f), fn, -0.5f);
-    float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
-    float ax = ux - flx, ay = uy - fly, az = uz - flz;
-    int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    uint32_t x0 = (uint32_t)min(max(ix, 0), nm1), x1 = (uint32_t)min(max(ix + 1, 0), nm1);
-    uint32_t y0 = (uint32_t)min(max(iy, 0), nm1), y1 = (uint32_t)min(max(iy + 1, 0), nm1);
-    uint32_t z0 = (uint32_t)min(max(iz, 0), nm1), z1 = (uint32_t)min(max(iz + 1, 0), nm1);
+    const AxisTaps ax = axis_taps(sx, fn, nm1), ay = axis_taps(sy, fn, nm1), az = axis_taps(sz, fn, nm1);
+    const float wx = ax.u - ax.fl, wy = ay.u - ay.fl, wz = az.u - az.fl;
     const uint32_t nb = L.nb;
-    uint32_t X0 = ((x0 >> 2) << 6) | spread2(x0 & 3u), X1 = ((x1 >> 2) << 6) | spread2(x1 & 3u);
-    uint32_t Y0 = (((y0 >> 2) * nb) << 6) | (spread2(y0 & 3u) << 1);
-    uint32_t Y1 = (((y1 >> 2) * nb) << 6) | (spread2(y1 & 3u) << 1);
-    uint32_t Z0 = (((z0 >> 2) * nb * nb) << 6) | (spread2(z0 & 3u) << 2);
-    uint32_t Z1 = (((z1 >> 2) * nb * nb) << 6) | (spread2(z1 & 3u) << 2);
+    const uint32_t X0 = ((ax.c0 >> 2) << 6) | spread2(ax.c0 & 3u), X1 = ((ax.c1 >> 2) << 6) | spread2(ax.c1 & 3u);
+    const uint32_t Y0 = (((ay.c0 >> 2) * nb) << 6) | (spread2(ay.c0 & 3u) << 1);
+    const uint32_t Y1 = (((ay.c1 >> 2) * nb) << 6) | (spread2(ay.c1 & 3u) << 1);
+    const uint32_t Z0 = (((az.c0 >> 2) * nb * nb) << 6) | (spread2(az.c0 & 3u) << 2);
+    const uint32_t Z1 = (((az.c1 >> 2) * nb * nb) << 6) | (spread2(az.c1 & 3u) << 2);
     const uint64_t base = L.base;
-    if (NCH == 1) {
-        float t000 = fetch1<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch1<FMT>(vol, base + (X1 + Y0 + Z0));
-        float t010 = fetch1<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch1<FMT>(vol, base + (X1 + Y1 + Z0));
-        float t001 = fetch1<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch1<FMT>(vol, base + (X1 + Y0 + Z1));
-        float t011 = fetch1<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch1<FMT>(vol, base + (X1 + Y1 + Z1));
-        float c00 = lerpF(t000, t100, ax), c10 = lerpF(t010, t110, ax);
-        float c01 = lerpF(t001, t101, ax), c11 = lerpF(t011, t111, ax);
-        float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);
-        return make_float4(lerpF(c0, c1, az), 0.0f, 0.0f, 0.0f);
-    }
-    if (NCH == 2) {
-        // grey texels (r = g = b): filter r and a, copy r to g and b
-        float2 t000 = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z0));
-        float2 t010 = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z0));
-        float2 t001 = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z1));
-        float2 t011 = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z1));
-        float2 o2;
-#define CV_TRI2(ch)                                                                    \
-    {                                                                                  \
-        float c00 = lerpF(t000.ch, t100.ch, ax), c10 = lerpF(t010.ch, t110.ch, ax);     \
-        float c01 = lerpF(t001.ch, t101.ch, ax), c11 = lerpF(t011.ch, t111.ch, ax);     \
-        float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);                      \
-        o2.ch = lerpF(c0, c1, az);                                                     \
-    }
-        CV_TRI2(x) CV_TRI2(y)
-#undef CV_TRI2
-        return make_float4(o2.x, o2.x, o2.x, o2.y);
-    }
-    float4 t000 = fetch4<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch4<FMT>(vol, base + (X1 + Y0 + Z0));
-    float4 t010 = fetch4<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch4<FMT>(vol, base + (X1 + Y1 + Z0));
-    float4 t001 = fetch4<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch4<FMT>(vol, base + (X1 + Y0 + Z1));
-    float4 t011 = fetch4<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch4<FMT>(vol, base + (X1 + Y1 + Z1));
-    float4 o;
-#define CV_TRI(ch)                                                                     \
-    {                                                                                  \
-        float c00 = lerpF(t000.ch, t100.ch, ax), c10 = lerpF(t010.ch, t110.ch, ax);     \
-        float c01 = lerpF(t001.ch, t101.ch, ax), c11 = lerpF(t011.ch, t111.ch, ax);     \
-        float c0 = lerpF(c00, c10, ay), c1 = lerpF(c01, c11, ay);                      \
-        o.ch = lerpF(c0, c1, az);                                                      \
-    }
-    CV_TRI(x) CV_TRI(y) CV_TRI(z) CV_TRI(w)
-#undef CV_TRI
-    return o;
+    float t000 = fetch1<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch1<FMT>(vol, base + (X1 + Y0 + Z0));
+    float t010 = fetch1<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch1<FMT>(vol, base + (X1 + Y1 + Z0));
+    float t001 = fetch1<FMT>(vol, base + (X0 + Y0 + Z1)), t101 = fetch1<FMT>(vol, base + (X1 + Y0 + Z1));
+    float t011 = fetch1<FMT>(vol, base + (X0 + Y1 + Z1)), t111 = fetch1<FMT>(vol, base + (X1 + Y1 + Z1));
+    float c00 = lerpF(t000, t100, wx), c10 = lerpF(t010, t110, wx);
+    float c01 = lerpF(t001, t101, wx), c11 = lerpF(t011, t111, wx);
+    float c0 = lerpF(c00, c10, wy), c1 = lerpF(c01, c11, wy);
+    return lerpF(c0, c1, wz);
 }
 
 // Grey sampling split into fetch and filter, so a two-level sample issues
-// the 16 loads of both footprints before waiting for any of them.
+// the loads of both footprints before waiting for any of them.
 struct GreyTaps {
     float2 t[8];  // (r, a) at x0y0z0, x1y0z0, x0y1z0, x1y1z0, x0y0z1, x1y0z1, x0y1z1, x1y1z1
     float ax, ay, az;
@@ -1483,34 +1579,40 @@ struct GreyTaps {
 template <int FMT>
 __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
                                            float sz, GreyTaps& T) {
+    static_assert(FMT == 1 || FMT == 2, "RGBA formats");
     const float fn = (float)L.n;
     const int nm1 = (int)L.n - 1;
-    float ux = fmaf(fminf(fmaxf(sx, -1.0f), 2.0f), fn, -0.5f);
-    float uy = fmaf(fminf(fmaxf(sy, -1.0f), 2.0f), fn, -0.5f);
-    float uz = fmaf(fminf(fmaxf(sz, -1.0f), 2.0f), fn, -0.5f);
-    float flx = floorf(ux), fly = floorf(uy), flz = floorf(uz);
-    T.ax = ux - flx; T.ay = uy - fly; T.az = uz - flz;
-    int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    uint32_t x0 = (uint32_t)min(max(ix, 0), nm1), x1 = (uint32_t)min(max(ix + 1, 0), nm1);
-    uint32_t y0 = (uint32_t)min(max(iy, 0), nm1), y1 = (uint32_t)min(max(iy + 1, 0), nm1);
-    uint32_t z0 = (uint32_t)min(max(iz, 0), nm1), z1 = (uint32_t)min(max(iz + 1, 0), nm1);
+    const AxisTaps ax = axis_taps(sx, fn, nm1), ay = axis_taps(sy, fn, nm1), az = axis_taps(sz, fn, nm1);
+    // x1 is the slot's neighbour x0+ except below the first texel, where both
+    // x corners clamp to texel 0: there the weight 0 returns texel 0 exactly,
+    // as lerp(t0, t0, w) does
+    T.ax = ax.i < 0 ? 0.0f : ax.u - ax.fl;
+    T.ay = ay.u - ay.fl;
+    T.az = az.u - az.fl;
     const uint32_t nb = L.nb;
-    uint32_t X0 = ((x0 >> 2) << 6) | spread2(x0 & 3u), X1 = ((x1 >> 2) << 6) | spread2(x1 & 3u);
-    uint32_t Y0 = (((y0 >> 2) * nb) << 6) | (spread2(y0 & 3u) << 1);
-    uint32_t Y1 = (((y1 >> 2) * nb) << 6) | (spread2(y1 & 3u) << 1);
-    uint32_t Z0 = (((z0 >> 2) * nb * nb) << 6) | (spread2(z0 & 3u) << 2);
-    uint32_t Z1 = (((z1 >> 2) * nb * nb) << 6) | (spread2(z1 & 3u) << 2);
-    const uint64_t base = L.base;
-    T.t[0] = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z0));
-    T.t[1] = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z0));
-    T.t[2] = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z0));
-    T.t[3] = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z0));
-    T.t[4] = fetch_ra<FMT>(vol, base + (X0 + Y0 + Z1));
-    T.t[5] = fetch_ra<FMT>(vol, base + (X1 + Y0 + Z1));
-    T.t[6] = fetch_ra<FMT>(vol, base + (X0 + Y1 + Z1));
-    T.t[7] = fetch_ra<FMT>(vol, base + (X1 + Y1 + Z1));
+    const uint32_t X0 = ((ax.c0 >> 2) << 6) | spread2(ax.c0 & 3u);
+    const uint32_t Y0 = (((ay.c0 >> 2) * nb) << 6) | (spread2(ay.c0 & 3u) << 1);
+    const uint32_t Y1 = (((ay.c1 >> 2) * nb) << 6) | (spread2(ay.c1 & 3u) << 1);
+    const uint32_t Z0 = (((az.c0 >> 2) * nb * nb) << 6) | (spread2(az.c0 & 3u) << 2);
+    const uint32_t Z1 = (((az.c1 >> 2) * nb * nb) << 6) | (spread2(az.c1 & 3u) << 2);
+    if constexpr (FMT == 1) {
+        const float4* v = (const float4*)vol + L.base;
+        const float4 q0 = v[X0 + Y0 + Z0], q1 = v[X0 + Y1 + Z0], q2 = v[X0 + Y0 + Z1], q3 = v[X0 + Y1 + Z1];
+        T.t[0] = make_float2(q0.x, q0.y); T.t[1] = make_float2(q0.z, q0.w);
+        T.t[2] = make_float2(q1.x, q1.y); T.t[3] = make_float2(q1.z, q1.w);
+        T.t[4] = make_float2(q2.x, q2.y); T.t[5] = make_float2(q2.z, q2.w);
+        T.t[6] = make_float2(q3.x, q3.y); T.t[7] = make_float2(q3.z, q3.w);
+    } else {
+        const uint2* v = (const uint2*)vol + L.base;
+        const uint2 q0 = v[X0 + Y0 + Z0], q1 = v[X0 + Y1 + Z0], q2 = v[X0 + Y0 + Z1], q3 = v[X0 + Y1 + Z1];
+        auto h2 = [](uint32_t u) { return make_float2(h2f(u & 0xffffu), h2f(u >> 16)); };
+        T.t[0] = h2(q0.x); T.t[1] = h2(q0.y);
+        T.t[2] = h2(q1.x); T.t[3] = h2(q1.y);
+        T.t[4] = h2(q2.x); T.t[5] = h2(q2.y);
+        T.t[6] = h2(q3.x); T.t[7] = h2(q3.y);
+    }
 }
-// lerps along x, then y, then z (the order of trilinear())
+// lerps along x, then y, then z (GL's trilinear order)
 __device__ __forceinline__ float2 grey_filter(const GreyTaps& T) {
     float2 o;
     {
@@ -1528,7 +1630,9 @@ __device__ __forceinline__ float2 grey_filter(const GreyTaps& T) {
     return o;
 }
 
-// textureLod: lambda -> level / fraction (GL 4.4 8.14.3), (1-f)*tau1 + f*tau2
+// textureLod: lambda -> level / fraction (GL 4.4 8.14.3), (1-f)*tau1 + f*tau2.
+// NCH 1: r only (the weighted R-sum cone); NCH 2: (r, a) of an RGBA volume,
+// returned as RGBA (r, r, r, a).
 template <int FMT, int NCH>
 __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const VolumeDesc& vd, float lam, float sx,
                                              float sy, float sz, bool* two) {
@@ -1537,9 +1641,9 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
     if (lam <= 0.0f) { lv = 0; f = 0.0f; }
     else if (lam >= (float)(vd.L - 1)) { lv = vd.L - 1; f = 0.0f; }
     else { float fl = floorf(lam); lv = (int)fl; f = lam - fl; }
-    if (NCH == 2) {
-        const bool tw = f != 0.0f;
-        *two = tw;
+    const bool tw = f != 0.0f;
+    *two = tw;
+    if constexpr (NCH == 2) {
         GreyTaps A, B;
         grey_fetch<FMT>(vol, vd.lv[lv], sx, sy, sz, A);
         if (tw) grey_fetch<FMT>(vol, vd.lv[lv + 1], sx, sy, sz, B);  // in flight with A
@@ -1550,23 +1654,11 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
             a.y = lerpF(a.y, b.y, f);
         }
         return make_float4(a.x, a.x, a.x, a.y);
+    } else {
+        float r = trilinear_r<FMT>(vol, vd.lv[lv], sx, sy, sz);
+        if (tw) r = lerpF(r, trilinear_r<FMT>(vol, vd.lv[lv + 1], sx, sy, sz), f);
+        return make_float4(r, 0.0f, 0.0f, 0.0f);
     }
-    float4 o = trilinear<FMT, NCH>(vol, vd.lv[lv], sx, sy, sz);
-    *two = f != 0.0f;
-    if (f != 0.0f) {
-        float4 o2 = trilinear<FMT, NCH>(vol, vd.lv[lv + 1], sx, sy, sz);
-        o.x = lerpF(o.x, o2.x, f);
-        if (NCH == 4) {
-            o.y = lerpF(o.y, o2.y, f);
-            o.z = lerpF(o.z, o2.z, f);
-            o.w = lerpF(o.w, o2.w, f);
-        } else if (NCH == 2) {
-            o.y = o.x;
-            o.z = o.x;
-            o.w = lerpF(o.w, o2.w, f);
-        }
-    }
-    return o;
 }
 
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
@@ -1581,10 +1673,9 @@ struct FtbState {
     float k0, k1, k2, ka;
 };
 
-// GREY: the volume's texels have r = g = b (rgba mode: every store is
-// (1,1,1,1) or (0,0,0,1), mips average channels alike, noise scales only a),
-// so only r and a are filtered -- bit-identical to filtering all four.
-template <int FMT, bool GREY>
+// RGBA volumes are grey (r = g = b), so only r and a are filtered --
+// bit-identical to filtering all four channels.
+template <int FMT>
 __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
                                          float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
                                          int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
@@ -1593,7 +1684,7 @@ __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __res
         float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dz, p0z);
         if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) return 1;
         bool two;
-        float4 sm = sample_lod<FMT, GREY ? 2 : 4>(vol, vd, cd.lam[i], sx, sy, sz, &two);
+        float4 sm = sample_lod<FMT, 2>(vol, vd, cd.lam[i], sx, sy, sz, &two);
         ntwo += two ? 1u : 0u;
         float al;
         if (cd.cone_mode == 1) {
@@ -1646,8 +1737,10 @@ __device__ __forceinline__ void farfield_features(const VolumeDesc& vd, const vo
         const float sx = fmaf(hj, dx, px), sy = fmaf(hj, dy, py), sz = fmaf(hj, dz, pz);
         float dens = 0.0f, light = 0.0f;
         if (sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f) {
-            float4 t = trilinear<FMT, 4>(vol, vd.lv[cd.feat_lvl[j]], sx, sy, sz);
-            dens = t.w;
+            GreyTaps T;
+            grey_fetch<FMT>(vol, vd.lv[cd.feat_lvl[j]], sx, sy, sz, T);
+            const float2 t = grey_filter(T);
+            dens = t.y;
             light = t.x;
         }
         f[9 + 2 * j] = dens;
@@ -1737,7 +1830,7 @@ constexpr int kConeChunk = 256;
 // two-level sampling branch) stays uniform across the wave, and in the
 // learned kernels the lanes reach step k together: their cones form the
 // wave's 64-row MLP batch.
-template <int FMT, int CONE, int COMP, int LEARNED, bool GREY>
+template <int FMT, int CONE, int COMP, int LEARNED>
 __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
                                           const ConeDesc& cd, const void* __restrict__ vol,
                                           const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
@@ -1835,7 +1928,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                 const float sc = sphere_contrib(du, dv, cw.w);
                 ++nfrag;
                 f = cone_frag(g, cv, cd, cw, du, dv, sc, fN);
-                if (CONE == 0) {
+                if constexpr (CONE == 0) {
                     float color = 0.0f;
                     for (int i = 0; i < cd.steps; ++i) {
                         const float h = cd.h[i];
@@ -1848,12 +1941,12 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                     nsamp += (uint32_t)cd.steps;
                     s0 = s1 = s2 = s3 = clamp01(color);
                 } else {
-                    const int rs = march_ftb<FMT, GREY>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0,
+                    const int rs = march_ftb<FMT>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0,
                                                         kk_steps, st, nsamp, ntwo);
                     alive = LEARNED && rs == 0 && kk_steps < cd.steps;
                 }
             }
-            if (LEARNED && CONE != 0) {
+            if constexpr (LEARNED && CONE != 0) {
                 // wave-uniform: all lanes take part in the batched MLP
                 if (__any(alive)) {
                     float x[kMlpIn];
@@ -1881,7 +1974,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                         const float rem = 1.0f - st.a;
                         if (cd.err_bound < 0.0f || !(rem * y[4] <= cd.err_bound)) {
                             ++nfall;
-                            march_ftb<FMT, GREY>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, kk_steps,
+                            march_ftb<FMT>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, kk_steps,
                                                  cd.steps, st, nsamp, ntwo);
                         } else {
                             st.c0 = fmaf(rem, clamp01(y[0]), st.c0);
@@ -1913,7 +2006,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
 
 // The learned variant asks for 3 waves per SIMD: the inline MLP needs ~100
 // registers beside the march state, and without the bound the kernel gets 2.
-template <int FMT, int CONE, int COMP, int LEARNED, bool GREY>
+template <int FMT, int CONE, int COMP, int LEARNED>
 __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
                                              const void* __restrict__ vol, const float4* __restrict__ sph_world,
                                              const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
@@ -1928,7 +2021,7 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, Volume
     const int lane = threadIdx.x;
     uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
     const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
-    cone_tile<FMT, CONE, COMP, LEARNED, GREY>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp,
+    cone_tile<FMT, CONE, COMP, LEARNED>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp,
                                               xs, cover, id, tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals);
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
@@ -1949,7 +2042,7 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, Volume
 
 // Batched cone trace: global tiles of many scenes (heavy first), each wave
 // finds its scene by binary search over the scenes' first tiles.
-template <int FMT, int CONE, int COMP, bool GREY>
+template <int FMT, int CONE, int COMP>
 __global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict__ scenes,
                                                     const int* __restrict__ tile0, int nscenes,
                                                     const int* __restrict__ gorder, int total,
@@ -1971,7 +2064,7 @@ __global__ void __launch_bounds__(256) k_cone_batch(const BatchScene* __restrict
         }
         const int sidx = __builtin_amdgcn_readfirstlane(lo);
         const BatchScene& B = scenes[sidx];
-        cone_tile<FMT, CONE, COMP, 0, GREY>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order, B.masks,
+        cone_tile<FMT, CONE, COMP, 0>(B.g, B.vd, B.cv, B.cd, B.levels, B.sph_world, B.sph_view, B.order, B.masks,
                                             B.words, B.image, none, nullptr, cover[wave], id[wave], gt - B.tile0, lane,
                                             nfrag, nsamp, ntwo, nrows, nfall, nevals);
     }
@@ -2001,14 +2094,14 @@ __global__ void __launch_bounds__(256) k_segments(GridDesc g, VolumeDesc vd, Con
     const int kk_steps = cd.learned_k;
     FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t ns = 0, nt = 0;
-    int rs = march_ftb<FMT, false>(vd, vol, cd, px, py, pz, dx, dy, dz, 0, min(kk_steps, cd.steps), st, ns, nt);
+    int rs = march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, 0, min(kk_steps, cd.steps), st, ns, nt);
     const bool rec = rs == 0 && kk_steps < cd.steps;
     float f[kMlpIn];
     float ck0 = 0.0f, ck1 = 0.0f, ck2 = 0.0f, ak = 0.0f;
     if (rec) {
         farfield_features<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, (float)g.N, f);
         ck0 = st.c0; ck1 = st.c1; ck2 = st.c2; ak = st.a;
-        march_ftb<FMT, false>(vd, vol, cd, px, py, pz, dx, dy, dz, kk_steps, cd.steps, st, ns, nt);
+        march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, kk_steps, cd.steps, st, ns, nt);
     }
     valid[i] = rec ? 1 : 0;
 #pragma unroll
@@ -2092,12 +2185,13 @@ __global__ void __launch_bounds__(256) k_debrick(LevelDesc L, int tb, const uint
     if (i >= n * n * n) return;
     uint32_t x = (uint32_t)(i % n), y = (uint32_t)((i / n) % n), z = (uint32_t)(i / (n * n));
     uint64_t a = brick_addr(L, x, y, z);
-    if (tb == 16) {  // stored (r, a, g, b) -> RGBA
-        float4 t = ((const float4*)levels)[a];
-        ((float4*)out)[i] = make_float4(t.x, t.z, t.w, t.y);
-    } else if (tb == 8) {
-        uint2 u = ((const uint2*)levels)[a];
-        ((uint2*)out)[i] = make_uint2((u.x & 0xffffu) | (u.y << 16), (u.y >> 16) | (u.x & 0xffff0000u));
+    if (tb == 16) {  // slot (r, a, r+, a+) -> RGBA (r, r, r, a)
+        const float2 t = ((const float2*)levels)[2 * a];
+        ((float4*)out)[i] = make_float4(t.x, t.x, t.x, t.y);
+    } else if (tb == 8) {  // halves (r, a, r+, a+) -> (r, r, r, a)
+        const uint32_t u = ((const uint32_t*)levels)[2 * a];
+        const uint32_t rr = u & 0xffffu;
+        ((uint2*)out)[i] = make_uint2(rr | (rr << 16), rr | (u & 0xffff0000u));
     } else {
         out[i] = levels[a];
     }
@@ -2167,7 +2261,7 @@ void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned
         const int nlev = vd.L < 5 ? vd.L : 5;
         const dim3 grid(rpa * rpa * rpa);
         if (vd.format == 0)
-            hipLaunchKernelGGL(k_build_fused<0>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
+            hipLaunchKernelGGL(k_build_fused_r8, grid, dim3(256), 0, s, g, vd, shellbits, levels, nlev, rpa);
         else if (vd.format == 1)
             hipLaunchKernelGGL(k_build_fused<1>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
         else
@@ -2296,71 +2390,55 @@ TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int til
     return q;
 }
 
-template <int FMT, int CONE, int COMP, bool GREY = false>
+template <int FMT, int CONE, int COMP>
 static void launch_cone_batch_t(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total,
                                 int wpg, int groups, unsigned long long* counters, hipStream_t s) {
     if (wpg < 1 || wpg > 4) return;  // k_cone_batch holds chunk lists for at most 4 waves
     const int items_per_group = wpg;
     unsigned grid = groups > 0 ? (unsigned)groups : (unsigned)((total + items_per_group - 1) / items_per_group);
     if (grid == 0) return;
-    hipLaunchKernelGGL((k_cone_batch<FMT, CONE, COMP, GREY>), dim3(grid), dim3(64 * wpg), 0, s, scenes, tile0, nscenes,
+    hipLaunchKernelGGL((k_cone_batch<FMT, CONE, COMP>), dim3(grid), dim3(64 * wpg), 0, s, scenes, tile0, nscenes,
                        order, total, counters);
 }
 
 void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, const int* order, int total_tiles,
-                       int format, int cone_mode, int composite, int grey, int waves_per_group, int groups,
+                       int format, int cone_mode, int composite, int waves_per_group, int groups,
                        unsigned long long* counters, hipStream_t s) {
     if (total_tiles <= 0) return;
     const bool w = cone_mode == 0;
-    if (format == 1) {
-        if (w) {
-            if (composite == 0) launch_cone_batch_t<1, 0, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-            else launch_cone_batch_t<1, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-        } else {
-            if (composite == 0) launch_cone_batch_t<1, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-            else if (grey) launch_cone_batch_t<1, 1, 1, true>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-            else launch_cone_batch_t<1, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-        }
-    } else {
-        if (w) {
-            if (composite == 0) launch_cone_batch_t<2, 0, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-            else launch_cone_batch_t<2, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-        } else {
-            if (composite == 0) launch_cone_batch_t<2, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-            else if (grey) launch_cone_batch_t<2, 1, 1, true>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-            else launch_cone_batch_t<2, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
-        }
-    }
+    const int c = composite == 0 ? 0 : 1;
+    auto go = [&](auto fmt) {
+        constexpr int F = decltype(fmt)::value;
+        if (w && c == 0) launch_cone_batch_t<F, 0, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        else if (w) launch_cone_batch_t<F, 0, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        else if (c == 0) launch_cone_batch_t<F, 1, 0>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+        else launch_cone_batch_t<F, 1, 1>(scenes, tile0, nscenes, order, total_tiles, waves_per_group, groups, counters, s);
+    };
+    if (format == 1) go(std::integral_constant<int, 1>{});
+    else go(std::integral_constant<int, 2>{});
 }
 
-template <int FMT, int CONE, int COMP, int LEARNED, bool GREY = false>
+template <int FMT, int CONE, int COMP, int LEARNED>
 static void launch_cone_t(const ConeLaunch& a, hipStream_t s) {
     unsigned ntiles = (unsigned)(a.cv.tiles_x * a.cv.tiles_y);
-    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED, GREY>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
+    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
                        a.levels, a.sph_world, a.sph_view, a.order, a.masks, a.words, a.image, a.counters,
                        a.tile_order, a.mlp);
 }
 
 // R8 volumes hold one channel: only the weighted R-sum cone reads them (the
-// runtime rejects the front-to-back modes for R8 before launching).  Volumes
-// in rgba mode are grey (r = g = b): their front-to-back kernels filter r and a.
+// runtime rejects the front-to-back modes for R8 before launching).
 template <int FMT>
 static void launch_cone_f(const ConeLaunch& a, hipStream_t s) {
     const bool weighted = a.cd.cone_mode == 0;
-    const bool grey = a.vd.mode == 1;
     if constexpr (FMT == 0) {
         if (a.cd.composite == 0) launch_cone_t<0, 0, 0, 0>(a, s);
         else launch_cone_t<0, 0, 1, 0>(a, s);
     } else {
-        if (a.learned) {
-            if (grey) launch_cone_t<FMT, 1, 1, 1, true>(a, s);
-            else launch_cone_t<FMT, 1, 1, 1>(a, s);
-            return;
-        }
-        if (weighted && a.cd.composite == 0) launch_cone_t<FMT, 0, 0, 0>(a, s);
+        if (a.learned) launch_cone_t<FMT, 1, 1, 1>(a, s);
+        else if (weighted && a.cd.composite == 0) launch_cone_t<FMT, 0, 0, 0>(a, s);
         else if (weighted) launch_cone_t<FMT, 0, 1, 0>(a, s);
         else if (a.cd.composite == 0) launch_cone_t<FMT, 1, 0, 0>(a, s);
-        else if (grey) launch_cone_t<FMT, 1, 1, 0, true>(a, s);
         else launch_cone_t<FMT, 1, 1, 0>(a, s);
     }
 }

@@ -1123,7 +1123,6 @@ struct cr_batch_s {
     cr_context ctx = nullptr;
     std::vector<cr_batch_item> items;
     int format = 0, cone_mode = 0, composite = 0;
-    int grey = 0;  // every volume in rgba mode (r = g = b)
     cr_batch_config cfg{8, 1, 0};
     BatchScene* d_scenes = nullptr;
     size_t scenes_cap = 0;
@@ -1231,7 +1230,7 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     launch_tile_order(b->d_counts, b->total_tiles, b->d_oscratch, b->d_order, s);
     if (ev) (void)hipEventRecord(ev[2], s);
     launch_cone_batch(b->d_scenes, b->d_tile0, (int)n, b->d_order, b->total_tiles, b->format, b->cone_mode,
-                      b->composite, b->grey, b->cfg.waves_per_group, b->cfg.groups, counters, s);
+                      b->composite, b->cfg.waves_per_group, b->cfg.groups, counters, s);
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
@@ -1294,9 +1293,6 @@ int cr_batch_create(cr_context ctx, const cr_batch_item* items, int count, cr_ba
     b->format = items[0].volume->format;
     b->cone_mode = items[0].params.cone_mode;
     b->composite = items[0].params.composite;
-    b->grey = 1;
-    for (int i = 0; i < count; ++i)
-        if (items[i].volume->mode != CR_MODE_RGBA) b->grey = 0;
     hipError_t e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete b;
