@@ -113,6 +113,12 @@ struct LightScratch {
 
 // ---- launch wrappers (cv_kernels.hip) ----
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s);
+struct FillSeg {
+    uint64_t* p;
+    uint64_t n;
+    uint64_t value;
+};
+void launch_fill_multi(const FillSeg* segs, int nseg, hipStream_t s);  // up to 4 segments, one launch
 size_t light_items_bound(int S, float max_radius, const LightView& lv);
 void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
                          const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,

@@ -158,6 +158,25 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t value)
     for (; i < n; i += stride) p[i] = value;
 }
 
+// up to four fills in one launch (segments concatenated)
+struct FillArgs {
+    uint64_t* p[4];
+    uint64_t n[4];
+    uint64_t v[4];
+    int nseg;
+};
+__global__ void k_fill_multi(FillArgs a) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t total = a.n[0] + a.n[1] + a.n[2] + a.n[3];
+    for (; i < total; i += stride) {
+        uint64_t r = i;
+        int k = 0;
+        while (k < 3 && r >= a.n[k]) r -= a.n[k++];
+        a.p[k][r] = a.v[k];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // pass 1: light-view raster (res/first_voxelize.glsl:39-64)
 // ---------------------------------------------------------------------------
@@ -200,11 +219,9 @@ __device__ __forceinline__ bool light_fragment(const LightView& lv, const LightS
 // Work decomposition of pass 1: every billboard's pixel box is cut into
 // kLightTile x kLightTile tiles; one workgroup rasterizes one (billboard, tile)
 // item, so large spheres spread over many CUs instead of forming the tail.
-__global__ void __launch_bounds__(256) k_light_setup(GridDesc g, LightView lv, const float4* __restrict__ boards, int S,
-                                                     int4* __restrict__ lbox, int* __restrict__ counts) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= S) return;
-    LightSphere o = light_sphere(g, lv, boards[j]);
+// light_box: the billboard's conservative pixel box and its tile count.
+__device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv, float4 board, int4* out) {
+    LightSphere o = light_sphere(g, lv, board);
     int4 box = make_int4(1, 1, 0, 0);
     int cnt = 0;
     if (o.live) {
@@ -223,44 +240,80 @@ __global__ void __launch_bounds__(256) k_light_setup(GridDesc g, LightView lv, c
             cnt = ((box.z - box.x) / kLightTile + 1) * ((box.w - box.y) / kLightTile + 1);
         }
     }
-    lbox[j] = box;
-    counts[j] = cnt;
+    *out = box;
+    return cnt;
 }
 
-// exclusive scan of counts[0..n) into offsets[0..n], one workgroup of 1024
-__global__ void __launch_bounds__(1024) k_scan_counts(const int* __restrict__ counts, int n, int* __restrict__ offsets) {
+// The pass-1 work list in one workgroup: every billboard's pixel box (lbox),
+// the exclusive scan of the tile counts and the (billboard, tile) items in
+// billboard order, cut at cap; *total receives the uncut item count.  Four
+// consecutive billboards per thread, 4096 per round; then each thread writes
+// an equal share of the round's items, finding its first billboard by a
+// binary search over the round's offsets.
+__global__ void __launch_bounds__(1024) k_light_items(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                      int S, int4* __restrict__ lbox, int2* __restrict__ items,
+                                                      int cap, int* __restrict__ total) {
     __shared__ int wsum[16];
-    __shared__ int carry;
+    __shared__ int cnt_l[4096], off_l[4096];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        int i = base + tid;
-        int v = i < n ? counts[i] : 0;
-        int x = v;  // inclusive wave scan
+    int carry = 0;
+    for (int base = 0; base < S; base += 4096) {
+        int c[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int j = base + tid * 4 + e;
+            c[e] = 0;
+            if (j < S) {
+                int4 box;
+                c[e] = light_box(g, lv, boards[j], &box);
+                lbox[j] = box;
+            }
+        }
+        const int tsum = (c[0] + c[1]) + (c[2] + c[3]);
+        int x = tsum;  // inclusive wave scan
         for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(x, off, 64);
+            const int y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
         if (lane == 63) wsum[wid] = x;
         __syncthreads();
-        int wpre = 0;
-        for (int w = 0; w < wid; ++w) wpre += wsum[w];
-        int c = carry;
-        if (i < n) offsets[i] = c + wpre + x - v;
+        int wpre = 0, round = 0;
+        for (int w = 0; w < 16; ++w) {
+            const int ws = wsum[w];
+            wpre += w < wid ? ws : 0;
+            round += ws;
+        }
+        int o = carry + wpre + x - tsum;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            cnt_l[tid * 4 + e] = c[e];
+            off_l[tid * 4 + e] = o;
+            o += c[e];
+        }
         __syncthreads();
-        if (tid == 1023) carry = c + wpre + x;
-        __syncthreads();
+        const int per = (round + 1023) >> 10;
+        const int i0 = carry + tid * per, i1 = min(i0 + per, min(carry + round, cap));
+        if (i0 < i1) {
+            int lo = 0, hi = 4095;  // last billboard with off_l <= i0
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off_l[mid] <= i0) lo = mid;
+                else hi = mid - 1;
+            }
+            int b = lo, ob = off_l[b], cb = cnt_l[b];
+            for (int i = i0; i < i1; ++i) {
+                while (i >= ob + cb) {
+                    ++b;
+                    ob = off_l[b];
+                    cb = cnt_l[b];
+                }
+                items[i] = make_int2(base + b, i - ob);
+            }
+        }
+        __syncthreads();  // wsum, cnt_l and off_l are rewritten next round
+        carry += round;
     }
-    if (tid == 0) offsets[n] = carry;
-}
-
-__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ counts, const int* __restrict__ offsets,
-                                                      int S, int2* __restrict__ items, int cap) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= S) return;
-    int c = counts[j], o = offsets[j];
-    for (int t = 0; t < c && o + t < cap; ++t) items[o + t] = make_int2(j, t);
+    if (tid == 0) *total = carry;
 }
 
 constexpr int kTilePixels = kLightTile * kLightTile;
@@ -1100,6 +1153,68 @@ __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restr
     const float2 own = mip_parent_at(vd, Cl, levels, x, y, z, np);
     const float2 nb = x + 1 < np ? mip_parent_at(vd, Cl, levels, x + 1, y, z, np) : own;
     store_pair_rt(levels, vd.format, out, own, nb);
+}
+
+// The small end of the chain in one workgroup: levels k0..L-1 (n <= 16, at
+// most 4096 texels each), every level kept in LDS for the next one (level
+// k0 - 1 is read from memory).  Same sums, order and rounding as k_mip.
+constexpr int kTailTexels = 4096;
+__global__ void __launch_bounds__(1024) k_mip_tail(VolumeDesc vd, int k0, void* __restrict__ levels) {
+    __shared__ float2 buf[2][kTailTexels];  // (r, a), or the R8 value in .x; linear x + n (y + n z)
+    const int tid = threadIdx.x;
+    int cur = 0;
+    for (int k = k0; k < vd.L; ++k) {
+        const LevelDesc P = vd.lv[k], Cl = vd.lv[k - 1];
+        const int np = (int)P.n, nc = (int)Cl.n;
+        const int ntex = np * np * np;
+        for (int t = tid; t < ntex; t += blockDim.x) {
+            const int x = t % np, y = (t / np) % np, z = t / (np * np);
+            int x0, x1, y0, y1, z0, z1;
+            child_range(x, np, nc, &x0, &x1);
+            child_range(y, np, nc, &y0, &y1);
+            child_range(z, np, nc, &z0, &z1);
+            const int cnt = (x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+            if (vd.format == 0) {
+                uint32_t sum = 0;
+                for (int cz = z0; cz <= z1; ++cz)
+                    for (int cy = y0; cy <= y1; ++cy)
+                        for (int cx = x0; cx <= x1; ++cx)
+                            sum += k == k0 ? (uint32_t)((const uint8_t*)levels)[brick_addr(Cl, cx, cy, cz)]
+                                           : (uint32_t)buf[cur ^ 1][cx + nc * (cy + nc * cz)].x;
+                buf[cur][t] = make_float2((float)((sum + (uint32_t)cnt / 2u) / (uint32_t)cnt), 0.0f);
+            } else {
+                float2 sm = make_float2(0.0f, 0.0f);
+                for (int cz = z0; cz <= z1; ++cz)
+                    for (int cy = y0; cy <= y1; ++cy)
+                        for (int cx = x0; cx <= x1; ++cx)
+                            sm = add2(sm, k == k0 ? load_ra(levels, vd.format, brick_addr(Cl, cx, cy, cz))
+                                                  : buf[cur ^ 1][cx + nc * (cy + nc * cz)]);
+                buf[cur][t] = vd.format == 1 ? mip_parent2<1>(sm, (float)cnt) : mip_parent2<2>(sm, (float)cnt);
+            }
+        }
+        __syncthreads();
+        // every slot of the level's bricks (zeros beyond the side)
+        const uint32_t nslots = P.nb * P.nb * P.nb * 64u;
+        for (uint32_t t = tid; t < nslots; t += blockDim.x) {
+            const uint32_t brick = t >> 6, m = t & 63u;
+            uint32_t lx, ly, lz;
+            demorton6(m, &lx, &ly, &lz);
+            const int x = (int)((brick % P.nb) * 4u + lx), y = (int)(((brick / P.nb) % P.nb) * 4u + ly);
+            const int z = (int)((brick / (P.nb * P.nb)) * 4u + lz);
+            const uint64_t out = P.base + t;
+            const bool in = x < np && y < np && z < np;
+            const int li = x + np * (y + np * z);
+            if (vd.format == 0) {
+                ((uint8_t*)levels)[out] = in ? (uint8_t)buf[cur][li].x : (uint8_t)0;
+            } else {
+                const float2 own = in ? buf[cur][li] : make_float2(0.0f, 0.0f);
+                const float2 nb = in && x + 1 < np ? buf[cur][li + 1] : own;
+                store_pair_rt(levels, vd.format, out, own, nb);
+            }
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2204,6 +2319,26 @@ static inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((
 
 size_t texel_bytes(int format) { return format == 0 ? 1 : (format == 1 ? 16 : 8); }
 
+void launch_fill_multi(const FillSeg* segs, int nseg, hipStream_t s) {
+    FillArgs a{};
+    uint64_t total = 0;
+    int k = 0;
+    for (int i = 0; i < nseg && k < 4; ++i) {
+        if (segs[i].n == 0 || !segs[i].p) continue;
+        a.p[k] = segs[i].p;
+        a.n[k] = segs[i].n;
+        a.v[k] = segs[i].value;
+        total += segs[i].n;
+        ++k;
+    }
+    a.nseg = k;
+    if (total == 0) return;
+    for (int i = k; i < 4; ++i) a.p[i] = a.p[0];  // empty segments (n = 0)
+    unsigned nb = blocks_for(total, 256);
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_fill_multi, dim3(nb), dim3(256), 0, s, a);
+}
+
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s) {
     if (n == 0) return;
     unsigned nb = blocks_for(n, 256);
@@ -2215,11 +2350,9 @@ void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const
                          const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
                          unsigned long long* counters, hipStream_t s) {
     if (S <= 0) return;
-    hipLaunchKernelGGL(k_light_setup, dim3(blocks_for(S, 256)), dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, ls.counts, S, ls.offsets);
     const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
-    hipLaunchKernelGGL(k_light_expand, dim3(blocks_for(S, 256)), dim3(256), 0, s, ls.counts, ls.offsets, S, ls.items,
-                       cap);
+    hipLaunchKernelGGL(k_light_items, dim3(1), dim3(1024), 0, s, g, lv, boards, S, ls.lbox, ls.items, cap,
+                       ls.offsets + S);
     unsigned grid = (unsigned)std::min<size_t>(ls.items_cap, 4096);
     const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
     const int chord = mode == 1 ? 1 : 0;
@@ -2272,7 +2405,9 @@ void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned
         hipLaunchKernelGGL(k_level0, dim3(blocks_for(nbricks, 4)), dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
                            nbricks);
     }
-    for (int k = first_generic; k < vd.L; ++k) launch_mip(vd, k, levels, s);
+    int k = first_generic;
+    for (; k < vd.L && vd.lv[k].n * vd.lv[k].n * vd.lv[k].n > (uint32_t)kTailTexels; ++k) launch_mip(vd, k, levels, s);
+    if (k < vd.L) hipLaunchKernelGGL(k_mip_tail, dim3(1), dim3(1024), 0, s, vd, k, levels);
 }
 
 void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s) {
@@ -2367,19 +2502,21 @@ void launch_tile_order(const int* tile_count, int ntiles, int* scratch, int* til
 }
 
 // Binning plus the heavy-first tile order.  tile scratch (ints): hist[32] |
-// cursor[32] | (2 unused) | tile_count[ntiles] | bucket_of | tile_order
+// cursor[32] | (2 unused) | tile_count[ntiles] | bucket_of | tile_order.  The
+// masks and the scratch up to tile_count are cleared by one fill launch.
 TileQueue launch_bin(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int tiles_y, int words,
                      unsigned long long* masks, int* scratch, hipStream_t s) {
     const int ntiles = tiles_x * tiles_y;
     unsigned int* counter = (unsigned int*)(scratch + 64);
     int* tile_count = scratch + 66;
     int* tile_order = tile_count + 2 * (size_t)ntiles;
-    launch_fill_u64((uint64_t*)tile_count, (uint64_t)(ntiles + 1) / 2, 0, s);
+    const FillSeg segs[2] = {{(uint64_t*)masks, (uint64_t)ntiles * (uint64_t)words, 0},
+                             {(uint64_t*)scratch, (uint64_t)(66 + ntiles + 1) / 2, 0}};
+    launch_fill_multi(segs, 2, s);
     launch_bin_counts(sph_tiles, order, S, tiles_x, words, masks, tile_count, s);
     int* hist = scratch;
     int* cursor = scratch + 32;
     int* bucket_of = tile_count + ntiles;
-    launch_fill_u64((uint64_t*)scratch, 32, 0, s);
     hipLaunchKernelGGL(k_tile_bucket, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tile_count, ntiles, bucket_of,
                        hist);
     hipLaunchKernelGGL(k_tile_scatter, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, bucket_of, ntiles, hist, cursor,

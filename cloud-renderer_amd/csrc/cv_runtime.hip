@@ -560,11 +560,12 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
                              hipStream_t s) {
     if (ev) ev->mark(0);
-    if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
-    // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187)
-    launch_fill_u64((uint64_t*)v->d_keys, plan.npx, ~0ull, s);
-    launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, s);
-    launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, s);
+    // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
+    const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
+                             {(uint64_t*)v->d_shell, (uint64_t)v->nbricks0, 0},
+                             {(uint64_t*)v->d_fill, (uint64_t)v->nbricks0, 0},
+                             {(uint64_t*)counters, counters ? 9u : 0u, 0}};
+    launch_fill_multi(segs, 4, s);
     if (ev) ev->mark(1);
     launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, v->d_fill, counters, s);
     if (ev) ev->mark(2);
@@ -773,7 +774,6 @@ static void cone_enqueue_sort(cr_volume v, const ConePlan& plan, StageEvents* ev
     if (ev) ev->mark(1);
     launch_sort(v->d_sort, plan.S, v->d_order, v->d_tie, s);
     if (ev) ev->mark(2);
-    launch_fill_u64((uint64_t*)v->d_masks, plan.nmask, 0, s);
 }
 
 static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
@@ -1223,6 +1223,7 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     for (size_t i = 0; i < n; ++i) {
         cr_volume v = b->items[i].volume;
         cone_enqueue_sort(v, cp[i], nullptr, s);
+        launch_fill_u64((uint64_t*)v->d_masks, cp[i].nmask, 0, s);
         launch_bin_counts(v->d_sph_tiles, v->d_order, cp[i].S, cp[i].cv.tiles_x, cp[i].words, v->d_masks,
                           b->d_counts + t0, s);
         t0 += (int)cp[i].ntiles;
