@@ -83,6 +83,13 @@ struct cr_volume_s {
     uint64_t vox_gen = 0;  // gen at the last voxelize
     LightView last_lv{};   // light view of the last voxelize
     size_t posmap_cap = 0;
+    // camera-side stream of cr_cone_trace: cam setup, sort and binning run
+    // there, overlapped with the caller's stream, which waits for them only
+    // before the cone-trace kernel; ev_free marks the end of the last cone
+    // trace (its sort and mask buffers may be rewritten after it)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_cam = nullptr, ev_free = nullptr;
+    bool free_valid = false;
 };
 
 // far-field MLP: FP32 master parameters + Adam moments, bf16 packed copies for
@@ -236,10 +243,11 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 int drain_pending(cr_context ctx) {
     for (const auto& p : ctx->pending) {
         hipEvent_t* e = &ctx->pool[p.base];
-        hipError_t err = hipEventSynchronize(e[4]);
+        hipError_t err = hipEventSynchronize(e[p.kind == 0 ? 4 : 5]);
         if (err != hipSuccess) return fail(ctx, CR_EDEVICE, "hipEventSynchronize: %s", hipGetErrorString(err));
         float t[4];
-        for (int k = 0; k < 4; ++k) t[k] = elapsed(e[k], e[k + 1]);
+        for (int k = 0; k < 3; ++k) t[k] = elapsed(e[k], e[k + 1]);
+        t[3] = p.kind == 0 ? elapsed(e[3], e[4]) : elapsed(e[4], e[5]);
         cr_stats& s = ctx->stats;
         if (p.kind == 0) {
             s.ms_clear += t[0]; s.ms_light_raster += t[1]; s.ms_splat += t[2]; s.ms_mip += t[3];
@@ -254,7 +262,9 @@ int drain_pending(cr_context ctx) {
     return CR_OK;
 }
 
-// Stage events of one API call: five events bracketing four stages.
+// Stage events of one API call, six slots: voxelize brackets its four
+// stages with slots 0..4; a cone trace brackets setup, sort and binning with
+// slots 0..3 (on the camera-side stream) and the cone kernel with 4..5.
 struct StageEvents {
     cr_context ctx = nullptr;
     hipStream_t s = nullptr;
@@ -266,9 +276,9 @@ struct StageEvents {
         ctx = c; s = st; kind = k; e = nullptr;
         if (!c->profiling) return CR_OK;
         if (c->profiling == 1) {
-            base = (size_t)(5 * k);
-            if (c->pool.size() < 10) {
-                while (c->pool.size() < 10) {
+            base = (size_t)(6 * k);
+            if (c->pool.size() < 12) {
+                while (c->pool.size() < 12) {
                     hipEvent_t ev;
                     hipError_t err = hipEventCreate(&ev);
                     if (err != hipSuccess) return fail(c, CR_EDEVICE, "hipEventCreate: %s", hipGetErrorString(err));
@@ -281,19 +291,22 @@ struct StageEvents {
                 if (rc) return rc;
             }
             base = c->pool_used;
-            while (c->pool.size() < base + 5) {
+            while (c->pool.size() < base + 6) {
                 hipEvent_t ev;
                 hipError_t err = hipEventCreate(&ev);
                 if (err != hipSuccess) return fail(c, CR_EDEVICE, "hipEventCreate: %s", hipGetErrorString(err));
                 c->pool.push_back(ev);
             }
-            c->pool_used += 5;
+            c->pool_used += 6;
         }
         e = &c->pool[base];
         return CR_OK;
     }
     void mark(int i) {
         if (e) (void)hipEventRecord(e[i], s);
+    }
+    void mark_on(int i, hipStream_t st) {
+        if (e) (void)hipEventRecord(e[i], st);
     }
     void end() {
         if (e && ctx->profiling == 2) ctx->pending.push_back({kind, base});
@@ -454,6 +467,12 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
 int cr_volume_destroy(cr_volume v) {
     if (!v) return CR_EINVAL;
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
+    if (v->side) {
+        (void)hipStreamSynchronize(v->side);
+        (void)hipStreamDestroy(v->side);
+    }
+    if (v->ev_cam) (void)hipEventDestroy(v->ev_cam);
+    if (v->ev_free) (void)hipEventDestroy(v->ev_free);
     void* ptrs[] = {v->d_levels, v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
@@ -495,6 +514,7 @@ int cr_volume_set_billboards(cr_volume v, const float* xyz, const float* scales,
         h[j] = make_float4(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], r);
         maxr = std::max(maxr, r);
     }
+    if (v->side) CR_HIP(ctx, hipStreamSynchronize(v->side));  // a pending camera pass reads the boards
     size_t cap = (size_t)v->boards_cap;
     rc = ensure(ctx, &v->d_boards, (size_t)std::max(count, 1), &cap);
     if (rc) return rc;
@@ -785,14 +805,35 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_cone_trace: volume not voxelized");
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
+    // Camera setup, sort and binning depend on the billboards and the camera
+    // only: they run on the volume's camera-side stream from the end of the
+    // previous cone trace on, overlapping the voxelization enqueued before
+    // this call; the cone kernel waits for them.  Counter runs stay serial.
+    const bool overlap = ctx->profiling != 1;
+    hipStream_t cs = s;
+    if (overlap) {
+        if (!v->side) {
+            CR_HIP(ctx, hipStreamCreateWithFlags(&v->side, hipStreamNonBlocking));
+            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_cam, hipEventDisableTiming));
+            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_free, hipEventDisableTiming));
+        }
+        cs = v->side;
+        if (!v->free_valid) CR_HIP(ctx, hipEventRecord(v->ev_free, s));  // first trace: after all prior work
+        CR_HIP(ctx, hipStreamWaitEvent(cs, v->ev_free, 0));
+    }
     StageEvents ev;
-    if ((rc = ev.begin(ctx, s, 1))) return rc;
+    if ((rc = ev.begin(ctx, cs, 1))) return rc;
     ev.mark(0);
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
-    cone_enqueue_sort(v, plan, &ev, s);
+    cone_enqueue_sort(v, plan, &ev, cs);
     TileQueue tq = launch_bin(v->d_sph_tiles, v->d_order, plan.S, plan.cv.tiles_x, plan.cv.tiles_y, plan.words,
-                              v->d_masks, v->d_tiles, s);
+                              v->d_masks, v->d_tiles, cs);
     ev.mark(3);
+    if (overlap) {
+        CR_HIP(ctx, hipEventRecord(v->ev_cam, cs));
+        CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_cam, 0));
+    }
+    ev.mark_on(4, s);
     ConeLaunch cl{};
     cl.g = v->g;
     cl.vd = v->vd;
@@ -810,8 +851,12 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     cl.learned = lo ? 1 : 0;
     if (lo) cl.mlp = mlp_dev(lo->mlp);
     launch_cone(cl, s);
-    ev.mark(4);
+    ev.mark_on(5, s);
     ev.end();
+    if (overlap) {
+        CR_HIP(ctx, hipEventRecord(v->ev_free, s));
+        v->free_valid = true;
+    }
     CR_HIP(ctx, hipGetLastError());
     v->sorted_count = plan.S;
     if (ctx->profiling == 1) {
@@ -827,7 +872,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
         ctx->stats.ms_setup = elapsed(ev.e[0], ev.e[1]);
         ctx->stats.ms_sort = elapsed(ev.e[1], ev.e[2]);
         ctx->stats.ms_bin = elapsed(ev.e[2], ev.e[3]);
-        ctx->stats.ms_cone = elapsed(ev.e[3], ev.e[4]);
+        ctx->stats.ms_cone = elapsed(ev.e[4], ev.e[5]);
         ctx->stats.n_cone = 1;
     }
     return CR_OK;
@@ -1345,6 +1390,14 @@ int cr_batch_run(cr_batch b, void* stream) {
     int rc = set_device(ctx);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    // the scenes' buffers are shared with their single cone traces: wait for
+    // any camera pass still running on a volume's camera-side stream
+    for (const auto& it : b->items) {
+        cr_volume v = it.volume;
+        if (!v->side) continue;
+        CR_HIP(ctx, hipEventRecord(v->ev_cam, v->side));
+        CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_cam, 0));
+    }
     if (ctx->profiling == 1) {
         // eager run with counters and phase times
         std::vector<VoxPlan> vp;
@@ -1372,6 +1425,13 @@ int cr_batch_run(cr_batch b, void* stream) {
     } else {
         if (!graph_current(b) && (rc = batch_capture(b))) return rc;
         CR_HIP(ctx, hipGraphLaunch(b->exec, s));
+    }
+    // and the next camera pass of a volume waits for this run
+    for (const auto& it : b->items) {
+        cr_volume v = it.volume;
+        if (!v->side) continue;
+        CR_HIP(ctx, hipEventRecord(v->ev_free, s));
+        v->free_valid = true;
     }
     for (auto& it : b->items) {
         it.volume->voxelized = true;
