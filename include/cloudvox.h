@@ -166,7 +166,11 @@ int cr_voxelize(cr_volume vol, const cr_light *light, int light_w, int light_h, 
 /* ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82):
  * sorts billboards far to near from the camera (CloudVolume::sortBoards),
  * shades every camera fragment with one cone toward the sun through the
- * mip chain and composites into d_image (W x H, row pitch in bytes). */
+ * mip chain and composites into d_image (W x H, row pitch in bytes).
+ * The camera-side work (setup, sort, binning) runs on a stream owned by the
+ * volume, after the volume's previous cone trace and concurrently with work
+ * already queued on `stream` (a voxelization); the cone kernel itself runs on
+ * `stream` after it.  d_image is written only by work on `stream`. */
 int cr_cone_trace(cr_volume vol, const cr_light *light, const cr_camera *camera,
                   const cr_cone_params *params, void *d_image, int width, int height,
                   size_t pitch_bytes, int pixel_format, void *stream);
