@@ -472,6 +472,43 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                 // register: a brick change costs one LDS atomic
                 int cur_w = -1;
                 unsigned long long cur_b = 0ull;
+                // Each axis' voxel coordinate is monotone in k (B fixed, every
+                // step rounds monotonically), so when the first and the last
+                // sample lie inside the volume and the slab box, all do: such
+                // chords skip the per-sample bounds tests.
+                bool inner;
+                {
+                    const float ie = (float)(n - 1) * g.stepSize;
+                    const float ax = (sx - g.lo[0]) * kx, ay = (sy - g.lo[1]) * ky, az = (sz - g.lo[2]) * kz;
+                    const float bx = ((sx + Bx * ie) - g.lo[0]) * kx, by = ((sy + By * ie) - g.lo[1]) * ky;
+                    const float bz = ((sz + Bz * ie) - g.lo[2]) * kz;
+                    auto ok = [&](float l, uint32_t b0, uint32_t nb) {
+                        if (!(l > -1.0f && l < fN)) return false;
+                        return ((uint32_t)(int)l >> 2) - b0 < nb;
+                    };
+                    inner = ok(ax, ubx, unx) && ok(bx, ubx, unx) && ok(ay, uby, uny) && ok(by, uby, uny) &&
+                            ok(az, ubz, unz) && ok(bz, ubz, unz);
+                }
+                if (inner) {
+                    for (int k = 0; k < n; ++k) {
+                        const float i = (float)k * g.stepSize;
+                        const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
+                        const uint32_t ix = (uint32_t)(int)((qx - g.lo[0]) * kx);
+                        const uint32_t iy = (uint32_t)(int)((qy - g.lo[1]) * ky);
+                        const uint32_t iz = (uint32_t)(int)((qz - g.lo[2]) * kz);
+                        const uint32_t ux = (ix >> 2) - ubx, uy = (iy >> 2) - uby, uz = (iz >> 2) - ubz;
+                        const int w = (int)(__umul24(__umul24(uz, uny) + uy, unx) + ux);
+                        const unsigned long long m = 1ull << ((ix & 3u) | ((iy << 2) & 0xcu) | ((iz << 4) & 0x30u));
+                        if (w != cur_w) {
+                            if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
+                            cur_w = w;
+                            cur_b = 0ull;
+                        }
+                        cur_b |= m;
+                    }
+                    if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
+                    continue;
+                }
                 for (int k = 0; k < n; ++k) {
                     const float i = (float)k * g.stepSize;
                     const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
