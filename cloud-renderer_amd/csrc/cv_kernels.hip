@@ -1696,6 +1696,17 @@ __device__ __forceinline__ AxisTaps axis_taps(float s, float fn, int nm1) {
     a.c1 = (uint32_t)min(max(a.i + 1, 0), nm1);
     return a;
 }
+// the same for s in [0, 1] (every front-to-back sample): the coordinate
+// clamp is the identity and i lies in [-1, n - 1], so one bound per corner
+__device__ __forceinline__ AxisTaps axis_taps_in(float s, float fn, int nm1) {
+    AxisTaps a;
+    a.u = fmaf(s, fn, -0.5f);
+    a.fl = floorf(a.u);
+    a.i = (int)a.fl;
+    a.c0 = (uint32_t)max(a.i, 0);
+    a.c1 = (uint32_t)min(a.i + 1, nm1);
+    return a;
+}
 
 // GL trilinear filtering of r in one level with CLAMP_TO_EDGE, lerps x, y, z
 template <int FMT>
@@ -1728,13 +1739,14 @@ struct GreyTaps {
     float2 t[8];  // (r, a) at x0y0z0, x1y0z0, x0y1z0, x1y1z0, x0y0z1, x1y0z1, x0y1z1, x1y1z1
     float ax, ay, az;
 };
+// (sx, sy, sz) in [0, 1]^3: callers stop a cone before it leaves the volume
 template <int FMT>
 __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
                                            float sz, GreyTaps& T) {
     static_assert(FMT == 1 || FMT == 2, "RGBA formats");
     const float fn = (float)L.n;
     const int nm1 = (int)L.n - 1;
-    const AxisTaps ax = axis_taps(sx, fn, nm1), ay = axis_taps(sy, fn, nm1), az = axis_taps(sz, fn, nm1);
+    const AxisTaps ax = axis_taps_in(sx, fn, nm1), ay = axis_taps_in(sy, fn, nm1), az = axis_taps_in(sz, fn, nm1);
     // x1 is the slot's neighbour x0+ except below the first texel, where both
     // x corners clamp to texel 0: there the weight 0 returns texel 0 exactly,
     // as lerp(t0, t0, w) does
