@@ -1680,7 +1680,10 @@ __device__ __forceinline__ float fetch1(const void* __restrict__ vol, uint64_t i
     return h2f(((const unsigned short*)vol)[i * 4]);
 }
 
-// bricked-layout offsets of the clamped corner coordinates of one axis
+// Clamped corner coordinates of one axis.  Texel indices inside a level are
+// 32-bit (the side is at most 1624: 406^3 bricks of 64 texels < 2^32) and
+// the brick-row and brick-slab strides fit the 24-bit multiplier
+// (64 * 406^2 < 2^24).
 struct AxisTaps {
     float u;      // texel-space coordinate - 0.5
     float fl;     // floor(u)
@@ -1716,12 +1719,12 @@ __device__ __forceinline__ float trilinear_r(const void* __restrict__ vol, const
     const int nm1 = (int)L.n - 1;
     const AxisTaps ax = axis_taps(sx, fn, nm1), ay = axis_taps(sy, fn, nm1), az = axis_taps(sz, fn, nm1);
     const float wx = ax.u - ax.fl, wy = ay.u - ay.fl, wz = az.u - az.fl;
-    const uint32_t nb = L.nb;
+    const uint32_t srow = L.nb << 6, sslab = (L.nb * L.nb) << 6;  // texels per brick row / slab (< 2^24)
     const uint32_t X0 = ((ax.c0 >> 2) << 6) | spread2(ax.c0 & 3u), X1 = ((ax.c1 >> 2) << 6) | spread2(ax.c1 & 3u);
-    const uint32_t Y0 = (((ay.c0 >> 2) * nb) << 6) | (spread2(ay.c0 & 3u) << 1);
-    const uint32_t Y1 = (((ay.c1 >> 2) * nb) << 6) | (spread2(ay.c1 & 3u) << 1);
-    const uint32_t Z0 = (((az.c0 >> 2) * nb * nb) << 6) | (spread2(az.c0 & 3u) << 2);
-    const uint32_t Z1 = (((az.c1 >> 2) * nb * nb) << 6) | (spread2(az.c1 & 3u) << 2);
+    const uint32_t Y0 = __umul24(ay.c0 >> 2, srow) | (spread2(ay.c0 & 3u) << 1);
+    const uint32_t Y1 = __umul24(ay.c1 >> 2, srow) | (spread2(ay.c1 & 3u) << 1);
+    const uint32_t Z0 = __umul24(az.c0 >> 2, sslab) | (spread2(az.c0 & 3u) << 2);
+    const uint32_t Z1 = __umul24(az.c1 >> 2, sslab) | (spread2(az.c1 & 3u) << 2);
     const uint64_t base = L.base;
     float t000 = fetch1<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch1<FMT>(vol, base + (X1 + Y0 + Z0));
     float t010 = fetch1<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch1<FMT>(vol, base + (X1 + Y1 + Z0));
@@ -1753,12 +1756,12 @@ __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const L
     T.ax = ax.i < 0 ? 0.0f : ax.u - ax.fl;
     T.ay = ay.u - ay.fl;
     T.az = az.u - az.fl;
-    const uint32_t nb = L.nb;
+    const uint32_t srow = L.nb << 6, sslab = (L.nb * L.nb) << 6;  // texels per brick row / slab (< 2^24)
     const uint32_t X0 = ((ax.c0 >> 2) << 6) | spread2(ax.c0 & 3u);
-    const uint32_t Y0 = (((ay.c0 >> 2) * nb) << 6) | (spread2(ay.c0 & 3u) << 1);
-    const uint32_t Y1 = (((ay.c1 >> 2) * nb) << 6) | (spread2(ay.c1 & 3u) << 1);
-    const uint32_t Z0 = (((az.c0 >> 2) * nb * nb) << 6) | (spread2(az.c0 & 3u) << 2);
-    const uint32_t Z1 = (((az.c1 >> 2) * nb * nb) << 6) | (spread2(az.c1 & 3u) << 2);
+    const uint32_t Y0 = __umul24(ay.c0 >> 2, srow) | (spread2(ay.c0 & 3u) << 1);
+    const uint32_t Y1 = __umul24(ay.c1 >> 2, srow) | (spread2(ay.c1 & 3u) << 1);
+    const uint32_t Z0 = __umul24(az.c0 >> 2, sslab) | (spread2(az.c0 & 3u) << 2);
+    const uint32_t Z1 = __umul24(az.c1 >> 2, sslab) | (spread2(az.c1 & 3u) << 2);
     if constexpr (FMT == 1) {
         const float4* v = (const float4*)vol + L.base;
         const float4 q0 = v[X0 + Y0 + Z0], q1 = v[X0 + Y1 + Z0], q2 = v[X0 + Y0 + Z1], q3 = v[X0 + Y1 + Z1];

@@ -382,7 +382,8 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     if (!ctx || !out || !bounds_lo || !bounds_hi || !position)
         return fail(ctx, CR_EINVAL, "cr_volume_create: NULL argument");
     *out = nullptr;
-    if (dim < 1 || dim > 2048) return fail(ctx, CR_EINVAL, "cr_volume_create: dim must be in [1, 2048]");
+    // texel indices inside a level are 32-bit: ceil(dim / 4)^3 * 64 < 2^32
+    if (dim < 1 || dim > 1624) return fail(ctx, CR_EINVAL, "cr_volume_create: dim must be in [1, 1624]");
     int maxl = 1;
     while ((dim >> maxl) >= 1) ++maxl;  // floor(log2(dim)) + 1
     if (levels < 1 || levels > maxl || levels > kMaxLevels)

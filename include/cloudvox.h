@@ -20,9 +20,11 @@
  *     and never aborts; cr_last_error() describes the last failure on the
  *     context.  No C++ exception crosses this boundary.
  *   - Host arrays passed in are copied before the call returns.
- *   - All device work is enqueued on the caller's hipStream_t (passed as
+ *   - All device work is ordered on the caller's hipStream_t (passed as
  *     void*, NULL = the null stream) and is complete after that stream is
- *     synchronised.  Calls on one volume must not overlap.
+ *     synchronised (cr_cone_trace runs part of it on a stream of the volume
+ *     that the caller's stream waits for).  Calls on one volume must not
+ *     overlap.
  *   - The library owns the volume, its mips, bitmaps and scratch buffers; the
  *     caller owns the output image.
  *   - Images are stored bottom row first (GL window coordinates).
@@ -144,7 +146,8 @@ int cr_abi_version(void);
 
 /* ---- volume (CloudVolume, src/CloudVolume.cpp:7-55) -------------------- */
 /* bounds_lo/hi are (xBounds.x, yBounds.x, zBounds.x) / (.y ...), relative
- * to position, as in CloudVolume.  levels in [1, log2(dim) + 1]. */
+ * to position, as in CloudVolume.  dim in [1, 1624] (texel indices inside a
+ * level are 32-bit), levels in [1, log2(dim) + 1]. */
 int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const float bounds_hi[3],
                      const float position[3], int levels, int format, int mode, cr_volume *out);
 int cr_volume_destroy(cr_volume vol);
