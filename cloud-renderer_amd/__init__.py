@@ -135,7 +135,6 @@ class CloudVolume:
         self.mode = int(mode)
         rng = hi - lo
         self.range = rng
-        self.voxelSize = rng / np.float32(dim)
         self.billboards = Billboards()
         self.fluffiness = 1.0
         h = C.c_void_p()
@@ -143,6 +142,7 @@ class CloudVolume:
         _check(lib().cr_volume_create(self.ctx.handle, self.dimension, lo3, hi3, p3, self.levels, self.format,
                                       self.mode, C.byref(h)), self.ctx.handle)
         self.handle = h
+        self.voxelSize = rng / np.float32(dim)
 
     # -- billboards -------------------------------------------------------
     def set_billboards(self, positions, scales) -> None:

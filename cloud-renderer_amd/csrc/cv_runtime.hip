@@ -90,6 +90,13 @@ struct cr_volume_s {
     hipStream_t side = nullptr;
     hipEvent_t ev_cam = nullptr, ev_free = nullptr;
     bool free_valid = false;
+    // voxelization stream of cr_voxelize: clear, pass 1 and pass 2 run there
+    // from the end of the previous level build on (so they overlap the cone
+    // trace that reads the previous levels); the level build itself runs on
+    // the caller's stream after them.  ev_built marks the end of the last build.
+    hipStream_t vside = nullptr;
+    hipEvent_t ev_splat = nullptr, ev_built = nullptr;
+    bool built_valid = false;
 };
 
 // far-field MLP: FP32 master parameters + Adam moments, bf16 packed copies for
@@ -247,7 +254,7 @@ int drain_pending(cr_context ctx) {
         if (err != hipSuccess) return fail(ctx, CR_EDEVICE, "hipEventSynchronize: %s", hipGetErrorString(err));
         float t[4];
         for (int k = 0; k < 3; ++k) t[k] = elapsed(e[k], e[k + 1]);
-        t[3] = p.kind == 0 ? elapsed(e[3], e[4]) : elapsed(e[4], e[5]);
+        t[3] = p.kind == 0 ? elapsed(e[5], e[4]) : elapsed(e[4], e[5]);
         cr_stats& s = ctx->stats;
         if (p.kind == 0) {
             s.ms_clear += t[0]; s.ms_light_raster += t[1]; s.ms_splat += t[2]; s.ms_mip += t[3];
@@ -262,9 +269,10 @@ int drain_pending(cr_context ctx) {
     return CR_OK;
 }
 
-// Stage events of one API call, six slots: voxelize brackets its four
-// stages with slots 0..4; a cone trace brackets setup, sort and binning with
-// slots 0..3 (on the camera-side stream) and the cone kernel with 4..5.
+// Stage events of one API call, six slots: voxelize brackets clear, pass 1
+// and pass 2 with slots 0..3 (on the voxelization stream) and the level
+// build with 5..4; a cone trace brackets setup, sort and binning with slots
+// 0..3 (on the camera-side stream) and the cone kernel with 4..5.
 struct StageEvents {
     cr_context ctx = nullptr;
     hipStream_t s = nullptr;
@@ -468,12 +476,13 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
 int cr_volume_destroy(cr_volume v) {
     if (!v) return CR_EINVAL;
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
-    if (v->side) {
-        (void)hipStreamSynchronize(v->side);
-        (void)hipStreamDestroy(v->side);
+    for (hipStream_t st : {v->side, v->vside}) {
+        if (!st) continue;
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
     }
-    if (v->ev_cam) (void)hipEventDestroy(v->ev_cam);
-    if (v->ev_free) (void)hipEventDestroy(v->ev_free);
+    for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built})
+        if (e) (void)hipEventDestroy(e);
     void* ptrs[] = {v->d_levels, v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
@@ -515,7 +524,9 @@ int cr_volume_set_billboards(cr_volume v, const float* xyz, const float* scales,
         h[j] = make_float4(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], r);
         maxr = std::max(maxr, r);
     }
-    if (v->side) CR_HIP(ctx, hipStreamSynchronize(v->side));  // a pending camera pass reads the boards
+    // pending camera or voxelization passes read the boards
+    if (v->side) CR_HIP(ctx, hipStreamSynchronize(v->side));
+    if (v->vside) CR_HIP(ctx, hipStreamSynchronize(v->vside));
     size_t cap = (size_t)v->boards_cap;
     rc = ensure(ctx, &v->d_boards, (size_t)std::max(count, 1), &cap);
     if (rc) return rc;
@@ -578,22 +589,29 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
 }
 
 // kernels only (no allocation, no host synchronisation): safe under graph capture
+// Passes 1 and 2 on `vs`, the level build on `s` (after them when the
+// streams differ).
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
-                             hipStream_t s) {
+                             hipStream_t vs, hipStream_t s) {
     if (ev) ev->mark(0);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
     const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
                              {(uint64_t*)v->d_shell, (uint64_t)v->nbricks0, 0},
                              {(uint64_t*)v->d_fill, (uint64_t)v->nbricks0, 0},
                              {(uint64_t*)counters, counters ? 9u : 0u, 0}};
-    launch_fill_multi(segs, 4, s);
+    launch_fill_multi(segs, 4, vs);
     if (ev) ev->mark(1);
-    launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, v->d_fill, counters, s);
+    launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, v->d_fill, counters, vs);
     if (ev) ev->mark(2);
-    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, v->d_shell, counters, s);
+    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, v->d_shell, counters, vs);
     if (ev) ev->mark(3);
+    if (vs != s) {
+        (void)hipEventRecord(v->ev_splat, vs);
+        (void)hipStreamWaitEvent(s, v->ev_splat, 0);
+    }
+    if (ev) ev->mark_on(5, s);
     launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
-    if (ev) ev->mark(4);
+    if (ev) ev->mark_on(4, s);
 }
 
 int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, void* stream) {
@@ -603,10 +621,31 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     cr_context ctx = v->ctx;
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
+    // Clear, pass 1 and pass 2 write only the keys, the bitmaps and the work
+    // list: they run on the volume's voxelization stream from the end of the
+    // previous level build on, overlapping the cone trace that reads the
+    // previous levels; the build waits for them on the caller's stream.
+    // Counter runs stay serial.
+    const bool overlap = ctx->profiling != 1;
+    hipStream_t vs = s;
+    if (overlap) {
+        if (!v->vside) {
+            CR_HIP(ctx, hipStreamCreateWithFlags(&v->vside, hipStreamNonBlocking));
+            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_splat, hipEventDisableTiming));
+            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_built, hipEventDisableTiming));
+        }
+        vs = v->vside;
+        if (!v->built_valid) CR_HIP(ctx, hipEventRecord(v->ev_built, s));  // first call: after all prior work
+        CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_built, 0));
+    }
     StageEvents ev;
-    if ((rc = ev.begin(ctx, s, 0))) return rc;
-    voxelize_enqueue(v, plan, counters, &ev, s);
+    if ((rc = ev.begin(ctx, vs, 0))) return rc;
+    voxelize_enqueue(v, plan, counters, &ev, vs, s);
     ev.end();
+    if (overlap) {
+        CR_HIP(ctx, hipEventRecord(v->ev_built, s));
+        v->built_valid = true;
+    }
     CR_HIP(ctx, hipGetLastError());
     if (ctx->profiling == 1) {
         CR_HIP(ctx, hipStreamSynchronize(s));
@@ -618,7 +657,7 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
         ctx->stats.ms_clear = elapsed(ev.e[0], ev.e[1]);
         ctx->stats.ms_light_raster = elapsed(ev.e[1], ev.e[2]);
         ctx->stats.ms_splat = elapsed(ev.e[2], ev.e[3]);
-        ctx->stats.ms_mip = elapsed(ev.e[3], ev.e[4]);
+        ctx->stats.ms_mip = elapsed(ev.e[5], ev.e[4]);
         ctx->stats.n_voxelize = 1;
     }
     v->voxelized = true;
@@ -1262,7 +1301,7 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     const size_t n = b->items.size();
     if (ev) (void)hipEventRecord(ev[0], s);
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
-    for (size_t i = 0; i < n; ++i) voxelize_enqueue(b->items[i].volume, vp[i], nullptr, nullptr, s);
+    for (size_t i = 0; i < n; ++i) voxelize_enqueue(b->items[i].volume, vp[i], nullptr, nullptr, s, s);
     if (ev) (void)hipEventRecord(ev[1], s);
     launch_fill_u64((uint64_t*)b->d_counts, (uint64_t)(b->total_tiles + 1) / 2, 0, s);
     int t0 = 0;
@@ -1395,9 +1434,14 @@ int cr_batch_run(cr_batch b, void* stream) {
     // any camera pass still running on a volume's camera-side stream
     for (const auto& it : b->items) {
         cr_volume v = it.volume;
-        if (!v->side) continue;
-        CR_HIP(ctx, hipEventRecord(v->ev_cam, v->side));
-        CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_cam, 0));
+        if (v->side) {
+            CR_HIP(ctx, hipEventRecord(v->ev_cam, v->side));
+            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_cam, 0));
+        }
+        if (v->vside) {
+            CR_HIP(ctx, hipEventRecord(v->ev_splat, v->vside));
+            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_splat, 0));
+        }
     }
     if (ctx->profiling == 1) {
         // eager run with counters and phase times
@@ -1427,12 +1471,17 @@ int cr_batch_run(cr_batch b, void* stream) {
         if (!graph_current(b) && (rc = batch_capture(b))) return rc;
         CR_HIP(ctx, hipGraphLaunch(b->exec, s));
     }
-    // and the next camera pass of a volume waits for this run
+    // and the next camera pass and voxelization of a volume wait for this run
     for (const auto& it : b->items) {
         cr_volume v = it.volume;
-        if (!v->side) continue;
-        CR_HIP(ctx, hipEventRecord(v->ev_free, s));
-        v->free_valid = true;
+        if (v->side) {
+            CR_HIP(ctx, hipEventRecord(v->ev_free, s));
+            v->free_valid = true;
+        }
+        if (v->vside) {
+            CR_HIP(ctx, hipEventRecord(v->ev_built, s));
+            v->built_valid = true;
+        }
     }
     for (auto& it : b->items) {
         it.volume->voxelized = true;

@@ -114,3 +114,13 @@ def test_rgba8_output(cv, gpu_ctx):
     g32 = run_gpu(cv, scene, ctx=gpu_ctx, read_levels=False, pixel_format=0)
     want = np.floor(np.clip(g32.image, 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
     assert_bit_equal("rgba8", g8.image, want)
+
+
+def test_volume_side_limit(cv, gpu_ctx):
+    # texel indices inside a level are 32-bit: sides above 1624 are refused
+    # before anything is allocated
+    log("[limits] cr_volume_create side 1625 and 0")
+    for dim in (1625, 0):
+        with pytest.raises(cv.CloudVoxError) as e:
+            cv.CloudVolume(dim, mips=1, context=gpu_ctx)
+        assert e.value.code == cv._abi.CR_EINVAL and "dim must be in [1, 1624]" in str(e.value)
