@@ -22,9 +22,9 @@
  *   - Host arrays passed in are copied before the call returns.
  *   - All device work is ordered on the caller's hipStream_t (passed as
  *     void*, NULL = the null stream) and is complete after that stream is
- *     synchronised (cr_cone_trace runs part of it on a stream of the volume
- *     that the caller's stream waits for).  Calls on one volume must not
- *     overlap.
+ *     synchronised (cr_voxelize and cr_cone_trace run part of it on streams
+ *     of the volume that the caller's stream waits for).  Calls on one
+ *     volume must not overlap.
  *   - The library owns the volume, its mips, bitmaps and scratch buffers; the
  *     caller owns the output image.
  *   - Images are stored bottom row first (GL window coordinates).
@@ -163,7 +163,11 @@ int cr_volume_set_billboards(cr_volume vol, const float *xyz, const float *scale
 /* ---- the two render passes ------------------------------------------- */
 /* VoxelizeShader::voxelize (src/Shaders/VoxelizeShader.cpp:18-118): clear,
  * pass 1 from the sun's orthographic view into a light_w x light_h position
- * map with a depth test, pass 2 splat of 9 voxels per texel, mip chain. */
+ * map with a depth test, pass 2 splat of 9 voxels per texel, mip chain.
+ * Clear and both passes run on a stream owned by the volume, after the
+ * volume's previous level build and concurrently with work already queued on
+ * `stream` (the previous cone trace, which reads only the levels); the level
+ * build runs on `stream` after them. */
 int cr_voxelize(cr_volume vol, const cr_light *light, int light_w, int light_h, void *stream);
 
 /* ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82):
