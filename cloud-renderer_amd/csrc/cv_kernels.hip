@@ -147,6 +147,14 @@ __device__ __forceinline__ float cv_exp2_neg(float x) {
     return p * scale;
 }
 
+// 1 << (b mod 64): v_lshlrev_b64 reads only the low 6 bits of its count, so
+// callers pass bit indices with unmasked high bits
+__device__ __forceinline__ unsigned long long bit64_mod(uint32_t b) {
+    unsigned long long r;
+    asm("v_lshlrev_b64 %0, %1, 1" : "=v"(r) : "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -200,11 +208,10 @@ __device__ __forceinline__ LightSphere light_sphere(const GridDesc& g, const Lig
     return o;
 }
 
-// fragment of sphere o at light pixel (px, py): fragPos and sphereContrib
-__device__ __forceinline__ bool light_fragment(const LightView& lv, const LightSphere& o, int px, int py, float* fx,
-                                               float* fy, float* fz, float* sc) {
-    float hx = pixel_ndc(px, lv.W) * lv.lsx + lv.lox;
-    float hy = pixel_ndc(py, lv.H) * lv.lsy + lv.loy;
+// fragment of sphere o at a light pixel: fragPos and sphereContrib;
+// (hx, hy) is the pixel centre on the light's view plane
+__device__ __forceinline__ bool light_fragment_at(const LightView& lv, const LightSphere& o, float hx, float hy,
+                                                  float* fx, float* fy, float* fz, float* sc) {
     float du = hx - o.vx;
     float dv = hy - o.vy;
     float c = sphere_contrib(du, dv, o.s);
@@ -214,6 +221,12 @@ __device__ __forceinline__ bool light_fragment(const LightView& lv, const LightS
     *fy = (o.cy + du * lv.S[1]) + dv * lv.U[1];
     *fz = (o.cz + du * lv.S[2]) + dv * lv.U[2];
     return true;
+}
+__device__ __forceinline__ float light_hx(const LightView& lv, int px) { return pixel_ndc(px, lv.W) * lv.lsx + lv.lox; }
+__device__ __forceinline__ float light_hy(const LightView& lv, int py) { return pixel_ndc(py, lv.H) * lv.lsy + lv.loy; }
+__device__ __forceinline__ bool light_fragment(const LightView& lv, const LightSphere& o, int px, int py, float* fx,
+                                               float* fy, float* fz, float* sc) {
+    return light_fragment_at(lv, o, light_hx(lv, px), light_hy(lv, py), fx, fy, fz, sc);
 }
 
 // Work decomposition of pass 1: every billboard's pixel box is cut into
@@ -338,9 +351,11 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     // exactly k * stepSize: covered fragments are queued in LDS with their
     // sample counts and the chords are walked from the queue, so no lane
     // idles on a pixel outside the disc.
+    static_assert(WORDS <= 4096, "the flush decodes word indices below 4096 with float products");
     __shared__ unsigned long long lbits[WORDS];
     __shared__ float f_sx[kTilePixels], f_sy[kTilePixels], f_sz[kTilePixels];
     __shared__ int f_n[kTilePixels];
+    __shared__ float h_x[kLightTile], h_y[kLightTile];  // the tile's pixel centres on the view plane
     __shared__ int nfr_s;
     const int tid = threadIdx.x;
     const int total = min(*total_ptr, cap);
@@ -384,8 +399,10 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         if (use_lds)
             for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) lbits[w] = 0ull;
         if (tid == 0) nfr_s = 0;
-        __syncthreads();
     }
+    if (tid < kLightTile) h_x[tid] = light_hx(lv, x0 + tid);
+    else if (tid < 2 * kLightTile) h_y[tid - kLightTile] = light_hy(lv, y0 + tid - kLightTile);
+    __syncthreads();
     // store one brick word of chord samples (LDS bitmap, or HBM outside it)
     auto flush = [&](int gx, int gy, int gz, unsigned long long bits) {
         if (gx < 0) return;
@@ -400,10 +417,14 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     };
     // pass over the tile's pixels: depth keys; chords queued (flat) or walked
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
+    const float inv_tw = 1.0f / (float)tw;
     for (int p = tid; p < tw * th; p += blockDim.x) {
-        const int px = x0 + p % tw, py = y0 + p / tw;
+        // p / tw as a float product: (p + 0.5) / tw lies at least 1 / (2 tw)
+        // from an integer, far above the product's error for p < 1024
+        const int ry = (int)(((float)p + 0.5f) * inv_tw), rx = p - ry * tw;
+        const int px = x0 + rx, py = y0 + ry;
         float fx, fy, fz, sc;
-        if (!light_fragment(lv, o, px, py, &fx, &fy, &fz, &sc)) continue;
+        if (!light_fragment_at(lv, o, h_x[rx], h_y[ry], &fx, &fy, &fz, &sc)) continue;
         ++nfrag;
         float dist = o.s * sc;  // radius * sphereContrib
         if (chord) {
@@ -464,6 +485,10 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         const uint32_t uN = (uint32_t)g.N;
         const uint32_t ubx = (uint32_t)bx0, uby = (uint32_t)by0, ubz = (uint32_t)bz0;
         const uint32_t unx = (uint32_t)nbx, uny = (uint32_t)nby, unz = (uint32_t)nbz;
+        // word of brick (X, Y, Z) in the box: Z * unxy + Y * unx + X - wbase
+        // (mod 2^32; the true value is in [0, words) for a brick in the box)
+        const uint32_t unxy = unx * uny, wbase = (ubz * uny + uby) * unx + ubx;
+        const float step = g.stepSize;
         for (int f = tid; f < nfr; f += blockDim.x) {
             const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
             const int n = f_n[f];
@@ -490,15 +515,17 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                             ok(az, ubz, unz) && ok(bz, ubz, unz);
                 }
                 if (inner) {
-                    for (int k = 0; k < n; ++k) {
-                        const float i = (float)k * g.stepSize;
+                    // i = k * stepSize accumulated: with a power-of-two step
+                    // every partial sum is exact
+                    float i = 0.0f;
+                    for (int k = 0; k < n; ++k, i += step) {
                         const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
                         const uint32_t ix = (uint32_t)(int)((qx - g.lo[0]) * kx);
                         const uint32_t iy = (uint32_t)(int)((qy - g.lo[1]) * ky);
                         const uint32_t iz = (uint32_t)(int)((qz - g.lo[2]) * kz);
-                        const uint32_t ux = (ix >> 2) - ubx, uy = (iy >> 2) - uby, uz = (iz >> 2) - ubz;
-                        const int w = (int)(__umul24(__umul24(uz, uny) + uy, unx) + ux);
-                        const unsigned long long m = 1ull << ((ix & 3u) | ((iy << 2) & 0xcu) | ((iz << 4) & 0x30u));
+                        const int w = (int)(__umul24(iz >> 2, unxy) + (__umul24(iy >> 2, unx) + ((ix >> 2) - wbase)));
+                        // lin6 bit (the shift drops iz's high bits)
+                        const unsigned long long m = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
                         if (w != cur_w) {
                             if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
                             cur_w = w;
@@ -509,8 +536,8 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                     if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
                     continue;
                 }
-                for (int k = 0; k < n; ++k) {
-                    const float i = (float)k * g.stepSize;
+                float i = 0.0f;
+                for (int k = 0; k < n; ++k, i += step) {
                     const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
                     const float lx = __builtin_amdgcn_fmed3f((qx - g.lo[0]) * kx, -2.0f, fN);
                     const float ly = __builtin_amdgcn_fmed3f((qy - g.lo[1]) * ky, -2.0f, fN);
@@ -520,8 +547,8 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                     // brick offsets in the slab's box (unsigned: below the box wraps high)
                     const uint32_t ux = (ix >> 2) - ubx, uy = (iy >> 2) - uby, uz = (iz >> 2) - ubz;
                     const bool inbox = (ux < unx) & (uy < uny) & (uz < unz);
-                    const int w = (int)(__umul24(__umul24(uz, uny) + uy, unx) + ux);  // used when inbox
-                    const unsigned long long m = 1ull << ((ix & 3u) | ((iy << 2) & 0xcu) | ((iz << 4) & 0x30u));
+                    const int w = (int)(__umul24(uz, unxy) + (__umul24(uy, unx) + ux));  // used when inbox
+                    const unsigned long long m = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
                     if (in && !inbox)  // outside the slab's box (not expected): straight to HBM
                         atomicOr(&fillbits[((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2)], m);
                     const bool take = in && inbox;
@@ -569,16 +596,21 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     if (chord) {
         __syncthreads();
         if (use_lds) {
+            // w / nbx and w / (nbx nby) as float products: exact for w < 4096
+            // (the error stays below the 1 / (2 n) margin of the floor)
+            const float inv_x = 1.0f / (float)nbx, inv_xy = 1.0f / (float)(nbx * nby);
             for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) {
                 unsigned long long bits = lbits[w];
                 if (!bits) continue;
-                int lx = w % nbx, ly = (w / nbx) % nby, lz = w / (nbx * nby);
+                const float fw = (float)w + 0.5f;
+                const int q = (int)(fw * inv_x), lz = (int)(fw * inv_xy);
+                const int lx = w - q * nbx, ly = q - lz * nby;
                 uint32_t gw = ((uint32_t)(lz + bz0) * g.nb0 + (uint32_t)(ly + by0)) * g.nb0 + (uint32_t)(lx + bx0);
                 atomicOr(&fillbits[gw], bits);
             }
         }
-        __syncthreads();  // the next item reuses the bitmap and the queue
     }
+    __syncthreads();  // the next item reuses the bitmap, the queue and the pixel centres
     }  // items
     if (counters) {
         nfrag = wave_sum_u64(nfrag);
