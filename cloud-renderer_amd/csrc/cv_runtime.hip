@@ -43,8 +43,18 @@ struct cr_volume_s {
     float blo[3] = {}, bhi[3] = {}, pos[3] = {};
     VolumeDesc vd{};
     GridDesc g{};
-    void* d_levels = nullptr;
+    void* d_levels = nullptr;  // the current level set: the last build's, read by every reader of the levels
     size_t levels_bytes = 0;
+    // two level sets: while the cone trace of frame f reads one, the build of
+    // frame f + 1 writes the other (the second is allocated by the first
+    // overlapped voxelization; without it the build runs on the caller's
+    // stream into the one set).  ev_release[i] marks, on the caller's
+    // stream, the end of the work enqueued while set i was current.
+    void* d_lset[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool lset_failed = false;
+    hipEvent_t ev_release[2] = {nullptr, nullptr};
+    bool release_valid[2] = {false, false};
     unsigned long long* d_shell = nullptr;
     unsigned long long* d_fill = nullptr;
     uint32_t nbricks0 = 0;
@@ -90,10 +100,10 @@ struct cr_volume_s {
     hipStream_t side = nullptr;
     hipEvent_t ev_cam = nullptr, ev_free = nullptr;
     bool free_valid = false;
-    // voxelization stream of cr_voxelize: clear, pass 1 and pass 2 run there
-    // from the end of the previous level build on (so they overlap the cone
-    // trace that reads the previous levels); the level build itself runs on
-    // the caller's stream after them.  ev_built marks the end of the last build.
+    // voxelization stream of cr_voxelize: clear, pass 1, pass 2 and the level
+    // build (into the set that is not current) run there, overlapping the
+    // cone trace that reads the current set; the caller's stream waits for
+    // the build.  ev_built marks the end of the last build.
     hipStream_t vside = nullptr;
     hipEvent_t ev_splat = nullptr, ev_built = nullptr;
     bool built_valid = false;
@@ -450,7 +460,8 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     v->g.N = dim;
     v->g.nb0 = v->vd.lv[0].nb;
     v->nbricks0 = v->g.nb0 * v->g.nb0 * v->g.nb0;
-    hipError_t e = hipMalloc(&v->d_levels, v->levels_bytes);
+    hipError_t e = hipMalloc(&v->d_lset[0], v->levels_bytes);
+    v->d_levels = v->d_lset[0];
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_shell, (size_t)v->nbricks0 * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_fill, (size_t)v->nbricks0 * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_counters, 16 * sizeof(unsigned long long));
@@ -481,9 +492,9 @@ int cr_volume_destroy(cr_volume v) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
-    for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built})
+    for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built, v->ev_release[0], v->ev_release[1]})
         if (e) (void)hipEventDestroy(e);
-    void* ptrs[] = {v->d_levels, v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
+    void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
     for (void* p : ptrs)
@@ -589,10 +600,10 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
 }
 
 // kernels only (no allocation, no host synchronisation): safe under graph capture
-// Passes 1 and 2 on `vs`, the level build on `s` (after them when the
-// streams differ).
+// Passes 1 and 2 on `vs`, the level build into `levels` on `bs` (after them
+// when the streams differ, and after `wait` when given).
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
-                             hipStream_t vs, hipStream_t s) {
+                             hipStream_t vs, hipStream_t bs, void* levels, hipEvent_t wait) {
     if (ev) ev->mark(0);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
     const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
@@ -605,13 +616,14 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
     if (ev) ev->mark(2);
     launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, v->d_shell, counters, vs);
     if (ev) ev->mark(3);
-    if (vs != s) {
+    if (vs != bs) {
         (void)hipEventRecord(v->ev_splat, vs);
-        (void)hipStreamWaitEvent(s, v->ev_splat, 0);
+        (void)hipStreamWaitEvent(bs, v->ev_splat, 0);
     }
-    if (ev) ev->mark_on(5, s);
-    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_levels, s);
-    if (ev) ev->mark_on(4, s);
+    if (wait) (void)hipStreamWaitEvent(bs, wait, 0);
+    if (ev) ev->mark_on(5, bs);
+    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, levels, bs);
+    if (ev) ev->mark_on(4, bs);
 }
 
 int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, void* stream) {
@@ -621,11 +633,13 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     cr_context ctx = v->ctx;
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
-    // Clear, pass 1 and pass 2 write only the keys, the bitmaps and the work
-    // list: they run on the volume's voxelization stream from the end of the
-    // previous level build on, overlapping the cone trace that reads the
-    // previous levels; the build waits for them on the caller's stream.
-    // Counter runs stay serial.
+    // Clear, pass 1, pass 2 and the build write only the keys, the bitmaps,
+    // the work list and the level set that is not current: they run on the
+    // volume's voxelization stream, overlapping the cone trace that reads the
+    // current set (the build waits for the end of the work enqueued while its
+    // set was last current), and the caller's stream waits for the build.
+    // With one level set the build runs on the caller's stream after passes
+    // 1-2; counter runs stay serial.
     const bool overlap = ctx->profiling != 1;
     hipStream_t vs = s;
     if (overlap) {
@@ -633,16 +647,38 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
             CR_HIP(ctx, hipStreamCreateWithFlags(&v->vside, hipStreamNonBlocking));
             CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_splat, hipEventDisableTiming));
             CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_built, hipEventDisableTiming));
+            for (int i = 0; i < 2; ++i) CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_release[i], hipEventDisableTiming));
         }
         vs = v->vside;
-        if (!v->built_valid) CR_HIP(ctx, hipEventRecord(v->ev_built, s));  // first call: after all prior work
+        if (!v->built_valid) CR_HIP(ctx, hipEventRecord(v->ev_built, s));  // after all prior work on s
         CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_built, 0));
+        if (!v->d_lset[1] && !v->lset_failed) {
+            if (hipMalloc(&v->d_lset[1], v->levels_bytes) != hipSuccess) {
+                (void)hipGetLastError();  // no room for a second set: build into the one set
+                v->d_lset[1] = nullptr;
+                v->lset_failed = true;
+            } else {
+                launch_fill_u64((uint64_t*)v->d_lset[1], v->levels_bytes / 8, 0, vs);
+            }
+        }
     }
+    const bool two = overlap && v->d_lset[1] != nullptr;
+    const int nxt = two ? v->cur ^ 1 : v->cur;
     StageEvents ev;
     if ((rc = ev.begin(ctx, vs, 0))) return rc;
-    voxelize_enqueue(v, plan, counters, &ev, vs, s);
+    voxelize_enqueue(v, plan, counters, &ev, vs, two ? vs : s, v->d_lset[nxt],
+                     two && v->release_valid[nxt] ? v->ev_release[nxt] : nullptr);
     ev.end();
-    if (overlap) {
+    if (two) {
+        // what was enqueued on s so far may read the current set
+        CR_HIP(ctx, hipEventRecord(v->ev_release[v->cur], s));
+        v->release_valid[v->cur] = true;
+        CR_HIP(ctx, hipEventRecord(v->ev_built, vs));
+        CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_built, 0));
+        v->built_valid = true;
+        v->cur = nxt;
+        v->d_levels = v->d_lset[nxt];
+    } else if (overlap) {
         CR_HIP(ctx, hipEventRecord(v->ev_built, s));
         v->built_valid = true;
     }
@@ -1267,7 +1303,7 @@ int batch_prepare(cr_batch b, std::vector<VoxPlan>& vp, std::vector<ConePlan>& c
         B.vd = v->vd;
         B.cv = cp[i].cv;
         B.cd = cp[i].cd;
-        B.levels = v->d_levels;
+        B.levels = v->d_lset[0];  // batches build and read set 0
         B.sph_world = v->d_sph_world;
         B.sph_view = v->d_sph_view;
         B.order = v->d_order;
@@ -1301,7 +1337,10 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     const size_t n = b->items.size();
     if (ev) (void)hipEventRecord(ev[0], s);
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
-    for (size_t i = 0; i < n; ++i) voxelize_enqueue(b->items[i].volume, vp[i], nullptr, nullptr, s, s);
+    for (size_t i = 0; i < n; ++i) {
+        cr_volume v = b->items[i].volume;
+        voxelize_enqueue(v, vp[i], nullptr, nullptr, s, s, v->d_lset[0], nullptr);
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
     launch_fill_u64((uint64_t*)b->d_counts, (uint64_t)(b->total_tiles + 1) / 2, 0, s);
     int t0 = 0;
@@ -1481,6 +1520,12 @@ int cr_batch_run(cr_batch b, void* stream) {
         if (v->vside) {
             CR_HIP(ctx, hipEventRecord(v->ev_built, s));
             v->built_valid = true;
+        }
+        if (v->cur != 0) {  // set 0 (the batch's) becomes current
+            CR_HIP(ctx, hipEventRecord(v->ev_release[v->cur], s));
+            v->release_valid[v->cur] = true;
+            v->cur = 0;
+            v->d_levels = v->d_lset[0];
         }
     }
     for (auto& it : b->items) {

@@ -124,3 +124,48 @@ def test_volume_side_limit(cv, gpu_ctx):
         with pytest.raises(cv.CloudVoxError) as e:
             cv.CloudVolume(dim, mips=1, context=gpu_ctx)
         assert e.value.code == cv._abi.CR_EINVAL and "dim must be in [1, 1624]" in str(e.value)
+
+
+def test_pipelined_frames_match_serial(cv, gpu_ctx):
+    """Frames queued back to back on one stream (the voxelization of frame
+    i + 1 overlaps the cone trace of frame i on the volume's own streams) give
+    the images of frames run one at a time.  The sun moves every frame, so a
+    pass that reads or writes the wrong frame's bitmaps or levels changes the
+    picture."""
+    import math
+    import torch
+
+    scene = cv.scenes.plain256(0)
+    F = 4
+    log(f"[pipeline] plain256 seed 0, {F} frames with a moving sun")
+    vol, sun0, _, vox, tr = cv.scene_objects(scene, 0, context=gpu_ctx)
+    try:
+        suns, cams = [], []
+        for i in range(F):
+            a = 0.7 * i
+            p = sun0.position
+            pos = (p[0] * math.cos(a) - p[2] * math.sin(a), p[1], p[0] * math.sin(a) + p[2] * math.cos(a))
+            n = math.sqrt(sum(x * x for x in pos))
+            suns.append(cv.Sun(position=pos, direction=tuple(x / n for x in pos), directional=sun0.directional))
+            c = scene.cameras[(3 * i) % len(scene.cameras)]
+            cams.append(cv.Camera(eye=c.eye, target=c.target, up=c.up, fovy=c.fovy, znear=c.znear, zfar=c.zfar))
+        shape = (scene.H, scene.W, 4)
+        serial = []
+        for i in range(F):
+            img = torch.zeros(shape, dtype=torch.float32, device="cuda")
+            vox.voxelize(vol, suns[i])
+            tr.coneTrace(vol, suns[i], cams[i], img, scene.W, scene.H)
+            torch.cuda.synchronize()
+            serial.append(img.cpu().numpy())
+        stream = torch.cuda.current_stream()
+        imgs = [torch.zeros(shape, dtype=torch.float32, device="cuda") for _ in range(F)]
+        for rep in range(2):
+            for i in range(F):
+                vox.voxelize(vol, suns[i], stream=stream)
+                tr.coneTrace(vol, suns[i], cams[i], imgs[i], scene.W, scene.H, stream=stream)
+            torch.cuda.synchronize()
+            for i in range(F):
+                assert_bit_equal(f"pipelined frame {i} (pass {rep})", imgs[i].cpu().numpy(), serial[i])
+        assert len({serial[i].tobytes() for i in range(F)}) == F, "the frames differ"
+    finally:
+        vol.close()
