@@ -489,7 +489,16 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         // (mod 2^32; the true value is in [0, words) for a brick in the box)
         const uint32_t unxy = unx * uny, wbase = (ubz * uny + uby) * unx + ubx;
         const float step = g.stepSize;
-        for (int f = tid; f < nfr; f += blockDim.x) {
+        // k is a power of two, so scaling the operands of each rounded step by
+        // k scales its result exactly: ((s + B i) - lo) k == (s k + (B k) i) - lo k
+        // bit for bit (a subnormal B i vanishes in the add in both forms)
+        const float Bkx = Bx * kx, Bky = By * ky, Bkz = Bz * kz;
+        const float lkx = g.lo[0] * kx, lky = g.lo[1] * ky, lkz = g.lo[2] * kz;
+        for (int f0 = tid & ~63; f0 < nfr; f0 += blockDim.x) {
+            // lanes past the end of the queue repeat its last chord (the
+            // stores are ORs, so repeating them changes nothing): every lane
+            // is active, and the inner walk's trip count is wave-uniform
+            const int f = min(f0 + (tid & 63), nfr - 1);
             const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
             const int n = f_n[f];
             if (use_lds) {
@@ -514,26 +523,43 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
                     inner = ok(ax, ubx, unx) && ok(bx, ubx, unx) && ok(ay, uby, uny) && ok(by, uby, uny) &&
                             ok(az, ubz, unz) && ok(bz, ubz, unz);
                 }
+                // the longest inner chord of the wave (every lane is active here)
+                int nmax = inner ? n : 0;
+                for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+                nmax = __builtin_amdgcn_readfirstlane(nmax);
                 if (inner) {
-                    // i = k * stepSize accumulated: with a power-of-two step
-                    // every partial sum is exact
-                    float i = 0.0f;
-                    for (int k = 0; k < n; ++k, i += step) {
-                        const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
-                        const uint32_t ix = (uint32_t)(int)((qx - g.lo[0]) * kx);
-                        const uint32_t iy = (uint32_t)(int)((qy - g.lo[1]) * ky);
-                        const uint32_t iz = (uint32_t)(int)((qz - g.lo[2]) * kz);
-                        const int w = (int)(__umul24(iz >> 2, unxy) + (__umul24(iy >> 2, unx) + ((ix >> 2) - wbase)));
-                        // lin6 bit (the shift drops iz's high bits)
+                    const float skx = sx * kx, sky = sy * ky, skz = sz * kz;
+                    // step k at i = min(k, n - 1) * stepSize: a lane past its
+                    // last sample repeats it.  i is accumulated: with a
+                    // power-of-two step every partial sum is exact.
+                    const float ilast = (float)(n - 1) * step;
+                    auto voxel = [&](float i, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
+                        ix = (uint32_t)(int)((skx + Bkx * i) - lkx);
+                        iy = (uint32_t)(int)((sky + Bky * i) - lky);
+                        iz = (uint32_t)(int)((skz + Bkz * i) - lkz);
+                    };
+                    // word of the brick in the box; lin6 bit (the shift drops iz's high bits)
+                    auto word = [&](uint32_t ix, uint32_t iy, uint32_t iz) {
+                        return (int)(__umul24(iz >> 2, unxy) + (__umul24(iy >> 2, unx) + ((ix >> 2) - wbase)));
+                    };
+                    uint32_t ix, iy, iz;
+                    voxel(0.0f, ix, iy, iz);
+                    cur_w = word(ix, iy, iz);
+                    cur_b = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
+                    float iacc = 0.0f;
+                    for (int k = 1; k < nmax; ++k) {
+                        iacc += step;
+                        voxel(fminf(iacc, ilast), ix, iy, iz);
+                        const int w = word(ix, iy, iz);
                         const unsigned long long m = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
                         if (w != cur_w) {
-                            if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
+                            atomicOr(&lbits[cur_w], cur_b);
                             cur_w = w;
                             cur_b = 0ull;
                         }
                         cur_b |= m;
                     }
-                    if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
+                    atomicOr(&lbits[cur_w], cur_b);
                     continue;
                 }
                 float i = 0.0f;
