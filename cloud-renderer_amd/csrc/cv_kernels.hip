@@ -361,11 +361,19 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     const int total = min(*total_ptr, cap);
     const bool flat = P2 && chord;
     unsigned long long nfrag = 0, nchord = 0;
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-    const int2 it = items[item];
+    // the item's (billboard, tile) and the billboard's data; the next item's
+    // are fetched while this one is rasterized (its work-list entry first,
+    // its billboard after the pixel pass), hiding the dependent loads.  The
+    // last item fetches itself again.
+    if ((int)blockIdx.x >= total) return;
+    int item = blockIdx.x;
+    int2 it = items[item];
+    float4 bd = boards[it.x];
+    int4 box = lbox[it.x];
+    for (; item < total; item += gridDim.x) {
+    const int2 nit = items[min(item + (int)gridDim.x, total - 1)];
     const int j = it.x;
-    LightSphere o = light_sphere(g, lv, boards[j]);
-    const int4 box = lbox[j];
+    LightSphere o = light_sphere(g, lv, bd);
     const int ntx = (box.z - box.x) / kLightTile + 1;
     const int x0 = box.x + (it.y % ntx) * kLightTile, y0 = box.y + (it.y / ntx) * kLightTile;
     const int x1 = min(x0 + kLightTile - 1, box.z), y1 = min(y0 + kLightTile - 1, box.w);
@@ -471,6 +479,8 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             atomicMin(&keys[(size_t)py * lv.W + px], key);
         }
     }
+    const float4 nbd = boards[nit.x];
+    const int4 nbox = lbox[nit.x];
     if (flat) {
         __syncthreads();
         // every lane walks one queued chord: no lane idles on a pixel
@@ -637,6 +647,9 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         }
     }
     __syncthreads();  // the next item reuses the bitmap, the queue and the pixel centres
+    it = nit;
+    bd = nbd;
+    box = nbox;
     }  // items
     if (counters) {
         nfrag = wave_sum_u64(nfrag);
