@@ -126,8 +126,12 @@ void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const
 void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards,
                         const unsigned long long* keys, float4* posmap, unsigned long long* shellbits,
                         unsigned long long* counters, hipStream_t s);
+// zflags (optional, RGBA formats with N % 16 == 0): one byte per 16^3
+// region of the level set, 1 when everything the region's build writes holds
+// zero; regions that stay empty then write nothing
+size_t build_zero_flags(const GridDesc& g, const VolumeDesc& vd);
 void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
-                         const unsigned long long* fillbits, void* levels, hipStream_t s);
+                         const unsigned long long* fillbits, void* levels, uint8_t* zflags, hipStream_t s);
 void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s);
 void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boards, int S, float4* sph_world,
                       float4* sph_view, int4* sph_tiles, unsigned long long* keys, hipStream_t s);

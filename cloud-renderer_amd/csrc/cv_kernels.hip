@@ -990,11 +990,19 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
 // region's last column; those are this region's first column, which it
 // writes into the previous region's last slots (second halves only).
 // ---------------------------------------------------------------------------
+//
+// Every value a region's workgroup writes (its texels of levels 0..4, and the
+// second halves of the previous region's last column) derives from the
+// region's own voxels, and an empty voxel is (0, 0) at every level.  With
+// zflags, a region whose occupancy words are all zero and whose flag says
+// that its slots already hold zero returns at once; otherwise it writes and
+// records whether it wrote zeros.
 template <int FMT>
 __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
                                                      const unsigned long long* __restrict__ shellbits,
                                                      const unsigned long long* __restrict__ fillbits,
-                                                     void* __restrict__ levels, int nlev, uint32_t rpa) {
+                                                     void* __restrict__ levels, uint8_t* __restrict__ zflags, int nlev,
+                                                     uint32_t rpa) {
     static_assert(FMT == 1 || FMT == 2, "RGBA formats");
     __shared__ float2 stage[2048];  // [brick of the half region][Morton]
     __shared__ float2 l1[512];
@@ -1005,6 +1013,20 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
     const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
     const bool last_x = rx + 1u == rpa;
     const uint32_t nb0 = g.nb0;
+    if (zflags) {
+        // threads 0..63: shell words of the region's 64 bricks, 64..127: fill words
+        unsigned long long w = 0ull;
+        if (tid < 128u) {
+            const uint32_t lb = tid & 63u;
+            const uint32_t brick = ((4u * rz + (lb >> 4)) * nb0 + 4u * ry + ((lb >> 2) & 3u)) * nb0 + 4u * rx + (lb & 3u);
+            if (tid < 64u) w = shellbits[brick];
+            else if (fillbits) w = fillbits[brick];
+        }
+        const bool stored_zero = zflags[r] != 0;
+        const bool any = __syncthreads_or(w != 0ull) != 0;
+        if (!any && stored_zero) return;
+        if (tid == 0) zflags[r] = any ? 0 : 1;
+    }
     const bool noisy = vd.mode != 0 && vd.noise_octaves > 0;
     for (uint32_t rep = 0; rep < 2; ++rep) {
         const uint32_t grp = tid + rep * 256u;
@@ -2509,8 +2531,14 @@ void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* bo
 
 // level 0 and the whole mip chain; levels 0..4 in one fused pass when the
 // side is a multiple of 16, the remaining levels one launch each
+size_t build_zero_flags(const GridDesc& g, const VolumeDesc& vd) {
+    if (g.N % 16 != 0 || vd.format == 0) return 0;
+    const size_t rpa = (size_t)g.N / 16;
+    return rpa * rpa * rpa;
+}
+
 void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
-                         const unsigned long long* fillbits, void* levels, hipStream_t s) {
+                         const unsigned long long* fillbits, void* levels, uint8_t* zflags, hipStream_t s) {
     int first_generic = 1;
     if (g.N % 16 == 0) {
         const uint32_t rpa = (uint32_t)g.N / 16u;
@@ -2519,9 +2547,11 @@ void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned
         if (vd.format == 0)
             hipLaunchKernelGGL(k_build_fused_r8, grid, dim3(256), 0, s, g, vd, shellbits, levels, nlev, rpa);
         else if (vd.format == 1)
-            hipLaunchKernelGGL(k_build_fused<1>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
+            hipLaunchKernelGGL(k_build_fused<1>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, zflags,
+                               nlev, rpa);
         else
-            hipLaunchKernelGGL(k_build_fused<2>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, nlev, rpa);
+            hipLaunchKernelGGL(k_build_fused<2>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, zflags,
+                               nlev, rpa);
         first_generic = nlev;
     } else {
         uint32_t nbricks = g.nb0 * g.nb0 * g.nb0;

@@ -51,6 +51,8 @@ struct cr_volume_s {
     // stream into the one set).  ev_release[i] marks, on the caller's
     // stream, the end of the work enqueued while set i was current.
     void* d_lset[2] = {nullptr, nullptr};
+    uint8_t* d_zflags[2] = {nullptr, nullptr};  // per set: regions whose slots hold zero (fused build)
+    size_t zflags_bytes = 0;
     int cur = 0;
     bool lset_failed = false;
     hipEvent_t ev_release[2] = {nullptr, nullptr};
@@ -462,6 +464,8 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     v->nbricks0 = v->g.nb0 * v->g.nb0 * v->g.nb0;
     hipError_t e = hipMalloc(&v->d_lset[0], v->levels_bytes);
     v->d_levels = v->d_lset[0];
+    v->zflags_bytes = (build_zero_flags(v->g, v->vd) + 7) & ~(size_t)7;
+    if (e == hipSuccess && v->zflags_bytes) e = hipMalloc((void**)&v->d_zflags[0], v->zflags_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_shell, (size_t)v->nbricks0 * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_fill, (size_t)v->nbricks0 * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_counters, 16 * sizeof(unsigned long long));
@@ -473,6 +477,7 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     }
     // zero the volume so reads before the first voxelize are defined
     launch_fill_u64((uint64_t*)v->d_levels, v->levels_bytes / 8, 0, nullptr);
+    if (v->d_zflags[0]) launch_fill_u64((uint64_t*)v->d_zflags[0], v->zflags_bytes / 8, 0x0101010101010101ull, nullptr);
     launch_fill_u64((uint64_t*)v->d_shell, v->nbricks0, 0, nullptr);
     launch_fill_u64((uint64_t*)v->d_fill, v->nbricks0, 0, nullptr);
     e = hipDeviceSynchronize();
@@ -494,7 +499,7 @@ int cr_volume_destroy(cr_volume v) {
     }
     for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built, v->ev_release[0], v->ev_release[1]})
         if (e) (void)hipEventDestroy(e);
-    void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
+    void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_zflags[0], v->d_zflags[1], v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
     for (void* p : ptrs)
@@ -600,10 +605,10 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
 }
 
 // kernels only (no allocation, no host synchronisation): safe under graph capture
-// Passes 1 and 2 on `vs`, the level build into `levels` on `bs` (after them
-// when the streams differ, and after `wait` when given).
+// Passes 1 and 2 on `vs`, the level build into level set `set` on `bs`
+// (after them when the streams differ, and after `wait` when given).
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
-                             hipStream_t vs, hipStream_t bs, void* levels, hipEvent_t wait) {
+                             hipStream_t vs, hipStream_t bs, int set, hipEvent_t wait) {
     if (ev) ev->mark(0);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
     const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
@@ -622,7 +627,8 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
     }
     if (wait) (void)hipStreamWaitEvent(bs, wait, 0);
     if (ev) ev->mark_on(5, bs);
-    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, levels, bs);
+    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_lset[set],
+                        v->d_zflags[set], bs);
     if (ev) ev->mark_on(4, bs);
 }
 
@@ -653,12 +659,18 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
         if (!v->built_valid) CR_HIP(ctx, hipEventRecord(v->ev_built, s));  // after all prior work on s
         CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_built, 0));
         if (!v->d_lset[1] && !v->lset_failed) {
-            if (hipMalloc(&v->d_lset[1], v->levels_bytes) != hipSuccess) {
+            void* p = nullptr;
+            uint8_t* z = nullptr;
+            if (hipMalloc(&p, v->levels_bytes) != hipSuccess ||
+                (v->zflags_bytes && hipMalloc((void**)&z, v->zflags_bytes) != hipSuccess)) {
                 (void)hipGetLastError();  // no room for a second set: build into the one set
-                v->d_lset[1] = nullptr;
+                if (p) (void)hipFree(p);
                 v->lset_failed = true;
             } else {
-                launch_fill_u64((uint64_t*)v->d_lset[1], v->levels_bytes / 8, 0, vs);
+                v->d_lset[1] = p;
+                v->d_zflags[1] = z;
+                launch_fill_u64((uint64_t*)p, v->levels_bytes / 8, 0, vs);
+                if (z) launch_fill_u64((uint64_t*)z, v->zflags_bytes / 8, 0x0101010101010101ull, vs);
             }
         }
     }
@@ -666,7 +678,7 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     const int nxt = two ? v->cur ^ 1 : v->cur;
     StageEvents ev;
     if ((rc = ev.begin(ctx, vs, 0))) return rc;
-    voxelize_enqueue(v, plan, counters, &ev, vs, two ? vs : s, v->d_lset[nxt],
+    voxelize_enqueue(v, plan, counters, &ev, vs, two ? vs : s, nxt,
                      two && v->release_valid[nxt] ? v->ev_release[nxt] : nullptr);
     ev.end();
     if (two) {
@@ -1339,7 +1351,7 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
     for (size_t i = 0; i < n; ++i) {
         cr_volume v = b->items[i].volume;
-        voxelize_enqueue(v, vp[i], nullptr, nullptr, s, s, v->d_lset[0], nullptr);
+        voxelize_enqueue(v, vp[i], nullptr, nullptr, s, s, 0, nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
     launch_fill_u64((uint64_t*)b->d_counts, (uint64_t)(b->total_tiles + 1) / 2, 0, s);
