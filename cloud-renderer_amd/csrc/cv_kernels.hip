@@ -43,6 +43,17 @@ __device__ __forceinline__ void demorton6(uint32_t m, uint32_t* x, uint32_t* y, 
     *z = ((m >> 2) & 1u) | ((m >> 4) & 2u);
 }
 
+// Morton offset of a clamped texel coordinate c along one axis of a level:
+// the brick index times the axis' brick stride, plus c's two low bits spread
+// to Morton bits (0, 3) << axis; t * 5 & 9 spreads t in [0, 3] to bits 0 and 3
+__device__ __forceinline__ uint32_t morton_off_x(uint32_t c) { return ((c >> 2) << 6) | (((c & 3u) * 5u) & 9u); }
+__device__ __forceinline__ uint32_t morton_off_y(uint32_t c, uint32_t srow) {
+    return __umul24(c >> 2, srow) + (__umul24(c & 3u, 10u) & 18u);
+}
+__device__ __forceinline__ uint32_t morton_off_z(uint32_t c, uint32_t sslab) {
+    return __umul24(c >> 2, sslab) + (__umul24(c & 3u, 20u) & 36u);
+}
+
 // texel (x, y, z) of a level in the bricked layout
 __device__ __forceinline__ uint64_t brick_addr(const LevelDesc& L, uint32_t x, uint32_t y, uint32_t z) {
     return L.base + (uint64_t)(((z >> 2) * L.nb + (y >> 2)) * L.nb + (x >> 2)) * 64u + morton6(x & 3u, y & 3u, z & 3u);
@@ -1813,11 +1824,9 @@ __device__ __forceinline__ float trilinear_r(const void* __restrict__ vol, const
     const AxisTaps ax = axis_taps(sx, fn, nm1), ay = axis_taps(sy, fn, nm1), az = axis_taps(sz, fn, nm1);
     const float wx = ax.u - ax.fl, wy = ay.u - ay.fl, wz = az.u - az.fl;
     const uint32_t srow = L.nb << 6, sslab = (L.nb * L.nb) << 6;  // texels per brick row / slab (< 2^24)
-    const uint32_t X0 = ((ax.c0 >> 2) << 6) | spread2(ax.c0 & 3u), X1 = ((ax.c1 >> 2) << 6) | spread2(ax.c1 & 3u);
-    const uint32_t Y0 = __umul24(ay.c0 >> 2, srow) | (spread2(ay.c0 & 3u) << 1);
-    const uint32_t Y1 = __umul24(ay.c1 >> 2, srow) | (spread2(ay.c1 & 3u) << 1);
-    const uint32_t Z0 = __umul24(az.c0 >> 2, sslab) | (spread2(az.c0 & 3u) << 2);
-    const uint32_t Z1 = __umul24(az.c1 >> 2, sslab) | (spread2(az.c1 & 3u) << 2);
+    const uint32_t X0 = morton_off_x(ax.c0), X1 = morton_off_x(ax.c1);
+    const uint32_t Y0 = morton_off_y(ay.c0, srow), Y1 = morton_off_y(ay.c1, srow);
+    const uint32_t Z0 = morton_off_z(az.c0, sslab), Z1 = morton_off_z(az.c1, sslab);
     const uint64_t base = L.base;
     float t000 = fetch1<FMT>(vol, base + (X0 + Y0 + Z0)), t100 = fetch1<FMT>(vol, base + (X1 + Y0 + Z0));
     float t010 = fetch1<FMT>(vol, base + (X0 + Y1 + Z0)), t110 = fetch1<FMT>(vol, base + (X1 + Y1 + Z0));
@@ -1850,11 +1859,9 @@ __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const L
     T.ay = ay.u - ay.fl;
     T.az = az.u - az.fl;
     const uint32_t srow = L.nb << 6, sslab = (L.nb * L.nb) << 6;  // texels per brick row / slab (< 2^24)
-    const uint32_t X0 = ((ax.c0 >> 2) << 6) | spread2(ax.c0 & 3u);
-    const uint32_t Y0 = __umul24(ay.c0 >> 2, srow) | (spread2(ay.c0 & 3u) << 1);
-    const uint32_t Y1 = __umul24(ay.c1 >> 2, srow) | (spread2(ay.c1 & 3u) << 1);
-    const uint32_t Z0 = __umul24(az.c0 >> 2, sslab) | (spread2(az.c0 & 3u) << 2);
-    const uint32_t Z1 = __umul24(az.c1 >> 2, sslab) | (spread2(az.c1 & 3u) << 2);
+    const uint32_t X0 = morton_off_x(ax.c0);
+    const uint32_t Y0 = morton_off_y(ay.c0, srow), Y1 = morton_off_y(ay.c1, srow);
+    const uint32_t Z0 = morton_off_z(az.c0, sslab), Z1 = morton_off_z(az.c1, sslab);
     if constexpr (FMT == 1) {
         const float4* v = (const float4*)vol + L.base;
         const float4 q0 = v[X0 + Y0 + Z0], q1 = v[X0 + Y1 + Z0], q2 = v[X0 + Y0 + Z1], q3 = v[X0 + Y1 + Z1];
