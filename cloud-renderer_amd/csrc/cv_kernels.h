@@ -77,7 +77,10 @@ struct CameraView {
 
 struct ConeDesc {
     float h[kMaxSteps];    // cone height before step i (voxels)
-    float lam[kMaxSteps];  // lod + lodOffset at step i
+    // textureLod's level and fraction at step i (lambda = lod + lodOffset,
+    // GL 4.4 8.14.3): level llev[i], blended with llev[i] + 1 by lfrac[i]
+    float lfrac[kMaxSteps];
+    uint32_t llev4[kMaxSteps / 4];  // llev[i] = byte i % 4 of llev4[i / 4] (read with word loads)
     int steps;
     float tanHalf;
     float den;             // steps * downScaling

@@ -1897,23 +1897,24 @@ __device__ __forceinline__ float2 grey_filter(const GreyTaps& T) {
     return o;
 }
 
-// textureLod: lambda -> level / fraction (GL 4.4 8.14.3), (1-f)*tau1 + f*tau2.
+__device__ __forceinline__ int step_level(const ConeDesc& cd, int i) {
+    return (int)((cd.llev4[i >> 2] >> (8 * (i & 3))) & 0xffu);
+}
+
+// textureLod at level lv with fraction f toward lv + 1 (the step's lambda
+// split on the host): (1-f)*tau1 + f*tau2.
 // NCH 1: r only (the weighted R-sum cone); NCH 2: (r, a) of an RGBA volume,
 // returned as RGBA (r, r, r, a).
+// LA, LB: descriptors of levels lv and lv + 1 (LB unused when f == 0).
 template <int FMT, int NCH>
-__device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const VolumeDesc& vd, float lam, float sx,
-                                             float sy, float sz, bool* two) {
-    int lv;
-    float f;
-    if (lam <= 0.0f) { lv = 0; f = 0.0f; }
-    else if (lam >= (float)(vd.L - 1)) { lv = vd.L - 1; f = 0.0f; }
-    else { float fl = floorf(lam); lv = (int)fl; f = lam - fl; }
+__device__ __forceinline__ float4 sample_levels(const void* __restrict__ vol, const LevelDesc& LA, const LevelDesc& LB,
+                                                float f, float sx, float sy, float sz, bool* two) {
     const bool tw = f != 0.0f;
     *two = tw;
     if constexpr (NCH == 2) {
         GreyTaps A, B;
-        grey_fetch<FMT>(vol, vd.lv[lv], sx, sy, sz, A);
-        if (tw) grey_fetch<FMT>(vol, vd.lv[lv + 1], sx, sy, sz, B);  // in flight with A
+        grey_fetch<FMT>(vol, LA, sx, sy, sz, A);
+        if (tw) grey_fetch<FMT>(vol, LB, sx, sy, sz, B);  // in flight with A
         float2 a = grey_filter(A);
         if (tw) {
             float2 b = grey_filter(B);
@@ -1922,10 +1923,15 @@ __device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const
         }
         return make_float4(a.x, a.x, a.x, a.y);
     } else {
-        float r = trilinear_r<FMT>(vol, vd.lv[lv], sx, sy, sz);
-        if (tw) r = lerpF(r, trilinear_r<FMT>(vol, vd.lv[lv + 1], sx, sy, sz), f);
+        float r = trilinear_r<FMT>(vol, LA, sx, sy, sz);
+        if (tw) r = lerpF(r, trilinear_r<FMT>(vol, LB, sx, sy, sz), f);
         return make_float4(r, 0.0f, 0.0f, 0.0f);
     }
+}
+template <int FMT, int NCH>
+__device__ __forceinline__ float4 sample_lod(const void* __restrict__ vol, const VolumeDesc& vd, int lv, float f,
+                                             float sx, float sy, float sz, bool* two) {
+    return sample_levels<FMT, NCH>(vol, vd.lv[lv], vd.lv[lv + 1 < kMaxLevels ? lv + 1 : lv], f, sx, sy, sz, two);
 }
 
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
@@ -1946,12 +1952,33 @@ template <int FMT>
 __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
                                          float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
                                          int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
+    if (i0 >= i1) return 0;
+    // a step's scalars are loaded one step ahead, its level descriptors only
+    // when the level changes (at most once per level over a march)
+    float h_n = cd.h[i0], f_n = cd.lfrac[i0];
+    uint32_t lw_n = cd.llev4[i0 >> 2];  // the level's word, shifted at its step
+    int cur = -1;
+    LevelDesc LA{}, LB{};
     for (int i = i0; i < i1; ++i) {
-        const float h = cd.h[i];
+        const float h = h_n, fr = f_n;
+        const int lv = (int)((lw_n >> (8 * (i & 3))) & 0xffu);
+        if (i + 1 < i1) {
+            h_n = cd.h[i + 1];
+            f_n = cd.lfrac[i + 1];
+            lw_n = cd.llev4[(i + 1) >> 2];
+        }
+        if (lv != cur) {
+            LA = vd.lv[lv];
+            LB = vd.lv[lv + 1 < kMaxLevels ? lv + 1 : lv];
+            cur = lv;
+            // wait here (scalar loads complete out of order, so a later wait
+            // for them would also wait for the step prefetch above)
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+        }
         float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dz, p0z);
         if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) return 1;
         bool two;
-        float4 sm = sample_lod<FMT, 2>(vol, vd, cd.lam[i], sx, sy, sz, &two);
+        float4 sm = sample_levels<FMT, 2>(vol, LA, LB, fr, sx, sy, sz, &two);
         ntwo += two ? 1u : 0u;
         float al;
         if (cd.cone_mode == 1) {
@@ -2200,7 +2227,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                     for (int i = 0; i < cd.steps; ++i) {
                         const float h = cd.h[i];
                         bool two;
-                        float4 sm = sample_lod<FMT, 1>(vol, vd, cd.lam[i], fmaf(h, f.dx, f.p0x), fmaf(h, f.dy, f.p0y),
+                        float4 sm = sample_lod<FMT, 1>(vol, vd, step_level(cd, i), cd.lfrac[i], fmaf(h, f.dx, f.p0x), fmaf(h, f.dy, f.p0y),
                                                        fmaf(h, f.dz, f.p0z), &two);
                         ntwo += two ? 1u : 0u;
                         color = color + (sm.x * (float)(i + 1)) / cd.den;

@@ -728,7 +728,20 @@ static void fill_cone_desc(cr_volume v, const cr_light* light, const cr_cone_par
         float r = h * tanHalf;
         float lod = std::log2(std::max(1.0f, 2.0f * r));
         cd.h[i] = h;
-        cd.lam[i] = lod + p->lod_offset;
+        // lambda -> level / fraction, in FP32 as the samplers did per sample
+        const float lam = lod + p->lod_offset;
+        int lv = 0;
+        float f = 0.0f;
+        if (lam <= 0.0f) {
+        } else if (lam >= (float)(v->L - 1)) {
+            lv = v->L - 1;
+        } else {
+            const float fl = std::floor(lam);
+            lv = (int)fl;
+            f = lam - fl;
+        }
+        cd.llev4[i >> 2] = (cd.llev4[i >> 2] & ~(0xffu << (8 * (i & 3)))) | ((uint32_t)lv << (8 * (i & 3)));
+        cd.lfrac[i] = f;
         h = h + r;
     }
     cd.den = (float)p->steps * p->down_scaling;
