@@ -10,9 +10,12 @@ cone trace + composite) at the next azimuth.  Inputs are resident in HBM
 before the timed region.
 
 value = W*H*steps / timed seconds / 1e6 (Mrays/s over whole frames).
-The roofline object covers the dominant kernel with its algorithmic bytes
-per launch (cone stage: 8 texels * 16 B per mip level fetched per sample,
-plus the 16 B/pixel image write) over its event-timed average duration.
+The roofline object covers the dominant kernel (the stage with the longest
+event-timed average in the timed frames, where the voxelization of frame f+1
+overlaps the cone trace of frame f and both stretch) with its algorithmic
+bytes per launch (light raster: 16 B per billboard + 8 B per depth key and
+per chord-sample store; cone: 8 texels * 16 B per mip level fetched per
+sample, plus the 16 B/pixel image write) over that average duration.
 The CPU baseline is the oracle's FP32 pass timed on this host.
 """
 from __future__ import annotations

@@ -268,32 +268,35 @@ __device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv,
     return cnt;
 }
 
-// The pass-1 work list in one workgroup: every billboard's pixel box (lbox),
-// the exclusive scan of the tile counts and the (billboard, tile) items in
-// billboard order, cut at cap; *total receives the uncut item count.  Four
-// consecutive billboards per thread, 4096 per round; then each thread writes
-// an equal share of the round's items, finding its first billboard by a
-// binary search over the round's offsets.
-__global__ void __launch_bounds__(1024) k_light_items(GridDesc g, LightView lv, const float4* __restrict__ boards,
-                                                      int S, int4* __restrict__ lbox, int2* __restrict__ items,
-                                                      int cap, int* __restrict__ total) {
-    __shared__ int wsum[16];
-    __shared__ int cnt_l[4096], off_l[4096];
+// The pass-1 work list: every billboard's pixel box (lbox) and tile count,
+// the exclusive scan of the counts (offsets[S] = the uncut item count) and
+// the (billboard, tile) items in billboard order, cut at cap.  Three small
+// kernels (256-thread workgroups): they find room on CUs that the cone trace
+// of the previous frame keeps busy, where one 1024-thread workgroup waited
+// for a whole CU to drain.
+__global__ void __launch_bounds__(256) k_light_boxes(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= S) return;
+    int4 box;
+    counts[j] = light_box(g, lv, boards[j], &box);
+    lbox[j] = box;
+}
+
+// one workgroup: 4096 counts per round, 16 consecutive per thread
+__global__ void __launch_bounds__(256) k_light_scan(const int* __restrict__ counts, int S, int* __restrict__ offsets) {
+    __shared__ int wsum[4];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int carry = 0;
     for (int base = 0; base < S; base += 4096) {
-        int c[4];
+        int c[16];
+        int tsum = 0;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int j = base + tid * 4 + e;
-            c[e] = 0;
-            if (j < S) {
-                int4 box;
-                c[e] = light_box(g, lv, boards[j], &box);
-                lbox[j] = box;
-            }
+        for (int e = 0; e < 16; ++e) {
+            const int j = base + tid * 16 + e;
+            c[e] = j < S ? counts[j] : 0;
+            tsum += c[e];
         }
-        const int tsum = (c[0] + c[1]) + (c[2] + c[3]);
         int x = tsum;  // inclusive wave scan
         for (int off = 1; off < 64; off <<= 1) {
             const int y = __shfl_up(x, off, 64);
@@ -302,42 +305,30 @@ __global__ void __launch_bounds__(1024) k_light_items(GridDesc g, LightView lv, 
         if (lane == 63) wsum[wid] = x;
         __syncthreads();
         int wpre = 0, round = 0;
-        for (int w = 0; w < 16; ++w) {
+        for (int w = 0; w < 4; ++w) {
             const int ws = wsum[w];
             wpre += w < wid ? ws : 0;
             round += ws;
         }
         int o = carry + wpre + x - tsum;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            cnt_l[tid * 4 + e] = c[e];
-            off_l[tid * 4 + e] = o;
+        for (int e = 0; e < 16; ++e) {
+            const int j = base + tid * 16 + e;
+            if (j < S) offsets[j] = o;
             o += c[e];
         }
-        __syncthreads();
-        const int per = (round + 1023) >> 10;
-        const int i0 = carry + tid * per, i1 = min(i0 + per, min(carry + round, cap));
-        if (i0 < i1) {
-            int lo = 0, hi = 4095;  // last billboard with off_l <= i0
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (off_l[mid] <= i0) lo = mid;
-                else hi = mid - 1;
-            }
-            int b = lo, ob = off_l[b], cb = cnt_l[b];
-            for (int i = i0; i < i1; ++i) {
-                while (i >= ob + cb) {
-                    ++b;
-                    ob = off_l[b];
-                    cb = cnt_l[b];
-                }
-                items[i] = make_int2(base + b, i - ob);
-            }
-        }
-        __syncthreads();  // wsum, cnt_l and off_l are rewritten next round
         carry += round;
+        __syncthreads();  // wsum is rewritten next round
     }
-    if (tid == 0) *total = carry;
+    if (tid == 0) offsets[S] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ offsets, int S, int2* __restrict__ items,
+                                                      int cap) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= S) return;
+    const int o = offsets[j], c = offsets[j + 1] - o;
+    for (int t = 0; t < c && o + t < cap; ++t) items[o + t] = make_int2(j, t);
 }
 
 constexpr int kTilePixels = kLightTile * kLightTile;
@@ -377,6 +368,9 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     // its billboard after the pixel pass), hiding the dependent loads.  The
     // last item fetches itself again.
     if ((int)blockIdx.x >= total) return;
+    // NDC per pixel and voxels per world unit, for the conservative slab boxes
+    const float kW = 2.0f / (float)lv.W, kH = 2.0f / (float)lv.H;
+    const float kvox[3] = {(float)g.N * g.inv_range[0], (float)g.N * g.inv_range[1], (float)g.N * g.inv_range[2]};
     int item = blockIdx.x;
     int2 it = items[item];
     float4 bd = boards[it.x];
@@ -393,12 +387,14 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     bool use_lds = false;
     if (chord) {
         // world box of the chords behind the tile: c + du*S + dv*U + t*B,
-        // du/dv over the tile's view-plane rectangle (within the disc), |t| <= s
-        float du0 = fmaxf((pixel_ndc(x0, lv.W) * lv.lsx + lv.lox) - o.vx, -o.s);
-        float du1 = fminf((pixel_ndc(x1, lv.W) * lv.lsx + lv.lox) - o.vx, o.s);
-        float dv0 = fmaxf((pixel_ndc(y0, lv.H) * lv.lsy + lv.loy) - o.vy, -o.s);
-        float dv1 = fminf((pixel_ndc(y1, lv.H) * lv.lsy + lv.loy) - o.vy, o.s);
-        const float fN = (float)g.N;
+        // du/dv over the tile's view-plane rectangle (within the disc), |t| <= s.
+        // Reciprocal products instead of divisions: the box keeps a 1-voxel
+        // margin, and samples outside it are stored directly, so it only has
+        // to be close.
+        float du0 = fmaxf(((((float)x0 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, -o.s);
+        float du1 = fminf(((((float)x1 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, o.s);
+        float dv0 = fmaxf(((((float)y0 + 0.5f) * kH - 1.0f) * lv.lsy + lv.loy) - o.vy, -o.s);
+        float dv1 = fminf(((((float)y1 + 0.5f) * kH - 1.0f) * lv.lsy + lv.loy) - o.vy, o.s);
         const float c3[3] = {o.cx, o.cy, o.cz};
         int b0[3], b1[3];
         for (int a = 0; a < 3; ++a) {
@@ -406,8 +402,8 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             float tb = o.s * fabsf(lv.B[a]);
             float wlo = c3[a] + fminf(su0, su1) + fminf(sv0, sv1) - tb;
             float whi = c3[a] + fmaxf(su0, su1) + fmaxf(sv0, sv1) + tb;
-            float l0 = fN * ((wlo - g.lo[a]) / g.range[a]) - 1.0f;
-            float l1 = fN * ((whi - g.lo[a]) / g.range[a]) + 1.0f;
+            float l0 = (wlo - g.lo[a]) * kvox[a] - 1.0f;
+            float l1 = (whi - g.lo[a]) * kvox[a] + 1.0f;
             float f0 = fmaxf(floorf(l0 * 0.25f), 0.0f), f1 = fminf(floorf(l1 * 0.25f), (float)(g.nb0 - 1));
             b0[a] = (int)f0;
             b1[a] = f1 >= f0 ? (int)f1 : (int)f0 - 1;
@@ -2530,8 +2526,10 @@ void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const
                          unsigned long long* counters, hipStream_t s) {
     if (S <= 0) return;
     const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
-    hipLaunchKernelGGL(k_light_items, dim3(1), dim3(1024), 0, s, g, lv, boards, S, ls.lbox, ls.items, cap,
-                       ls.offsets + S);
+    const dim3 bgrid((unsigned)((S + 255) / 256));
+    hipLaunchKernelGGL(k_light_boxes, bgrid, dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts);
+    hipLaunchKernelGGL(k_light_scan, dim3(1), dim3(256), 0, s, ls.counts, S, ls.offsets);
+    hipLaunchKernelGGL(k_light_expand, bgrid, dim3(256), 0, s, ls.offsets, S, ls.items, cap);
     unsigned grid = (unsigned)std::min<size_t>(ls.items_cap, 4096);
     const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
     const int chord = mode == 1 ? 1 : 0;
