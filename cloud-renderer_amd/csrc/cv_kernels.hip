@@ -2297,7 +2297,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
 // The learned variant asks for 3 waves per SIMD: the inline MLP needs ~100
 // registers beside the march state, and without the bound the kernel gets 2.
 template <int FMT, int CONE, int COMP, int LEARNED>
-__global__ void __launch_bounds__(64, LEARNED ? 3 : 1) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
+__global__ void __launch_bounds__(64, LEARNED ? 3 : 6) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
                                              const void* __restrict__ vol, const float4* __restrict__ sph_world,
                                              const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
                                              const unsigned long long* __restrict__ masks, int words,
