@@ -57,8 +57,12 @@ struct cr_volume_s {
     bool lset_failed = false;
     hipEvent_t ev_release[2] = {nullptr, nullptr};
     bool release_valid[2] = {false, false};
-    unsigned long long* d_shell = nullptr;
+    unsigned long long* d_shell = nullptr;  // occupancy words of the current set (the last voxelization's)
     unsigned long long* d_fill = nullptr;
+    // occupancy words per set, paired with the level sets: frame f + 1's
+    // passes write one pair while frame f's build reads the other
+    unsigned long long* d_shell_s[2] = {nullptr, nullptr};
+    unsigned long long* d_fill_s[2] = {nullptr, nullptr};
     uint32_t nbricks0 = 0;
     int S = 0;
     float max_radius = 0.0f;
@@ -102,13 +106,18 @@ struct cr_volume_s {
     hipStream_t side = nullptr;
     hipEvent_t ev_cam = nullptr, ev_free = nullptr;
     bool free_valid = false;
-    // voxelization stream of cr_voxelize: clear, pass 1, pass 2 and the level
-    // build (into the set that is not current) run there, overlapping the
-    // cone trace that reads the current set; the caller's stream waits for
-    // the build.  ev_built marks the end of the last build.
-    hipStream_t vside = nullptr;
-    hipEvent_t ev_splat = nullptr, ev_built = nullptr;
-    bool built_valid = false;
+    // voxelization streams of cr_voxelize: clear, pass 1 and pass 2 run on
+    // vside, the level build on bside, both into the set that is not current,
+    // overlapping the cone trace that reads the current set; the caller's
+    // stream waits for the build (ev_built).  ev_bfree[i]: the end of the
+    // last build that read occupancy set i.  ev_sync orders the streams after
+    // the caller's stream when s_sync_needed (first call, after a serial call
+    // or a batch).
+    hipStream_t vside = nullptr, bside = nullptr;
+    hipEvent_t ev_splat = nullptr, ev_built = nullptr, ev_sync = nullptr;
+    hipEvent_t ev_bfree[2] = {nullptr, nullptr};
+    bool bfree_valid[2] = {false, false};
+    bool s_sync_needed = true;
 };
 
 // far-field MLP: FP32 master parameters + Adam moments, bf16 packed copies for
@@ -466,8 +475,10 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     v->d_levels = v->d_lset[0];
     v->zflags_bytes = (build_zero_flags(v->g, v->vd) + 7) & ~(size_t)7;
     if (e == hipSuccess && v->zflags_bytes) e = hipMalloc((void**)&v->d_zflags[0], v->zflags_bytes);
-    if (e == hipSuccess) e = hipMalloc((void**)&v->d_shell, (size_t)v->nbricks0 * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&v->d_fill, (size_t)v->nbricks0 * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_shell_s[0], (size_t)v->nbricks0 * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&v->d_fill_s[0], (size_t)v->nbricks0 * 8);
+    v->d_shell = v->d_shell_s[0];
+    v->d_fill = v->d_fill_s[0];
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_counters, 16 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&v->d_tie, 64);
     if (e != hipSuccess) {
@@ -492,14 +503,15 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
 int cr_volume_destroy(cr_volume v) {
     if (!v) return CR_EINVAL;
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
-    for (hipStream_t st : {v->side, v->vside}) {
+    for (hipStream_t st : {v->side, v->vside, v->bside}) {
         if (!st) continue;
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
-    for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built, v->ev_release[0], v->ev_release[1]})
+    for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built, v->ev_sync, v->ev_release[0],
+                         v->ev_release[1], v->ev_bfree[0], v->ev_bfree[1]})
         if (e) (void)hipEventDestroy(e);
-    void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_zflags[0], v->d_zflags[1], v->d_shell,   v->d_fill,    v->d_boards,    v->d_keys,      v->d_posmap,
+    void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_zflags[0], v->d_zflags[1], v->d_shell_s[0], v->d_shell_s[1], v->d_fill_s[0], v->d_fill_s[1],    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
     for (void* p : ptrs)
@@ -543,6 +555,7 @@ int cr_volume_set_billboards(cr_volume v, const float* xyz, const float* scales,
     // pending camera or voxelization passes read the boards
     if (v->side) CR_HIP(ctx, hipStreamSynchronize(v->side));
     if (v->vside) CR_HIP(ctx, hipStreamSynchronize(v->vside));
+    if (v->bside) CR_HIP(ctx, hipStreamSynchronize(v->bside));
     size_t cap = (size_t)v->boards_cap;
     rc = ensure(ctx, &v->d_boards, (size_t)std::max(count, 1), &cap);
     if (rc) return rc;
@@ -605,21 +618,24 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
 }
 
 // kernels only (no allocation, no host synchronisation): safe under graph capture
-// Passes 1 and 2 on `vs`, the level build into level set `set` on `bs`
-// (after them when the streams differ, and after `wait` when given).
+// Passes 1 and 2 into occupancy set `set` on `vs`, the level build into
+// level set `set` on `bs` (after them when the streams differ, and after
+// `wait` when given).
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
                              hipStream_t vs, hipStream_t bs, int set, hipEvent_t wait) {
+    unsigned long long* shell = v->d_shell_s[set];
+    unsigned long long* fill = v->d_fill_s[set];
     if (ev) ev->mark(0);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
     const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
-                             {(uint64_t*)v->d_shell, (uint64_t)v->nbricks0, 0},
-                             {(uint64_t*)v->d_fill, (uint64_t)v->nbricks0, 0},
+                             {(uint64_t*)shell, (uint64_t)v->nbricks0, 0},
+                             {(uint64_t*)fill, (uint64_t)v->nbricks0, 0},
                              {(uint64_t*)counters, counters ? 9u : 0u, 0}};
     launch_fill_multi(segs, 4, vs);
     if (ev) ev->mark(1);
-    launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, v->d_fill, counters, vs);
+    launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, fill, counters, vs);
     if (ev) ev->mark(2);
-    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, v->d_shell, counters, vs);
+    launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, shell, counters, vs);
     if (ev) ev->mark(3);
     if (vs != bs) {
         (void)hipEventRecord(v->ev_splat, vs);
@@ -627,7 +643,7 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
     }
     if (wait) (void)hipStreamWaitEvent(bs, wait, 0);
     if (ev) ev->mark_on(5, bs);
-    launch_build_levels(v->g, v->vd, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_lset[set],
+    launch_build_levels(v->g, v->vd, shell, v->mode == CR_MODE_RGBA ? fill : nullptr, v->d_lset[set],
                         v->d_zflags[set], bs);
     if (ev) ev->mark_on(4, bs);
 }
@@ -639,60 +655,77 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     cr_context ctx = v->ctx;
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
-    // Clear, pass 1, pass 2 and the build write only the keys, the bitmaps,
-    // the work list and the level set that is not current: they run on the
-    // volume's voxelization stream, overlapping the cone trace that reads the
-    // current set (the build waits for the end of the work enqueued while its
-    // set was last current), and the caller's stream waits for the build.
-    // With one level set the build runs on the caller's stream after passes
-    // 1-2; counter runs stay serial.
+    // Clear, pass 1 and pass 2 (on the voxelization stream) and the build
+    // (on the build stream) write only the keys, the work list and the
+    // occupancy and level sets that are not current, so they overlap the cone
+    // trace that reads the current sets; the passes wait for the end of the
+    // build that last read their occupancy set, the build for the end of the
+    // work enqueued while its level set was last current, and the caller's
+    // stream waits for the build.  Without room for second sets, or in counter
+    // runs, everything runs on the caller's stream.
     const bool overlap = ctx->profiling != 1;
-    hipStream_t vs = s;
     if (overlap) {
         if (!v->vside) {
             CR_HIP(ctx, hipStreamCreateWithFlags(&v->vside, hipStreamNonBlocking));
-            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_splat, hipEventDisableTiming));
-            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_built, hipEventDisableTiming));
-            for (int i = 0; i < 2; ++i) CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_release[i], hipEventDisableTiming));
+            CR_HIP(ctx, hipStreamCreateWithFlags(&v->bside, hipStreamNonBlocking));
+            for (hipEvent_t* e : {&v->ev_splat, &v->ev_built, &v->ev_sync, &v->ev_release[0], &v->ev_release[1],
+                                  &v->ev_bfree[0], &v->ev_bfree[1]})
+                CR_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
         }
-        vs = v->vside;
-        if (!v->built_valid) CR_HIP(ctx, hipEventRecord(v->ev_built, s));  // after all prior work on s
-        CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_built, 0));
         if (!v->d_lset[1] && !v->lset_failed) {
             void* p = nullptr;
             uint8_t* z = nullptr;
+            unsigned long long *sh = nullptr, *fl = nullptr;
+            const size_t wb = (size_t)v->nbricks0 * 8;
             if (hipMalloc(&p, v->levels_bytes) != hipSuccess ||
-                (v->zflags_bytes && hipMalloc((void**)&z, v->zflags_bytes) != hipSuccess)) {
-                (void)hipGetLastError();  // no room for a second set: build into the one set
-                if (p) (void)hipFree(p);
+                (v->zflags_bytes && hipMalloc((void**)&z, v->zflags_bytes) != hipSuccess) ||
+                hipMalloc((void**)&sh, wb) != hipSuccess || hipMalloc((void**)&fl, wb) != hipSuccess) {
+                (void)hipGetLastError();  // no room for second sets: one set on the caller's stream
+                for (void* q : {(void*)p, (void*)z, (void*)sh, (void*)fl})
+                    if (q) (void)hipFree(q);
                 v->lset_failed = true;
             } else {
                 v->d_lset[1] = p;
                 v->d_zflags[1] = z;
-                launch_fill_u64((uint64_t*)p, v->levels_bytes / 8, 0, vs);
-                if (z) launch_fill_u64((uint64_t*)z, v->zflags_bytes / 8, 0x0101010101010101ull, vs);
+                v->d_shell_s[1] = sh;
+                v->d_fill_s[1] = fl;
+                launch_fill_u64((uint64_t*)p, v->levels_bytes / 8, 0, s);
+                if (z) launch_fill_u64((uint64_t*)z, v->zflags_bytes / 8, 0x0101010101010101ull, s);
+                v->s_sync_needed = true;  // the streams wait for these fills
             }
         }
     }
     const bool two = overlap && v->d_lset[1] != nullptr;
     const int nxt = two ? v->cur ^ 1 : v->cur;
+    hipStream_t vs = two ? v->vside : s, bs = two ? v->bside : s;
+    if (two) {
+        if (v->s_sync_needed) {  // after all prior work on s
+            CR_HIP(ctx, hipEventRecord(v->ev_sync, s));
+            CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_sync, 0));
+            CR_HIP(ctx, hipStreamWaitEvent(bs, v->ev_sync, 0));
+            v->s_sync_needed = false;
+        }
+        if (v->bfree_valid[nxt]) CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_bfree[nxt], 0));
+    }
     StageEvents ev;
     if ((rc = ev.begin(ctx, vs, 0))) return rc;
-    voxelize_enqueue(v, plan, counters, &ev, vs, two ? vs : s, nxt,
+    voxelize_enqueue(v, plan, counters, &ev, vs, bs, nxt,
                      two && v->release_valid[nxt] ? v->ev_release[nxt] : nullptr);
     ev.end();
     if (two) {
-        // what was enqueued on s so far may read the current set
+        CR_HIP(ctx, hipEventRecord(v->ev_bfree[nxt], bs));
+        v->bfree_valid[nxt] = true;
+        // what was enqueued on s so far may read the current level set
         CR_HIP(ctx, hipEventRecord(v->ev_release[v->cur], s));
         v->release_valid[v->cur] = true;
-        CR_HIP(ctx, hipEventRecord(v->ev_built, vs));
+        CR_HIP(ctx, hipEventRecord(v->ev_built, bs));
         CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_built, 0));
-        v->built_valid = true;
         v->cur = nxt;
         v->d_levels = v->d_lset[nxt];
-    } else if (overlap) {
-        CR_HIP(ctx, hipEventRecord(v->ev_built, s));
-        v->built_valid = true;
+        v->d_shell = v->d_shell_s[nxt];
+        v->d_fill = v->d_fill_s[nxt];
+    } else {
+        v->s_sync_needed = true;
     }
     CR_HIP(ctx, hipGetLastError());
     if (ctx->profiling == 1) {
@@ -1505,6 +1538,8 @@ int cr_batch_run(cr_batch b, void* stream) {
         if (v->vside) {
             CR_HIP(ctx, hipEventRecord(v->ev_splat, v->vside));
             CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_splat, 0));
+            CR_HIP(ctx, hipEventRecord(v->ev_sync, v->bside));
+            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_sync, 0));
         }
     }
     if (ctx->profiling == 1) {
@@ -1542,15 +1577,14 @@ int cr_batch_run(cr_batch b, void* stream) {
             CR_HIP(ctx, hipEventRecord(v->ev_free, s));
             v->free_valid = true;
         }
-        if (v->vside) {
-            CR_HIP(ctx, hipEventRecord(v->ev_built, s));
-            v->built_valid = true;
-        }
-        if (v->cur != 0) {  // set 0 (the batch's) becomes current
+        v->s_sync_needed = true;  // its next voxelization follows this run
+        if (v->cur != 0) {  // the batch's sets (0) become current
             CR_HIP(ctx, hipEventRecord(v->ev_release[v->cur], s));
             v->release_valid[v->cur] = true;
             v->cur = 0;
             v->d_levels = v->d_lset[0];
+            v->d_shell = v->d_shell_s[0];
+            v->d_fill = v->d_fill_s[0];
         }
     }
     for (auto& it : b->items) {
