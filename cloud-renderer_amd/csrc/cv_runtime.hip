@@ -37,6 +37,21 @@ struct cr_context_s {
     std::vector<Pending> pending;
 };
 
+// camera-pass buffers of one cone trace (billboard view data, draw order,
+// tile masks, sort and binning scratch)
+struct CamBufs {
+    float4* sph_world = nullptr;
+    float4* sph_view = nullptr;
+    int4* sph_tiles = nullptr;
+    unsigned long long* sort = nullptr;
+    int32_t* order = nullptr;
+    int* tie = nullptr;
+    int* tiles = nullptr;
+    unsigned long long* masks = nullptr;
+    int sph_cap = 0;
+    size_t sort_cap = 0, masks_cap = 0, tiles_cap = 0;
+};
+
 struct cr_volume_s {
     cr_context ctx = nullptr;
     int N = 0, L = 0, format = 0, mode = 0;
@@ -101,11 +116,18 @@ struct cr_volume_s {
     size_t posmap_cap = 0;
     // camera-side stream of cr_cone_trace: cam setup, sort and binning run
     // there, overlapped with the caller's stream, which waits for them only
-    // before the cone-trace kernel; ev_free marks the end of the last cone
-    // trace (its sort and mask buffers may be rewritten after it)
+    // before the cone-trace kernel.  Consecutive traces alternate between the
+    // buffers above and cam_spare (cset: which of the two pairs is above), so
+    // a trace's camera pass overlaps the previous trace; ev_cfree[i] marks the
+    // end of the last cone trace that read pair i, ev_csync orders the
+    // camera stream after the caller's stream when cam_sync_needed.
+    CamBufs cam_spare;
+    int cset = 0;
     hipStream_t side = nullptr;
-    hipEvent_t ev_cam = nullptr, ev_free = nullptr;
-    bool free_valid = false;
+    hipEvent_t ev_cam = nullptr, ev_csync = nullptr;
+    hipEvent_t ev_cfree[2] = {nullptr, nullptr};
+    bool cfree_valid[2] = {false, false};
+    bool cam_sync_needed = true;
     // voxelization streams of cr_voxelize: clear, pass 1 and pass 2 run on
     // vside, the level build on bside, both into the set that is not current,
     // overlapping the cone trace that reads the current set; the caller's
@@ -508,12 +530,14 @@ int cr_volume_destroy(cr_volume v) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
-    for (hipEvent_t e : {v->ev_cam, v->ev_free, v->ev_splat, v->ev_built, v->ev_sync, v->ev_release[0],
+    for (hipEvent_t e : {v->ev_cam, v->ev_csync, v->ev_cfree[0], v->ev_cfree[1], v->ev_splat, v->ev_built, v->ev_sync, v->ev_release[0],
                          v->ev_release[1], v->ev_bfree[0], v->ev_bfree[1]})
         if (e) (void)hipEventDestroy(e);
     void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_zflags[0], v->d_zflags[1], v->d_shell_s[0], v->d_shell_s[1], v->d_fill_s[0], v->d_fill_s[1],    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
-                    v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles};
+                    v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles,
+                    v->cam_spare.sph_world, v->cam_spare.sph_view, v->cam_spare.sph_tiles, v->cam_spare.sort,
+                    v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -930,30 +954,69 @@ static void cone_enqueue_sort(cr_volume v, const ConePlan& plan, StageEvents* ev
     if (ev) ev->mark(2);
 }
 
+// exchange the volume's camera buffers with the spare pair
+static void swap_cam(cr_volume v) {
+    CamBufs& o = v->cam_spare;
+    std::swap(v->d_sph_world, o.sph_world);
+    std::swap(v->d_sph_view, o.sph_view);
+    std::swap(v->d_sph_tiles, o.sph_tiles);
+    std::swap(v->d_sort, o.sort);
+    std::swap(v->d_order, o.order);
+    std::swap(v->d_tie, o.tie);
+    std::swap(v->d_tiles, o.tiles);
+    std::swap(v->d_masks, o.masks);
+    std::swap(v->sph_cap, o.sph_cap);
+    std::swap(v->sort_cap, o.sort_cap);
+    std::swap(v->masks_cap, o.masks_cap);
+    std::swap(v->tiles_cap, o.tiles_cap);
+    v->cset ^= 1;
+}
+
 static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
                            const LearnedOpts* lo, void* d_image, int W, int H, size_t pitch, int pixfmt, void* stream) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_cone_trace: vol is NULL");
+    cr_context ctx = v->ctx;
+    // Camera setup, sort and binning depend on the billboards and the camera
+    // only: they run on the volume's camera-side stream into the buffer pair
+    // the previous trace did not use, from the end of the trace before it
+    // on, so they overlap the previous cone trace and the voxelization
+    // enqueued before this call; the cone kernel waits for them.  Counter
+    // runs stay serial.
+    const bool overlap = ctx->profiling != 1;
+    bool swapped = false;
+    if (overlap && v->voxelized) {
+        if (!v->cam_spare.tie && hipMalloc((void**)&v->cam_spare.tie, 64) != hipSuccess) {
+            (void)hipGetLastError();  // no second pair: the camera pass waits for the previous trace
+            v->cam_spare.tie = nullptr;
+        }
+        if (v->cam_spare.tie) {
+            swap_cam(v);
+            swapped = true;
+        }
+    }
     ConePlan plan;
     int rc = cone_prepare(v, light, cam, p, lo, d_image, W, H, pitch, pixfmt, kTile, plan);
-    if (rc) return rc;
-    cr_context ctx = v->ctx;
+    if (rc) {
+        if (swapped) swap_cam(v);
+        return rc;
+    }
     if (!v->voxelized) return fail(ctx, CR_ESTATE, "cr_cone_trace: volume not voxelized");
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
-    // Camera setup, sort and binning depend on the billboards and the camera
-    // only: they run on the volume's camera-side stream from the end of the
-    // previous cone trace on, overlapping the voxelization enqueued before
-    // this call; the cone kernel waits for them.  Counter runs stay serial.
-    const bool overlap = ctx->profiling != 1;
     hipStream_t cs = s;
     if (overlap) {
         if (!v->side) {
             CR_HIP(ctx, hipStreamCreateWithFlags(&v->side, hipStreamNonBlocking));
-            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_cam, hipEventDisableTiming));
-            CR_HIP(ctx, hipEventCreateWithFlags(&v->ev_free, hipEventDisableTiming));
+            for (hipEvent_t* e : {&v->ev_cam, &v->ev_csync, &v->ev_cfree[0], &v->ev_cfree[1]})
+                CR_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
         }
         cs = v->side;
-        if (!v->free_valid) CR_HIP(ctx, hipEventRecord(v->ev_free, s));  // first trace: after all prior work
-        CR_HIP(ctx, hipStreamWaitEvent(cs, v->ev_free, 0));
+        if (v->cam_sync_needed) {  // after all prior work on s
+            CR_HIP(ctx, hipEventRecord(v->ev_csync, s));
+            CR_HIP(ctx, hipStreamWaitEvent(cs, v->ev_csync, 0));
+            v->cam_sync_needed = false;
+        }
+        if (v->cfree_valid[v->cset]) CR_HIP(ctx, hipStreamWaitEvent(cs, v->ev_cfree[v->cset], 0));
     }
     StageEvents ev;
     if ((rc = ev.begin(ctx, cs, 1))) return rc;
@@ -988,8 +1051,10 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     ev.mark_on(5, s);
     ev.end();
     if (overlap) {
-        CR_HIP(ctx, hipEventRecord(v->ev_free, s));
-        v->free_valid = true;
+        CR_HIP(ctx, hipEventRecord(v->ev_cfree[v->cset], s));
+        v->cfree_valid[v->cset] = true;
+    } else {
+        v->cam_sync_needed = true;
     }
     CR_HIP(ctx, hipGetLastError());
     v->sorted_count = plan.S;
@@ -1320,6 +1385,7 @@ struct cr_batch_s {
     bool have_graph = false;
     cr_batch_config graph_cfg{0, 0, 0};
     std::vector<uint64_t> graph_gen;
+    std::vector<const void*> graph_ptrs;  // per scene: the camera buffers the graph was captured with
     int total_tiles = 0;
 };
 
@@ -1417,10 +1483,22 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
+// the camera buffers a batch graph bakes in (a single trace may reallocate them)
+static void cam_ptrs(cr_volume v, const void** out) {
+    const void* p[8] = {v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort,
+                        v->d_order,     v->d_tie,      v->d_tiles,     v->d_masks};
+    for (int k = 0; k < 8; ++k) out[k] = p[k];
+}
+
 bool graph_current(cr_batch b) {
     if (!b->have_graph || !same_cfg(b->cfg, b->graph_cfg)) return false;
-    for (size_t i = 0; i < b->items.size(); ++i)
+    for (size_t i = 0; i < b->items.size(); ++i) {
         if (b->items[i].volume->gen != b->graph_gen[i]) return false;
+        const void* p[8];
+        cam_ptrs(b->items[i].volume, p);
+        for (int k = 0; k < 8; ++k)
+            if (p[k] != b->graph_ptrs[8 * i + k]) return false;
+    }
     return true;
 }
 
@@ -1445,7 +1523,11 @@ int batch_capture(cr_batch b) {
     b->have_graph = true;
     b->graph_cfg = b->cfg;
     b->graph_gen.resize(b->items.size());
-    for (size_t i = 0; i < b->items.size(); ++i) b->graph_gen[i] = b->items[i].volume->gen;
+    b->graph_ptrs.resize(8 * b->items.size());
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        b->graph_gen[i] = b->items[i].volume->gen;
+        cam_ptrs(b->items[i].volume, &b->graph_ptrs[8 * i]);
+    }
     return CR_OK;
 }
 
@@ -1541,6 +1623,7 @@ int cr_batch_run(cr_batch b, void* stream) {
             CR_HIP(ctx, hipEventRecord(v->ev_sync, v->bside));
             CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_sync, 0));
         }
+        if (v->cset != 0) swap_cam(v);  // batches use camera pair 0
     }
     if (ctx->profiling == 1) {
         // eager run with counters and phase times
@@ -1573,10 +1656,7 @@ int cr_batch_run(cr_batch b, void* stream) {
     // and the next camera pass and voxelization of a volume wait for this run
     for (const auto& it : b->items) {
         cr_volume v = it.volume;
-        if (v->side) {
-            CR_HIP(ctx, hipEventRecord(v->ev_free, s));
-            v->free_valid = true;
-        }
+        v->cam_sync_needed = true;  // its next camera pass follows this run
         v->s_sync_needed = true;  // its next voxelization follows this run
         if (v->cur != 0) {  // the batch's sets (0) become current
             CR_HIP(ctx, hipEventRecord(v->ev_release[v->cur], s));
