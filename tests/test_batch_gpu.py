@@ -135,3 +135,36 @@ def test_batch_autotune_and_counters(cv, gpu_ctx):
         _check(entries, want, "eager profiled run")
     finally:
         b.close()
+
+
+def test_batch_after_single_traces_on_its_volumes(cv, gpu_ctx):
+    """Single voxelize + trace calls on a batch's volumes between runs, one
+    with a larger image (its camera buffers grow and are reallocated): the
+    next run captures again where needed and still reproduces every scene."""
+    import torch
+    scenes = _scenes(cv, n=3, seed=6)
+    want = [_single(cv, gpu_ctx, sc) for sc in scenes]
+    b, entries = _batch(cv, gpu_ctx, scenes)
+    try:
+        b.run()
+        torch.cuda.synchronize()
+        _check(entries, want, "first run")
+        vol, sun, cam, tr, _, vox = entries[0]
+        sc = scenes[0]
+        for k in range(3):
+            # two traces, so both camera buffer pairs are used; the second
+            # one of round 1 is larger than anything the batch traced
+            for t in range(2):
+                w, h = (2 * sc.W, 2 * sc.H) if (k == 1 and t == 1) else (sc.W, sc.H)
+                big = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+                vox.voxelize(vol, sun)
+                tr.coneTrace(vol, sun, cam, big, w, h)
+            torch.cuda.synchronize()
+            for _, _, _, _, img, _ in entries:
+                img.fill_(-1.0)
+            b.run()
+            torch.cuda.synchronize()
+            _check(entries, want, f"run after single trace {k}")
+        log("[batch] runs after single traces on a batch volume equal the per-scene images")
+    finally:
+        b.close()
