@@ -164,10 +164,11 @@ int cr_volume_set_billboards(cr_volume vol, const float *xyz, const float *scale
 /* VoxelizeShader::voxelize (src/Shaders/VoxelizeShader.cpp:18-118): clear,
  * pass 1 from the sun's orthographic view into a light_w x light_h position
  * map with a depth test, pass 2 splat of 9 voxels per texel, mip chain.
- * Clear and both passes run on a stream owned by the volume, after the
- * volume's previous level build and concurrently with work already queued on
- * `stream` (the previous cone trace, which reads only the levels); the level
- * build runs on `stream` after them. */
+ * The volume holds two sets of occupancy words and mip levels: the clear and
+ * both passes (on one stream owned by the volume) and the level build (on a
+ * second) write the set that is not current, concurrently with work already
+ * queued on `stream` (the previous cone trace reads the current set);
+ * `stream` waits for the build, after which the new set is current. */
 int cr_voxelize(cr_volume vol, const cr_light *light, int light_w, int light_h, void *stream);
 
 /* ConeTraceShader::coneTrace (src/Shaders/ConeTraceShader.cpp:15-82):
@@ -175,9 +176,10 @@ int cr_voxelize(cr_volume vol, const cr_light *light, int light_w, int light_h, 
  * shades every camera fragment with one cone toward the sun through the
  * mip chain and composites into d_image (W x H, row pitch in bytes).
  * The camera-side work (setup, sort, binning) runs on a stream owned by the
- * volume, after the volume's previous cone trace and concurrently with work
- * already queued on `stream` (a voxelization); the cone kernel itself runs on
- * `stream` after it.  d_image is written only by work on `stream`. */
+ * volume into one of two buffer pairs that consecutive traces alternate
+ * between, concurrently with work already queued on `stream` (the previous
+ * cone trace, a voxelization); the cone kernel itself runs on `stream` after
+ * it.  d_image is written only by work on `stream`. */
 int cr_cone_trace(cr_volume vol, const cr_light *light, const cr_camera *camera,
                   const cr_cone_params *params, void *d_image, int width, int height,
                   size_t pitch_bytes, int pixel_format, void *stream);
