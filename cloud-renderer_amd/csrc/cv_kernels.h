@@ -123,6 +123,9 @@ struct FillSeg {
 };
 void launch_fill_multi(const FillSeg* segs, int nseg, hipStream_t s);  // up to 4 segments, one launch
 size_t light_items_bound(int S, float max_radius, const LightView& lv);
+// the pass-1 work list (boxes, scan, items); launch_light_raster reads it
+void launch_light_items(const GridDesc& g, const LightView& lv, const float4* boards, int S, const LightScratch& ls,
+                        hipStream_t s);
 void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
                          const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
                          unsigned long long* counters, hipStream_t s);

@@ -2521,15 +2521,21 @@ void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, s, p, n, value);
 }
 
-void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
-                         const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
-                         unsigned long long* counters, hipStream_t s) {
+void launch_light_items(const GridDesc& g, const LightView& lv, const float4* boards, int S, const LightScratch& ls,
+                        hipStream_t s) {
     if (S <= 0) return;
     const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
     const dim3 bgrid((unsigned)((S + 255) / 256));
     hipLaunchKernelGGL(k_light_boxes, bgrid, dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts);
     hipLaunchKernelGGL(k_light_scan, dim3(1), dim3(256), 0, s, ls.counts, S, ls.offsets);
     hipLaunchKernelGGL(k_light_expand, bgrid, dim3(256), 0, s, ls.offsets, S, ls.items, cap);
+}
+
+void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
+                         const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
+                         unsigned long long* counters, hipStream_t s) {
+    if (S <= 0) return;
+    const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
     unsigned grid = (unsigned)std::min<size_t>(ls.items_cap, 4096);
     const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
     const int chord = mode == 1 ? 1 : 0;

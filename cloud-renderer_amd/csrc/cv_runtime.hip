@@ -37,6 +37,14 @@ struct cr_context_s {
     std::vector<Pending> pending;
 };
 
+// pass-1 work-list buffers of one voxelization
+struct LightBufs {
+    int4* lbox = nullptr;
+    int* lcounts = nullptr;
+    int2* litems = nullptr;
+    size_t caps[3] = {0, 0, 0};  // lbox, lcounts, litems
+};
+
 // camera-pass buffers of one cone trace (billboard view data, draw order,
 // tile masks, sort and binning scratch)
 struct CamBufs {
@@ -92,6 +100,16 @@ struct cr_volume_s {
     int* d_lcounts = nullptr;  // counts (S) followed by offsets (S + 1)
     int2* d_litems = nullptr;
     size_t lbox_cap = 0, lcounts_cap = 0, litems_cap = 0;
+    // consecutive overlapped voxelizations alternate between the work-list
+    // buffers above and light_spare (lset_i: which pair is above); the work
+    // list is built on lside, ev_items marks it ready, ev_lfree[i] the end
+    // of the last raster that read pair i
+    LightBufs light_spare;
+    int lset_i = 0;
+    hipStream_t lside = nullptr;
+    hipEvent_t ev_items = nullptr;
+    hipEvent_t ev_lfree[2] = {nullptr, nullptr};
+    bool lfree_valid[2] = {false, false};
     // camera pass
     float4* d_sph_world = nullptr;
     float4* d_sph_view = nullptr;
@@ -525,19 +543,21 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
 int cr_volume_destroy(cr_volume v) {
     if (!v) return CR_EINVAL;
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
-    for (hipStream_t st : {v->side, v->vside, v->bside}) {
+    for (hipStream_t st : {v->side, v->vside, v->bside, v->lside}) {
         if (!st) continue;
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
     for (hipEvent_t e : {v->ev_cam, v->ev_csync, v->ev_cfree[0], v->ev_cfree[1], v->ev_splat, v->ev_built, v->ev_sync, v->ev_release[0],
-                         v->ev_release[1], v->ev_bfree[0], v->ev_bfree[1]})
+                         v->ev_release[1], v->ev_bfree[0], v->ev_bfree[1], v->ev_items, v->ev_lfree[0],
+                         v->ev_lfree[1]})
         if (e) (void)hipEventDestroy(e);
     void* ptrs[] = {v->d_lset[0], v->d_lset[1], v->d_zflags[0], v->d_zflags[1], v->d_shell_s[0], v->d_shell_s[1], v->d_fill_s[0], v->d_fill_s[1],    v->d_boards,    v->d_keys,      v->d_posmap,
                     v->d_lbox,   v->d_lcounts, v->d_litems,  v->d_sph_world, v->d_sph_view,  v->d_sph_tiles,
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles,
                     v->cam_spare.sph_world, v->cam_spare.sph_view, v->cam_spare.sph_tiles, v->cam_spare.sort,
-                    v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks};
+                    v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks,
+                    v->light_spare.lbox, v->light_spare.lcounts, v->light_spare.litems};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -580,6 +600,7 @@ int cr_volume_set_billboards(cr_volume v, const float* xyz, const float* scales,
     if (v->side) CR_HIP(ctx, hipStreamSynchronize(v->side));
     if (v->vside) CR_HIP(ctx, hipStreamSynchronize(v->vside));
     if (v->bside) CR_HIP(ctx, hipStreamSynchronize(v->bside));
+    if (v->lside) CR_HIP(ctx, hipStreamSynchronize(v->lside));
     size_t cap = (size_t)v->boards_cap;
     rc = ensure(ctx, &v->d_boards, (size_t)std::max(count, 1), &cap);
     if (rc) return rc;
@@ -597,6 +618,7 @@ struct VoxPlan {
     LightView lv;
     LightScratch ls;
     size_t npx;
+    size_t nitems;  // work-list capacity
 };
 
 static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int light_h, VoxPlan& plan) {
@@ -627,28 +649,53 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
     v->last_lv = plan.lv;  // the position map is rebuilt from the keys on request
     v->vox_gen = v->gen;
     plan.ls = LightScratch{};
-    if (v->S > 0) {
-        if ((rc = ensure(ctx, &v->d_lbox, (size_t)v->S, &v->lbox_cap))) return rc;
-        if ((rc = ensure(ctx, &v->d_lcounts, 2 * (size_t)v->S + 1, &v->lcounts_cap))) return rc;
-        size_t nitems = light_items_bound(v->S, v->max_radius, plan.lv);
-        if ((rc = ensure(ctx, &v->d_litems, nitems, &v->litems_cap))) return rc;
-        plan.ls.lbox = v->d_lbox;
-        plan.ls.counts = v->d_lcounts;
-        plan.ls.offsets = v->d_lcounts + v->S;
-        plan.ls.items = v->d_litems;
-        plan.ls.items_cap = nitems;
-    }
+    plan.nitems = v->S > 0 ? light_items_bound(v->S, v->max_radius, plan.lv) : 0;
     return CR_OK;
 }
 
+// the volume's current work-list buffers, grown to the plan
+static int light_scratch(cr_volume v, VoxPlan& plan) {
+    cr_context ctx = v->ctx;
+    plan.ls = LightScratch{};
+    if (v->S <= 0) return CR_OK;
+    int rc;
+    if ((rc = ensure(ctx, &v->d_lbox, (size_t)v->S, &v->lbox_cap))) return rc;
+    if ((rc = ensure(ctx, &v->d_lcounts, 2 * (size_t)v->S + 1, &v->lcounts_cap))) return rc;
+    if ((rc = ensure(ctx, &v->d_litems, plan.nitems, &v->litems_cap))) return rc;
+    plan.ls.lbox = v->d_lbox;
+    plan.ls.counts = v->d_lcounts;
+    plan.ls.offsets = v->d_lcounts + v->S;
+    plan.ls.items = v->d_litems;
+    plan.ls.items_cap = plan.nitems;
+    return CR_OK;
+}
+
+// exchange the volume's work-list buffers with the spare pair
+static void swap_light(cr_volume v) {
+    LightBufs& o = v->light_spare;
+    std::swap(v->d_lbox, o.lbox);
+    std::swap(v->d_lcounts, o.lcounts);
+    std::swap(v->d_litems, o.litems);
+    std::swap(v->lbox_cap, o.caps[0]);
+    std::swap(v->lcounts_cap, o.caps[1]);
+    std::swap(v->litems_cap, o.caps[2]);
+    v->lset_i ^= 1;
+}
+
 // kernels only (no allocation, no host synchronisation): safe under graph capture
-// Passes 1 and 2 into occupancy set `set` on `vs`, the level build into
-// level set `set` on `bs` (after them when the streams differ, and after
-// `wait` when given).
+// The work list on `ls` (after `lwait` when given), passes 1 and 2 into
+// occupancy set `set` on `vs`, the level build into level set `set` on `bs`
+// (after them when the streams differ, and after `wait` when given).
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
-                             hipStream_t vs, hipStream_t bs, int set, hipEvent_t wait) {
+                             hipStream_t ls, hipEvent_t lwait, hipStream_t vs, hipStream_t bs, int set,
+                             hipEvent_t wait) {
     unsigned long long* shell = v->d_shell_s[set];
     unsigned long long* fill = v->d_fill_s[set];
+    if (ls != vs) {
+        if (lwait) (void)hipStreamWaitEvent(ls, lwait, 0);
+        launch_light_items(v->g, plan.lv, v->d_boards, v->S, plan.ls, ls);
+        (void)hipEventRecord(v->ev_items, ls);
+    }
     if (ev) ev->mark(0);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
     const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
@@ -657,7 +704,10 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
                              {(uint64_t*)counters, counters ? 9u : 0u, 0}};
     launch_fill_multi(segs, 4, vs);
     if (ev) ev->mark(1);
+    if (ls != vs) (void)hipStreamWaitEvent(vs, v->ev_items, 0);
+    else launch_light_items(v->g, plan.lv, v->d_boards, v->S, plan.ls, vs);
     launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, fill, counters, vs);
+    if (ls != vs) (void)hipEventRecord(v->ev_lfree[v->lset_i], vs);
     if (ev) ev->mark(2);
     launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, shell, counters, vs);
     if (ev) ev->mark(3);
@@ -692,8 +742,9 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
         if (!v->vside) {
             CR_HIP(ctx, hipStreamCreateWithFlags(&v->vside, hipStreamNonBlocking));
             CR_HIP(ctx, hipStreamCreateWithFlags(&v->bside, hipStreamNonBlocking));
+            CR_HIP(ctx, hipStreamCreateWithFlags(&v->lside, hipStreamNonBlocking));
             for (hipEvent_t* e : {&v->ev_splat, &v->ev_built, &v->ev_sync, &v->ev_release[0], &v->ev_release[1],
-                                  &v->ev_bfree[0], &v->ev_bfree[1]})
+                                  &v->ev_bfree[0], &v->ev_bfree[1], &v->ev_items, &v->ev_lfree[0], &v->ev_lfree[1]})
                 CR_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
         }
         if (!v->d_lset[1] && !v->lset_failed) {
@@ -721,21 +772,28 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     }
     const bool two = overlap && v->d_lset[1] != nullptr;
     const int nxt = two ? v->cur ^ 1 : v->cur;
-    hipStream_t vs = two ? v->vside : s, bs = two ? v->bside : s;
+    hipStream_t vs = two ? v->vside : s, bs = two ? v->bside : s, ls = two ? v->lside : s;
+    // the work list goes into the pair the previous voxelization did not use
+    if (two) swap_light(v);
+    if ((rc = light_scratch(v, plan))) {
+        if (two) swap_light(v);
+        return rc;
+    }
     if (two) {
         if (v->s_sync_needed) {  // after all prior work on s
             CR_HIP(ctx, hipEventRecord(v->ev_sync, s));
-            CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_sync, 0));
-            CR_HIP(ctx, hipStreamWaitEvent(bs, v->ev_sync, 0));
+            for (hipStream_t st : {vs, bs, ls}) CR_HIP(ctx, hipStreamWaitEvent(st, v->ev_sync, 0));
             v->s_sync_needed = false;
         }
         if (v->bfree_valid[nxt]) CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_bfree[nxt], 0));
     }
     StageEvents ev;
     if ((rc = ev.begin(ctx, vs, 0))) return rc;
-    voxelize_enqueue(v, plan, counters, &ev, vs, bs, nxt,
+    voxelize_enqueue(v, plan, counters, &ev, ls,
+                     two && v->lfree_valid[v->lset_i] ? v->ev_lfree[v->lset_i] : nullptr, vs, bs, nxt,
                      two && v->release_valid[nxt] ? v->ev_release[nxt] : nullptr);
     ev.end();
+    if (two) v->lfree_valid[v->lset_i] = true;
     if (two) {
         CR_HIP(ctx, hipEventRecord(v->ev_bfree[nxt], bs));
         v->bfree_valid[nxt] = true;
@@ -1385,7 +1443,7 @@ struct cr_batch_s {
     bool have_graph = false;
     cr_batch_config graph_cfg{0, 0, 0};
     std::vector<uint64_t> graph_gen;
-    std::vector<const void*> graph_ptrs;  // per scene: the camera buffers the graph was captured with
+    std::vector<const void*> graph_ptrs;  // per scene: the camera and work-list buffers the graph was captured with
     int total_tiles = 0;
 };
 
@@ -1417,6 +1475,7 @@ int batch_prepare(cr_batch b, std::vector<VoxPlan>& vp, std::vector<ConePlan>& c
         cr_volume v = it.volume;
         int rc = voxelize_prepare(v, &it.light, it.light_w, it.light_h, vp[i]);
         if (rc) return rc;
+        if ((rc = light_scratch(v, vp[i]))) return rc;
         rc = cone_prepare(v, &it.light, &it.camera, &it.params, nullptr, it.d_image, it.width, it.height,
                           it.pitch_bytes, it.pixel_format, b->cfg.tile_w, cp[i]);
         if (rc) return rc;
@@ -1463,7 +1522,7 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
     for (size_t i = 0; i < n; ++i) {
         cr_volume v = b->items[i].volume;
-        voxelize_enqueue(v, vp[i], nullptr, nullptr, s, s, 0, nullptr);
+        voxelize_enqueue(v, vp[i], nullptr, nullptr, s, nullptr, s, s, 0, nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
     launch_fill_u64((uint64_t*)b->d_counts, (uint64_t)(b->total_tiles + 1) / 2, 0, s);
@@ -1484,20 +1543,22 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
 }
 
 // the camera buffers a batch graph bakes in (a single trace may reallocate them)
+constexpr int kGraphPtrs = 11;
 static void cam_ptrs(cr_volume v, const void** out) {
-    const void* p[8] = {v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort,
-                        v->d_order,     v->d_tie,      v->d_tiles,     v->d_masks};
-    for (int k = 0; k < 8; ++k) out[k] = p[k];
+    const void* p[kGraphPtrs] = {v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort,
+                                 v->d_order,     v->d_tie,      v->d_tiles,     v->d_masks,
+                                 v->d_lbox,      v->d_lcounts,  v->d_litems};
+    for (int k = 0; k < kGraphPtrs; ++k) out[k] = p[k];
 }
 
 bool graph_current(cr_batch b) {
     if (!b->have_graph || !same_cfg(b->cfg, b->graph_cfg)) return false;
     for (size_t i = 0; i < b->items.size(); ++i) {
         if (b->items[i].volume->gen != b->graph_gen[i]) return false;
-        const void* p[8];
+        const void* p[kGraphPtrs];
         cam_ptrs(b->items[i].volume, p);
-        for (int k = 0; k < 8; ++k)
-            if (p[k] != b->graph_ptrs[8 * i + k]) return false;
+        for (int k = 0; k < kGraphPtrs; ++k)
+            if (p[k] != b->graph_ptrs[kGraphPtrs * i + k]) return false;
     }
     return true;
 }
@@ -1523,10 +1584,10 @@ int batch_capture(cr_batch b) {
     b->have_graph = true;
     b->graph_cfg = b->cfg;
     b->graph_gen.resize(b->items.size());
-    b->graph_ptrs.resize(8 * b->items.size());
+    b->graph_ptrs.resize(kGraphPtrs * b->items.size());
     for (size_t i = 0; i < b->items.size(); ++i) {
         b->graph_gen[i] = b->items[i].volume->gen;
-        cam_ptrs(b->items[i].volume, &b->graph_ptrs[8 * i]);
+        cam_ptrs(b->items[i].volume, &b->graph_ptrs[kGraphPtrs * i]);
     }
     return CR_OK;
 }
@@ -1622,8 +1683,11 @@ int cr_batch_run(cr_batch b, void* stream) {
             CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_splat, 0));
             CR_HIP(ctx, hipEventRecord(v->ev_sync, v->bside));
             CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_sync, 0));
+            CR_HIP(ctx, hipEventRecord(v->ev_items, v->lside));
+            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_items, 0));
         }
         if (v->cset != 0) swap_cam(v);  // batches use camera pair 0
+        if (v->lset_i != 0) swap_light(v);  // and work-list pair 0
     }
     if (ctx->profiling == 1) {
         // eager run with counters and phase times
