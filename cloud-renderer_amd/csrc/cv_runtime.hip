@@ -42,7 +42,8 @@ struct LightBufs {
     int4* lbox = nullptr;
     int* lcounts = nullptr;
     int2* litems = nullptr;
-    size_t caps[3] = {0, 0, 0};  // lbox, lcounts, litems
+    unsigned long long* keys = nullptr;
+    size_t caps[4] = {0, 0, 0, 0};  // lbox, lcounts, litems, keys
 };
 
 // camera-pass buffers of one cone trace (billboard view data, draw order,
@@ -101,9 +102,10 @@ struct cr_volume_s {
     int2* d_litems = nullptr;
     size_t lbox_cap = 0, lcounts_cap = 0, litems_cap = 0;
     // consecutive overlapped voxelizations alternate between the work-list
-    // buffers above and light_spare (lset_i: which pair is above); the work
-    // list is built on lside, ev_items marks it ready, ev_lfree[i] the end
-    // of the last raster that read pair i
+    // and depth-key buffers above and light_spare (lset_i: which pair is
+    // above); the clear and the work list run on lside ahead of the frame's
+    // raster, ev_items marks them done, ev_lfree[i] the end of the last
+    // passes that read pair i
     LightBufs light_spare;
     int lset_i = 0;
     hipStream_t lside = nullptr;
@@ -557,7 +559,8 @@ int cr_volume_destroy(cr_volume v) {
                     v->d_sort,   v->d_order,   v->d_tie,     v->d_masks,     v->d_counters,  v->d_scratch, v->d_tiles,
                     v->cam_spare.sph_world, v->cam_spare.sph_view, v->cam_spare.sph_tiles, v->cam_spare.sort,
                     v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks,
-                    v->light_spare.lbox, v->light_spare.lcounts, v->light_spare.litems};
+                    v->light_spare.lbox, v->light_spare.lcounts, v->light_spare.litems,
+                    v->light_spare.keys};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -643,7 +646,6 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
     sun_update(v, light->position, light_w, light_h, &plan.lv);
     size_t npx = (size_t)light_w * light_h;
     plan.npx = npx;
-    if ((rc = ensure(ctx, &v->d_keys, npx, &v->light_cap))) return rc;
     v->lw = light_w;
     v->lh = light_h;
     v->last_lv = plan.lv;  // the position map is rebuilt from the keys on request
@@ -657,8 +659,9 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
 static int light_scratch(cr_volume v, VoxPlan& plan) {
     cr_context ctx = v->ctx;
     plan.ls = LightScratch{};
-    if (v->S <= 0) return CR_OK;
     int rc;
+    if ((rc = ensure(ctx, &v->d_keys, plan.npx, &v->light_cap))) return rc;
+    if (v->S <= 0) return CR_OK;
     if ((rc = ensure(ctx, &v->d_lbox, (size_t)v->S, &v->lbox_cap))) return rc;
     if ((rc = ensure(ctx, &v->d_lcounts, 2 * (size_t)v->S + 1, &v->lcounts_cap))) return rc;
     if ((rc = ensure(ctx, &v->d_litems, plan.nitems, &v->litems_cap))) return rc;
@@ -679,38 +682,42 @@ static void swap_light(cr_volume v) {
     std::swap(v->lbox_cap, o.caps[0]);
     std::swap(v->lcounts_cap, o.caps[1]);
     std::swap(v->litems_cap, o.caps[2]);
+    std::swap(v->d_keys, o.keys);
+    std::swap(v->light_cap, o.caps[3]);
     v->lset_i ^= 1;
 }
 
 // kernels only (no allocation, no host synchronisation): safe under graph capture
-// The work list on `ls` (after `lwait` when given), passes 1 and 2 into
-// occupancy set `set` on `vs`, the level build into level set `set` on `bs`
-// (after them when the streams differ, and after `wait` when given).
+// The clear and the work list on `ls` (after `lwaits` when given), passes 1
+// and 2 into occupancy set `set` on `vs`, the level build into level set
+// `set` on `bs` (after them when the streams differ, and after `wait` when
+// given).
 static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long long* counters, StageEvents* ev,
-                             hipStream_t ls, hipEvent_t lwait, hipStream_t vs, hipStream_t bs, int set,
+                             hipStream_t ls, const hipEvent_t* lwaits, hipStream_t vs, hipStream_t bs, int set,
                              hipEvent_t wait) {
     unsigned long long* shell = v->d_shell_s[set];
     unsigned long long* fill = v->d_fill_s[set];
-    if (ls != vs) {
-        if (lwait) (void)hipStreamWaitEvent(ls, lwait, 0);
-        launch_light_items(v->g, plan.lv, v->d_boards, v->S, plan.ls, ls);
-        (void)hipEventRecord(v->ev_items, ls);
-    }
-    if (ev) ev->mark(0);
+    if (lwaits)
+        for (int i = 0; i < 2; ++i)
+            if (lwaits[i]) (void)hipStreamWaitEvent(ls, lwaits[i], 0);
+    if (ev) ev->mark_on(0, ls);
     // clearGPU + clearPositionMap (src/CloudVolume.cpp:96-100, VoxelizeShader.cpp:185-187), one launch
     const FillSeg segs[4] = {{(uint64_t*)v->d_keys, (uint64_t)plan.npx, ~0ull},
                              {(uint64_t*)shell, (uint64_t)v->nbricks0, 0},
                              {(uint64_t*)fill, (uint64_t)v->nbricks0, 0},
                              {(uint64_t*)counters, counters ? 9u : 0u, 0}};
-    launch_fill_multi(segs, 4, vs);
-    if (ev) ev->mark(1);
-    if (ls != vs) (void)hipStreamWaitEvent(vs, v->ev_items, 0);
-    else launch_light_items(v->g, plan.lv, v->d_boards, v->S, plan.ls, vs);
+    launch_fill_multi(segs, 4, ls);
+    if (ev) ev->mark_on(1, ls);
+    launch_light_items(v->g, plan.lv, v->d_boards, v->S, plan.ls, ls);
+    if (ls != vs) {
+        (void)hipEventRecord(v->ev_items, ls);
+        (void)hipStreamWaitEvent(vs, v->ev_items, 0);
+    }
     launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, fill, counters, vs);
-    if (ls != vs) (void)hipEventRecord(v->ev_lfree[v->lset_i], vs);
-    if (ev) ev->mark(2);
+    if (ev) ev->mark_on(2, vs);
     launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, shell, counters, vs);
-    if (ev) ev->mark(3);
+    if (ls != vs) (void)hipEventRecord(v->ev_lfree[v->lset_i], vs);
+    if (ev) ev->mark_on(3, vs);
     if (vs != bs) {
         (void)hipEventRecord(v->ev_splat, vs);
         (void)hipStreamWaitEvent(bs, v->ev_splat, 0);
@@ -785,12 +792,14 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
             for (hipStream_t st : {vs, bs, ls}) CR_HIP(ctx, hipStreamWaitEvent(st, v->ev_sync, 0));
             v->s_sync_needed = false;
         }
-        if (v->bfree_valid[nxt]) CR_HIP(ctx, hipStreamWaitEvent(vs, v->ev_bfree[nxt], 0));
     }
+    // the clear waits for the passes that last read its key pair and the
+    // build that last read its occupancy set
+    const hipEvent_t lwaits[2] = {two && v->lfree_valid[v->lset_i] ? v->ev_lfree[v->lset_i] : nullptr,
+                                  two && v->bfree_valid[nxt] ? v->ev_bfree[nxt] : nullptr};
     StageEvents ev;
     if ((rc = ev.begin(ctx, vs, 0))) return rc;
-    voxelize_enqueue(v, plan, counters, &ev, ls,
-                     two && v->lfree_valid[v->lset_i] ? v->ev_lfree[v->lset_i] : nullptr, vs, bs, nxt,
+    voxelize_enqueue(v, plan, counters, &ev, ls, lwaits, vs, bs, nxt,
                      two && v->release_valid[nxt] ? v->ev_release[nxt] : nullptr);
     ev.end();
     if (two) v->lfree_valid[v->lset_i] = true;
@@ -1543,11 +1552,11 @@ void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector
 }
 
 // the camera buffers a batch graph bakes in (a single trace may reallocate them)
-constexpr int kGraphPtrs = 11;
+constexpr int kGraphPtrs = 12;
 static void cam_ptrs(cr_volume v, const void** out) {
     const void* p[kGraphPtrs] = {v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort,
                                  v->d_order,     v->d_tie,      v->d_tiles,     v->d_masks,
-                                 v->d_lbox,      v->d_lcounts,  v->d_litems};
+                                 v->d_lbox,      v->d_lcounts,  v->d_litems,    v->d_keys};
     for (int k = 0; k < kGraphPtrs; ++k) out[k] = p[k];
 }
 
