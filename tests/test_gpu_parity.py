@@ -157,6 +157,12 @@ def test_pipelined_frames_match_serial(cv, gpu_ctx):
             tr.coneTrace(vol, suns[i], cams[i], img, scene.W, scene.H)
             torch.cuda.synchronize()
             serial.append(img.cpu().numpy())
+        # the volume state after the last serial frame
+        want_state = vol.read_occupancy()
+        want_count = vol.occupancy()
+        want_levels = [vol.read_level(k) for k in range(scene.levels)]
+        want_pm = vol.read_position_map(scene.light_w, scene.light_h)
+        want_order = vol.read_draw_order()
         stream = torch.cuda.current_stream()
         imgs = [torch.zeros(shape, dtype=torch.float32, device="cuda") for _ in range(F)]
         for rep in range(2):
@@ -166,6 +172,29 @@ def test_pipelined_frames_match_serial(cv, gpu_ctx):
             torch.cuda.synchronize()
             for i in range(F):
                 assert_bit_equal(f"pipelined frame {i} (pass {rep})", imgs[i].cpu().numpy(), serial[i])
+            # readbacks see the last queued frame
+            assert_bit_equal(f"occupancy state (pass {rep})", vol.read_occupancy(), want_state)
+            assert vol.occupancy() == want_count
+            for k in range(scene.levels):
+                assert_bit_equal(f"level {k} (pass {rep})", vol.read_level(k), want_levels[k])
+            assert_bit_equal(f"position map (pass {rep})", vol.read_position_map(scene.light_w, scene.light_h),
+                             want_pm)
+            assert np.array_equal(vol.read_draw_order(), want_order)
+        # a counter run (serial, on the caller's stream) between queued frames
+        for img in imgs:
+            img.zero_()
+        for i in range(F):
+            if i == 2:
+                gpu_ctx.set_profiling(1)
+            try:
+                vox.voxelize(vol, suns[i], stream=stream)
+                tr.coneTrace(vol, suns[i], cams[i], imgs[i], scene.W, scene.H, stream=stream)
+            finally:
+                if i == 2:
+                    gpu_ctx.set_profiling(0)
+        torch.cuda.synchronize()
+        for i in range(F):
+            assert_bit_equal(f"frame {i} around a counter run", imgs[i].cpu().numpy(), serial[i])
         assert len({serial[i].tobytes() for i in range(F)}) == F, "the frames differ"
     finally:
         vol.close()
