@@ -1800,12 +1800,20 @@ __device__ __forceinline__ AxisTaps axis_taps(float s, float fn, int nm1) {
     return a;
 }
 // the same for s in [0, 1] (every front-to-back sample): the coordinate
-// clamp is the identity and i lies in [-1, n - 1], so one bound per corner
-__device__ __forceinline__ AxisTaps axis_taps_in(float s, float fn, int nm1) {
-    AxisTaps a;
-    a.u = fmaf(s, fn, -0.5f);
-    a.fl = floorf(a.u);
-    a.i = (int)a.fl;
+// clamp is the identity and i lies in [-1, n - 1], so one bound per corner.
+// w is u - floor(u) from v_fract_f32: exact for u >= 0 (Sterbenz), and for
+// u in [-0.5, 0) both corners clamp to texel 0, where any weight returns
+// that texel exactly; i is floor(u) converted in one instruction.
+struct AxisTapsIn {
+    float w;
+    int i;
+    uint32_t c0, c1;
+};
+__device__ __forceinline__ AxisTapsIn axis_taps_in(float s, float fn, int nm1) {
+    AxisTapsIn a;
+    const float u = fmaf(s, fn, -0.5f);
+    a.w = __builtin_amdgcn_fractf(u);
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(a.i) : "v"(u));  // (int)floor(u), u finite and small
     a.c0 = (uint32_t)max(a.i, 0);
     a.c1 = (uint32_t)min(a.i + 1, nm1);
     return a;
@@ -1847,13 +1855,13 @@ __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const L
     static_assert(FMT == 1 || FMT == 2, "RGBA formats");
     const float fn = (float)L.n;
     const int nm1 = (int)L.n - 1;
-    const AxisTaps ax = axis_taps_in(sx, fn, nm1), ay = axis_taps_in(sy, fn, nm1), az = axis_taps_in(sz, fn, nm1);
+    const AxisTapsIn ax = axis_taps_in(sx, fn, nm1), ay = axis_taps_in(sy, fn, nm1), az = axis_taps_in(sz, fn, nm1);
     // x1 is the slot's neighbour x0+ except below the first texel, where both
     // x corners clamp to texel 0: there the weight 0 returns texel 0 exactly,
     // as lerp(t0, t0, w) does
-    T.ax = ax.i < 0 ? 0.0f : ax.u - ax.fl;
-    T.ay = ay.u - ay.fl;
-    T.az = az.u - az.fl;
+    T.ax = ax.i < 0 ? 0.0f : ax.w;
+    T.ay = ay.w;
+    T.az = az.w;
     const uint32_t srow = L.nb << 6, sslab = (L.nb * L.nb) << 6;  // texels per brick row / slab (< 2^24)
     const uint32_t X0 = morton_off_x(ax.c0);
     const uint32_t Y0 = morton_off_y(ay.c0, srow), Y1 = morton_off_y(ay.c1, srow);
