@@ -24,6 +24,7 @@ struct LevelDesc {
     uint32_t n;      // texels per side
     uint32_t nb;     // bricks per side = ceil(n / 4)
     uint64_t base;   // first texel of the level (in texels)
+    float fn;        // (float)n (the samplers' scale, without a per-sample conversion)
 };
 
 // Voxel grid of the volume (world bounds, index map)

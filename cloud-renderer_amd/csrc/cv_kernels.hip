@@ -1823,7 +1823,7 @@ __device__ __forceinline__ AxisTapsIn axis_taps_in(float s, float fn, int nm1) {
 template <int FMT>
 __device__ __forceinline__ float trilinear_r(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
                                              float sz) {
-    const float fn = (float)L.n;
+    const float fn = L.fn;
     const int nm1 = (int)L.n - 1;
     const AxisTaps ax = axis_taps(sx, fn, nm1), ay = axis_taps(sy, fn, nm1), az = axis_taps(sz, fn, nm1);
     const float wx = ax.u - ax.fl, wy = ay.u - ay.fl, wz = az.u - az.fl;
@@ -1853,7 +1853,7 @@ template <int FMT>
 __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
                                            float sz, GreyTaps& T) {
     static_assert(FMT == 1 || FMT == 2, "RGBA formats");
-    const float fn = (float)L.n;
+    const float fn = L.fn;
     const int nm1 = (int)L.n - 1;
     const AxisTapsIn ax = axis_taps_in(sx, fn, nm1), ay = axis_taps_in(sy, fn, nm1), az = axis_taps_in(sz, fn, nm1);
     // x1 is the slot's neighbour x0+ except below the first texel, where both
@@ -1913,7 +1913,7 @@ __device__ __forceinline__ int step_level(const ConeDesc& cd, int i) {
 template <int FMT, int NCH>
 __device__ __forceinline__ float4 sample_levels(const void* __restrict__ vol, const LevelDesc& LA, const LevelDesc& LB,
                                                 float f, float sx, float sy, float sz, bool* two) {
-    const bool tw = f != 0.0f;
+    const bool tw = (__float_as_uint(f) << 1) != 0u;  // f != 0, as an integer test on the uniform f
     *two = tw;
     if constexpr (NCH == 2) {
         GreyTaps A, B;
@@ -1945,17 +1945,20 @@ __device__ __forceinline__ uint32_t q8(float v) { return (uint32_t)(int)floorf(c
 //   stop before a sample outside [0,1]^3 (returns 1), accumulate
 //   w = (1 - a) * alpha_i;  c += w * s.rgb;  a += w  (Kahan-compensated when
 //   requested), stop once a >= alpha_stop (returns 2); returns 0 after i1.
+// RGBA volumes are grey (r = g = b), so every sample has s.r = s.g = s.b and
+// the three colour sums run the same operations on the same values: one sum
+// c stands for all three, bit for bit.
 struct FtbState {
-    float c0, c1, c2, a;
-    float k0, k1, k2, ka;
+    float c, a;
+    float k, ka;  // compensation terms
 };
 
 // RGBA volumes are grey (r = g = b), so only r and a are filtered --
-// bit-identical to filtering all four channels.
-template <int FMT>
-__device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
-                                         float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
-                                         int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
+// bit-identical to filtering all four channels.  KAHAN: compensated sums.
+template <int FMT, bool KAHAN>
+__device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
+                                           float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
+                                           int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
     if (i0 >= i1) return 0;
     // a step's scalars are loaded one step ahead, its level descriptors only
     // when the level changes (at most once per level over a march)
@@ -1993,22 +1996,25 @@ __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __res
             al = 1.0f - cv_exp2_neg(-(kk * sm.w));
         }
         float wgt = (1.0f - st.a) * al;
-        if (cd.compensated) {
+        if constexpr (KAHAN) {
             float y, t;
-            y = fmaf(wgt, sm.x, -st.k0); t = st.c0 + y; st.k0 = (t - st.c0) - y; st.c0 = t;
-            y = fmaf(wgt, sm.y, -st.k1); t = st.c1 + y; st.k1 = (t - st.c1) - y; st.c1 = t;
-            y = fmaf(wgt, sm.z, -st.k2); t = st.c2 + y; st.k2 = (t - st.c2) - y; st.c2 = t;
+            y = fmaf(wgt, sm.x, -st.k); t = st.c + y; st.k = (t - st.c) - y; st.c = t;
             y = wgt - st.ka; t = st.a + y; st.ka = (t - st.a) - y; st.a = t;
         } else {
-            st.c0 = fmaf(wgt, sm.x, st.c0);
-            st.c1 = fmaf(wgt, sm.y, st.c1);
-            st.c2 = fmaf(wgt, sm.z, st.c2);
+            st.c = fmaf(wgt, sm.x, st.c);
             st.a = st.a + wgt;
         }
         ++nsamp;
         if (st.a >= cd.alpha_stop) return 2;
     }
     return 0;
+}
+template <int FMT>
+__device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
+                                         float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
+                                         int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
+    return cd.compensated ? march_ftb_t<FMT, true>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dz, i0, i1, st, nsamp, ntwo)
+                          : march_ftb_t<FMT, false>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dz, i0, i1, st, nsamp, ntwo);
 }
 
 // Far-field features after k samples: last sample position (2x - 1), cone
@@ -2214,8 +2220,10 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
             }
             if (!__any(have)) break;
             ConeFrag f = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            FtbState st = {0.0f, 0.0f, 0.0f, 0.0f};
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+            float fr = 0.0f, fg = 0.0f, fb = 0.0f;  // colour with the predicted far field
+            bool pred = false;
             bool alive = false;
             if (have) {
                 const float4 cw = sph_world[j];
@@ -2275,16 +2283,21 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                             march_ftb<FMT>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, kk_steps,
                                                  cd.steps, st, nsamp, ntwo);
                         } else {
-                            st.c0 = fmaf(rem, clamp01(y[0]), st.c0);
-                            st.c1 = fmaf(rem, clamp01(y[1]), st.c1);
-                            st.c2 = fmaf(rem, clamp01(y[2]), st.c2);
+                            fr = fmaf(rem, clamp01(y[0]), st.c);
+                            fg = fmaf(rem, clamp01(y[1]), st.c);
+                            fb = fmaf(rem, clamp01(y[2]), st.c);
                             st.a = st.a + rem * clamp01(y[3]);
+                            pred = true;
                         }
                     }
                 }
             }
             if (have) {
-                if (CONE != 0) { s0 = clamp01(st.c0); s1 = clamp01(st.c1); s2 = clamp01(st.c2); s3 = clamp01(st.a); }
+                if (CONE != 0) {
+                    s0 = s1 = s2 = clamp01(st.c);
+                    if (pred) { s0 = clamp01(fr); s1 = clamp01(fg); s2 = clamp01(fb); }
+                    s3 = clamp01(st.a);
+                }
                 blend_frag<COMP>(cd, s0, s1, s2, s3, C0, C1, C2, C3, done);
             }
         }
@@ -2390,15 +2403,15 @@ __global__ void __launch_bounds__(256) k_segments(GridDesc g, VolumeDesc vd, Con
     const float px = p0[3 * i], py = p0[3 * i + 1], pz = p0[3 * i + 2];
     const float dx = cd.ldir[0], dy = cd.ldir[1], dz = cd.ldir[2];
     const int kk_steps = cd.learned_k;
-    FtbState st = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    FtbState st = {0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t ns = 0, nt = 0;
     int rs = march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, 0, min(kk_steps, cd.steps), st, ns, nt);
     const bool rec = rs == 0 && kk_steps < cd.steps;
     float f[kMlpIn];
-    float ck0 = 0.0f, ck1 = 0.0f, ck2 = 0.0f, ak = 0.0f;
+    float ck = 0.0f, ak = 0.0f;
     if (rec) {
         farfield_features<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, (float)g.N, f);
-        ck0 = st.c0; ck1 = st.c1; ck2 = st.c2; ak = st.a;
+        ck = st.c; ak = st.a;
         march_ftb<FMT>(vd, vol, cd, px, py, pz, dx, dy, dz, kk_steps, cd.steps, st, ns, nt);
     }
     valid[i] = rec ? 1 : 0;
@@ -2407,9 +2420,7 @@ __global__ void __launch_bounds__(256) k_segments(GridDesc g, VolumeDesc vd, Con
     float* t = tgt + (size_t)i * kMlpOut;
     if (rec) {
         const float om = 1.0f - ak;
-        t[0] = (st.c0 - ck0) / om;
-        t[1] = (st.c1 - ck1) / om;
-        t[2] = (st.c2 - ck2) / om;
+        t[0] = t[1] = t[2] = (st.c - ck) / om;  // grey
         t[3] = (st.a - ak) / om;
         t[4] = om;
     } else {

@@ -480,6 +480,7 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
         v->vd.lv[k].n = n;
         v->vd.lv[k].nb = nb;
         v->vd.lv[k].base = base;
+        v->vd.lv[k].fn = (float)n;
         base += (uint64_t)nb * nb * nb * 64u;
     }
     v->vd.L = levels;
