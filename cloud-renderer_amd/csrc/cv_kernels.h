@@ -107,12 +107,30 @@ struct ConeDesc {
 
 // pass-1 work list: per-billboard pixel box and tile count, exclusive scan
 // (offsets[S] = total), and (billboard, tile) items
+// One pass-1 work item: a kLightTile^2 tile of a billboard's pixel box, the
+// billboard's light-view data and the brick box of the chord slab behind the
+// tile, prepared once per item by the work-list kernel.
+struct RasterItem {
+    int j;                    // billboard
+    int x0, y0, x1, y1;       // pixel rectangle, inclusive
+    int bx0, by0, bz0;        // slab brick box origin
+    int nbx, nby, nbz;        // and extents
+    int use_lds;              // the slab's words fit the LDS bitmap
+    float cx, cy, cz, s;      // sphere centre (world) and radius
+    float vx, vy;             // centre in light view
+    float inv_tw;             // 1 / (x1 - x0 + 1)
+    float inv_x, inv_xy;      // 1 / nbx, 1 / (nbx * nby) (bitmap flush decode)
+    int pad[3];
+};
+static_assert(sizeof(RasterItem) == 96, "work items are 96 bytes");
+
 struct LightScratch {
     int4* lbox;
     int* counts;
     int* offsets;   // S + 1
-    int2* items;
+    RasterItem* items;
     size_t items_cap;
+    int words;      // LDS bitmap words of the raster kernel
 };
 
 // ---- launch wrappers (cv_kernels.hip) ----
@@ -125,8 +143,8 @@ struct FillSeg {
 void launch_fill_multi(const FillSeg* segs, int nseg, hipStream_t s);  // up to 4 segments, one launch
 size_t light_items_bound(int S, float max_radius, const LightView& lv);
 // the pass-1 work list (boxes, scan, items); launch_light_raster reads it
-void launch_light_items(const GridDesc& g, const LightView& lv, const float4* boards, int S, const LightScratch& ls,
-                        hipStream_t s);
+void launch_light_items(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
+                        const LightScratch& ls, hipStream_t s);
 void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
                          const LightScratch& ls, unsigned long long* keys, unsigned long long* fillbits,
                          unsigned long long* counters, hipStream_t s);

@@ -323,12 +323,73 @@ __global__ void __launch_bounds__(256) k_light_scan(const int* __restrict__ coun
     if (tid == 0) offsets[S] = carry;
 }
 
-__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ offsets, int S, int2* __restrict__ items,
-                                                      int cap) {
+// The brick box of the chord slab behind a tile (chord fill only): the world
+// box of c + du*S + dv*U + t*B, du/dv over the tile's view-plane rectangle
+// (within the disc), |t| <= s.  Reciprocal products instead of divisions:
+// the box keeps a 1-voxel margin, and the raster stores samples outside it
+// directly, so it only has to be close.
+__device__ void slab_box(const GridDesc& g, const LightView& lv, const LightSphere& o, int words, RasterItem& it) {
+    const float kW = 2.0f / (float)lv.W, kH = 2.0f / (float)lv.H;
+    float du0 = fmaxf(((((float)it.x0 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, -o.s);
+    float du1 = fminf(((((float)it.x1 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, o.s);
+    float dv0 = fmaxf(((((float)it.y0 + 0.5f) * kH - 1.0f) * lv.lsy + lv.loy) - o.vy, -o.s);
+    float dv1 = fminf(((((float)it.y1 + 0.5f) * kH - 1.0f) * lv.lsy + lv.loy) - o.vy, o.s);
+    const float c3[3] = {o.cx, o.cy, o.cz};
+    int b0[3], b1[3];
+    for (int a = 0; a < 3; ++a) {
+        const float kvox = (float)g.N * g.inv_range[a];
+        float su0 = du0 * lv.S[a], su1 = du1 * lv.S[a], sv0 = dv0 * lv.U[a], sv1 = dv1 * lv.U[a];
+        float tb = o.s * fabsf(lv.B[a]);
+        float wlo = c3[a] + fminf(su0, su1) + fminf(sv0, sv1) - tb;
+        float whi = c3[a] + fmaxf(su0, su1) + fmaxf(sv0, sv1) + tb;
+        float l0 = (wlo - g.lo[a]) * kvox - 1.0f;
+        float l1 = (whi - g.lo[a]) * kvox + 1.0f;
+        float f0 = fmaxf(floorf(l0 * 0.25f), 0.0f), f1 = fminf(floorf(l1 * 0.25f), (float)(g.nb0 - 1));
+        b0[a] = (int)f0;
+        b1[a] = f1 >= f0 ? (int)f1 : (int)f0 - 1;
+    }
+    it.bx0 = b0[0]; it.by0 = b0[1]; it.bz0 = b0[2];
+    it.nbx = b1[0] - b0[0] + 1; it.nby = b1[1] - b0[1] + 1; it.nbz = b1[2] - b0[2] + 1;
+    it.use_lds = du0 <= du1 && dv0 <= dv1 && it.nbx > 0 && it.nby > 0 && it.nbz > 0 &&
+                 it.nbx * it.nby * it.nbz <= words;
+    it.inv_x = it.use_lds ? 1.0f / (float)it.nbx : 0.0f;
+    it.inv_xy = it.use_lds ? 1.0f / (float)(it.nbx * it.nby) : 0.0f;
+}
+
+// one thread per billboard enumerates its items (tiles in row order):
+// (billboard, tile) into the first two words of each
+__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ offsets, int S,
+                                                      RasterItem* __restrict__ items, int cap) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= S) return;
-    const int o = offsets[j], c = offsets[j + 1] - o;
-    for (int t = 0; t < c && o + t < cap; ++t) items[o + t] = make_int2(j, t);
+    const int o0 = offsets[j], c = offsets[j + 1] - o0;
+    for (int t = 0; t < c && o0 + t < cap; ++t) *(int2*)&items[o0 + t] = make_int2(j, t);
+}
+
+// one thread per item: the tile's rectangle, the billboard's light-view data
+// and the slab box
+__global__ void __launch_bounds__(256) k_light_prep(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                    const int4* __restrict__ lbox, const int* __restrict__ total_ptr,
+                                                    RasterItem* __restrict__ items, int cap, int chord, int words) {
+    const int n = min(*total_ptr, cap);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int2 jt = *(const int2*)&items[i];
+    const int j = jt.x, t = jt.y;
+    const LightSphere o = light_sphere(g, lv, boards[j]);
+    const int4 box = lbox[j];
+    const int ntx = (box.z - box.x) / kLightTile + 1;
+    RasterItem it{};
+    it.j = j;
+    it.x0 = box.x + (t % ntx) * kLightTile;
+    it.y0 = box.y + (t / ntx) * kLightTile;
+    it.x1 = min(it.x0 + kLightTile - 1, box.z);
+    it.y1 = min(it.y0 + kLightTile - 1, box.w);
+    it.cx = o.cx; it.cy = o.cy; it.cz = o.cz; it.s = o.s;
+    it.vx = o.vx; it.vy = o.vy;
+    it.inv_tw = 1.0f / (float)(it.x1 - it.x0 + 1);
+    if (chord) slab_box(g, lv, o, words, it);
+    items[i] = it;
+    }
 }
 
 constexpr int kTilePixels = kLightTile * kLightTile;
@@ -338,9 +399,8 @@ constexpr int kTilePixels = kLightTile * kLightTile;
 // LDS chord bitmap (bigger volumes have bigger slabs behind a tile)
 template <bool P2, int WORDS>
 __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, int chord,
-                                                      const float4* __restrict__ boards,
-                                                      const int4* __restrict__ lbox, const int* __restrict__ total_ptr,
-                                                      const int2* __restrict__ items, int cap,
+                                                      const int* __restrict__ total_ptr,
+                                                      const RasterItem* __restrict__ items, int cap,
                                                       unsigned long long* __restrict__ keys,
                                                       unsigned long long* __restrict__ fillbits,
                                                       unsigned long long* __restrict__ counters) {
@@ -363,54 +423,33 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     const int total = min(*total_ptr, cap);
     const bool flat = P2 && chord;
     unsigned long long nfrag = 0, nchord = 0;
-    // the item's (billboard, tile) and the billboard's data; the next item's
-    // are fetched while this one is rasterized (its work-list entry first,
-    // its billboard after the pixel pass), hiding the dependent loads.  The
-    // last item fetches itself again.
+    // The item's setup comes from the work list.  The next item is fetched
+    // while this one is rasterized: lanes 0..23 of every wave load one word
+    // each (vector loads, so the LDS waits in between do not wait for them)
+    // and the fields are read off those lanes at the next item; the last
+    // item fetches itself again.
     if ((int)blockIdx.x >= total) return;
-    // NDC per pixel and voxels per world unit, for the conservative slab boxes
-    const float kW = 2.0f / (float)lv.W, kH = 2.0f / (float)lv.H;
-    const float kvox[3] = {(float)g.N * g.inv_range[0], (float)g.N * g.inv_range[1], (float)g.N * g.inv_range[2]};
+    constexpr int kItemWords = (int)(sizeof(RasterItem) / 4);
+    const int lane = tid & 63;
     int item = blockIdx.x;
-    int2 it = items[item];
-    float4 bd = boards[it.x];
-    int4 box = lbox[it.x];
+    int iw = lane < kItemWords ? ((const int*)(items + item))[lane] : 0;
     for (; item < total; item += gridDim.x) {
-    const int2 nit = items[min(item + (int)gridDim.x, total - 1)];
-    const int j = it.x;
-    LightSphere o = light_sphere(g, lv, bd);
-    const int ntx = (box.z - box.x) / kLightTile + 1;
-    const int x0 = box.x + (it.y % ntx) * kLightTile, y0 = box.y + (it.y / ntx) * kLightTile;
-    const int x1 = min(x0 + kLightTile - 1, box.z), y1 = min(y0 + kLightTile - 1, box.w);
-
+    const int iwn = lane < kItemWords ? ((const int*)(items + min(item + (int)gridDim.x, total - 1)))[lane] : 0;
+    auto field_i = [&](int k) { return __builtin_amdgcn_readlane(iw, k); };
+    auto field_f = [&](int k) { return __int_as_float(__builtin_amdgcn_readlane(iw, k)); };
+    const int j = field_i(0);
+    const int x0 = field_i(1), y0 = field_i(2), x1 = field_i(3), y1 = field_i(4);
+    LightSphere o;
+    o.cx = field_f(12); o.cy = field_f(13); o.cz = field_f(14); o.s = field_f(15);
+    o.vx = field_f(16); o.vy = field_f(17);
+    o.vz = 0.0f;
+    o.live = true;
     int bx0 = 0, by0 = 0, bz0 = 0, nbx = 0, nby = 0, nbz = 0;
     bool use_lds = false;
     if (chord) {
-        // world box of the chords behind the tile: c + du*S + dv*U + t*B,
-        // du/dv over the tile's view-plane rectangle (within the disc), |t| <= s.
-        // Reciprocal products instead of divisions: the box keeps a 1-voxel
-        // margin, and samples outside it are stored directly, so it only has
-        // to be close.
-        float du0 = fmaxf(((((float)x0 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, -o.s);
-        float du1 = fminf(((((float)x1 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, o.s);
-        float dv0 = fmaxf(((((float)y0 + 0.5f) * kH - 1.0f) * lv.lsy + lv.loy) - o.vy, -o.s);
-        float dv1 = fminf(((((float)y1 + 0.5f) * kH - 1.0f) * lv.lsy + lv.loy) - o.vy, o.s);
-        const float c3[3] = {o.cx, o.cy, o.cz};
-        int b0[3], b1[3];
-        for (int a = 0; a < 3; ++a) {
-            float su0 = du0 * lv.S[a], su1 = du1 * lv.S[a], sv0 = dv0 * lv.U[a], sv1 = dv1 * lv.U[a];
-            float tb = o.s * fabsf(lv.B[a]);
-            float wlo = c3[a] + fminf(su0, su1) + fminf(sv0, sv1) - tb;
-            float whi = c3[a] + fmaxf(su0, su1) + fmaxf(sv0, sv1) + tb;
-            float l0 = (wlo - g.lo[a]) * kvox[a] - 1.0f;
-            float l1 = (whi - g.lo[a]) * kvox[a] + 1.0f;
-            float f0 = fmaxf(floorf(l0 * 0.25f), 0.0f), f1 = fminf(floorf(l1 * 0.25f), (float)(g.nb0 - 1));
-            b0[a] = (int)f0;
-            b1[a] = f1 >= f0 ? (int)f1 : (int)f0 - 1;
-        }
-        bx0 = b0[0]; by0 = b0[1]; bz0 = b0[2];
-        nbx = b1[0] - b0[0] + 1; nby = b1[1] - b0[1] + 1; nbz = b1[2] - b0[2] + 1;
-        use_lds = du0 <= du1 && dv0 <= dv1 && nbx > 0 && nby > 0 && nbz > 0 && nbx * nby * nbz <= WORDS;
+        bx0 = field_i(5); by0 = field_i(6); bz0 = field_i(7);
+        nbx = field_i(8); nby = field_i(9); nbz = field_i(10);
+        use_lds = field_i(11) != 0;
         if (use_lds)
             for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) lbits[w] = 0ull;
         if (tid == 0) nfr_s = 0;
@@ -432,7 +471,7 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     };
     // pass over the tile's pixels: depth keys; chords queued (flat) or walked
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
-    const float inv_tw = 1.0f / (float)tw;
+    const float inv_tw = field_f(18);
     for (int p = tid; p < tw * th; p += blockDim.x) {
         // p / tw as a float product: (p + 0.5) / tw lies at least 1 / (2 tw)
         // from an integer, far above the product's error for p < 1024
@@ -486,8 +525,6 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
             atomicMin(&keys[(size_t)py * lv.W + px], key);
         }
     }
-    const float4 nbd = boards[nit.x];
-    const int4 nbox = lbox[nit.x];
     if (flat) {
         __syncthreads();
         // every lane walks one queued chord: no lane idles on a pixel
@@ -641,7 +678,7 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         if (use_lds) {
             // w / nbx and w / (nbx nby) as float products: exact for w < 4096
             // (the error stays below the 1 / (2 n) margin of the floor)
-            const float inv_x = 1.0f / (float)nbx, inv_xy = 1.0f / (float)(nbx * nby);
+            const float inv_x = field_f(19), inv_xy = field_f(20);
             for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) {
                 unsigned long long bits = lbits[w];
                 if (!bits) continue;
@@ -654,9 +691,7 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         }
     }
     __syncthreads();  // the next item reuses the bitmap, the queue and the pixel centres
-    it = nit;
-    bd = nbd;
-    box = nbox;
+    iw = iwn;
     }  // items
     if (counters) {
         nfrag = wave_sum_u64(nfrag);
@@ -2540,14 +2575,23 @@ void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, s, p, n, value);
 }
 
-void launch_light_items(const GridDesc& g, const LightView& lv, const float4* boards, int S, const LightScratch& ls,
-                        hipStream_t s) {
+// LDS bitmap words of the raster kernel for this grid
+static int raster_words(const GridDesc& g) {
+    const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
+    return p2 && g.nb0 <= 64 ? 2048 : 4096;
+}
+
+void launch_light_items(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
+                        const LightScratch& ls, hipStream_t s) {
     if (S <= 0) return;
     const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
     const dim3 bgrid((unsigned)((S + 255) / 256));
     hipLaunchKernelGGL(k_light_boxes, bgrid, dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts);
     hipLaunchKernelGGL(k_light_scan, dim3(1), dim3(256), 0, s, ls.counts, S, ls.offsets);
     hipLaunchKernelGGL(k_light_expand, bgrid, dim3(256), 0, s, ls.offsets, S, ls.items, cap);
+    const dim3 igrid((unsigned)std::min<size_t>((ls.items_cap + 255) / 256, 2048));
+    hipLaunchKernelGGL(k_light_prep, igrid, dim3(256), 0, s, g, lv, boards, ls.lbox, ls.offsets + S, ls.items, cap,
+                       mode == 1 ? 1 : 0, raster_words(g));
 }
 
 void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
@@ -2558,15 +2602,15 @@ void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const
     unsigned grid = (unsigned)std::min<size_t>(ls.items_cap, 4096);
     const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
     const int chord = mode == 1 ? 1 : 0;
-    if (p2 && g.nb0 <= 64)
-        hipLaunchKernelGGL((k_light_raster<true, 2048>), dim3(grid), dim3(256), 0, s, g, lv, chord, boards, ls.lbox,
-                           ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+    if (p2 && raster_words(g) == 2048)
+        hipLaunchKernelGGL((k_light_raster<true, 2048>), dim3(grid), dim3(256), 0, s, g, lv, chord, ls.offsets + S,
+                           ls.items, cap, keys, fillbits, counters);
     else if (p2)
-        hipLaunchKernelGGL((k_light_raster<true, 4096>), dim3(grid), dim3(256), 0, s, g, lv, chord, boards, ls.lbox,
-                           ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+        hipLaunchKernelGGL((k_light_raster<true, 4096>), dim3(grid), dim3(256), 0, s, g, lv, chord, ls.offsets + S,
+                           ls.items, cap, keys, fillbits, counters);
     else
-        hipLaunchKernelGGL((k_light_raster<false, 4096>), dim3(grid), dim3(256), 0, s, g, lv, chord, boards, ls.lbox,
-                           ls.offsets + S, ls.items, cap, keys, fillbits, counters);
+        hipLaunchKernelGGL((k_light_raster<false, 4096>), dim3(grid), dim3(256), 0, s, g, lv, chord, ls.offsets + S,
+                           ls.items, cap, keys, fillbits, counters);
 }
 
 size_t light_items_bound(int S, float max_radius, const LightView& lv) {

@@ -41,7 +41,7 @@ struct cr_context_s {
 struct LightBufs {
     int4* lbox = nullptr;
     int* lcounts = nullptr;
-    int2* litems = nullptr;
+    RasterItem* litems = nullptr;
     unsigned long long* keys = nullptr;
     size_t caps[4] = {0, 0, 0, 0};  // lbox, lcounts, litems, keys
 };
@@ -99,7 +99,7 @@ struct cr_volume_s {
     size_t light_cap = 0;
     int4* d_lbox = nullptr;
     int* d_lcounts = nullptr;  // counts (S) followed by offsets (S + 1)
-    int2* d_litems = nullptr;
+    RasterItem* d_litems = nullptr;
     size_t lbox_cap = 0, lcounts_cap = 0, litems_cap = 0;
     // consecutive overlapped voxelizations alternate between the work-list
     // and depth-key buffers above and light_spare (lset_i: which pair is
@@ -709,7 +709,7 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
                              {(uint64_t*)counters, counters ? 9u : 0u, 0}};
     launch_fill_multi(segs, 4, ls);
     if (ev) ev->mark_on(1, ls);
-    launch_light_items(v->g, plan.lv, v->d_boards, v->S, plan.ls, ls);
+    launch_light_items(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, ls);
     if (ls != vs) {
         (void)hipEventRecord(v->ev_items, ls);
         (void)hipStreamWaitEvent(vs, v->ev_items, 0);
