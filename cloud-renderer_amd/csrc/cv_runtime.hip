@@ -288,7 +288,14 @@ int level_dim(int N, int k) { return std::max(1, N >> k); }
 template <class T>
 int ensure(cr_context ctx, T** p, size_t need, size_t* cap) {
     if (need <= *cap && *p) return CR_OK;
-    if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
+    if (*p) {
+        // work queued on the volume's streams may still read the old buffer
+        // (growth is rare: wait for the device before freeing it)
+        (void)hipDeviceSynchronize();
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
     size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
     hipError_t e = hipMalloc((void**)p, bytes);
     if (e == hipErrorOutOfMemory) { *p = nullptr; return fail(ctx, CR_ENOMEM, "hipMalloc: out of device memory"); }
