@@ -198,3 +198,37 @@ def test_pipelined_frames_match_serial(cv, gpu_ctx):
         assert len({serial[i].tobytes() for i in range(F)}) == F, "the frames differ"
     finally:
         vol.close()
+
+
+def test_pipelined_frames_with_growing_buffers(cv, gpu_ctx):
+    """Queued frames whose light map and image sizes grow from frame to frame
+    (the depth keys, the work list and the camera buffers are reallocated
+    while earlier frames may still be queued) give the images of frames run
+    one at a time."""
+    import torch
+
+    scene = cv.scenes.small32(4)
+    sizes = [(48, 40, 64, 48), (96, 80, 128, 96), (64, 64, 200, 150), (160, 120, 256, 200)]
+    log(f"[pipeline] small32 seed 4, {len(sizes)} frames with growing light maps and images")
+
+    def run(queued):
+        vol, sun, cam, _, tr = cv.scene_objects(scene, 0, context=gpu_ctx)
+        try:
+            out = []
+            for (lw, lh, w, h) in sizes:
+                vox = cv.VoxelizeShader(lw, lh)
+                img = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+                vox.voxelize(vol, sun)
+                tr.coneTrace(vol, sun, cam, img, w, h)
+                if not queued:
+                    torch.cuda.synchronize()
+                out.append(img)
+            torch.cuda.synchronize()
+            return [o.cpu().numpy() for o in out]
+        finally:
+            vol.close()
+
+    want = run(False)
+    got = run(True)
+    for i in range(len(sizes)):
+        assert_bit_equal(f"frame {i} ({sizes[i]})", got[i], want[i])
