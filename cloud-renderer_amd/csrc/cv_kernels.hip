@@ -823,6 +823,17 @@ __device__ __forceinline__ void cell_corners(uint32_t seed, int ix, int iy, int 
 #pragma unroll
     for (int k = 0; k < 8; ++k) c[k] = lattice_value(seed, ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2));
 }
+// cell_corners for a cell every active lane of the wave shares, the hash on
+// scalar registers (each value passes through readfirstlane, which keeps the
+// compiler from merging this hash with the per-lane one)
+__device__ __forceinline__ void cell_corners_wave(uint32_t seed, int ix, int iy, int iz, float c[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t h = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)lattice_hash(seed, ix + (k & 1), iy + ((k >> 1) & 1), iz + (k >> 2)));
+        c[k] = (float)(h >> 8) * (1.0f / 16777216.0f);
+    }
+}
 // smoothstep-weighted trilinear blend of the corners: x, then y, then z
 __device__ __forceinline__ float cell_blend(const float c[8], float wx, float wy, float wz) {
     float x00 = lerpf(c[0], c[1], wx), x10 = lerpf(c[2], c[3], wx);
@@ -855,12 +866,15 @@ __device__ float fbm_voxel(const VolumeDesc& vd, int N, int x, int y, int z) {
 }
 
 // fbm_voxel of the 2x2x2 block at even (x0, y0, z0), voxel v at
-// (x0 + (v & 1), y0 + ((v >> 1) & 1), z0 + (v >> 2)), for the voxels whose bit
-// is set in `need`.  Per octave the corners of the first voxel's cell are
-// hashed once and reused by every voxel of the block in that cell (all of
-// them unless a cell boundary cuts the block); the per-voxel arithmetic is
-// that of fbm_voxel, octave by octave, so the results are identical.
-__device__ void fbm_block8(const VolumeDesc& vd, int N, int x0, int y0, int z0, uint32_t need, float out[8]) {
+// (x0 + (v & 1), y0 + ((v >> 1) & 1), z0 + (v >> 2)); every voxel's value is
+// computed (a caller ignores those it does not need).  The per-voxel arithmetic
+// is that of fbm_voxel, octave by octave, so the results are identical.  When
+// the block lies in one lattice cell (always, for cells of two voxels or more
+// aligned to even coordinates) the corners are hashed once, the smoothstep
+// weights once per axis coordinate and the x and y lerps once per distinct
+// (x) and (x, y) pair; when every active lane of the wave is in the same cell
+// the hash inputs are wave-uniform and the hash runs on scalar registers.
+__device__ void fbm_block8(const VolumeDesc& vd, int N, int x0, int y0, int z0, float out[8]) {
     const float fN = (float)N;
     float u[2], w[2], t[2];
 #pragma unroll
@@ -883,22 +897,43 @@ __device__ void fbm_block8(const VolumeDesc& vd, int N, int x0, int y0, int z0, 
             fx[d] = floorf(px[d]); fy[d] = floorf(py[d]); fz[d] = floorf(pz[d]);
             ix[d] = (int)fx[d]; iy[d] = (int)fy[d]; iz[d] = (int)fz[d];
         }
-        float c[8];
-        cell_corners(seed, ix[0], iy[0], iz[0], c);
+        float wx[2], wy[2], wz[2];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            if (!((need >> v) & 1u)) continue;
-            const int dx = v & 1, dy = (v >> 1) & 1, dz = v >> 2;
-            const float wx = smoothw(px[dx] - fx[dx]), wy = smoothw(py[dy] - fy[dy]), wz = smoothw(pz[dz] - fz[dz]);
-            float val;
-            if (ix[dx] == ix[0] && iy[dy] == iy[0] && iz[dz] == iz[0]) {
-                val = cell_blend(c, wx, wy, wz);
-            } else {
+        for (int d = 0; d < 2; ++d) {
+            wx[d] = smoothw(px[d] - fx[d]); wy[d] = smoothw(py[d] - fy[d]); wz[d] = smoothw(pz[d] - fz[d]);
+        }
+        if (ix[1] == ix[0] && iy[1] == iy[0] && iz[1] == iz[0]) {
+            float c[8];
+            const int sx = __builtin_amdgcn_readfirstlane(ix[0]);
+            const int sy = __builtin_amdgcn_readfirstlane(iy[0]);
+            const int sz = __builtin_amdgcn_readfirstlane(iz[0]);
+            if (__builtin_amdgcn_ballot_w64(ix[0] != sx || iy[0] != sy || iz[0] != sz) == 0ull)
+                cell_corners_wave(seed, sx, sy, sz, c);
+            else
+                cell_corners(seed, ix[0], iy[0], iz[0], c);
+            // cell_blend's x lerps per x coordinate, y lerps per (x, y)
+            float x00[2], x10[2], x01[2], x11[2], y0[4], y1[4];
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                x00[dx] = lerpf(c[0], c[1], wx[dx]); x10[dx] = lerpf(c[2], c[3], wx[dx]);
+                x01[dx] = lerpf(c[4], c[5], wx[dx]); x11[dx] = lerpf(c[6], c[7], wx[dx]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int dx = k & 1, dy = k >> 1;
+                y0[k] = lerpf(x00[dx], x10[dx], wy[dy]);
+                y1[k] = lerpf(x01[dx], x11[dx], wy[dy]);
+            }
+#pragma unroll
+            for (int v = 0; v < 8; ++v) sum[v] = sum[v] + amp * lerpf(y0[v & 3], y1[v & 3], wz[v >> 2]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int dx = v & 1, dy = (v >> 1) & 1, dz = v >> 2;
                 float e[8];
                 cell_corners(seed, ix[dx], iy[dy], iz[dz], e);
-                val = cell_blend(e, wx, wy, wz);
+                sum[v] = sum[v] + amp * cell_blend(e, wx[dx], wy[dy], wz[dz]);
             }
-            sum[v] = sum[v] + amp * val;
         }
         norm = norm + amp;
         freq = freq * vd.noise_lacunarity;
@@ -1092,7 +1127,7 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
 #pragma unroll
         for (int c = 0; c < 8; ++c) nz[c] = 1.0f;
         if (noisy && occ)
-            fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), occ, nz);
+            fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), nz);
         float2 acc = make_float2(0.0f, 0.0f);
 #pragma unroll
         for (uint32_t c = 0; c < 8; ++c) {

@@ -35,6 +35,10 @@ def _scenes(cv):
         "small32_fp16_noise": replace(base, fmt=2, noise=sc.Noise(seed=7), cone=replace(base.cone, compensated=1)),
         # cells of 32/3, 32/7.5, 32/18.75 voxels: boundaries cut the 2x2x2 blocks
         "small32_fp32_noise_odd": replace(base, fmt=1, noise=sc.Noise(octaves=3, seed=9, base_freq=3.0, lacunarity=2.5)),
+        # cells of 32 and 16 voxels: every lane of a wave in one cell (lattice
+        # hash on scalar registers), then cells of 8 (one cell per lane)
+        "small32_fp16_noise_wide": replace(base, fmt=2, noise=sc.Noise(octaves=3, seed=12, base_freq=1.0),
+                                           cone=replace(base.cone, compensated=1)),
         "small32_comp_blend": replace(base, cone=replace(base.cone, composite=0)),
         "small32_rect": replace(base, W=200, H=120, light_w=130, light_h=210),
         "npot24": replace(base, N=24, levels=5),
@@ -68,7 +72,7 @@ def tie_scene(sc):
 
 
 CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_rgba32f", "small32_ref_float_blend",
-         "small32_fp16_noise", "small32_fp32_noise_odd", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
+         "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
 
 
 @pytest.mark.parametrize("case", CASES)
