@@ -28,7 +28,7 @@ from . import _abi
 from ._abi import (CR_COMPOSITE_FTB, CR_COMPOSITE_GL_BLEND, CR_CONE_FTB, CR_CONE_FTB_DENSITY,  # noqa: F401
                    CR_CONE_WEIGHTED, CR_FORMAT_R8, CR_FORMAT_RGBA16F, CR_FORMAT_RGBA32F, CR_MODE_REF,
                    CR_MODE_RGBA, CR_PIXEL_RGBA8, CR_PIXEL_RGBA32F, CloudVoxError, cr_camera, cr_cone_params,
-                   cr_light, cr_stats)
+                   cr_light, cr_noise_march, cr_stats)
 
 __all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "FarFieldMLP", "Batch",
            "CloudVoxError", "lib", "scenes"]
@@ -164,6 +164,18 @@ class CloudVolume:
         _check(lib().cr_volume_set_noise(self.handle, int(octaves), int(seed) & 0xffffffff, float(base_freq),
                                          float(lacunarity), float(gain)), self.ctx.handle)
 
+    def set_noise_march(self, nm: Optional[cr_noise_march]) -> None:
+        """Switch the soft-particle noise march of coneTrace on (nm.enabled) or off (None)."""
+        _check(lib().cr_volume_set_noise_march(self.handle, C.byref(nm) if nm is not None else None),
+               self.ctx.handle)
+
+    def read_noise_map(self) -> np.ndarray:
+        """The noise map bytes, (32, 32, 32, 4) int8 indexed [z, y, x, (r, g, b, a)]."""
+        out = np.empty((32, 32, 32, 4), np.int8)
+        _check(lib().cr_volume_read_noise_map(self.handle, out.ctypes.data_as(C.c_void_p), out.nbytes),
+               self.ctx.handle)
+        return out
+
     # -- readback ----------------------------------------------------------
     def occupancy(self, stream=None):
         cnt, h = C.c_uint64(), C.c_uint64()
@@ -274,6 +286,23 @@ class ConeTraceShader:
         self.quantizeRGBA8 = True
         self.compensated = False
         self.clearColor = (0.2, 0.3, 0.5, 1.0)   # src/main.cpp:112
+        # noise march, src/Shaders/ConeTraceShader.hpp:14-25 (the reference
+        # starts with doNoiseSample = true; here it starts off, the cone
+        # output alone being the default look)
+        self.doNoiseSample = False
+        self.noiseSeed = 0            # replaces rand() in initNoiseMap
+        self.stepSize = 0.01
+        self.noiseOpacity = 4.0
+        self.numOctaves = 4
+        self.freqStep = 3.0
+        self.persStep = 0.5
+        self.adjustSize = 40.0
+        self.minNoiseSteps = 2
+        self.maxNoiseSteps = 8
+        self.minNoiseColor = 0.2
+        self.noiseColorScale = 0.45
+        self.windVel = (0.01, 0.0, 0.0)
+        self.runTime = 0.0            # Window::runTime: octaveOffsets = windVel * runTime
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -285,6 +314,15 @@ class ConeTraceShader:
                               float(self.alphaStop), float(self.sigma), int(self.composite),
                               1 if self.quantizeRGBA8 else 0, 1 if self.compensated else 0,
                               _abi.F4(*[float(c) for c in self.clearColor]))
+
+    def noise_params(self) -> Optional[cr_noise_march]:
+        if not self.doNoiseSample:
+            return None
+        off = (C.c_float * 3)(*[float(np.float32(w) * np.float64(self.runTime)) for w in self.windVel])
+        return cr_noise_march(1, int(self.noiseSeed) & 0xffffffff, float(self.stepSize), float(self.noiseOpacity),
+                              int(self.numOctaves), float(self.freqStep), float(self.persStep),
+                              float(self.adjustSize), int(self.minNoiseSteps), int(self.maxNoiseSteps),
+                              float(self.minNoiseColor), float(self.noiseColorScale), off)
 
     @staticmethod
     def _image_args(image, width, pixel_format, pitch, stream):
@@ -308,6 +346,7 @@ class ConeTraceShader:
         light = sun.to_c()
         cam = camera.to_c()
         p = self.params()
+        volume.set_noise_march(self.noise_params())
         _check(lib().cr_cone_trace(volume.handle, C.byref(light), C.byref(cam), C.byref(p), C.c_void_p(ptr),
                                    int(width), int(height), C.c_size_t(pitch), int(pixel_format),
                                    _stream_handle(stream)), volume.ctx.handle)
@@ -322,6 +361,7 @@ class ConeTraceShader:
         light = sun.to_c()
         cam = camera.to_c()
         p = self.params()
+        volume.set_noise_march(self.noise_params())
         _check(lib().cr_cone_trace_learned(volume.handle, C.byref(light), C.byref(cam), C.byref(p), mlp.handle,
                                            int(k), float(err_bound), C.c_void_p(ptr), int(width), int(height),
                                            C.c_size_t(pitch), int(pixel_format), _stream_handle(stream)),
@@ -530,4 +570,13 @@ def scene_objects(scene, camera_index: int = 0, context: Optional[Context] = Non
                          alphaStop=k.alpha_stop, sigma=k.sigma, composite=k.composite,
                          quantizeRGBA8=bool(k.quantize_rgba8), compensated=bool(k.compensated),
                          clearColor=k.clear_color)
+    if scene.noise_march is not None:
+        m = scene.noise_march
+        tr.doNoiseSample = True
+        tr.noiseSeed, tr.stepSize, tr.noiseOpacity, tr.numOctaves = m.seed, m.step_size, m.opacity, m.octaves
+        tr.freqStep, tr.persStep, tr.adjustSize = m.freq_step, m.pers_step, m.adjust_size
+        tr.minNoiseSteps, tr.maxNoiseSteps, tr.minNoiseColor, tr.noiseColorScale = (m.min_steps, m.max_steps,
+                                                                                    m.min_color, m.color_scale)
+        # offsets given directly: windVel = offsets, runTime = 1
+        tr.windVel, tr.runTime = tuple(m.offsets), 1.0
     return vol, sun, cam, vox, tr

@@ -55,6 +55,13 @@ class cr_cone_params(C.Structure):
                 ("quantize_rgba8", C.c_int), ("compensated", C.c_int), ("clear_color", F4)]
 
 
+class cr_noise_march(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("seed", C.c_uint32), ("step_size", C.c_float), ("opacity", C.c_float),
+                ("octaves", C.c_int), ("freq_step", C.c_float), ("pers_step", C.c_float),
+                ("adjust_size", C.c_float), ("min_steps", C.c_int), ("max_steps", C.c_int),
+                ("min_color", C.c_float), ("color_scale", C.c_float), ("offsets", C.c_float * 3)]
+
+
 class cr_stats(C.Structure):
     _fields_ = [("light_fragments", C.c_int64), ("shell_texels", C.c_int64), ("fragments", C.c_int64),
                 ("samples", C.c_int64), ("two_level", C.c_int64),
@@ -92,6 +99,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "cr_volume_destroy": (C.c_int, [C.c_void_p]),
     "cr_volume_set_noise": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "cr_volume_set_noise_march": (C.c_int, [C.c_void_p, C.POINTER(cr_noise_march)]),
+    "cr_volume_read_noise_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "cr_volume_set_billboards": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                            C.c_float]),
     "cr_voxelize": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.c_int, C.c_int, C.c_void_p]),

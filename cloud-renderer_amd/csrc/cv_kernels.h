@@ -103,6 +103,12 @@ struct ConeDesc {
     float feat_h[4];       // feature sampling heights (voxels) for LODs 3..6
     int feat_lvl[4];       // their levels, clamped to L - 1
     float sun[3];          // sun direction feature
+    // soft-particle noise march (NM kernels): the ConeTraceShader noise
+    // parameters, the octaveOffsets uniform and normalize(V's third row)
+    int nm_octaves, nm_min_steps, nm_max_steps;
+    float nm_step, nm_opacity, nm_freq_step, nm_pers_step, nm_adjust, nm_min_color, nm_color_scale;
+    float nm_off[3];
+    float nm_ray[3];
 };
 
 // pass-1 work list: per-billboard pixel box and tile count, exclusive scan
@@ -216,6 +222,7 @@ struct ConeLaunch {
     const int* tile_order;
     MlpDev mlp;
     int learned;
+    const float4* nmap;  // noise-march map (32^3 decoded RGBA8_SNORM texels) or null: no noise march
 };
 void launch_cone(const ConeLaunch& a, hipStream_t s);
 void launch_segments(const GridDesc& g, const VolumeDesc& vd, const ConeDesc& cd, const void* levels,

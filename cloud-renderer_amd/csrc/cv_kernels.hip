@@ -2180,6 +2180,104 @@ __device__ __forceinline__ void blend_frag(const ConeDesc& cd, float s0, float s
     }
 }
 
+// ---------------------------------------------------------------------------
+// Soft-particle noise march (res/conetrace_frag.glsl:103-120,136-174)
+// ---------------------------------------------------------------------------
+// floor(u) mod 32 as fl - 32 * floor(fl / 32): exact for every float fl (the
+// scalings by 32 are exact and the difference is exact); NaN gives texel 0.
+__device__ __forceinline__ int nm_wrap(float fl) {
+    const float r = fl - 32.0f * floorf(fl * 0.03125f);
+    return (r == r ? (int)r : 0) & 31;
+}
+// texture(noiseMap, s): GL_LINEAR, GL_REPEAT on the 32^3 map of decoded
+// RGBA8_SNORM texels (x fastest); u = fma(s, 32, -1/2), alpha = u - floor(u),
+// lerps along x, then y, then z
+__device__ __forceinline__ float4 nm_tex(const float4* __restrict__ map, float sx, float sy, float sz) {
+    const float ux = fmaf(sx, 32.0f, -0.5f), uy = fmaf(sy, 32.0f, -0.5f), uz = fmaf(sz, 32.0f, -0.5f);
+    const float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+    const float ax = ux - fx, ay = uy - fy, az = uz - fz;
+    const int x0 = nm_wrap(fx), y0 = nm_wrap(fy), z0 = nm_wrap(fz);
+    const int x1 = (x0 + 1) & 31, y1 = (y0 + 1) & 31, z1 = (z0 + 1) & 31;
+    const int r00 = (z0 * 32 + y0) * 32, r10 = (z0 * 32 + y1) * 32, r01 = (z1 * 32 + y0) * 32, r11 = (z1 * 32 + y1) * 32;
+    const float4 t000 = map[r00 + x0], t100 = map[r00 + x1], t010 = map[r10 + x0], t110 = map[r10 + x1];
+    const float4 t001 = map[r01 + x0], t101 = map[r01 + x1], t011 = map[r11 + x0], t111 = map[r11 + x1];
+    auto tri = [&](float a000, float a100, float a010, float a110, float a001, float a101, float a011, float a111) {
+        const float c00 = lerpF(a000, a100, ax), c10 = lerpF(a010, a110, ax);
+        const float c01 = lerpF(a001, a101, ax), c11 = lerpF(a011, a111, ax);
+        return lerpF(lerpF(c00, c10, ay), lerpF(c01, c11, ay), az);
+    };
+    return make_float4(tri(t000.x, t100.x, t010.x, t110.x, t001.x, t101.x, t011.x, t111.x),
+                       tri(t000.y, t100.y, t010.y, t110.y, t001.y, t101.y, t011.y, t111.y),
+                       tri(t000.z, t100.z, t010.z, t110.z, t001.z, t101.z, t011.z, t111.z),
+                       tri(t000.w, t100.w, t010.w, t110.w, t001.w, t101.w, t011.w, t111.w));
+}
+// glm dot: (x*x' + y*y') + z*z'
+__device__ __forceinline__ float nm_dot(float ax, float ay, float az, float bx, float by, float bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+// The noise branch of main() (res/conetrace_frag.glsl:136-174) for the
+// fragment of billboard cw (centre, radius) at in-plane offset (du, dv):
+// false where the view ray misses the sphere (disc < 0.01: discard); else col
+// and runningOpacity * alpha.  noise3D (:103-120): octave i samples
+// (uv + octaveOffsets[i]) * freq, the vec3 uniform indexed by component, 0 past
+// the third; alpha = |sum|.  fragTex - 0.5 = (vertPos.xy + 1) / 2 - 0.5 with
+// vertPos.xy = (du, dv) / radius.
+__device__ __forceinline__ bool noise_fragment(const float4* __restrict__ map, const ConeDesc& cd, const CameraView& cv,
+                                            float4 cw, float du, float dv, float& col_out, float& alpha_out) {
+    const float r = cw.w;
+    const float fpx = (cw.x + du * cv.S[0]) + dv * cv.U[0];
+    const float fpy = (cw.y + du * cv.S[1]) + dv * cv.U[1];
+    const float fpz = (cw.z + du * cv.S[2]) + dv * cv.U[2];
+    const float rx = cd.nm_ray[0], ry = cd.nm_ray[1], rz = cd.nm_ray[2];
+    const float ex = fpx - cw.x, ey = fpy - cw.y, ez = fpz - cw.z;
+    const float A = nm_dot(rx, ry, rz, rx, ry, rz);
+    const float B = 2.0f * nm_dot(ex, ey, ez, rx, ry, rz);
+    const float C = nm_dot(ex, ey, ez, ex, ey, ez) - r * r;
+    const float disc = B * B - (4.0f * A) * C;
+    if (disc < 0.01f) return false;
+    const float sq = sqrtf(disc);
+    const float tnear = (-B - sq) / (2.0f * A);
+    const float tfar = (-B + sq) / (2.0f * A);
+    const float wnx = fpx + rx * tnear, wny = fpy + ry * tnear, wnz = fpz + rz * tnear;
+    const float wfx = fpx + rx * tfar, wfy = fpy + ry * tfar, wfz = fpz + rz * tfar;
+    float ux = (wnx - cw.x) / r, uy = (wny - cw.y) / r, uz = (wnz - cw.z) / r;
+    const float fadj = 1.0f / cd.nm_adjust;
+    float cx = wnx * fadj, cy = wny * fadj, cz = wnz * fadj;
+    const float dlx = wfx * fadj - cx, dly = wfy * fadj - cy, dlz = wfz * fadj - cz;
+    float steps = sqrtf(nm_dot(dlx, dly, dlz, dlx, dly, dlz)) / cd.nm_step;
+    const float span = (float)(cd.nm_max_steps - cd.nm_min_steps);
+    steps = (span < steps ? span : steps) + (float)cd.nm_min_steps;
+    const float den = steps - 1.0f;
+    const float dtx = dlx / den, dty = dly / den, dtz = dlz / den;
+    const float opAdj = cd.nm_opacity / den;
+    const float lightAdj = 1.0f / den;
+    float ro = 0.0f, rl = 0.0f;
+    for (int i = 0; (float)i < steps; ++i) {
+        float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, n3 = 0.0f;
+        float freq = 1.0f, pers = 1.0f;
+        for (int o = 0; o < cd.nm_octaves; ++o) {
+            const float off = o < 3 ? cd.nm_off[o] : 0.0f;
+            const float4 t = nm_tex(map, (cx + off) * freq, (cy + off) * freq, (cz + off) * freq);
+            n0 = n0 + pers * t.x; n1 = n1 + pers * t.y; n2 = n2 + pers * t.z; n3 = n3 + pers * t.w;
+            freq = freq * cd.nm_freq_step;
+            pers = pers * cd.nm_pers_step;
+        }
+        n3 = fabsf(n3);
+        const float uu = nm_dot(ux, uy, uz, ux, uy, uz);
+        const float inv = 1.0f / sqrtf(uu);
+        const float nx = n0 + ux * inv, ny = n1 + uy * inv, nz = n2 + uz * inv;
+        ro = ro + n3 * (1.0f - uu);
+        rl = rl + clamp01(nm_dot(nx, ny, nz, 0.0f, 1.0f, 0.0f) * 0.5f + 0.5f);
+        cx = cx + dtx; cy = cy + dty; cz = cz + dtz;
+        ux = ux + dtx; uy = uy + dty; uz = uz + dtz;
+    }
+    col_out = cd.nm_min_color + (cd.nm_color_scale * rl) * lightAdj;
+    const float tx = (du / r + 1.0f) / 2.0f - 0.5f, ty = (dv / r + 1.0f) / 2.0f - 0.5f;
+    const float alpha = 1.0f - sqrtf(tx * tx + ty * ty) * 2.0f;
+    alpha_out = clamp01(ro * opAdj) * alpha;
+    return true;
+}
+
 // Ranks gathered per pass of the cone walk.
 constexpr int kConeChunk = 256;
 
@@ -2204,7 +2302,7 @@ constexpr int kConeChunk = 256;
 // two-level sampling branch) stays uniform across the wave, and in the
 // learned kernels the lanes reach step k together: their cones form the
 // wave's 64-row MLP batch.
-template <int FMT, int CONE, int COMP, int LEARNED>
+template <int FMT, int CONE, int COMP, int LEARNED, int NM = 0>
 __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
                                           const ConeDesc& cd, const void* __restrict__ vol,
                                           const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
@@ -2213,7 +2311,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                                           void* __restrict__ image, const MlpDev& mlp, uint16_t* xs,
                                           unsigned long long* cover, int* id, int tile, int lane, uint32_t& nfrag,
                                           uint32_t& nsamp, uint32_t& ntwo, uint32_t& nrows, uint32_t& nfall,
-                                          uint32_t& nevals) {
+                                          uint32_t& nevals, const float4* __restrict__ nmap = nullptr) {
     const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
     const int px = tx * cv.tile_w + (lane & (cv.tile_w - 1)), py = ty * cv.tile_h + (lane >> cv.tile_shift);
     const bool valid = px < cv.W && py < cv.H;
@@ -2292,9 +2390,20 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
             ConeFrag f = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             FtbState st = {0.0f, 0.0f, 0.0f, 0.0f};
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+            float s0u = 0.0f;  // weighted cone: the unclamped indirect
             float fr = 0.0f, fg = 0.0f, fb = 0.0f;  // colour with the predicted far field
             bool pred = false;
             bool alive = false;
+            float nm_col = 1.0f, nm_alpha = 0.0f;  // noise march (NM): colour factor and fragment alpha
+            if constexpr (NM) {
+                // the noise branch runs first and may discard the fragment
+                if (have) {
+                    const float4 cw = sph_world[j];
+                    const float4 vv = sph_view[j];
+                    const float dz = -vv.z;
+                    have = noise_fragment(nmap, cd, cv, cw, rx * dz - vv.x, ry * dz - vv.y, nm_col, nm_alpha);
+                }
+            }
             if (have) {
                 const float4 cw = sph_world[j];
                 const float4 vv = sph_view[j];
@@ -2316,6 +2425,7 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                     }
                     nsamp += (uint32_t)cd.steps;
                     s0 = s1 = s2 = s3 = clamp01(color);
+                    s0u = color;
                 } else {
                     const int rs = march_ftb<FMT>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0,
                                                         kk_steps, st, nsamp, ntwo);
@@ -2368,6 +2478,13 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                     if (pred) { s0 = clamp01(fr); s1 = clamp01(fg); s2 = clamp01(fb); }
                     s3 = clamp01(st.a);
                 }
+                if constexpr (NM) {
+                    // color = vec4(vec3(col), runningOpacity * alpha); color.rgb *= indirect
+                    // (res/conetrace_frag.glsl:172, :194-196), clamped
+                    const float ind = CONE != 0 ? st.c : s0u;
+                    s0 = s1 = s2 = clamp01(nm_col * ind);
+                    s3 = clamp01(nm_alpha);
+                }
                 blend_frag<COMP>(cd, s0, s1, s2, s3, C0, C1, C2, C3, done);
             }
         }
@@ -2387,13 +2504,14 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
 
 // The learned variant asks for 3 waves per SIMD: the inline MLP needs ~100
 // registers beside the march state, and without the bound the kernel gets 2.
-template <int FMT, int CONE, int COMP, int LEARNED>
-__global__ void __launch_bounds__(64, LEARNED ? 3 : 6) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
+template <int FMT, int CONE, int COMP, int LEARNED, int NM>
+__global__ void __launch_bounds__(64, LEARNED ? 3 : (NM ? 4 : 6)) k_cone(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd,
                                              const void* __restrict__ vol, const float4* __restrict__ sph_world,
                                              const float4* __restrict__ sph_view, const int32_t* __restrict__ order,
                                              const unsigned long long* __restrict__ masks, int words,
                                              void* __restrict__ image, unsigned long long* __restrict__ counters,
-                                             const int* __restrict__ tile_order, MlpDev mlp) {
+                                             const int* __restrict__ tile_order, MlpDev mlp,
+                                             const float4* __restrict__ nmap) {
     // One wavefront per tile; tiles are dispatched heaviest first
     // (tile_order), so the costly clump-centre tiles do not form the tail.
     __shared__ __align__(16) uint16_t xs[LEARNED ? 64 * kMlpInPad : 8];
@@ -2402,8 +2520,9 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : 6) k_cone(GridDesc g, Volume
     const int lane = threadIdx.x;
     uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
     const int tile = tile_order ? tile_order[blockIdx.x] : (int)blockIdx.x;
-    cone_tile<FMT, CONE, COMP, LEARNED>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, mlp,
-                                              xs, cover, id, tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals);
+    cone_tile<FMT, CONE, COMP, LEARNED, NM>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image,
+                                            mlp, xs, cover, id, tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals,
+                                            nmap);
     if (counters) {
         unsigned long long a = wave_sum_u64(nfrag), bsum = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
         unsigned long long e0 = 0, e1 = 0;
@@ -2844,12 +2963,22 @@ void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, 
     else go(std::integral_constant<int, 2>{});
 }
 
+// The noise-march kernels (NM = 1) are separate instantiations, so the plain
+// kernels keep their register budget.
 template <int FMT, int CONE, int COMP, int LEARNED>
 static void launch_cone_t(const ConeLaunch& a, hipStream_t s) {
     unsigned ntiles = (unsigned)(a.cv.tiles_x * a.cv.tiles_y);
-    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
+    if constexpr (!LEARNED) {
+        if (a.nmap) {
+            hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, 0, 1>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
+                               a.levels, a.sph_world, a.sph_view, a.order, a.masks, a.words, a.image, a.counters,
+                               a.tile_order, a.mlp, a.nmap);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, LEARNED, 0>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
                        a.levels, a.sph_world, a.sph_view, a.order, a.masks, a.words, a.image, a.counters,
-                       a.tile_order, a.mlp);
+                       a.tile_order, a.mlp, a.nmap);
 }
 
 // R8 volumes hold one channel: only the weighted R-sum cone reads them (the

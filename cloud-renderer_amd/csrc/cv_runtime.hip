@@ -64,6 +64,12 @@ struct CamBufs {
 struct cr_volume_s {
     cr_context ctx = nullptr;
     int N = 0, L = 0, format = 0, mode = 0;
+    // soft-particle noise march: parameters, the map's bytes and its decoded
+    // texels on the device (null while the march is off)
+    bool nm_on = false;
+    cr_noise_march nm{};
+    std::vector<int8_t> nm_bytes;
+    float4* d_nmap = nullptr;
     float blo[3] = {}, bhi[3] = {}, pos[3] = {};
     VolumeDesc vd{};
     GridDesc g{};
@@ -568,7 +574,7 @@ int cr_volume_destroy(cr_volume v) {
                     v->cam_spare.sph_world, v->cam_spare.sph_view, v->cam_spare.sph_tiles, v->cam_spare.sort,
                     v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks,
                     v->light_spare.lbox, v->light_spare.lcounts, v->light_spare.litems,
-                    v->light_spare.keys};
+                    v->light_spare.keys, v->d_nmap};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -586,6 +592,105 @@ int cr_volume_set_noise(cr_volume v, int octaves, uint32_t seed, float base_freq
     v->vd.noise_lacunarity = lacunarity;
     v->vd.noise_gain = gain;
     v->gen += 1;
+    return CR_OK;
+}
+
+// initNoiseMap(32) (src/Shaders/ConeTraceShader.cpp:127-159) with a seeded
+// stand-in for rand(): m[4 * i + c] = (r, g, b, a) bytes, i = x + 32 * (y + 32 * z).
+//   a = (char)(u * (128 - -128) + -128) (Util::genRandom(-128, 128),
+//       src/Util.hpp:23-33), u = (hash(seed, x, y, z) >> 8) / 2^24 with the
+//       lattice hash of the density noise;
+//   rgb = (char)(normalize(g) * 128), g_x = d(x + 1) - d(x - 1) / 0.5 (the
+//       division binds to the second term, as the reference writes it),
+//       d = a / 128, neighbours wrapped (getIndex, :100-113).
+// (char)f is the x86 conversion: truncate to int32 (NaN and out-of-range
+// values give INT_MIN), keep the low byte.
+static uint32_t nm_hash(uint32_t seed, int ix, int iy, int iz) {
+    uint32_t h = seed;
+    h ^= (uint32_t)ix * 0x8da6b343u;
+    h ^= (uint32_t)iy * 0xd8163841u;
+    h ^= (uint32_t)iz * 0xcb1ab31fu;
+    h ^= h >> 16; h *= 0x85ebca6bu;
+    h ^= h >> 13; h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+static int8_t nm_char(float v) {
+    const int32_t i = (v == v && std::fabs(v) < 2147483648.0f) ? (int32_t)v : INT32_MIN;
+    return (int8_t)(uint8_t)((uint32_t)i & 0xffu);
+}
+static void noise_map_build(uint32_t seed, std::vector<int8_t>& m) {
+    constexpr int D = 32;
+    m.assign((size_t)D * D * D * 4, 0);
+    auto at = [](int x, int y, int z) {
+        x = (x + D) % D; y = (y + D) % D; z = (z + D) % D;
+        return x + D * (y + D * z);
+    };
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < D; ++y)
+            for (int x = 0; x < D; ++x) {
+                const float u = (float)(nm_hash(seed, x, y, z) >> 8) * (1.0f / 16777216.0f);
+                m[4 * at(x, y, z) + 3] = nm_char(u * (128.0f - -128.0f) + -128.0f);
+            }
+    auto d = [&](int i) { return (float)m[4 * i + 3] / 128.0f; };
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < D; ++y)
+            for (int x = 0; x < D; ++x) {
+                const F3 g = f3(d(at(x + 1, y, z)) - d(at(x - 1, y, z)) / 0.5f,
+                                d(at(x, y + 1, z)) - d(at(x, y - 1, z)) / 0.5f,
+                                d(at(x, y, z + 1)) - d(at(x, y, z - 1)) / 0.5f);
+                const F3 n = norm3(g);
+                const int i = at(x, y, z);
+                m[4 * i + 0] = nm_char(n.x * 128.0f);
+                m[4 * i + 1] = nm_char(n.y * 128.0f);
+                m[4 * i + 2] = nm_char(n.z * 128.0f);
+            }
+}
+
+int cr_volume_set_noise_march(cr_volume v, const cr_noise_march* nm) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_set_noise_march: vol is NULL");
+    cr_context ctx = v->ctx;
+    if (!nm || !nm->enabled) {
+        v->nm_on = false;
+        return CR_OK;
+    }
+    if (nm->octaves < 1 || nm->octaves > 16)
+        return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: octaves must be in [1, 16]");
+    if (nm->min_steps < 1 || nm->max_steps < nm->min_steps || nm->max_steps > 64)
+        return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: need 1 <= min_steps <= max_steps <= 64");
+    const float fs[] = {nm->step_size, nm->opacity, nm->freq_step, nm->pers_step, nm->adjust_size, nm->min_color,
+                        nm->color_scale, nm->offsets[0], nm->offsets[1], nm->offsets[2]};
+    for (float f : fs)
+        if (!std::isfinite(f)) return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: non-finite parameter");
+    if (!(nm->step_size > 0.0f) || nm->adjust_size == 0.0f)
+        return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: step_size must be > 0 and adjust_size != 0");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if (!v->d_nmap || !v->nm_on || v->nm.seed != nm->seed || v->nm_bytes.empty()) {
+        std::vector<int8_t> m;
+        noise_map_build(nm->seed, m);
+        // decoded texels: RGBA8_SNORM -> max(c / 127, -1) (GL 4.4 eq. 2.2)
+        std::vector<float4> t(m.size() / 4);
+        for (size_t i = 0; i < t.size(); ++i)
+            t[i] = make_float4(std::max((float)m[4 * i] / 127.0f, -1.0f), std::max((float)m[4 * i + 1] / 127.0f, -1.0f),
+                               std::max((float)m[4 * i + 2] / 127.0f, -1.0f), std::max((float)m[4 * i + 3] / 127.0f, -1.0f));
+        // cone traces still queued may read the old map
+        CR_HIP(ctx, hipDeviceSynchronize());
+        if (!v->d_nmap) CR_HIP(ctx, hipMalloc((void**)&v->d_nmap, t.size() * sizeof(float4)));
+        CR_HIP(ctx, hipMemcpy(v->d_nmap, t.data(), t.size() * sizeof(float4), hipMemcpyHostToDevice));
+        v->nm_bytes.swap(m);
+    }
+    v->nm = *nm;
+    v->nm_on = true;
+    return CR_OK;
+}
+
+int cr_volume_read_noise_map(cr_volume v, int8_t* host_dst, size_t bytes) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_read_noise_map: vol is NULL");
+    if (!host_dst) return fail(v->ctx, CR_EINVAL, "cr_volume_read_noise_map: NULL destination");
+    if (v->nm_bytes.empty()) return fail(v->ctx, CR_ESTATE, "cr_volume_read_noise_map: the noise march was never on");
+    if (bytes < v->nm_bytes.size()) return fail(v->ctx, CR_EINVAL, "cr_volume_read_noise_map: need 131072 bytes");
+    std::memcpy(host_dst, v->nm_bytes.data(), v->nm_bytes.size());
     return CR_OK;
 }
 
@@ -1122,6 +1227,24 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     cl.tile_order = tq.order;
     cl.learned = lo ? 1 : 0;
     if (lo) cl.mlp = mlp_dev(lo->mlp);
+    if (v->nm_on) {
+        ConeDesc& cd = cl.cd;
+        cd.nm_octaves = v->nm.octaves;
+        cd.nm_min_steps = v->nm.min_steps;
+        cd.nm_max_steps = v->nm.max_steps;
+        cd.nm_step = v->nm.step_size;
+        cd.nm_opacity = v->nm.opacity;
+        cd.nm_freq_step = v->nm.freq_step;
+        cd.nm_pers_step = v->nm.pers_step;
+        cd.nm_adjust = v->nm.adjust_size;
+        cd.nm_min_color = v->nm.min_color;
+        cd.nm_color_scale = v->nm.color_scale;
+        for (int a = 0; a < 3; ++a) cd.nm_off[a] = v->nm.offsets[a];
+        // viewRay = normalize(vec3(V[0][2], V[1][2], V[2][2])) (res/conetrace_frag.glsl:138)
+        const F3 ray = norm3(f3(cl.cv.V[2], cl.cv.V[6], cl.cv.V[10]));
+        cd.nm_ray[0] = ray.x; cd.nm_ray[1] = ray.y; cd.nm_ray[2] = ray.z;
+        cl.nmap = v->d_nmap;
+    }
     launch_cone(cl, s);
     ev.mark_on(5, s);
     ev.end();
@@ -1160,6 +1283,8 @@ int cr_cone_trace(cr_volume v, const cr_light* light, const cr_camera* cam, cons
 int cr_cone_trace_learned(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
                           cr_mlp mlp, int k, float err_bound, void* d_image, int W, int H, size_t pitch, int pixfmt,
                           void* stream) {
+    if (v && v->nm_on)
+        return fail(v->ctx, CR_EUNSUPPORTED, "cr_cone_trace_learned: the noise march is on for this volume");
     LearnedOpts lo{mlp, k, err_bound};
     return cone_trace_impl(v, light, cam, p, &lo, d_image, W, H, pitch, pixfmt, stream);
 }
@@ -1171,6 +1296,7 @@ int cr_volume_segments(cr_volume v, const cr_light* light, const cr_cone_params*
     if (!light || !p || (n > 0 && (!d_p0 || !d_features || !d_targets || !d_valid)))
         return fail(ctx, CR_EINVAL, "cr_volume_segments: NULL argument");
     if (n < 0) return fail(ctx, CR_EINVAL, "cr_volume_segments: n must be >= 0");
+    if (v->nm_on) return fail(ctx, CR_EUNSUPPORTED, "cr_volume_segments: the noise march is on for this volume");
     int rc = cone_params_check(ctx, v, p, "cr_volume_segments");
     if (rc) return rc;
     if (p->cone_mode == CR_CONE_WEIGHTED) return fail(ctx, CR_EUNSUPPORTED, "cr_volume_segments: needs a front-to-back cone");
@@ -1621,6 +1747,7 @@ int cr_batch_create(cr_context ctx, const cr_batch_item* items, int count, cr_ba
         const cr_batch_item& it = items[i];
         if (!it.volume || it.volume->ctx != ctx) return fail(ctx, CR_EINVAL, "cr_batch_create: volume of another context or NULL");
         if (it.volume->format == CR_FORMAT_R8) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: R8 volumes are not batched");
+        if (it.volume->nm_on) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: the noise march is not batched");
         if (it.volume->format != items[0].volume->format || it.params.cone_mode != items[0].params.cone_mode ||
             it.params.composite != items[0].params.composite)
             return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: scenes must share format, cone mode and composite");
@@ -1687,6 +1814,8 @@ int cr_batch_run(cr_batch b, void* stream) {
     int rc = set_device(ctx);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    for (const auto& it : b->items)
+        if (it.volume->nm_on) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_run: the noise march is not batched");
     // the scenes' buffers are shared with their single cone traces: wait for
     // any camera pass still running on a volume's camera-side stream
     for (const auto& it : b->items) {

@@ -62,6 +62,23 @@ class Noise:
     gain: float = 0.5
 
 
+@dataclass(frozen=True)
+class NoiseMarch:
+    """Soft-particle noise march (ConeTraceShader defaults, src/Shaders/ConeTraceShader.hpp:14-25)."""
+    seed: int = 0
+    step_size: float = 0.01
+    opacity: float = 4.0
+    octaves: int = 4
+    freq_step: float = 3.0
+    pers_step: float = 0.5
+    adjust_size: float = 40.0
+    min_steps: int = 2
+    max_steps: int = 8
+    min_color: float = 0.2
+    color_scale: float = 0.45
+    offsets: tuple = (0.0, 0.0, 0.0)
+
+
 @dataclass
 class Scene:
     name: str
@@ -83,6 +100,7 @@ class Scene:
     position: tuple = (0.0, 0.0, 0.0)
     fluffiness: float = 1.0
     noise: Optional[Noise] = None
+    noise_march: Optional[NoiseMarch] = None
     directional: int = 1
     seed: int = 0
     meta: dict = field(default_factory=dict)

@@ -154,6 +154,40 @@ int cr_volume_destroy(cr_volume vol);
 /* Density modulation by fractal value noise (rgba mode), octaves = 0 disables. */
 int cr_volume_set_noise(cr_volume vol, int octaves, uint32_t seed, float base_freq, float lacunarity,
                         float gain);
+/* Soft-particle noise march: the noise branch of the cone-trace fragment
+ * shader (res/conetrace_frag.glsl:103-174, colour :194-196) with the
+ * parameters of ConeTraceShader (src/Shaders/ConeTraceShader.hpp:14-25,
+ * defaults in brackets) and its 32^3 RGBA8_SNORM noise map
+ * (src/Shaders/ConeTraceShader.cpp:127-159).  The map's random alpha bytes
+ * come from `seed` in place of rand(); normals follow the map's gradient as
+ * the reference computes it.  offsets is the octaveOffsets uniform
+ * (windVel * runTime, ConeTraceShader.cpp:54-61); octave i adds offsets[i]
+ * to all three coordinates, 0 past the third. */
+typedef struct {
+    int enabled;         /* doNoiseSample */
+    uint32_t seed;
+    float step_size;     /* stepSize [0.01] */
+    float opacity;       /* noiseOpacity [4] */
+    int octaves;         /* numOctaves [4], 1..16 */
+    float freq_step;     /* freqStep [3] */
+    float pers_step;     /* persStep [0.5] */
+    float adjust_size;   /* adjustSize [40] */
+    int min_steps;       /* minNoiseSteps [2], >= 1 */
+    int max_steps;       /* maxNoiseSteps [8], min_steps..64 */
+    float min_color;     /* minNoiseColor [0.2] */
+    float color_scale;   /* noiseColorScale [0.45] */
+    float offsets[3];
+} cr_noise_march;
+
+/* Switch the noise march of cr_cone_trace on (nm->enabled != 0, the noise map
+ * is built on the host and uploaded) or off (nm NULL or enabled 0).  With it
+ * on, a camera fragment whose view ray misses its sphere is discarded and the
+ * others blend (col * indirect, runningOpacity * alpha), clamped to [0, 1].
+ * cr_cone_trace_learned and cr_volume_segments reject a volume with it on. */
+int cr_volume_set_noise_march(cr_volume vol, const cr_noise_march *nm);
+/* The noise map bytes of the last cr_volume_set_noise_march that switched it
+ * on: 32^3 texels of (r, g, b, a), x fastest (bytes >= 131072). */
+int cr_volume_read_noise_map(cr_volume vol, int8_t *host_dst, size_t bytes);
 /* CloudVolume::uploadBillboards (src/CloudVolume.cpp:139-164): billboard
  * centres relative to position (xyz, 3*count floats) and scales; the radius
  * used is scale * fluffiness.  count = 0 is allowed (nothing is drawn). */

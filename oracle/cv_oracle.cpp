@@ -83,6 +83,24 @@ typedef struct {
     float clear_color[4]; // framebuffer clear (composite 0)
 } cvo_cone;
 
+// soft-particle noise march (res/conetrace_frag.glsl:29-41 uniforms;
+// defaults src/Shaders/ConeTraceShader.hpp:14-25)
+typedef struct {
+    int enabled;          // doNoiseSample
+    uint32_t seed;        // noise-map alpha bytes (a seeded stand-in for rand())
+    float step_size;      // stepSize
+    float opacity;        // noiseOpacity
+    int octaves;          // numOctaves
+    float freq_step;      // freqStep
+    float pers_step;      // persStep
+    float adjust_size;    // adjustSize
+    int min_steps;        // minNoiseSteps
+    int max_steps;        // maxNoiseSteps
+    float min_color;      // minNoiseColor
+    float color_scale;    // noiseColorScale
+    float offsets[3];     // the octaveOffsets uniform (windVel * runTime)
+} cvo_noise_march;
+
 typedef struct {
     float V[16];          // glm column-major view matrix
     float near_plane[3];  // Sun::nearPlane
@@ -968,6 +986,178 @@ inline uint8_t q8(float v) {  // GL float -> UNORM8 conversion (round to nearest
     return (uint8_t)(int)std::floor(v * 255.0f + 0.5f);
 }
 
+// ------------------------------------------------------------------------
+// Soft-particle noise march (res/conetrace_frag.glsl:103-120,136-174) and its
+// noise map (src/Shaders/ConeTraceShader.cpp:100-159)
+// ------------------------------------------------------------------------
+constexpr int kNoiseDim = 32;
+
+// (char)v for a float v, as the x86 conversion does it: truncate to a 32-bit
+// integer (NaN and out-of-range values give INT_MIN), keep the low byte.
+inline int8_t to_char(float v) {
+    int32_t i = (v == v && std::fabs(v) < 2147483648.0f) ? (int32_t)v : INT32_MIN;
+    return (int8_t)(uint8_t)((uint32_t)i & 0xffu);
+}
+// getIndex (src/Shaders/ConeTraceShader.cpp:100-113): wrap into [0, dim)
+inline int noise_index(int x, int y, int z) {
+    x = (x + kNoiseDim) % kNoiseDim; y = (y + kNoiseDim) % kNoiseDim; z = (z + kNoiseDim) % kNoiseDim;
+    return x + y * kNoiseDim + z * kNoiseDim * kNoiseDim;
+}
+
+// initNoiseMap(32) (src/Shaders/ConeTraceShader.cpp:127-159), m[4 * i + c]
+// (r, g, b, a) bytes, i = x + 32 * (y + 32 * z):
+//   a = (char)genRandom(-128, 128) = (char)(u * (128 - -128) + -128)
+//     (src/Util.hpp:23-33); u = lattice_value(seed, x, y, z) in [0, 1)
+//     stands in for rand() / RAND_MAX;
+//   gradient component = getDensity(+1) - getDensity(-1) / heightAdjust (the
+//     division binds to the second term only, as written), getDensity = a / 128,
+//     heightAdjust = 0.5; rgb = (char)(normalize(gradient) * 128).
+// Pinned by: test_oracle_noise_map_alpha (uniform bytes over [-127, 127]),
+// test_oracle_noise_map_normals (unit length; along the gradient as written,
+// not along the central difference).
+void noise_map_bytes(uint32_t seed, int8_t* m) {
+    const int D = kNoiseDim;
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < D; ++y)
+            for (int x = 0; x < D; ++x) {
+                float u = lattice_value<float>(seed, x, y, z);
+                m[4 * noise_index(x, y, z) + 3] = to_char(u * (128.0f - -128.0f) + -128.0f);
+            }
+    auto dens = [&](int i) { return (float)m[4 * i + 3] / 128.0f; };
+    const float heightAdjust = 0.5f;
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < D; ++y)
+            for (int x = 0; x < D; ++x) {
+                V3 g = v3(dens(noise_index(x + 1, y, z)) - dens(noise_index(x - 1, y, z)) / heightAdjust,
+                          dens(noise_index(x, y + 1, z)) - dens(noise_index(x, y - 1, z)) / heightAdjust,
+                          dens(noise_index(x, y, z + 1)) - dens(noise_index(x, y, z - 1)) / heightAdjust);
+                V3 n = normalize(g);
+                int i = noise_index(x, y, z);
+                m[4 * i + 0] = to_char(n.x * 128.0f);
+                m[4 * i + 1] = to_char(n.y * 128.0f);
+                m[4 * i + 2] = to_char(n.z * 128.0f);
+            }
+}
+
+// RGBA8_SNORM texel -> float (GL 4.4 eq. 2.2): max(c / 127, -1)
+template <class T> inline T snorm8(int8_t c) { return std::max((T)c / (T)127, (T)-1); }
+
+// texture(noiseMap, s) with GL_LINEAR and GL_REPEAT on the one-level 32^3 map
+// (GL 4.4 8.14.2): u = s * 32 - 1/2 as fma(s, 32, -1/2), i0 = floor(u) mod 32,
+// i1 = (i0 + 1) mod 32, alpha = u - floor(u); lerps along x, then y, then z as
+// in trilinear() (lerp(a, b, t) = fma(t, b - a, a)).  floor(u) mod 32 is
+// taken as fl - 32 * floor(fl / 32), exact for every float fl.
+// Pinned by: test_oracle_noise_sample (weight-product form, wrap-around).
+template <class T> inline int wrap32(T fl) {
+    T r = fl - (T)32 * std::floor(fl * (T)(1.0 / 32.0));
+    int i = (r == r) ? (int)r : 0;
+    return i & 31;
+}
+template <class T> void noise_tex(const int8_t* m, T sx, T sy, T sz, T out[4]) {
+    T u[3] = {std::fma(sx, (T)32, (T)-0.5), std::fma(sy, (T)32, (T)-0.5), std::fma(sz, (T)32, (T)-0.5)};
+    int i0[3], i1[3];
+    T a[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        T fl = std::floor(u[ax]);
+        a[ax] = u[ax] - fl;
+        i0[ax] = wrap32(fl);
+        i1[ax] = (i0[ax] + 1) & 31;
+    }
+    auto tx = [&](int x, int y, int z, int c) { return snorm8<T>(m[4 * (x + 32 * (y + 32 * z)) + c]); };
+    for (int c = 0; c < 4; ++c) {
+        T c00 = lerpF(tx(i0[0], i0[1], i0[2], c), tx(i1[0], i0[1], i0[2], c), a[0]);
+        T c10 = lerpF(tx(i0[0], i1[1], i0[2], c), tx(i1[0], i1[1], i0[2], c), a[0]);
+        T c01 = lerpF(tx(i0[0], i0[1], i1[2], c), tx(i1[0], i0[1], i1[2], c), a[0]);
+        T c11 = lerpF(tx(i0[0], i1[1], i1[2], c), tx(i1[0], i1[1], i1[2], c), a[0]);
+        T c0 = lerpF(c00, c10, a[1]);
+        T c1 = lerpF(c01, c11, a[1]);
+        out[c] = lerpF(c0, c1, a[2]);
+    }
+}
+
+template <class T> struct NV3 { T x, y, z; };
+template <class T> inline T ndot(NV3<T> a, NV3<T> b) { T x = a.x * b.x, y = a.y * b.y, z = a.z * b.z; return (x + y) + z; }
+
+// noise3D (res/conetrace_frag.glsl:103-120): octave i samples at
+// (uv + octaveOffsets[i]) * freq, octaveOffsets being the vec3 uniform indexed
+// by component (uv + a scalar); components past the third (GLSL leaves a
+// dynamic index out of range undefined) add 0.  alpha = |sum of alphas|.
+template <class T> void noise3d(const int8_t* m, const cvo_noise_march* nm, NV3<T> uv, T out[4]) {
+    for (int c = 0; c < 4; ++c) out[c] = (T)0;
+    T freq = (T)1, pers = (T)1;
+    for (int i = 0; i < nm->octaves; ++i) {
+        T off = i < 3 ? (T)nm->offsets[i] : (T)0;
+        T ov[4];
+        noise_tex<T>(m, (uv.x + off) * freq, (uv.y + off) * freq, (uv.z + off) * freq, ov);
+        for (int c = 0; c < 4; ++c) out[c] = out[c] + pers * ov[c];
+        freq = freq * (T)nm->freq_step;
+        pers = pers * (T)nm->pers_step;
+    }
+    out[3] = std::fabs(out[3]);
+}
+
+template <class T> inline T sat(T x) { return std::fmin(std::fmax(x, (T)0), (T)1); }
+
+// The noise branch of main() (res/conetrace_frag.glsl:136-174) for one
+// fragment: fp = fragPos, c = center, r = radius, (tu, tv) = vertPos.xy (the
+// fragment's offset in billboard units, fragTex = (vertPos.xy + 1) / 2),
+// ray = normalize(V's third row).  Returns false where the fragment is
+// discarded (ray misses the sphere: disc < DIST_BIAS); else col and the
+// fragment alpha runningOpacity * alpha.  raySphereIntersect is :83-100.
+// Pinned by: test_oracle_noise_fragment (an FP64 restatement from the GLSL
+// text, with the step count, offsets and discard decisions checked).
+// replay >= 0 (FP64 replay of an FP32 fragment): skip the discard test and
+// take exactly `replay` steps.
+template <class T>
+bool noise_fragment(const int8_t* m, const cvo_noise_march* nm, NV3<T> fp, NV3<T> c, T r, T tu, T tv, NV3<T> ray,
+                    T* col_out, T* alpha_out, int* steps_out, int replay = -1) {
+    NV3<T> delta = {fp.x - c.x, fp.y - c.y, fp.z - c.z};
+    T A = ndot(ray, ray);
+    T B = (T)2 * ndot(delta, ray);
+    T C = ndot(delta, delta) - r * r;
+    T disc = B * B - ((T)4 * A) * C;
+    if (replay < 0 && disc < (T)0.01) return false;
+    T sq = std::sqrt(std::max(disc, (T)0));
+    T tnear = (-B - sq) / ((T)2 * A);
+    T tfar = (-B + sq) / ((T)2 * A);
+    NV3<T> wn = {fp.x + ray.x * tnear, fp.y + ray.y * tnear, fp.z + ray.z * tnear};
+    NV3<T> wf = {fp.x + ray.x * tfar, fp.y + ray.y * tfar, fp.z + ray.z * tfar};
+    NV3<T> unit = {(wn.x - c.x) / r, (wn.y - c.y) / r, (wn.z - c.z) / r};
+    T fadj = (T)1 / (T)nm->adjust_size;
+    NV3<T> tn = {wn.x * fadj, wn.y * fadj, wn.z * fadj};
+    NV3<T> tf = {wf.x * fadj, wf.y * fadj, wf.z * fadj};
+    NV3<T> dl = {tf.x - tn.x, tf.y - tn.y, tf.z - tn.z};
+    T iSteps = std::sqrt(ndot(dl, dl)) / (T)nm->step_size;
+    iSteps = std::min(iSteps, (T)(nm->max_steps - nm->min_steps)) + (T)nm->min_steps;
+    T den = iSteps - (T)1;
+    NV3<T> dt = {dl.x / den, dl.y / den, dl.z / den};
+    T opAdj = (T)nm->opacity / den;
+    T lightAdj = (T)1 / den;
+    T ro = (T)0, rl = (T)0;
+    NV3<T> cur = tn;
+    int n = 0;
+    for (int i = 0; replay < 0 ? (T)i < iSteps : i < replay; ++i) {
+        T nc[4];
+        noise3d<T>(m, nm, cur, nc);
+        T inv = (T)1 / std::sqrt(ndot(unit, unit));
+        NV3<T> nx = {nc[0] + unit.x * inv, nc[1] + unit.y * inv, nc[2] + unit.z * inv};
+        ro = ro + nc[3] * ((T)1 - ndot(unit, unit));
+        NV3<T> up = {(T)0, (T)1, (T)0};
+        rl = rl + sat(ndot(nx, up) * (T)0.5 + (T)0.5);
+        cur = {cur.x + dt.x, cur.y + dt.y, cur.z + dt.z};
+        unit = {unit.x + dt.x, unit.y + dt.y, unit.z + dt.z};
+        ++n;
+    }
+    T col = (T)nm->min_color + ((T)nm->color_scale * rl) * lightAdj;
+    T fx = (tu + (T)1) / (T)2 - (T)0.5, fy = (tv + (T)1) / (T)2 - (T)0.5;
+    T alpha = (T)1 - std::sqrt(fx * fx + fy * fy) * (T)2;
+    ro = sat(ro * opAdj);
+    *col_out = col;
+    *alpha_out = ro * alpha;
+    if (steps_out) *steps_out = n;
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -988,11 +1178,14 @@ extern "C" {
 // Only rows [row0, row1) are shaded (bounded CPU timing).  img32: W*H*4
 // floats (all rows written; unshaded rows keep the clear/zero value).
 // img64: FP64 replay (optional).
-int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
-                   const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
-                   const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1, int row_step,
-                   float* img32, double* img64, cvo_stats* stats) {
+static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
+                           const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
+                           const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1,
+                           int row_step, float* img32, double* img64, cvo_stats* stats, const cvo_noise_march* nm) {
     if (!vol || W <= 0 || H <= 0 || !cp || cp->steps < 0 || row_step < 1) return -1;
+    const bool use_nm = nm && nm->enabled;
+    std::vector<int8_t> nmap(use_nm ? kNoiseDim * kNoiseDim * kNoiseDim * 4 : 0);
+    if (use_nm) noise_map_bytes(nm->seed, nmap.data());
     Grid g = make_grid(vol);
     int N = vol->dim, L = vol->levels, C = channels(vol->format);
     Volume32 v32;
@@ -1017,6 +1210,11 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
     double tanHalfD = std::tan((double)cam->fovy / 2.0);
     cs.kxd = tanHalfD * ((double)W / (double)H);
     cs.kyd = tanHalfD;
+    // noise march view ray: normalize(vec3(V[0][2], V[1][2], V[2][2])) (res/conetrace_frag.glsl:138)
+    V3 ray32 = normalize(v3(cs.V[2], cs.V[6], cs.V[10]));
+    NV3<float> nray = {ray32.x, ray32.y, ray32.z};
+    double rl_ = std::sqrt(cs.Vd[2] * cs.Vd[2] + cs.Vd[6] * cs.Vd[6] + cs.Vd[10] * cs.Vd[10]);
+    NV3<double> nrayd = {cs.Vd[2] / rl_, cs.Vd[6] / rl_, cs.Vd[10] / rl_};
 
     // directional cone direction: normalize(L) / voxelDim
     V3 ld = normalize(v3(light->direction[0], light->direction[1], light->direction[2]));
@@ -1090,10 +1288,17 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
                 float du = hx - o.cv.x;
                 float sc;
                 if (!sphere_contrib(du, dv, o.s, &sc)) continue;
-                ++nfrag;
                 // fragment position, FP32
                 V3 fp = v3((o.c.x + du * cs.bb.S.x) + dv * cs.bb.U.x, (o.c.y + du * cs.bb.S.y) + dv * cs.bb.U.y,
                            (o.c.z + du * cs.bb.S.z) + dv * cs.bb.U.z);
+                // noise branch first (it may discard the fragment)
+                float nm_col = 1.0f, nm_alpha = 0.0f;
+                int nm_steps = 0;
+                if (use_nm && !noise_fragment<float>(nmap.data(), nm, NV3<float>{fp.x, fp.y, fp.z},
+                                                     NV3<float>{o.c.x, o.c.y, o.c.z}, o.s, du / o.s, dv / o.s, nray,
+                                                     &nm_col, &nm_alpha, &nm_steps))
+                    continue;
+                ++nfrag;
                 // fragPos + billboardNormal * radius * sphereContrib: (n * r) * sc
                 V3 pos = v3(fp.x + (cs.bb.B.x * o.s) * sc, fp.y + (cs.bb.B.y * o.s) * sc, fp.z + (cs.bb.B.z * o.s) * sc);
                 V3 vp = voxel_lerp(g, pos);
@@ -1110,6 +1315,11 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
                 for (int i = 0; i < cr.nsamp; ++i) ntwo += two[i];
                 float src[4];
                 for (int q = 0; q < 4; ++q) src[q] = std::min(std::max(cr.rgba[q], 0.0f), 1.0f);
+                if (use_nm) {
+                    // color = vec4(vec3(col), runningOpacity * alpha); color.rgb *= indirect (:172, :194-196)
+                    for (int q = 0; q < 3; ++q) src[q] = sat(nm_col * cr.rgba[q]);
+                    src[3] = sat(nm_alpha);
+                }
 
                 // FP64 replay of the same fragment
                 double srcd[4] = {0, 0, 0, 0};
@@ -1140,6 +1350,17 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
                     }
                     ConeResult<double> crd = trace_cone<double>(v64, st64, cp, p0d, dd, cr.nsamp);
                     for (int q = 0; q < 4; ++q) srcd[q] = std::min(std::max(crd.rgba[q], 0.0), 1.0);
+                    if (use_nm) {
+                        double fpx = o.c.x + dud * cs.Sd.x + dvd * cs.Ud.x;
+                        double fpy = o.c.y + dud * cs.Sd.y + dvd * cs.Ud.y;
+                        double fpz = o.c.z + dud * cs.Sd.z + dvd * cs.Ud.z;
+                        double cold = 1.0, alphad = 0.0;
+                        noise_fragment<double>(nmap.data(), nm, NV3<double>{fpx, fpy, fpz},
+                                               NV3<double>{o.c.x, o.c.y, o.c.z}, (double)o.s, dud / o.s, dvd / o.s,
+                                               nrayd, &cold, &alphad, nullptr, nm_steps);
+                        for (int q = 0; q < 3; ++q) srcd[q] = sat(cold * crd.rgba[q]);
+                        srcd[3] = sat(alphad);
+                    }
                 }
                 (void)j;
 
@@ -1180,6 +1401,52 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
     if (dbl) std::memcpy(img64, C64.data(), npx * 4 * sizeof(double));
     if (stats) { stats->fragments = nfrag; stats->samples = nsamp; stats->two_level = ntwo; }
     return 0;
+}
+
+int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
+                   const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
+                   const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1, int row_step,
+                   float* img32, double* img64, cvo_stats* stats) {
+    return cone_trace_impl(vol, levels_native, levels_f64, S, xyz, radius, order, cam, light, cp, W, H, row0, row1,
+                           row_step, img32, img64, stats, nullptr);
+}
+
+// cvo_cone_trace with the soft-particle noise branch of res/conetrace_frag.glsl
+// (:136-174, :194-196) in front of the cone: a fragment the noise branch
+// discards is dropped; otherwise its colour is (col * indirect rgb,
+// runningOpacity * alpha), clamped to [0, 1].
+int cvo_cone_trace_nm(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
+                      const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
+                      const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1, int row_step,
+                      float* img32, double* img64, cvo_stats* stats, const cvo_noise_march* nm) {
+    return cone_trace_impl(vol, levels_native, levels_f64, S, xyz, radius, order, cam, light, cp, W, H, row0, row1,
+                           row_step, img32, img64, stats, nm);
+}
+
+// the noise map bytes (32^3 x (r, g, b, a), x fastest)
+void cvo_noise_map(uint32_t seed, int8_t* out) { noise_map_bytes(seed, out); }
+
+// texture(noiseMap, p) at n points (FP32 and FP64), for tests
+void cvo_noise_sample(uint32_t seed, const float* pts, int64_t n, float* out32, double* out64) {
+    std::vector<int8_t> m(kNoiseDim * kNoiseDim * kNoiseDim * 4);
+    noise_map_bytes(seed, m.data());
+    for (int64_t i = 0; i < n; ++i) {
+        noise_tex<float>(m.data(), pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], out32 + 4 * i);
+        if (out64) noise_tex<double>(m.data(), pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], out64 + 4 * i);
+    }
+}
+
+// one fragment of the noise branch (FP32), for tests: in[0..2] fragPos,
+// in[3..5] center, in[6] radius, in[7..8] vertPos.xy, in[9..11] view ray;
+// out = (col, alpha, steps, kept)
+void cvo_noise_fragment(const cvo_noise_march* nm, const float* in, float* out) {
+    std::vector<int8_t> m(kNoiseDim * kNoiseDim * kNoiseDim * 4);
+    noise_map_bytes(nm->seed, m.data());
+    float col = 0.0f, alpha = 0.0f;
+    int steps = 0;
+    bool kept = noise_fragment<float>(m.data(), nm, NV3<float>{in[0], in[1], in[2]}, NV3<float>{in[3], in[4], in[5]},
+                                      in[6], in[7], in[8], NV3<float>{in[9], in[10], in[11]}, &col, &alpha, &steps);
+    out[0] = col; out[1] = alpha; out[2] = (float)steps; out[3] = kept ? 1.0f : 0.0f;
 }
 
 // the FP32 cone schedule, for tests: h_i, r_i, level_i, frac_i

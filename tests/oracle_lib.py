@@ -33,6 +33,19 @@ class cvo_camera(C.Structure):
                 ("zfar", C.c_float)]
 
 
+class cvo_noise_march(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("seed", C.c_uint32), ("step_size", C.c_float), ("opacity", C.c_float),
+                ("octaves", C.c_int), ("freq_step", C.c_float), ("pers_step", C.c_float),
+                ("adjust_size", C.c_float), ("min_steps", C.c_int), ("max_steps", C.c_int),
+                ("min_color", C.c_float), ("color_scale", C.c_float), ("offsets", C.c_float * 3)]
+
+
+def noise_march_of(m) -> cvo_noise_march:
+    return cvo_noise_march(1, int(m.seed) & 0xffffffff, m.step_size, m.opacity, int(m.octaves), m.freq_step,
+                           m.pers_step, m.adjust_size, int(m.min_steps), int(m.max_steps), m.min_color,
+                           m.color_scale, (C.c_float * 3)(*[float(o) for o in m.offsets]))
+
+
 class cvo_cone(C.Structure):
     _fields_ = [("steps", C.c_int), ("cone_angle", C.c_float), ("initial_height", C.c_float),
                 ("lod_offset", C.c_float), ("down_scaling", C.c_float), ("cone_mode", C.c_int),
@@ -79,6 +92,11 @@ def lib():
                                      P(cvo_cone), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp,
                                      P(cvo_stats)]
         L.cvo_cone_trace.restype = C.c_int
+        L.cvo_cone_trace_nm.argtypes = L.cvo_cone_trace.argtypes + [P(cvo_noise_march)]
+        L.cvo_cone_trace_nm.restype = C.c_int
+        L.cvo_noise_map.argtypes = [C.c_uint32, vp]
+        L.cvo_noise_sample.argtypes = [C.c_uint32, vp, C.c_int64, vp, vp]
+        L.cvo_noise_fragment.argtypes = [P(cvo_noise_march), vp, vp]
         L.cvo_step_table.argtypes = [P(cvo_cone), C.c_int, C.c_int, vp, vp, vp, vp]
         L.cvo_sample.argtypes = [P(cvo_volume), vp, vp, C.c_float, vp, C.c_int64, vp, vp]
         L.cvo_trace_one.argtypes = [P(cvo_volume), vp, vp, P(cvo_cone), P(C.c_float), P(C.c_float), vp, vp,
@@ -234,11 +252,42 @@ def cone_trace(scene, nat, f64, order, cam, row0=0, row1=None, want_f64=True, ro
     k = camera_of(cam)
     L = light_of(scene)
     cp = cone_of(scene.cone)
-    rc = lib().cvo_cone_trace(C.byref(vol), ptr(nat), ptr(f64 if img64 is not None else None), scene.S, ptr(xyz),
-                              ptr(r), ptr(order), C.byref(k), C.byref(L), C.byref(cp), W, H, int(row0), int(row1),
-                              int(row_step), ptr(img32), ptr(img64), C.byref(st))
+    args = (C.byref(vol), ptr(nat), ptr(f64 if img64 is not None else None), scene.S, ptr(xyz), ptr(r), ptr(order),
+            C.byref(k), C.byref(L), C.byref(cp), W, H, int(row0), int(row1), int(row_step), ptr(img32), ptr(img64),
+            C.byref(st))
+    nm = getattr(scene, "noise_march", None)
+    if nm is not None:
+        m = noise_march_of(nm)
+        rc = lib().cvo_cone_trace_nm(*args, C.byref(m))
+    else:
+        rc = lib().cvo_cone_trace(*args)
     assert rc == 0
     return img32, img64, st
+
+
+def noise_map(seed: int) -> np.ndarray:
+    """The noise-march map bytes, (32, 32, 32, 4) int8 indexed [z, y, x, (r, g, b, a)]."""
+    out = np.zeros((32, 32, 32, 4), np.int8)
+    lib().cvo_noise_map(int(seed) & 0xffffffff, ptr(out))
+    return out
+
+
+def noise_sample(seed: int, pts: np.ndarray):
+    pts = np.ascontiguousarray(pts, np.float32)
+    n = pts.shape[0]
+    o32 = np.zeros((n, 4), np.float32)
+    o64 = np.zeros((n, 4), np.float64)
+    lib().cvo_noise_sample(int(seed) & 0xffffffff, ptr(pts), n, ptr(o32), ptr(o64))
+    return o32, o64
+
+
+def noise_fragment(m, fp, c, r, tu, tv, ray):
+    """One fragment of the noise branch (FP32): (col, alpha, steps, kept)."""
+    nm = noise_march_of(m)
+    inp = np.array(list(fp) + list(c) + [r, tu, tv] + list(ray), np.float32)
+    out = np.zeros(4, np.float32)
+    lib().cvo_noise_fragment(C.byref(nm), ptr(inp), ptr(out))
+    return float(out[0]), float(out[1]), int(out[2]), bool(out[3])
 
 
 def f2h(a: np.ndarray) -> np.ndarray:

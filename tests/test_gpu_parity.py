@@ -13,6 +13,7 @@ from dataclasses import replace
 import numpy as np
 import pytest
 
+import oracle_lib as O
 from gpu_common import assert_bit_equal, log, psnr, run_gpu, run_oracle
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +40,13 @@ def _scenes(cv):
         # hash on scalar registers), then cells of 8 (one cell per lane)
         "small32_fp16_noise_wide": replace(base, fmt=2, noise=sc.Noise(octaves=3, seed=12, base_freq=1.0),
                                            cone=replace(base.cone, compensated=1)),
+        # soft-particle noise march in front of the cone (reference defaults;
+        # then wind offsets, a fifth octave and 1..12 steps on the rgba path)
+        "small32_ref_noise_march": replace(sc.reference_mode(base), noise_march=sc.NoiseMarch(seed=3)),
+        "small32_ftb_noise_march": replace(base, cone=replace(base.cone, cone_mode=1),
+                                           noise_march=sc.NoiseMarch(seed=9, octaves=5, min_steps=1, max_steps=12,
+                                                                     offsets=(0.013, -0.02, 0.005), adjust_size=2.0,
+                                                                     step_size=0.02)),
         "small32_comp_blend": replace(base, cone=replace(base.cone, composite=0)),
         "small32_rect": replace(base, W=200, H=120, light_w=130, light_h=210),
         "npot24": replace(base, N=24, levels=5),
@@ -72,7 +80,8 @@ def tie_scene(sc):
 
 
 CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_rgba32f", "small32_ref_float_blend",
-         "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
+         "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_ref_noise_march",
+         "small32_ftb_noise_march", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -236,3 +245,35 @@ def test_pipelined_frames_with_growing_buffers(cv, gpu_ctx):
     got = run(True)
     for i in range(len(sizes)):
         assert_bit_equal(f"frame {i} ({sizes[i]})", got[i], want[i])
+
+
+def test_noise_march_map_and_limits(cv, gpu_ctx):
+    """The runtime's noise map equals the oracle's byte for byte; the learned
+    trace and the batch refuse a volume with the march on; switching it off
+    gives the plain image back."""
+    sc = cv.scenes
+    base = sc.small32(6)
+    log("[noise march] map bytes seed 21, refusals, switch off")
+    scene = replace(base, noise_march=sc.NoiseMarch(seed=21))
+    vol, sun, cam, vox, tr = cv.scene_objects(scene, 0, context=gpu_ctx)
+    try:
+        vol.set_noise_march(tr.noise_params())
+        assert_bit_equal("noise map", vol.read_noise_map(), O.noise_map(21))
+        vox.voxelize(vol, sun)
+        import torch
+        img = torch.zeros((scene.H, scene.W, 4), dtype=torch.float32, device="cuda")
+        with pytest.raises(cv.CloudVoxError) as e:
+            tr.coneTraceLearned(vol, sun, cam, cv.FarFieldMLP(context=gpu_ctx), img, scene.W, scene.H, k=8)
+        assert e.value.code == cv._abi.CR_EUNSUPPORTED
+        tr.coneTrace(vol, sun, cam, img, scene.W, scene.H)
+        torch.cuda.synchronize()
+        with_nm = img.cpu().numpy()
+        tr.doNoiseSample = False
+        tr.coneTrace(vol, sun, cam, img, scene.W, scene.H)
+        torch.cuda.synchronize()
+        plain = img.cpu().numpy()
+    finally:
+        vol.close()
+    want = run_gpu(cv, base, ctx=gpu_ctx, read_levels=False).image
+    assert_bit_equal("noise march off", plain, want)
+    assert not np.array_equal(with_nm, plain)

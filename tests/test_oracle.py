@@ -566,3 +566,167 @@ def test_oracle_sort_distinct_and_ties():
     O.lib().cvo_board_distances(48, O.ptr(np.ascontiguousarray(pts)), O.f3((0, 0, 0)), O.f3((0, 0, 3)), O.ptr(d32))
     assert len(np.unique(d32)) < 48
     assert list(order) == _literal_selection(d32.tolist())
+
+
+# ---------------------------------------------------------------------------
+# soft-particle noise march (res/conetrace_frag.glsl:103-174) and its map
+# (src/Shaders/ConeTraceShader.cpp:100-159)
+# ---------------------------------------------------------------------------
+def _hash_u(seed, x, y, z):
+    h = seed
+    h ^= (x * 0x8DA6B343) & 0xFFFFFFFF
+    h ^= (y * 0xD8163841) & 0xFFFFFFFF
+    h ^= (z * 0xCB1AB31F) & 0xFFFFFFFF
+    return (_fmix32(h) >> 8) / 16777216.0
+
+
+def test_oracle_noise_map_alpha():
+    m = O.noise_map(7)
+    a = m[..., 3].astype(np.int64)
+    # (char)genRandom(-128, 128): truncation toward zero of u * 256 - 128
+    for (x, y, z) in [(0, 0, 0), (31, 0, 5), (3, 17, 29), (31, 31, 31)]:
+        v = np.float32(np.float32(_hash_u(7, x, y, z)) * np.float32(256.0)) + np.float32(-128.0)
+        assert a[z, y, x] == int(np.trunc(v))
+    # uniform bytes over [-128, 127]: every value from -127 to 126 occurs, mean ~ -0.5
+    vals = set(np.unique(a).tolist())
+    assert set(range(-127, 127)) <= vals and vals <= set(range(-128, 128))
+    assert abs(a.mean() + 0.5) < 2.0
+    assert not np.array_equal(m, O.noise_map(8))
+
+
+def test_oracle_noise_map_normals():
+    m = O.noise_map(3)
+    d = m[..., 3].astype(np.float64) / 128.0                       # [z, y, x]
+    def nb(ax, s):
+        return np.roll(d, -s, axis=ax)
+    # as written: d(+1) - d(-1) / heightAdjust (the division binds to the second term)
+    g = np.stack([nb(2, 1) - nb(2, -1) / 0.5, nb(1, 1) - nb(1, -1) / 0.5, nb(0, 1) - nb(0, -1) / 0.5], -1)
+    central = np.stack([nb(2, 1) - nb(2, -1), nb(1, 1) - nb(1, -1), nb(0, 1) - nb(0, -1)], -1)
+    n = m[..., :3].astype(np.float64) / 128.0
+    ok = ~np.any(m[..., :3] == -128, axis=-1)   # +-1.0 components wrap to -128
+    assert ok.mean() > 0.99
+    ln = np.linalg.norm(n, axis=-1)
+    assert np.all(np.abs(ln[ok] - 1.0) < 0.03)
+    cos_g = np.sum(n * g, -1) / (ln * np.linalg.norm(g, axis=-1))
+    cos_c = np.sum(n * central, -1) / (ln * np.linalg.norm(central, axis=-1))
+    assert cos_g[ok].min() > 0.99
+    assert np.mean(cos_c[ok] < 0.99) > 0.5   # the central difference points elsewhere
+    # alpha bytes are untouched by the normal pass
+    assert np.array_equal(m[..., 3], O.noise_map(3)[..., 3])
+
+
+def _gl_repeat_trilinear(m, p):
+    """GL_LINEAR + GL_REPEAT, weight-product form, FP64 (GL 4.4 eq. 8.10)."""
+    t = np.maximum(m.astype(np.float64) / 127.0, -1.0)             # [z, y, x, c]
+    u = np.asarray(p, np.float64) * 32.0 - 0.5
+    i0 = np.floor(u).astype(np.int64)
+    a = u - i0
+    out = np.zeros(4)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                w = (a[0] if dx else 1 - a[0]) * (a[1] if dy else 1 - a[1]) * (a[2] if dz else 1 - a[2])
+                out += w * t[(i0[2] + dz) % 32, (i0[1] + dy) % 32, (i0[0] + dx) % 32]
+    return out
+
+
+def test_oracle_noise_sample():
+    seed = 11
+    m = O.noise_map(seed)
+    rng = np.random.default_rng(1)
+    pts = np.concatenate([rng.uniform(-3, 3, (300, 3)), rng.uniform(0, 1, (100, 3)),
+                          [[0.5 / 32, 0.5 / 32, 0.5 / 32], [31.5 / 32, 1.5 / 32, -0.5 / 32], [0.0, 1.0, 2.0],
+                           [-1e-9, 1e-9, 33.0]]]).astype(np.float32)
+    o32, o64 = O.noise_sample(seed, pts)
+    for p, a, b in zip(pts, o32, o64):
+        want = _gl_repeat_trilinear(m, p.astype(np.float64))
+        assert np.abs(b - want).max() < 1e-12
+        assert np.abs(a - want).max() < 1e-5   # FP32 weights: u = 32 s - 0.5 holds ~2^-17 of fraction at |s| = 3
+    # texel centres return the texel; wrap-around joins texel 31 to texel 0
+    c = O.noise_sample(seed, np.array([[3.5 / 32, 7.5 / 32, 30.5 / 32]], np.float32))[0][0]
+    assert np.array_equal(c, np.maximum(m[30, 7, 3].astype(np.float32) / np.float32(127), -1))
+    edge = O.noise_sample(seed, np.array([[0.0, 0.5 / 32, 0.5 / 32]], np.float32))[1][0]
+    t = np.maximum(m.astype(np.float64) / 127.0, -1.0)
+    assert np.abs(edge - 0.5 * (t[0, 0, 31] + t[0, 0, 0])).max() < 1e-12
+
+
+def _noise_fragment_f64(m, P, fp, c, r, tu, tv, ray):
+    """res/conetrace_frag.glsl:136-172 restated in FP64 from the GLSL text."""
+    fp, c, ray = (np.asarray(v, np.float64) for v in (fp, c, ray))
+    delta = fp - c
+    A = ray @ ray
+    B = 2 * (delta @ ray)
+    Cc = delta @ delta - r * r
+    disc = B * B - 4.0 * A * Cc
+    if disc < 0.01:
+        return None
+    sq = math.sqrt(disc)
+    tnear, tfar = (-B - sq) / (2 * A), (-B + sq) / (2 * A)
+    wn, wf = fp + ray * tnear, fp + ray * tfar
+    unit = (wn - c) / r
+    ltn, ltf = wn / P.adjust_size, wf / P.adjust_size
+    it = np.linalg.norm(ltf - ltn) / P.step_size
+    it = min(it, P.max_steps - P.min_steps) + P.min_steps
+    cur = ltn.copy()
+    dt = (ltf - ltn) / (it - 1)
+    ro = rl = 0.0
+    i = 0
+    while i < it:
+        nv = np.zeros(4)
+        freq = pers = 1.0
+        for o in range(P.octaves):
+            off = P.offsets[o] if o < 3 else 0.0
+            nv += pers * _gl_repeat_trilinear(m, (cur + off) * freq)
+            freq *= P.freq_step
+            pers *= P.pers_step
+        nv[3] = abs(nv[3])
+        nxyz = nv[:3] + unit / np.linalg.norm(unit)
+        ro += nv[3] * (1 - unit @ unit)
+        rl += min(max(nxyz[1] * 0.5 + 0.5, 0.0), 1.0)
+        cur += dt
+        unit += dt
+        i += 1
+    col = P.min_color + P.color_scale * rl / (it - 1)
+    alpha = 1 - math.hypot((tu + 1) / 2 - 0.5, (tv + 1) / 2 - 0.5) * 2
+    return col, min(max(ro * P.opacity / (it - 1), 0.0), 1.0) * alpha, i
+
+
+def _march(**kw):
+    from types import SimpleNamespace
+    base = dict(seed=5, step_size=0.01, opacity=4.0, octaves=4, freq_step=3.0, pers_step=0.5, adjust_size=40.0,
+                min_steps=2, max_steps=8, min_color=0.2, color_scale=0.45, offsets=(0.0, 0.0, 0.0))
+    base.update(kw)
+    return SimpleNamespace(**base)
+
+
+@pytest.mark.parametrize("kw", [{}, dict(offsets=(0.013, -0.02, 0.005), octaves=5, min_steps=1, max_steps=12,
+                                         adjust_size=0.7, step_size=0.05)])
+def test_oracle_noise_fragment(kw):
+    P = _march(**kw)
+    m = O.noise_map(P.seed)
+    rng = np.random.default_rng(4)
+    kept = dropped = 0
+    for _ in range(40):
+        c = rng.uniform(-0.6, 0.6, 3).astype(np.float32)
+        r = np.float32(rng.uniform(0.05, 0.3))
+        ray = rng.normal(size=3)
+        ray = (ray / np.linalg.norm(ray)).astype(np.float32)
+        # a point of the billboard plane through c, perpendicular to the ray
+        e1 = np.cross(ray, [0.0, 0.0, 1.0] if abs(ray[2]) < 0.9 else [1.0, 0.0, 0.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(ray, e1)
+        tu, tv = rng.uniform(-1, 1, 2).astype(np.float32)
+        fp = (c + r * tu * e1 + r * tv * e2).astype(np.float32)
+        col, alpha, steps, ok = O.noise_fragment(P, fp, c, float(r), float(tu), float(tv), ray)
+        want = _noise_fragment_f64(m, P, fp, c, float(r), float(tu), float(tv), ray)
+        if want is None or not ok:
+            # the discard decision (disc < 0.01) agrees away from its boundary
+            assert want is None and not ok or abs(4 * (float(r) ** 2 - float(tu * tu + tv * tv) * float(r) ** 2)
+                                                   - 0.01) < 1e-4
+            dropped += 1
+            continue
+        kept += 1
+        assert steps == want[2]
+        assert abs(col - want[0]) < 1e-4 and abs(alpha - want[1]) < 1e-4, (col, alpha, want)
+        assert 0.0 <= alpha <= 1.0 and col >= P.min_color
+    assert kept >= 10 and dropped >= 3
