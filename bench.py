@@ -298,6 +298,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ref-mode", action="store_true", help="skip the reference-semantics frame")
     ap.add_argument("--no-fp16", action="store_true", help="skip the 512^3 RGBA16F workload")
+    ap.add_argument("--no-noise-march", action="store_true", help="skip the reference frame with the noise march")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned far-field frames")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-scene batched workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -372,6 +373,18 @@ def main() -> int:
                            "frame_ms": round(rr["secs"] * 1e3 / max(4, args.steps // 4), 4),
                            "stage_ms": {k: round(v, 4) for k, v in rr["stage_ms"].items()},
                            "counters_per_frame": {k: int(v) for k, v in rr["counters"].items()}}
+    if not args.no_noise_march:
+        from dataclasses import replace
+        nmsc = replace(cv.scenes.reference_mode(scene), noise_march=cv.scenes.NoiseMarch(seed=args.seed))
+        kn = max(4, args.steps // 4)
+        rn = run_gpu(cv, nmsc, kn, 2, ctx, "plain256/ref+noise")
+        out["ref_noise_march"] = {"desc": "the reference's default look: shipped-shader semantics (as ref_mode) with "
+                                          "the soft-particle noise march in front of every cone (4 octaves, 2..8 "
+                                          "steps, 32^3 noise map)",
+                                  "mrays_per_s": round(W * H * kn / rn["secs"] / 1e6, 2),
+                                  "frame_ms": round(rn["secs"] * 1e3 / kn, 4),
+                                  "stage_ms": {k: round(v, 4) for k, v in rn["stage_ms"].items()},
+                                  "counters_per_frame": {k: int(v) for k, v in rn["counters"].items()}}
     if not args.no_fp16:
         f16 = cv.scenes.fp16_512(args.seed)
         k = max(4, args.steps // 4)
