@@ -28,7 +28,7 @@ from . import _abi
 from ._abi import (CR_COMPOSITE_FTB, CR_COMPOSITE_GL_BLEND, CR_CONE_FTB, CR_CONE_FTB_DENSITY,  # noqa: F401
                    CR_CONE_WEIGHTED, CR_FORMAT_R8, CR_FORMAT_RGBA16F, CR_FORMAT_RGBA32F, CR_MODE_REF,
                    CR_MODE_RGBA, CR_PIXEL_RGBA8, CR_PIXEL_RGBA32F, CloudVoxError, cr_camera, cr_cone_params,
-                   cr_light, cr_noise_march, cr_stats)
+                   cr_light, cr_noise_march, cr_stats, cr_sun_sprite)
 
 __all__ = ["Context", "CloudVolume", "Sun", "Camera", "VoxelizeShader", "ConeTraceShader", "FarFieldMLP", "Batch",
            "CloudVoxError", "lib", "scenes"]
@@ -168,6 +168,14 @@ class CloudVolume:
         """Switch the soft-particle noise march of coneTrace on (nm.enabled) or off (None)."""
         _check(lib().cr_volume_set_noise_march(self.handle, C.byref(nm) if nm is not None else None),
                self.ctx.handle)
+
+    def set_sun_sprite(self, inner_color=(1.0, 1.0, 1.0), outer_color=(1.0, 1.0, 0.0), inner_radius=1.0,
+                       outer_radius=2.0, enabled=True) -> None:
+        """SunShader::render before the clouds (Sun::innerColor ... defaults, src/main.cpp:40-43);
+        enabled=False switches it off."""
+        sp = cr_sun_sprite(1 if enabled else 0, (C.c_float * 3)(*map(float, inner_color)),
+                           (C.c_float * 3)(*map(float, outer_color)), float(inner_radius), float(outer_radius))
+        _check(lib().cr_volume_set_sun_sprite(self.handle, C.byref(sp)), self.ctx.handle)
 
     def read_noise_map(self) -> np.ndarray:
         """The noise map bytes, (32, 32, 32, 4) int8 indexed [z, y, x, (r, g, b, a)]."""
@@ -570,6 +578,9 @@ def scene_objects(scene, camera_index: int = 0, context: Optional[Context] = Non
                          alphaStop=k.alpha_stop, sigma=k.sigma, composite=k.composite,
                          quantizeRGBA8=bool(k.quantize_rgba8), compensated=bool(k.compensated),
                          clearColor=k.clear_color)
+    if scene.sun_sprite is not None:
+        q = scene.sun_sprite
+        vol.set_sun_sprite(q.inner_color, q.outer_color, q.inner_radius, q.outer_radius)
     if scene.noise_march is not None:
         m = scene.noise_march
         tr.doNoiseSample = True

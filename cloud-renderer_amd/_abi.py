@@ -62,6 +62,11 @@ class cr_noise_march(C.Structure):
                 ("min_color", C.c_float), ("color_scale", C.c_float), ("offsets", C.c_float * 3)]
 
 
+class cr_sun_sprite(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("inner_color", C.c_float * 3), ("outer_color", C.c_float * 3),
+                ("inner_radius", C.c_float), ("outer_radius", C.c_float)]
+
+
 class cr_stats(C.Structure):
     _fields_ = [("light_fragments", C.c_int64), ("shell_texels", C.c_int64), ("fragments", C.c_int64),
                 ("samples", C.c_int64), ("two_level", C.c_int64),
@@ -101,6 +106,7 @@ SIGNATURES = {
     "cr_volume_set_noise": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "cr_volume_set_noise_march": (C.c_int, [C.c_void_p, C.POINTER(cr_noise_march)]),
     "cr_volume_read_noise_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "cr_volume_set_sun_sprite": (C.c_int, [C.c_void_p, C.POINTER(cr_sun_sprite)]),
     "cr_volume_set_billboards": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                            C.c_float]),
     "cr_voxelize": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.c_int, C.c_int, C.c_void_p]),

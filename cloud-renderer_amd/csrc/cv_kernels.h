@@ -109,6 +109,11 @@ struct ConeDesc {
     float nm_step, nm_opacity, nm_freq_step, nm_pers_step, nm_adjust, nm_min_color, nm_color_scale;
     float nm_off[3];
     float nm_ray[3];
+    // sun sprite (GL blend composite): view-space centre, colours, radii
+    int sun_on;
+    float sun_v[3];
+    float sun_in[3], sun_out[3];
+    float sun_r0, sun_r1;
 };
 
 // pass-1 work list: per-billboard pixel box and tile count, exclusive scan

@@ -2327,6 +2327,32 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
             C0 = (float)q8(C0) / 255.0f; C1 = (float)q8(C1) / 255.0f;
             C2 = (float)q8(C2) / 255.0f; C3 = (float)q8(C3) / 255.0f;
         }
+        if (cd.sun_on && valid) {
+            // the sun sprite, blended over the clear colour before the clouds
+            // (res/sun_frag.glsl:14-28): fragment at the pixel ray's hit with
+            // the sprite's view-aligned plane, dist = distance(center, fragPos)
+            const float dz = -cd.sun_v[2];
+            const float du = rx * dz - cd.sun_v[0], dv = ry * dz - cd.sun_v[1];
+            if (fabsf(du) <= cd.sun_r1 && fabsf(dv) <= cd.sun_r1) {
+                const float dist = sqrtf(du * du + dv * dv);
+                float q0, q1, q2, q3;
+                bool keep = true;
+                if (dist < cd.sun_r0) {
+                    q0 = cd.sun_in[0]; q1 = cd.sun_in[1]; q2 = cd.sun_in[2]; q3 = 1.0f;
+                } else {
+                    const float sc = (dist - cd.sun_r0) / (cd.sun_r1 - cd.sun_r0);
+                    keep = !(sc > 0.99f);
+                    q0 = cd.sun_out[0] * sc + cd.sun_in[0] * (1.0f - sc);
+                    q1 = cd.sun_out[1] * sc + cd.sun_in[1] * (1.0f - sc);
+                    q2 = cd.sun_out[2] * sc + cd.sun_in[2] * (1.0f - sc);
+                    q3 = 1.0f - sc;
+                }
+                if (keep) {
+                    bool unused = false;
+                    blend_frag<0>(cd, clamp01(q0), clamp01(q1), clamp01(q2), clamp01(q3), C0, C1, C2, C3, unused);
+                }
+            }
+        }
     } else {
         C0 = C1 = C2 = C3 = 0.0f;
     }

@@ -70,6 +70,9 @@ struct cr_volume_s {
     cr_noise_march nm{};
     std::vector<int8_t> nm_bytes;
     float4* d_nmap = nullptr;
+    // sun sprite drawn before the clouds (GL blend composite)
+    bool sun_on = false;
+    cr_sun_sprite sun{};
     float blo[3] = {}, bhi[3] = {}, pos[3] = {};
     VolumeDesc vd{};
     GridDesc g{};
@@ -685,6 +688,20 @@ int cr_volume_set_noise_march(cr_volume v, const cr_noise_march* nm) {
     return CR_OK;
 }
 
+int cr_volume_set_sun_sprite(cr_volume v, const cr_sun_sprite* sp) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_set_sun_sprite: vol is NULL");
+    if (!sp || !sp->enabled) {
+        v->sun_on = false;
+        return CR_OK;
+    }
+    if (!finite3(sp->inner_color) || !finite3(sp->outer_color) || !std::isfinite(sp->inner_radius) ||
+        !std::isfinite(sp->outer_radius) || !(sp->inner_radius >= 0.0f) || !(sp->outer_radius > sp->inner_radius))
+        return fail(v->ctx, CR_EINVAL, "cr_volume_set_sun_sprite: need finite colours and 0 <= inner < outer radius");
+    v->sun = *sp;
+    v->sun_on = true;
+    return CR_OK;
+}
+
 int cr_volume_read_noise_map(cr_volume v, int8_t* host_dst, size_t bytes) {
     if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_read_noise_map: vol is NULL");
     if (!host_dst) return fail(v->ctx, CR_EINVAL, "cr_volume_read_noise_map: NULL destination");
@@ -1048,6 +1065,8 @@ static int cone_prepare(cr_volume v, const cr_light* light, const cr_camera* cam
     if (!v) return fail(nullptr, CR_EINVAL, "cr_cone_trace: vol is NULL");
     cr_context ctx = v->ctx;
     if (!light || !cam || !p || !d_image) return fail(ctx, CR_EINVAL, "cr_cone_trace: NULL argument");
+    if (v->sun_on && p->composite != CR_COMPOSITE_GL_BLEND)
+        return fail(ctx, CR_EUNSUPPORTED, "cr_cone_trace: the sun sprite needs the GL-blend composite");
     if (W <= 0 || H <= 0 || W > 16384 || H > 16384) return fail(ctx, CR_EINVAL, "cr_cone_trace: image size must be in [1, 16384]");
     if (pixfmt != CR_PIXEL_RGBA32F && pixfmt != CR_PIXEL_RGBA8) return fail(ctx, CR_EINVAL, "cr_cone_trace: bad pixel format");
     size_t px_bytes = pixfmt == CR_PIXEL_RGBA32F ? 16 : 4;
@@ -1227,6 +1246,24 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     cl.tile_order = tq.order;
     cl.learned = lo ? 1 : 0;
     if (lo) cl.mlp = mlp_dev(lo->mlp);
+    if (v->sun_on) {
+        // SunShader::render before the clouds: centre Sun::position, clipped
+        // like any primitive by the camera's depth range
+        const float* Vm = cl.cv.V;
+        const float px = light->position[0], py = light->position[1], pz = light->position[2];
+        const float vx = ((Vm[0] * px + Vm[4] * py) + Vm[8] * pz) + Vm[12];
+        const float vy = ((Vm[1] * px + Vm[5] * py) + Vm[9] * pz) + Vm[13];
+        const float vz = ((Vm[2] * px + Vm[6] * py) + Vm[10] * pz) + Vm[14];
+        const float dz = -vz;
+        if (dz >= cam->znear && dz <= cam->zfar) {
+            ConeDesc& cd = cl.cd;
+            cd.sun_on = 1;
+            cd.sun_v[0] = vx; cd.sun_v[1] = vy; cd.sun_v[2] = vz;
+            for (int a = 0; a < 3; ++a) { cd.sun_in[a] = v->sun.inner_color[a]; cd.sun_out[a] = v->sun.outer_color[a]; }
+            cd.sun_r0 = v->sun.inner_radius;
+            cd.sun_r1 = v->sun.outer_radius;
+        }
+    }
     if (v->nm_on) {
         ConeDesc& cd = cl.cd;
         cd.nm_octaves = v->nm.octaves;
@@ -1747,7 +1784,8 @@ int cr_batch_create(cr_context ctx, const cr_batch_item* items, int count, cr_ba
         const cr_batch_item& it = items[i];
         if (!it.volume || it.volume->ctx != ctx) return fail(ctx, CR_EINVAL, "cr_batch_create: volume of another context or NULL");
         if (it.volume->format == CR_FORMAT_R8) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: R8 volumes are not batched");
-        if (it.volume->nm_on) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: the noise march is not batched");
+        if (it.volume->nm_on || it.volume->sun_on)
+            return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: the noise march and the sun sprite are not batched");
         if (it.volume->format != items[0].volume->format || it.params.cone_mode != items[0].params.cone_mode ||
             it.params.composite != items[0].params.composite)
             return fail(ctx, CR_EUNSUPPORTED, "cr_batch_create: scenes must share format, cone mode and composite");
@@ -1815,7 +1853,8 @@ int cr_batch_run(cr_batch b, void* stream) {
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     for (const auto& it : b->items)
-        if (it.volume->nm_on) return fail(ctx, CR_EUNSUPPORTED, "cr_batch_run: the noise march is not batched");
+        if (it.volume->nm_on || it.volume->sun_on)
+            return fail(ctx, CR_EUNSUPPORTED, "cr_batch_run: the noise march and the sun sprite are not batched");
     // the scenes' buffers are shared with their single cone traces: wait for
     // any camera pass still running on a volume's camera-side stream
     for (const auto& it : b->items) {

@@ -79,6 +79,15 @@ class NoiseMarch:
     offsets: tuple = (0.0, 0.0, 0.0)
 
 
+@dataclass(frozen=True)
+class SunSprite:
+    """Sun sprite drawn before the clouds (src/main.cpp:40-43 defaults)."""
+    inner_color: tuple = (1.0, 1.0, 1.0)
+    outer_color: tuple = (1.0, 1.0, 0.0)
+    inner_radius: float = 1.0
+    outer_radius: float = 2.0
+
+
 @dataclass
 class Scene:
     name: str
@@ -101,6 +110,7 @@ class Scene:
     fluffiness: float = 1.0
     noise: Optional[Noise] = None
     noise_march: Optional[NoiseMarch] = None
+    sun_sprite: Optional[SunSprite] = None
     directional: int = 1
     seed: int = 0
     meta: dict = field(default_factory=dict)

@@ -188,6 +188,22 @@ int cr_volume_set_noise_march(cr_volume vol, const cr_noise_march *nm);
 /* The noise map bytes of the last cr_volume_set_noise_march that switched it
  * on: 32^3 texels of (r, g, b, a), x fastest (bytes >= 131072). */
 int cr_volume_read_noise_map(cr_volume vol, int8_t *host_dst, size_t bytes);
+/* Sun sprite (SunShader::render, src/Shaders/SunShader.cpp:7-42,
+ * res/sun_frag.glsl:14-28; defaults src/main.cpp:40-43 in brackets): a
+ * camera-facing quad of half-size outer_radius at the light position, drawn
+ * after the clear and before the clouds (src/main.cpp:112-116) with the GL
+ * blend; inside inner_radius inner_color at alpha 1, then a ring blending to
+ * outer_color with alpha 1 - scale, discarded where scale > 0.99.  GL-blend
+ * composite only (cr_cone_trace refuses it with the front-to-back composite);
+ * enabled 0 or NULL switches it off. */
+typedef struct {
+    int enabled;
+    float inner_color[3]; /* [1, 1, 1] */
+    float outer_color[3]; /* [1, 1, 0] */
+    float inner_radius;   /* [1], >= 0 */
+    float outer_radius;   /* [2], > inner_radius */
+} cr_sun_sprite;
+int cr_volume_set_sun_sprite(cr_volume vol, const cr_sun_sprite *sprite);
 /* CloudVolume::uploadBillboards (src/CloudVolume.cpp:139-164): billboard
  * centres relative to position (xyz, 3*count floats) and scales; the radius
  * used is scale * fluffiness.  count = 0 is allowed (nothing is drawn). */

@@ -101,6 +101,15 @@ typedef struct {
     float offsets[3];     // the octaveOffsets uniform (windVel * runTime)
 } cvo_noise_march;
 
+// sun sprite (res/sun_frag.glsl uniforms, defaults src/main.cpp:40-43)
+typedef struct {
+    int enabled;
+    float inner_color[3];
+    float outer_color[3];
+    float inner_radius;
+    float outer_radius;
+} cvo_sun_sprite;
+
 typedef struct {
     float V[16];          // glm column-major view matrix
     float near_plane[3];  // Sun::nearPlane
@@ -1181,7 +1190,8 @@ extern "C" {
 static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
                            const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
                            const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1,
-                           int row_step, float* img32, double* img64, cvo_stats* stats, const cvo_noise_march* nm) {
+                           int row_step, float* img32, double* img64, cvo_stats* stats, const cvo_noise_march* nm,
+                           const cvo_sun_sprite* sun = nullptr) {
     if (!vol || W <= 0 || H <= 0 || !cp || cp->steps < 0 || row_step < 1) return -1;
     const bool use_nm = nm && nm->enabled;
     std::vector<int8_t> nmap(use_nm ? kNoiseDim * kNoiseDim * kNoiseDim * 4 : 0);
@@ -1240,6 +1250,73 @@ static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, con
             }
             C32[4 * p + q] = init;
         }
+
+    // SunShader::render (src/Shaders/SunShader.cpp:7-42) after the clear and
+    // before the clouds (src/main.cpp:112-116), GL blend composite only:
+    // fragment at the pixel ray's hit with the view-aligned quad of half-size
+    // outerRadius centred on Sun::position, dist = distance(center, fragPos)
+    // (res/sun_frag.glsl:14-28), clipped by the camera's depth range.
+    if (sun && sun->enabled && cp->composite == 0) {
+        V3 sp = v3(light->position[0], light->position[1], light->position[2]);
+        V3 sv = view_point(cs.V, sp);
+        float dz = -sv.z;
+        double svx = ((cs.Vd[0] * sp.x + cs.Vd[4] * sp.y) + cs.Vd[8] * sp.z) + cs.Vd[12];
+        double svy = ((cs.Vd[1] * sp.x + cs.Vd[5] * sp.y) + cs.Vd[9] * sp.z) + cs.Vd[13];
+        double dzd = -(((cs.Vd[2] * sp.x + cs.Vd[6] * sp.y) + cs.Vd[10] * sp.z) + cs.Vd[14]);
+        const float r0 = sun->inner_radius, r1 = sun->outer_radius;
+        if (dz >= cam->znear && dz <= cam->zfar) {
+            for (int py = std::max(row0, 0); py < std::min(row1, H); ++py) {
+                float ry = pixel_ndc(py, H) * cs.ky;
+                double ryd = ((2.0 * ((double)py + 0.5)) / (double)H - 1.0) * cs.kyd;
+                for (int px = 0; px < W; ++px) {
+                    float rx = pixel_ndc(px, W) * cs.kx;
+                    float du = rx * dz - sv.x, dv = ry * dz - sv.y;
+                    if (!(std::fabs(du) <= r1 && std::fabs(dv) <= r1)) continue;
+                    float dist = std::sqrt(du * du + dv * dv);
+                    float q[4];
+                    if (dist < r0) {
+                        for (int c = 0; c < 3; ++c) q[c] = sun->inner_color[c];
+                        q[3] = 1.0f;
+                    } else {
+                        float sc = (dist - r0) / (r1 - r0);
+                        if (sc > 0.99f) continue;
+                        for (int c = 0; c < 3; ++c) q[c] = sun->outer_color[c] * sc + sun->inner_color[c] * (1.0f - sc);
+                        q[3] = 1.0f - sc;
+                    }
+                    size_t pix = (size_t)py * W + px;
+                    float a = sat(q[3]);
+                    for (int c = 0; c < 4; ++c) {
+                        float dst = cp->quantize_rgba8 ? (float)fb8[4 * pix + c] / 255.0f : C32[4 * pix + c];
+                        float res = sat(q[c]) * a + dst * (1.0f - a);
+                        if (cp->quantize_rgba8) {
+                            fb8[4 * pix + c] = q8(res);
+                            C32[4 * pix + c] = (float)fb8[4 * pix + c] / 255.0f;
+                        } else {
+                            C32[4 * pix + c] = res;
+                        }
+                    }
+                    if (dbl) {
+                        // FP64 replay of the same (FP32-decided) fragment
+                        double rxd = ((2.0 * ((double)px + 0.5)) / (double)W - 1.0) * cs.kxd;
+                        double dud = rxd * dzd - svx, dvd = ryd * dzd - svy;
+                        double dd = std::sqrt(dud * dud + dvd * dvd);
+                        double qd[4];
+                        if (dist < r0) {
+                            for (int c = 0; c < 3; ++c) qd[c] = sun->inner_color[c];
+                            qd[3] = 1.0;
+                        } else {
+                            double sc = (dd - r0) / ((double)r1 - r0);
+                            for (int c = 0; c < 3; ++c)
+                                qd[c] = (double)sun->outer_color[c] * sc + (double)sun->inner_color[c] * (1.0 - sc);
+                            qd[3] = 1.0 - sc;
+                        }
+                        double ad = sat(qd[3]);
+                        for (int c = 0; c < 4; ++c) C64[4 * pix + c] = sat(qd[c]) * ad + C64[4 * pix + c] * (1.0 - ad);
+                    }
+                }
+            }
+        }
+    }
 
     // per-sphere setup in draw order
     struct Sph { V3 c, cv; float s; PixBox box; bool live; };
@@ -1415,6 +1492,16 @@ int cvo_cone_trace(const cvo_volume* vol, const void* levels_native, const doubl
 // (:136-174, :194-196) in front of the cone: a fragment the noise branch
 // discards is dropped; otherwise its colour is (col * indirect rgb,
 // runningOpacity * alpha), clamped to [0, 1].
+// cvo_cone_trace_nm with the sun sprite drawn first (sun may be NULL)
+int cvo_cone_trace_ex(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
+                      const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
+                      const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1, int row_step,
+                      float* img32, double* img64, cvo_stats* stats, const cvo_noise_march* nm,
+                      const cvo_sun_sprite* sun) {
+    return cone_trace_impl(vol, levels_native, levels_f64, S, xyz, radius, order, cam, light, cp, W, H, row0, row1,
+                           row_step, img32, img64, stats, nm, sun);
+}
+
 int cvo_cone_trace_nm(const cvo_volume* vol, const void* levels_native, const double* levels_f64, int S,
                       const float* xyz, const float* radius, const int32_t* order, const cvo_camera* cam,
                       const cvo_light* light, const cvo_cone* cp, int W, int H, int row0, int row1, int row_step,

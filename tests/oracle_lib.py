@@ -46,6 +46,11 @@ def noise_march_of(m) -> cvo_noise_march:
                            m.color_scale, (C.c_float * 3)(*[float(o) for o in m.offsets]))
 
 
+class cvo_sun_sprite(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("inner_color", C.c_float * 3), ("outer_color", C.c_float * 3),
+                ("inner_radius", C.c_float), ("outer_radius", C.c_float)]
+
+
 class cvo_cone(C.Structure):
     _fields_ = [("steps", C.c_int), ("cone_angle", C.c_float), ("initial_height", C.c_float),
                 ("lod_offset", C.c_float), ("down_scaling", C.c_float), ("cone_mode", C.c_int),
@@ -94,6 +99,8 @@ def lib():
         L.cvo_cone_trace.restype = C.c_int
         L.cvo_cone_trace_nm.argtypes = L.cvo_cone_trace.argtypes + [P(cvo_noise_march)]
         L.cvo_cone_trace_nm.restype = C.c_int
+        L.cvo_cone_trace_ex.argtypes = L.cvo_cone_trace_nm.argtypes + [P(cvo_sun_sprite)]
+        L.cvo_cone_trace_ex.restype = C.c_int
         L.cvo_noise_map.argtypes = [C.c_uint32, vp]
         L.cvo_noise_sample.argtypes = [C.c_uint32, vp, C.c_int64, vp, vp]
         L.cvo_noise_fragment.argtypes = [P(cvo_noise_march), vp, vp]
@@ -256,9 +263,13 @@ def cone_trace(scene, nat, f64, order, cam, row0=0, row1=None, want_f64=True, ro
             C.byref(k), C.byref(L), C.byref(cp), W, H, int(row0), int(row1), int(row_step), ptr(img32), ptr(img64),
             C.byref(st))
     nm = getattr(scene, "noise_march", None)
-    if nm is not None:
-        m = noise_march_of(nm)
-        rc = lib().cvo_cone_trace_nm(*args, C.byref(m))
+    sun = getattr(scene, "sun_sprite", None)
+    if nm is not None or sun is not None:
+        m = noise_march_of(nm) if nm is not None else None
+        q = (cvo_sun_sprite(1, (C.c_float * 3)(*sun.inner_color), (C.c_float * 3)(*sun.outer_color),
+                            sun.inner_radius, sun.outer_radius) if sun is not None else None)
+        rc = lib().cvo_cone_trace_ex(*args, C.byref(m) if m is not None else None,
+                                     C.byref(q) if q is not None else None)
     else:
         rc = lib().cvo_cone_trace(*args)
     assert rc == 0

@@ -47,6 +47,16 @@ def _scenes(cv):
                                            noise_march=sc.NoiseMarch(seed=9, octaves=5, min_steps=1, max_steps=12,
                                                                      offsets=(0.013, -0.02, 0.005), adjust_size=2.0,
                                                                      step_size=0.02)),
+        # sun sprite before the clouds: sun behind the volume, in view; the
+        # default radii, then a wide ring over float blending with the march
+        "small32_ref_sun": replace(sc.reference_mode(base), sun_dir=_unit3((0.15, 0.25, -1.0)),
+                                   sun_sprite=sc.SunSprite()),
+        "small32_blend_sun_march": replace(base, sun_dir=_unit3((-0.2, 0.1, -1.0)),
+                                           cone=replace(base.cone, composite=0, quantize_rgba8=0),
+                                           sun_sprite=sc.SunSprite(inner_color=(0.9, 0.6, 0.3),
+                                                                   outer_color=(0.1, 0.2, 1.0), inner_radius=0.7,
+                                                                   outer_radius=3.5),
+                                           noise_march=sc.NoiseMarch(seed=4)),
         "small32_comp_blend": replace(base, cone=replace(base.cone, composite=0)),
         "small32_rect": replace(base, W=200, H=120, light_w=130, light_h=210),
         "npot24": replace(base, N=24, levels=5),
@@ -55,6 +65,11 @@ def _scenes(cv):
         "ties_ref": sc.reference_mode(tie_scene(sc)),
     }
     return cases
+
+
+def _unit3(v):
+    v = np.asarray(v, np.float64)
+    return (v / np.linalg.norm(v)).astype(np.float32)
 
 
 def tie_scene(sc):
@@ -81,7 +96,7 @@ def tie_scene(sc):
 
 CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_rgba32f", "small32_ref_float_blend",
          "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_ref_noise_march",
-         "small32_ftb_noise_march", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
+         "small32_ftb_noise_march", "small32_ref_sun", "small32_blend_sun_march", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
 
 
 @pytest.mark.parametrize("case", CASES)
