@@ -730,3 +730,62 @@ def test_oracle_noise_fragment(kw):
         assert abs(col - want[0]) < 1e-4 and abs(alpha - want[1]) < 1e-4, (col, alpha, want)
         assert 0.0 <= alpha <= 1.0 and col >= P.min_color
     assert kept >= 10 and dropped >= 3
+
+
+# ---------------------------------------------------------------------------
+# sun sprite (res/sun_frag.glsl:14-28, src/Shaders/SunShader.cpp:7-42)
+# ---------------------------------------------------------------------------
+def test_oracle_sun_sprite(cv):
+    """Empty scene, float GL blend: the image is the sprite over the clear
+    colour, restated in FP64 from the GLSL text (glm::lookAt + perspective,
+    pixel ray hit with the sprite's view-aligned plane)."""
+    from dataclasses import replace
+    from gpu_common import run_oracle
+    sc = cv.scenes
+    sun_dir = np.array([0.3, -0.2, -1.0]) / np.linalg.norm([0.3, -0.2, -1.0])
+    sp = sc.SunSprite(inner_color=(0.9, 0.5, 0.1), outer_color=(0.2, 0.4, 1.0), inner_radius=0.8, outer_radius=2.5)
+    base = sc.small32(0)
+    scene = replace(sc.reference_mode(base), xyz=np.zeros((0, 3), np.float32), scales=np.zeros(0, np.float32),
+                    sun_dir=sun_dir.astype(np.float32), sun_sprite=sp, W=96, H=64, light_w=32, light_h=32,
+                    cone=replace(sc.reference_mode(base).cone, quantize_rgba8=0))
+    o = run_oracle(scene)
+    cam = scene.cameras[0]
+    eye, tgt, up = (np.asarray(v, np.float64) for v in (cam.eye, cam.target, cam.up))
+    f = (tgt - eye) / np.linalg.norm(tgt - eye)
+    s = np.cross(f, up); s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    p = np.asarray(scene.sun_pos, np.float64)
+    sv = np.array([s @ (p - eye), u @ (p - eye), -(f @ (p - eye))])     # view space
+    th = math.tan(cam.fovy / 2)
+    clear = np.asarray(scene.cone.clear_color, np.float64)
+    want = np.tile(clear, (scene.H, scene.W, 1))
+    inside = boundary = 0
+    for py in range(scene.H):
+        for px in range(scene.W):
+            rx = ((2 * (px + 0.5)) / scene.W - 1) * th * scene.W / scene.H
+            ry = ((2 * (py + 0.5)) / scene.H - 1) * th
+            dz = -sv[2]
+            du, dv = rx * dz - sv[0], ry * dz - sv[1]
+            if max(abs(du), abs(dv)) > sp.outer_radius:
+                continue
+            dist = math.hypot(du, dv)
+            if dist < sp.inner_radius:
+                q, a = np.asarray(sp.inner_color), 1.0
+            else:
+                k = (dist - sp.inner_radius) / (sp.outer_radius - sp.inner_radius)
+                if abs(k - 0.99) < 1e-4 or abs(dist - sp.inner_radius) < 1e-5:
+                    boundary += 1
+                    want[py, px] = o.img64[py, px]   # FP32 decision at the edge
+                    continue
+                if k > 0.99:
+                    continue
+                q, a = np.asarray(sp.outer_color) * k + np.asarray(sp.inner_color) * (1 - k), 1 - k
+            inside += 1
+            want[py, px, :3] = q * a + clear[:3] * (1 - a)
+            want[py, px, 3] = a * a + clear[3] * (1 - a)
+    assert inside > 200
+    assert np.abs(o.img64 - want).max() < 1e-6
+    assert np.abs(o.img32 - want).max() < 1e-5
+    # the pixel nearest the sprite centre is the inner colour
+    cy, cx = np.unravel_index(np.argmax(np.all(np.isclose(want[..., :3], sp.inner_color), -1)), want.shape[:2])
+    assert np.allclose(o.img32[cy, cx, :3], sp.inner_color)
