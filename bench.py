@@ -10,13 +10,25 @@ cone trace + composite) at the next azimuth.  Inputs are resident in HBM
 before the timed region.
 
 value = W*H*steps / timed seconds / 1e6 (Mrays/s over whole frames).
-The roofline object covers the dominant kernel (the stage with the longest
-event-timed average in the timed frames, where the voxelization of frame f+1
-overlaps the cone trace of frame f and both stretch) with its algorithmic
-bytes per launch (light raster: 16 B per billboard + 8 B per depth key and
-per chord-sample store; cone: 8 texels * 16 B per mip level fetched per
-sample, plus the 16 B/pixel image write) over that average duration.
-The CPU baseline is the oracle's FP32 pass timed on this host.
+
+The roofline object covers the dominant kernel: the one with the longest
+hipEvent span in the timed frames, each span bracketing that kernel alone on
+the stream it runs on (the cone kernel; the light raster; the splat; the
+level build).  The cone kernel reads a volume that stays in the Infinity
+Cache and L2, so what bounds it is vector-instruction issue: achieved =
+VALU wave instructions per launch (SQ_INSTS_VALU, from the committed rocprofv3
+summary profiles/cone_plain256_valu.md) / the live average duration, against
+MI355X's issue peak of 256 CUs x 1 wave64 VALU instruction per cycle x
+2.4 GHz = 614.4 G instructions/s.  Beside it: the cache-free gather model
+(8 texels x 16 B per mip level fetched per sample + 16 B per pixel written),
+which exceeds HBM peak because the volume is cache-resident, and the HBM
+bytes rocprofv3 measured per launch (FETCH_SIZE + WRITE_SIZE, calibrated).
+A kernel without a VALU summary is priced against HBM with its algorithmic
+bytes.
+
+The CPU baseline is the oracle's FP32 pass timed on this host, on all its
+cores and on one.  The same leg uses the oracle as the checker of the
+learned frames (PSNR against its FP64 replay on every 8th image row).
 """
 from __future__ import annotations
 
@@ -32,6 +44,9 @@ from pathlib import Path
 REPO = Path(__file__).resolve().parent
 METRIC = "Mrays/s + frame ms (256³, 1920×1080); learned vs plain at PSNR≥40 dB"
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 4 SIMDs x (one wave64 VALU instruction per 4 cycles per SIMD) x 2.4 GHz
+VALU_PEAK_GINST = 256 * 2.4
+ROW_STEP = 8
 
 
 def _load_pkg():
@@ -102,6 +117,13 @@ def run_gpu(cv, scene, steps: int, warmup: int, ctx, label: str):
     for i in range(warmup):
         frame(i)
     torch.cuda.synchronize()
+    # frame latency: frames one at a time, the device idle between them
+    ns = min(len(cams), max(2, steps // 2))
+    t0 = time.perf_counter()
+    for i in range(ns):
+        frame(i)
+        torch.cuda.synchronize()
+    serial_ms = (time.perf_counter() - t0) * 1e3 / ns
     ctx.set_profiling(2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -122,14 +144,16 @@ def run_gpu(cv, scene, steps: int, warmup: int, ctx, label: str):
     avg = {k: sum(u[k] for u in used) / len(used) for k in used[0]}
     bytes_ = stage_bytes(scene, avg)
     log(f"[bench] {label}: {steps} frames in {secs * 1e3:.1f} ms; stages {json.dumps({k: round(v, 4) for k, v in stage_ms.items()})}")
-    return {"secs": secs, "stage_ms": stage_ms, "counters": avg, "bytes": bytes_}
+    return {"secs": secs, "stage_ms": stage_ms, "counters": avg, "bytes": bytes_, "serial_ms": serial_ms}
 
 
-def learned_entry(cv, ctx, scene, args):
-    """Learned far field (k = 32, committed parameters) on the bench workload:
+def learned_entry(cv, ctx, scene, args, keep):
+    """Learned far field (k = 32, committed parameters) on a 256^3 workload:
     whole frames at every azimuth at two error bounds (2e-3, and no
     fallback), frame time and cone-stage time, PSNR of the images against the
-    plain trace of the same azimuths, fallback rate."""
+    plain trace of the same azimuths, fallback rate.  The azimuth-0 images
+    (learned and plain) are appended to `keep` for the FP64 check of the CPU
+    leg."""
     import torch
     from cloud_renderer_amd import farfield as ff
 
@@ -145,8 +169,27 @@ def learned_entry(cv, ctx, scene, args):
     vox.voxelize(vol, sun, stream=stream)
     for a, cam in enumerate(cams):
         tr.coneTrace(vol, sun, cam, plain[a], W, H, stream=stream)
-    res = {"k": 32, "params": "cloud-renderer_amd/weights/farfield_k32.npz"}
+    torch.cuda.synchronize()
+    # the plain march of this volume, timed like the learned frames
     steps = max(4, args.steps // 2)
+    for i in range(2):
+        vox.voxelize(vol, sun, stream=stream)
+        tr.coneTrace(vol, sun, cams[i % len(cams)], img, W, H, stream=stream)
+    torch.cuda.synchronize()
+    ctx.set_profiling(2)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        vox.voxelize(vol, sun, stream=stream)
+        tr.coneTrace(vol, sun, cams[i % len(cams)], img, W, H, stream=stream)
+    torch.cuda.synchronize()
+    psecs = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.set_profiling(0)
+    pcone = st["ms_cone"] / max(1, st["n_cone"])
+    res = {"k": 32, "params": "cloud-renderer_amd/weights/farfield_k32.npz",
+           "plain": {"mrays_per_s": round(W * H * steps / psecs / 1e6, 2), "frame_ms": round(psecs * 1e3 / steps, 4),
+                     "cone_ms": round(pcone, 4), "cone_mrays_per_s": round(W * H / (pcone * 1e-3) / 1e6, 2)}}
+    kept = {"scene": scene, "plain": plain[0].cpu().numpy(), "learned": {}}
     for name, bound in (("err_bound_2e-3", 2e-3), ("no_fallback", 1e30)):
         # quality and counters per azimuth
         ctx.set_profiling(1)
@@ -157,6 +200,8 @@ def learned_entry(cv, ctx, scene, args):
             rows += st["mlp_rows"]
             falls += st["mlp_fallbacks"]
             se += float(((img - plain[a]).double() ** 2).mean())
+            if a == 0:
+                kept["learned"][name] = img.cpu().numpy()
         ctx.set_profiling(0)
         mse = se / len(cams)
         for i in range(2):
@@ -178,7 +223,8 @@ def learned_entry(cv, ctx, scene, args):
                      "cone_mrays_per_s": round(W * H / (cone_ms * 1e-3) / 1e6, 2),
                      "psnr_vs_plain_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None,
                      "fallback_rate": round(falls / max(1, rows), 4), "mlp_rows_per_frame": rows // len(cams)}
-        log(f"[bench] learned {name}: {json.dumps(res[name])}")
+        log(f"[bench] learned {scene.name} {name}: {json.dumps(res[name])}")
+    keep.append(kept)
     vol.close()
     mlp.close()
     return res
@@ -245,32 +291,90 @@ def batched_entry(cv, ctx, args):
 
 
 def cpu_baseline(cv, scene, budget_s: float):
-    """Oracle FP32 pass (test infrastructure, the checker) timed on the host."""
+    """Oracle FP32 pass (test infrastructure, the checker) timed on the host:
+    voxelize + mips once, then sort + cone trace + composite of whole frames
+    at distinct azimuths within the budget, on every OpenMP thread and then
+    on one thread (one frame)."""
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_lib as O
 
+    def timed(max_frames):
+        t0 = time.perf_counter()
+        vr = O.voxelize(scene)
+        t1 = time.perf_counter()
+        nat, _, _, _ = O.build_levels(scene, vr.shell, vr.fill, want_f64=False)
+        t2 = time.perf_counter()
+        frames, t_cone = 0, 0.0
+        for cam in scene.cameras[:max_frames]:
+            s0 = time.perf_counter()
+            order = O.sort_boards(scene, cam.eye)
+            O.cone_trace(scene, nat, None, order, cam, want_f64=False)
+            t_cone += time.perf_counter() - s0
+            frames += 1
+            if (t2 - t0) + t_cone > budget_s:
+                break
+        return (t1 - t0) + (t2 - t1) + t_cone / frames, frames
+
     threads = int(O.lib().cvo_num_threads())
-    t0 = time.perf_counter()
-    vr = O.voxelize(scene)
-    t1 = time.perf_counter()
-    nat, _, _, _ = O.build_levels(scene, vr.shell, vr.fill, want_f64=False)
-    t2 = time.perf_counter()
-    frames = 0
-    t_cone = 0.0
-    for a, cam in enumerate(scene.cameras):
-        s0 = time.perf_counter()
-        order = O.sort_boards(scene, cam.eye)
-        O.cone_trace(scene, nat, None, order, cam, want_f64=False)
-        t_cone += time.perf_counter() - s0
-        frames += 1
-        if (t2 - t0) + t_cone > budget_s:
-            break
-    per_frame = (t1 - t0) + (t2 - t1) + t_cone / frames
+    per_frame, frames = timed(len(scene.cameras))
+    O.lib().cvo_set_num_threads(1)
+    try:
+        per_frame1, _ = timed(1)
+    finally:
+        O.lib().cvo_set_num_threads(threads)
     npx = scene.W * scene.H
     return {"value": round(npx / per_frame / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP32 pass: voxelize + mips once, then sort + cone trace + composite of "
                       f"{frames} full {scene.W}x{scene.H} frame(s) at distinct azimuths; "
-                      f"{per_frame * 1e3:.0f} ms per frame on {threads} OpenMP threads"}
+                      f"{per_frame * 1e3:.0f} ms per frame on {threads} OpenMP threads",
+            "single_thread": {"value": round(npx / per_frame1 / 1e6, 4), "cores": 1,
+                              "sample": f"the same pass, one frame on one thread: {per_frame1 * 1e3:.0f} ms"}}
+
+
+def fp64_checks(kept):
+    """PSNR and max-abs of the kept learned / plain images against the oracle's
+    FP64 replay (the checker) on every ROW_STEP-th row of azimuth 0."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import numpy as np
+    import oracle_lib as O
+
+    out = {}
+    for k in kept:
+        scene = k["scene"]
+        vr = O.voxelize(scene)
+        nat, f64, _, _ = O.build_levels(scene, vr.shell, vr.fill)
+        cam = scene.cameras[0]
+        order = O.sort_boards(scene, cam.eye)
+        _, i64, _ = O.cone_trace(scene, nat, f64, order, cam, row_step=ROW_STEP)
+        rows = np.arange(0, scene.H, ROW_STEP)
+        ref = i64[rows]
+
+        def score(img):
+            d = img[rows].astype(np.float64) - ref
+            mse = float(np.mean(d * d))
+            return {"psnr_vs_fp64_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None,
+                    "maxabs_vs_fp64": float(np.abs(d).max())}
+
+        ent = {"plain": score(k["plain"])}
+        for name, img in k["learned"].items():
+            ent[name] = score(img)
+        out[scene.name] = ent
+        log(f"[bench] FP64 check {scene.name}: {json.dumps(ent)}")
+    return out
+
+
+def load_profile_value(name: str, key: str):
+    """A `key: <number>` line of a committed rocprofv3 summary under profiles/."""
+    p = REPO / "profiles" / f"{name}.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.strip().startswith(key + ":"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
 
 
 def load_traffic(name: str):
@@ -333,15 +437,33 @@ def main() -> int:
     W, H = scene.W, scene.H
     value = W * H * args.steps / r["secs"] / 1e6
     ms_per_step = r["secs"] * 1e3 / args.steps
-    # dominant kernel
-    dom = max(("light_raster", "splat", "mip", "cone"), key=lambda k: r["stage_ms"][k])
+    # dominant kernel: the longest kernel-only span
+    kernels = {"cone": "k_cone", "light_raster": "k_light_raster", "splat": "k_light_splat",
+               "mip": "k_build_fused + k_mip_tail"}
+    dom = max(kernels, key=lambda k: r["stage_ms"][k])
     dom_ms = r["stage_ms"][dom]
-    achieved = r["bytes"][dom] / (dom_ms * 1e-3) / 1e9
     traffic = load_traffic(f"{dom}_plain256")
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": (traffic or {}).get("bytes_per_launch"),
-                "algorithmic_bytes": int(r["bytes"][dom]), "avg_ms": round(dom_ms, 4)}
+    hbm_alg = r["bytes"][dom] / (dom_ms * 1e-3) / 1e9
+    valu = load_profile_value(f"{dom}_plain256_valu", "valu_insts_per_launch")
+    if valu:
+        achieved = valu / (dom_ms * 1e-3) / 1e9
+        roofline = {"bound": "valu", "kernel": kernels[dom], "achieved": round(achieved, 1), "peak": VALU_PEAK_GINST,
+                    "unit": "Ginst/s", "frac": round(achieved / VALU_PEAK_GINST, 4),
+                    "traffic": (traffic or {}).get("bytes_per_launch"), "avg_ms": round(dom_ms, 4),
+                    "valu_insts_per_launch": int(valu),
+                    "valu_source": f"profiles/{dom}_plain256_valu.md (rocprofv3 --pmc SQ_INSTS_VALU)"}
+    else:
+        roofline = {"bound": "hbm", "kernel": kernels[dom], "achieved": round(hbm_alg, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(hbm_alg / HBM_PEAK_GBS, 4),
+                    "traffic": (traffic or {}).get("bytes_per_launch"), "avg_ms": round(dom_ms, 4)}
+    # the gather model of the survey (cache-free: can exceed HBM peak) and the
+    # HBM bytes rocprofv3 measured, both over the same live duration
+    roofline["hbm_gather_model"] = {"algorithmic_bytes": int(r["bytes"][dom]), "achieved_gbs": round(hbm_alg, 1),
+                                    "frac_of_hbm_peak": round(hbm_alg / HBM_PEAK_GBS, 4)}
+    if traffic:
+        tb = traffic["bytes_per_launch"] / (dom_ms * 1e-3) / 1e9
+        roofline["hbm_measured"] = {"bytes_per_launch": traffic["bytes_per_launch"], "achieved_gbs": round(tb, 1),
+                                    "frac_of_hbm_peak": round(tb / HBM_PEAK_GBS, 4)}
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -359,6 +481,7 @@ def main() -> int:
                                "20 deg cone, <=256 steps, front-to-back alpha 0.99, 16 orbit azimuths",
                    "parallelism": "single GPU"},
         "frame_ms": round(ms_per_step, 4),
+        "serial_frame_ms": round(r["serial_ms"], 4),
         "cone_mrays_per_s": round(W * H / (r["stage_ms"]["cone"] * 1e-3) / 1e6, 2),
         "stage_ms": {k: round(v, 4) for k, v in r["stage_ms"].items()},
         "counters_per_frame": {k: int(v) for k, v in r["counters"].items()},
@@ -398,13 +521,32 @@ def main() -> int:
                            "stage_ms": {kk: round(v, 4) for kk, v in rf["stage_ms"].items()},
                            "counters_per_frame": {kk: int(v) for kk, v in rf["counters"].items()},
                            "cone_gather_gbs": round(rf["bytes"]["cone"] / (rf["stage_ms"]["cone"] * 1e-3) / 1e9, 1)}
+    kept = []
     if not args.no_learned:
-        out["learned"] = learned_entry(cv, ctx, scene, args)
+        out["learned"] = learned_entry(cv, ctx, scene, args, kept)
+        # learned + FP16: the same scenes with the volume stored as RGBA16F and
+        # compensated accumulation, against the plain FP32 frames above
+        from dataclasses import replace
+        s16 = replace(scene, name="plain256_fp16", fmt=2, cone=replace(scene.cone, compensated=1))
+        l16 = learned_entry(cv, ctx, s16, args, kept)
+        for name in ("err_bound_2e-3", "no_fallback"):
+            l16[name]["mrays_ratio_vs_plain_fp32"] = round(l16[name]["mrays_per_s"] / value, 3)
+            l16[name]["cone_ratio_vs_plain_fp32"] = round(l16[name]["cone_mrays_per_s"] / out["cone_mrays_per_s"], 3)
+        out["learned_fp16"] = l16
+        for name in ("err_bound_2e-3", "no_fallback"):
+            out["learned"][name]["mrays_ratio_vs_plain_fp32"] = round(out["learned"][name]["mrays_per_s"] / value, 3)
     if not args.no_batched:
         out["batched64"] = batched_entry(cv, ctx, args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle FP32 pass)")
         out["cpu_baseline"] = cpu_baseline(cv, scene, args.cpu_budget)
+        if kept:
+            chk = fp64_checks(kept)
+            for key, sname in (("learned", scene.name), ("learned_fp16", "plain256_fp16")):
+                if key in out and sname in chk:
+                    out[key]["fp64_check"] = {"rows": f"every {ROW_STEP}th row of azimuth 0", **chk[sname]}
+                    for name in ("err_bound_2e-3", "no_fallback"):
+                        out[key][name]["psnr_vs_fp64_db"] = chk[sname][name]["psnr_vs_fp64_db"]
     print(json.dumps(out), flush=True)
     return 0
 

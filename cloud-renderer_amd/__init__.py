@@ -191,6 +191,12 @@ class CloudVolume:
                self.ctx.handle)
         return int(cnt.value), int(h.value)
 
+    def active_voxels(self, stream=None) -> int:
+        """VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115): level-0 texels with r != 0."""
+        cnt = C.c_uint64()
+        _check(lib().cr_volume_active_voxels(self.handle, C.byref(cnt), _stream_handle(stream)), self.ctx.handle)
+        return int(cnt.value)
+
     def level_dim(self, k: int) -> int:
         return max(1, self.dimension >> k)
 
@@ -251,16 +257,29 @@ class Sun:
 
 @dataclass
 class Camera:
+    """Camera (src/Camera.hpp:25-45, src/Camera.cpp:59-60): V = lookAt(eye,
+    target, up), P = perspective(fovy, aspect, znear, zfar) with glm's
+    arithmetic (fovy in radians as glm takes it, so the reference's 45 is
+    accepted as is; aspect 0 = width / height).  P and V (column-major 4x4,
+    as ConeTraceShader uploads them) replace everything but eye, the sort
+    point, when both are given."""
     eye: tuple = (0.0, 0.0, 3.0)
     target: tuple = (0.0, 0.0, 0.0)
     up: tuple = (0.0, 1.0, 0.0)
     fovy: float = math.radians(60.0)
     znear: float = 0.01
     zfar: float = 2500.0
+    aspect: float = 0.0
+    P: Optional[np.ndarray] = None
+    V: Optional[np.ndarray] = None
 
     def to_c(self) -> cr_camera:
+        use = self.P is not None and self.V is not None
+        Pm = np.asarray(self.P, np.float32).reshape(16) if use else np.zeros(16, np.float32)
+        Vm = np.asarray(self.V, np.float32).reshape(16) if use else np.zeros(16, np.float32)
         return cr_camera(_f3(self.eye), _f3(self.target), _f3(self.up), float(self.fovy), float(self.znear),
-                         float(self.zfar))
+                         float(self.zfar), float(self.aspect), 1 if use else 0,
+                         (C.c_float * 16)(*[float(x) for x in Pm]), (C.c_float * 16)(*[float(x) for x in Vm]))
 
 
 class VoxelizeShader:
@@ -570,7 +589,8 @@ def scene_objects(scene, camera_index: int = 0, context: Optional[Context] = Non
     sun = Sun(position=tuple(float(x) for x in scene.sun_pos), direction=tuple(float(x) for x in scene.sun_dir),
               directional=bool(scene.directional))
     c = scene.cameras[camera_index]
-    cam = Camera(eye=c.eye, target=c.target, up=c.up, fovy=c.fovy, znear=c.znear, zfar=c.zfar)
+    cam = Camera(eye=c.eye, target=c.target, up=c.up, fovy=c.fovy, znear=c.znear, zfar=c.zfar,
+                 aspect=getattr(c, "aspect", 0.0), P=getattr(c, "P", None), V=getattr(c, "V", None))
     vox = VoxelizeShader(scene.light_w, scene.light_h)
     k = scene.cone
     tr = ConeTraceShader(vctSteps=k.steps, vctConeAngle=k.cone_angle, vctConeInitialHeight=k.initial_height,

@@ -45,7 +45,8 @@ class cr_light(C.Structure):
 
 class cr_camera(C.Structure):
     _fields_ = [("eye", F3), ("target", F3), ("up", F3), ("fovy", C.c_float),
-                ("znear", C.c_float), ("zfar", C.c_float)]
+                ("znear", C.c_float), ("zfar", C.c_float), ("aspect", C.c_float), ("use_matrices", C.c_int),
+                ("P", C.c_float * 16), ("V", C.c_float * 16)]
 
 
 class cr_cone_params(C.Structure):
@@ -113,6 +114,7 @@ SIGNATURES = {
     "cr_cone_trace": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_camera), C.POINTER(cr_cone_params),
                                 C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_void_p]),
     "cr_volume_occupancy": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_void_p]),
+    "cr_volume_active_voxels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
     "cr_volume_read_level": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p]),
     "cr_volume_read_occupancy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "cr_volume_read_position_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -73,6 +73,28 @@ def build_library(verbose: bool = False) -> Path:
     return LIB_PATH
 
 
+CPP_FRAME_SRC = REPO_DIR / "tests" / "cpp" / "reference_frame.cpp"
+CPP_FRAME_BIN = PKG_DIR / "_build" / "reference_frame"
+
+
+def cpp_frame_command(out: Path = CPP_FRAME_BIN) -> list:
+    """g++ line of the C++ facade test program (links libcloudvox.so and the HIP runtime)."""
+    rocm = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+    return ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I", str(REPO_DIR / "include"),
+            "-I", str(rocm / "include"), str(CPP_FRAME_SRC), "-o", str(out), "-L", str(PKG_DIR), "-lcloudvox",
+            "-L", str(rocm / "lib"), "-lamdhip64", f"-Wl,-rpath,$ORIGIN/..:{rocm / 'lib'}"]
+
+
+def build_cpp_frame(verbose: bool = False) -> Path:
+    """Compile the reference-frame program that drives the C++ facade (a test of the boundary)."""
+    CPP_FRAME_BIN.parent.mkdir(exist_ok=True)
+    cmd = cpp_frame_command()
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd, REPO_DIR)
+    return CPP_FRAME_BIN
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the CPU oracle (test infrastructure) with the recipe in oracle/Makefile."""
     cmd = ["make", "-s", "-C", str(ORACLE_DIR)]
@@ -85,6 +107,7 @@ def build_oracle(verbose: bool = False) -> Path:
 def build_all(verbose: bool = False) -> None:
     build_library(verbose)
     build_oracle(verbose)
+    build_cpp_frame(verbose)
 
 
 if __name__ == "__main__":

@@ -659,8 +659,11 @@ int cr_volume_set_noise_march(cr_volume v, const cr_noise_march* nm) {
     }
     if (nm->octaves < 1 || nm->octaves > 16)
         return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: octaves must be in [1, 16]");
-    if (nm->min_steps < 1 || nm->max_steps < nm->min_steps || nm->max_steps > 64)
-        return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: need 1 <= min_steps <= max_steps <= 64");
+    // iSteps = min(len / stepSize, max - min) + min and the march divides by
+    // iSteps - 1 (res/conetrace_frag.glsl:154-158): min = max = 1 would divide
+    // by zero, so max_steps >= 2 (then iSteps - 1 > 0 for every kept fragment)
+    if (nm->min_steps < 1 || nm->max_steps < nm->min_steps || nm->max_steps > 64 || nm->max_steps < 2)
+        return fail(ctx, CR_EINVAL, "cr_volume_set_noise_march: need 1 <= min_steps <= max_steps <= 64, max_steps >= 2");
     const float fs[] = {nm->step_size, nm->opacity, nm->freq_step, nm->pers_step, nm->adjust_size, nm->min_color,
                         nm->color_scale, nm->offsets[0], nm->offsets[1], nm->offsets[2]};
     for (float f : fs)
@@ -837,12 +840,14 @@ static void voxelize_enqueue(cr_volume v, const VoxPlan& plan, unsigned long lon
                              {(uint64_t*)fill, (uint64_t)v->nbricks0, 0},
                              {(uint64_t*)counters, counters ? 9u : 0u, 0}};
     launch_fill_multi(segs, 4, ls);
-    if (ev) ev->mark_on(1, ls);
     launch_light_items(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, ls);
     if (ls != vs) {
         (void)hipEventRecord(v->ev_items, ls);
         (void)hipStreamWaitEvent(vs, v->ev_items, 0);
     }
+    // stage spans: [0, 1] clear + pass-1 work list, [1, 2] the raster kernel
+    // alone, [2, 3] the splat kernel alone, [5, 4] the level build
+    if (ev) ev->mark_on(1, vs);
     launch_light_raster(v->g, plan.lv, v->mode, v->d_boards, v->S, plan.ls, v->d_keys, fill, counters, vs);
     if (ev) ev->mark_on(2, vs);
     launch_light_splat(v->g, plan.lv, v->d_boards, v->d_keys, nullptr, shell, counters, vs);
@@ -1057,6 +1062,58 @@ struct ConePlan {
     size_t nmask;
 };
 
+// Camera (src/Camera.cpp:59-60) as ConeTraceShader uploads it
+// (src/Shaders/ConeTraceShader.cpp:64-69): V = lookAt(eye, target, up) and
+// glm::perspective's scales tanf(fovy / 2) * aspect, tanf(fovy / 2), or V and
+// P given as matrices (scales 1 / P[0], 1 / P[5]; near and far from P[10],
+// P[14]).  The billboard basis is V's rotation rows; eye is the sort point.
+static int camera_view(cr_context ctx, const cr_camera* cam, int W, int H, CameraView& cv) {
+    if (!finite3(cam->eye)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite camera");
+    if (cam->use_matrices) {
+        const float* P = cam->P;
+        for (int i = 0; i < 16; ++i)
+            if (!std::isfinite(P[i]) || !std::isfinite(cam->V[i]))
+                return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite camera matrix");
+        static const int zero[] = {1, 2, 3, 4, 6, 7, 8, 9, 12, 13, 15};
+        for (int i : zero)
+            if (P[i] != 0.0f) return fail(ctx, CR_EUNSUPPORTED, "cr_cone_trace: P is not a symmetric perspective projection");
+        if (P[11] != -1.0f || !(P[0] > 0.0f) || !(P[5] > 0.0f))
+            return fail(ctx, CR_EUNSUPPORTED, "cr_cone_trace: P is not a symmetric perspective projection");
+        for (int i = 0; i < 16; ++i) cv.V[i] = cam->V[i];
+        cv.kx = 1.0f / P[0];
+        cv.ky = 1.0f / P[5];
+        cv.znear = P[14] / (P[10] - 1.0f);
+        cv.zfar = P[14] / (P[10] + 1.0f);
+        if (!(cv.znear > 0.0f) || !(cv.zfar > cv.znear) || !std::isfinite(cv.zfar))
+            return fail(ctx, CR_EINVAL, "cr_cone_trace: P gives no 0 < znear < zfar");
+        float r0 = cv.V[0] * cv.V[0] + cv.V[4] * cv.V[4] + cv.V[8] * cv.V[8];
+        float r2 = cv.V[2] * cv.V[2] + cv.V[6] * cv.V[6] + cv.V[10] * cv.V[10];
+        if (!(r0 > 0.0f) || !(r2 > 0.0f)) return fail(ctx, CR_EINVAL, "cr_cone_trace: degenerate view matrix");
+    } else {
+        if (!finite3(cam->target) || !finite3(cam->up)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite camera");
+        float tanHalfFovy = std::tan(cam->fovy / 2.0f);
+        if (!(tanHalfFovy > 0.0f) || !std::isfinite(tanHalfFovy))
+            return fail(ctx, CR_EINVAL, "cr_cone_trace: fovy needs tan(fovy / 2) > 0");
+        if (!(cam->aspect >= 0.0f) || !std::isfinite(cam->aspect))
+            return fail(ctx, CR_EINVAL, "cr_cone_trace: aspect must be >= 0 (0: width / height)");
+        if (!(cam->znear > 0.0f) || !(cam->zfar > cam->znear)) return fail(ctx, CR_EINVAL, "cr_cone_trace: need 0 < znear < zfar");
+        F3 e = f3(cam->eye[0], cam->eye[1], cam->eye[2]), t = f3(cam->target[0], cam->target[1], cam->target[2]);
+        F3 d = t - e, upv = f3(cam->up[0], cam->up[1], cam->up[2]);
+        F3 sd = cross3(d, upv);
+        if (!(dot3(d, d) > 0.0f) || !(dot3(sd, sd) > 0.0f))
+            return fail(ctx, CR_EINVAL, "cr_cone_trace: eye, target and up give no view basis");
+        look_at(e, t, upv, cv.V);
+        float aspect = cam->aspect > 0.0f ? cam->aspect : (float)W / (float)H;
+        cv.kx = tanHalfFovy * aspect;
+        cv.ky = tanHalfFovy;
+        cv.znear = cam->znear;
+        cv.zfar = cam->zfar;
+    }
+    basis_from_view(cv.V, cv.S, cv.U, cv.B);
+    for (int a = 0; a < 3; ++a) cv.eye[a] = cam->eye[a];
+    return CR_OK;
+}
+
 // validation, camera view, cone schedule and buffer allocation (tile_w x
 // 64 / tile_w camera tiles); the volume's voxelized state is not checked here
 static int cone_prepare(cr_volume v, const cr_light* light, const cr_camera* cam, const cr_cone_params* p,
@@ -1084,27 +1141,13 @@ static int cone_prepare(cr_volume v, const cr_light* light, const cr_camera* cam
         if (lo->k < 1 || lo->k > kMaxSteps) return fail(ctx, CR_EINVAL, "cr_cone_trace_learned: k must be in [1, 256]");
         if (std::isnan(lo->err_bound)) return fail(ctx, CR_EINVAL, "cr_cone_trace_learned: error bound is NaN");
     }
-    if (!finite3(cam->eye) || !finite3(cam->target) || !finite3(cam->up)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite camera");
-    if (!(cam->fovy > 0.0f && cam->fovy < 3.14159265f)) return fail(ctx, CR_EINVAL, "cr_cone_trace: fovy must be in (0, pi)");
-    if (!(cam->znear > 0.0f) || !(cam->zfar > cam->znear)) return fail(ctx, CR_EINVAL, "cr_cone_trace: need 0 < znear < zfar");
     if (light->directional && !finite3(light->direction)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light direction");
     if (!light->directional && !finite3(light->position)) return fail(ctx, CR_EINVAL, "cr_cone_trace: non-finite light position");
-    rc = set_device(ctx);
-    if (rc) return rc;
-
-    // camera view (src/Camera.cpp:59-60)
     CameraView& cv = plan.cv;
     cv = CameraView{};
-    look_at(f3(cam->eye[0], cam->eye[1], cam->eye[2]), f3(cam->target[0], cam->target[1], cam->target[2]),
-            f3(cam->up[0], cam->up[1], cam->up[2]), cv.V);
-    basis_from_view(cv.V, cv.S, cv.U, cv.B);
-    for (int a = 0; a < 3; ++a) cv.eye[a] = cam->eye[a];
-    float tanHalfFovy = std::tan(cam->fovy / 2.0f);
-    float aspect = (float)W / (float)H;
-    cv.kx = tanHalfFovy * aspect;
-    cv.ky = tanHalfFovy;
-    cv.znear = cam->znear;
-    cv.zfar = cam->zfar;
+    if ((rc = camera_view(ctx, cam, W, H, cv))) return rc;
+    rc = set_device(ctx);
+    if (rc) return rc;
     cv.W = W;
     cv.H = H;
     cv.tile_w = tile_w;
@@ -1255,7 +1298,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
         const float vy = ((Vm[1] * px + Vm[5] * py) + Vm[9] * pz) + Vm[13];
         const float vz = ((Vm[2] * px + Vm[6] * py) + Vm[10] * pz) + Vm[14];
         const float dz = -vz;
-        if (dz >= cam->znear && dz <= cam->zfar) {
+        if (dz >= plan.cv.znear && dz <= plan.cv.zfar) {
             ConeDesc& cd = cl.cd;
             cd.sun_on = 1;
             cd.sun_v[0] = vx; cd.sun_v[1] = vy; cd.sun_v[2] = vz;
@@ -1364,6 +1407,25 @@ int cr_volume_occupancy(cr_volume v, uint64_t* count, uint64_t* hash, void* stre
     CR_HIP(ctx, hipStreamSynchronize(s));
     *count = c[0];
     *hash = c[1];
+    return CR_OK;
+}
+
+// VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115): level-0
+// texels with r != 0, which are exactly the lit-shell voxels in every mode
+// (chord-fill texels are (0, 0, 0, 1); noise scales a only)
+int cr_volume_active_voxels(cr_volume v, uint64_t* count, void* stream) {
+    if (!v || !count) return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_active_voxels: NULL argument");
+    cr_context ctx = v->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_fill_u64((uint64_t*)v->d_counters, 2, 0, s);
+    launch_occupancy(v->g, v->d_shell, nullptr, v->d_counters, s);
+    CR_HIP(ctx, hipGetLastError());
+    unsigned long long c[2];
+    CR_HIP(ctx, hipMemcpyAsync(c, v->d_counters, sizeof(c), hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    *count = c[0];
     return CR_OK;
 }
 

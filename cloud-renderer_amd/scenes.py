@@ -35,6 +35,9 @@ class Camera:
     fovy: float = math.radians(60.0)
     znear: float = 0.01
     zfar: float = 2500.0   # src/Camera.cpp:59
+    aspect: float = 0.0    # 0: W / H
+    P: Optional[np.ndarray] = None  # column-major matrices replacing the above (eye stays the sort point)
+    V: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -113,6 +116,7 @@ class Scene:
     sun_sprite: Optional[SunSprite] = None
     directional: int = 1
     seed: int = 0
+    sun_position: Optional[tuple] = None   # Sun::position when given (else centre + sun_dir * SUN_DISTANCE)
     meta: dict = field(default_factory=dict)
 
     @property
@@ -121,6 +125,8 @@ class Scene:
 
     @property
     def sun_pos(self) -> np.ndarray:
+        if self.sun_position is not None:
+            return np.asarray(self.sun_position, np.float32)
         return (np.asarray(self.position, np.float32) + self.sun_dir * np.float32(SUN_DISTANCE)).astype(np.float32)
 
 
@@ -213,6 +219,48 @@ def reference_mode(scene: Scene) -> Scene:
     return replace(scene, name=scene.name + "/ref", fmt=0, mode=0, directional=0, cone=cone, noise=None)
 
 
+def reference_camera(W: int, H: int) -> Camera:
+    """The reference's start-up camera (src/Camera.cpp:12-61 with phi = theta
+    = 0 and position = 0): lookAt = normalize((cos 0 cos 0, sin 0,
+    cos 0 cos(PI/2.f - 0))) = (1, 0, -4.371139e-08), V = lookAt(0, that,
+    +y), P = perspective(45.f, (float)(W / H), .01f, 2500.f) with glm taking
+    45 as radians and W / H an integer division."""
+    lz = float(np.float32(math.cos(float(np.float32(3.14159265359) / np.float32(2.0)))))
+    return Camera(eye=(0.0, 0.0, 0.0), target=(1.0, 0.0, lz), up=(0.0, 1.0, 0.0), fovy=45.0, znear=0.01,
+                  zfar=2500.0, aspect=float(W // H))
+
+
+def reference_default(seed: int = 0, W: int = 1280, H: int = 720, noise_march: bool = True, run_time: float = 0.0,
+                      fluffiness: float = 1.0, sun_sprite: bool = True, quantize_rgba8: int = 1) -> Scene:
+    """The reference application's default frame (src/main.cpp:25-46, 81-82,
+    112-123): a 32^3 GL_R8 volume with 4 mips at position (25, 0, 0) with
+    bounds (-5, 5) on every axis, 200 billboards with offsets uniform in
+    [-2.5, 2.5]^3 and scales uniform in [1, 2.5] (regenerateBillboards,
+    src/CloudVolume.cpp:120-133, seeded here in place of rand()), the point
+    sun at (5, 20, -5) with its sprite, the cone defaults vctSteps 16,
+    vctConeAngle 0.9, vctConeInitialHeight 0.1, vctLodOffset 0,
+    vctDownScaling 1, the noise march on with its defaults and octave offsets
+    windVel * runTime = (0.01 * run_time, 0, 0)
+    (src/Shaders/ConeTraceShader.hpp:14-36, ConeTraceShader.cpp:54-61), the
+    GL blend into an RGBA8 framebuffer cleared to (0.2, 0.3, 0.5, 1), a
+    light map the size of the window (src/Shaders/VoxelizeShader.cpp:20-22)
+    and the start-up camera (reference_camera)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xyz = rng.uniform(-2.5, 2.5, size=(200, 3)).astype(np.float32)
+    r = rng.uniform(1.0, 2.5, size=200).astype(np.float32)
+    pos = (25.0, 0.0, 0.0)
+    sun = (5.0, 20.0, -5.0)
+    sd = _unit(np.asarray(sun, np.float64) - np.asarray(pos, np.float64))
+    nm = NoiseMarch(seed=seed, offsets=(float(np.float32(0.01) * np.float32(run_time)), 0.0, 0.0)) if noise_march else None
+    return Scene(name="reference_default", N=32, levels=4, fmt=0, mode=0, xyz=xyz, scales=r, sun_dir=sd, W=W, H=H,
+                 light_w=W, light_h=H, cameras=[reference_camera(W, H)],
+                 cone=Cone(steps=16, cone_angle=0.9, initial_height=0.1, lod_offset=0.0, down_scaling=1.0,
+                           cone_mode=0, composite=0, quantize_rgba8=quantize_rgba8),
+                 bounds_lo=(-5.0, -5.0, -5.0), bounds_hi=(5.0, 5.0, 5.0), position=pos, fluffiness=fluffiness,
+                 noise_march=nm, sun_sprite=SunSprite() if sun_sprite else None, directional=0, seed=seed,
+                 sun_position=sun)
+
+
 def scaled(scene: Scene, W: int, H: int) -> Scene:
     """Same scene rendered (and light-mapped) at another resolution."""
     return replace(scene, W=W, H=H, light_w=W, light_h=H)
@@ -223,4 +271,5 @@ CONFIGS = {
     "plain256": plain256,
     "fp16_512": fp16_512,
     "learned256": learned256,
+    "reference_default": reference_default,
 }

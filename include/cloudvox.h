@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define CR_ABI_VERSION 2
+#define CR_ABI_VERSION 3
 
 /* status codes */
 #define CR_OK 0
@@ -87,7 +87,21 @@ typedef struct {
 } cr_light;
 
 /* Camera (src/Camera.cpp:59-60): V = lookAt(eye, target, up),
- * P = perspective(fovy, W/H, znear, zfar). */
+ * P = perspective(fovy, aspect, znear, zfar), glm's arithmetic: the
+ * projection scales are tanf(fovy / 2) * aspect and tanf(fovy / 2), so any
+ * fovy with a positive tangent of its half is accepted (the reference passes
+ * fovy = 45, which glm takes as radians, and aspect = (float)(1280 / 720) = 1
+ * after an integer division).  aspect 0 means width / height of the image.
+ *
+ * use_matrices = 1 takes the matrices ConeTraceShader uploads instead
+ * (src/Shaders/ConeTraceShader.cpp:64-69), column-major as glm stores them:
+ * V is used as given (its rotation rows orient the billboards, its third row
+ * is the noise march's view ray) and P must be a symmetric perspective
+ * projection (P[11] = -1, P[15] = 0, no shear or offset terms): the scales
+ * are 1 / P[0] and 1 / P[5], znear = P[14] / (P[10] - 1) and
+ * zfar = P[14] / (P[10] + 1); target, up, fovy, aspect, znear and zfar are
+ * ignored.  eye stays the point the billboards are sorted from
+ * (Camera::getPosition(), src/Shaders/ConeTraceShader.cpp:20). */
 typedef struct {
     float eye[3];
     float target[3];
@@ -95,6 +109,10 @@ typedef struct {
     float fovy; /* radians */
     float znear;
     float zfar;
+    float aspect;     /* 0: width / height */
+    int use_matrices; /* 1: P and V below replace target, up, fovy, aspect, znear, zfar */
+    float P[16];      /* column-major projection */
+    float V[16];      /* column-major view */
 } cr_camera;
 
 /* Cone-trace parameters (src/Shaders/ConeTraceShader.hpp:28-32 defaults:
@@ -116,7 +134,10 @@ typedef struct {
 
 /* Profiling results.  Level 1: counters and stage times of the last call.
  * Level 2: stage times summed over every call since the last cr_get_stats
- * (n_voxelize / n_cone calls), counters zero. */
+ * (n_voxelize / n_cone calls), counters zero.  Stage times are hipEvent
+ * spans on the stream each stage runs on: ms_clear covers the clear and the
+ * pass-1 work list, ms_light_raster, ms_splat and ms_cone one kernel each,
+ * ms_mip the level build, ms_setup / ms_sort / ms_bin the camera pass. */
 typedef struct {
     int64_t light_fragments; /* covered pass-1 fragments */
     int64_t shell_texels;    /* position-map texels with a > 0 */
@@ -235,11 +256,16 @@ int cr_cone_trace(cr_volume vol, const cr_light *light, const cr_camera *camera,
                   size_t pitch_bytes, int pixel_format, void *stream);
 
 /* ---- readback (synchronise `stream`) ---------------------------------- */
-/* Occupied level-0 voxels and an order-independent 64-bit hash:
- * sum over occupied voxels of splitmix64(linear_index * 4 + state),
- * state 2 = lit shell, 1 = chord fill only; linear_index = x + N*(y + N*z).
- * The reference's figure is VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115). */
+/* Occupied level-0 voxels (lit shell and chord fill) and an
+ * order-independent 64-bit hash: sum over occupied voxels of
+ * splitmix64(linear_index * 4 + state), state 2 = lit shell, 1 = chord fill
+ * only; linear_index = x + N*(y + N*z). */
 int cr_volume_occupancy(cr_volume vol, uint64_t *count, uint64_t *hash, void *stream);
+/* VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115): level-0
+ * texels whose r is non-zero.  Those are the lit-shell voxels (chord-fill
+ * voxels hold (0, 0, 0, 1) and noise scales only a), so in reference mode it
+ * equals cr_volume_occupancy's count and in rgba mode it leaves the fill out. */
+int cr_volume_active_voxels(cr_volume vol, uint64_t *count, void *stream);
 /* Level `level` in x-fastest linear order and native element type
  * (u8 / float / binary16), channels 1 (R8) or 4. */
 int cr_volume_read_level(cr_volume vol, int level, void *host_dst, size_t bytes, void *stream);

@@ -5,8 +5,12 @@
 //   cr::CloudVolume      CloudVolume (src/CloudVolume.hpp:12-55): position,
 //                        x/y/zBounds, dimension, levels, voxelSize,
 //                        billboards {positions, scales}, fluffiness, update()
-//   cr::Sun              Sun (src/Sun.hpp:12-19): position (+ directional sun)
-//   cr::Camera           Camera (src/Camera.hpp:25-45): eye, lookAt, P/V inputs
+//   cr::Sun              Sun (src/Sun.hpp:12-19): position (+ directional sun),
+//                        sprite colours and radii
+//   cr::SunShader        SunShader::render() (src/Shaders/SunShader.cpp:7-42):
+//                        the sprite drawn before the clouds
+//   cr::Camera           Camera (src/Camera.hpp:25-45): position, lookAt, and
+//                        the P/V the reference uploads (src/Camera.cpp:59-60)
 //   cr::VoxelizeShader   VoxelizeShader::voxelize(CloudVolume*)
 //                        (src/Shaders/VoxelizeShader.hpp:18)
 //   cr::ConeTraceShader  ConeTraceShader::coneTrace(CloudVolume*) with the
@@ -95,11 +99,21 @@ class CloudVolume {
     void setNoise(int octaves, uint32_t seed, float baseFreq = 4.0f, float lacunarity = 2.0f, float gain = 0.5f) {
         check(cr_volume_set_noise(h_, octaves, seed, baseFreq, lacunarity, gain), ctx_->handle());
     }
-    // VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115) plus an order-free hash
+    // occupied voxels (shell and chord fill) plus an order-free hash
     std::pair<uint64_t, uint64_t> occupancy(void* stream = nullptr) const {
         uint64_t c = 0, h = 0;
         check(cr_volume_occupancy(h_, &c, &h, stream), ctx_->handle());
         return {c, h};
+    }
+    // VoxelShader's activeVoxels (src/Shaders/VoxelShader.cpp:110-115): level-0 texels with r != 0
+    uint64_t activeVoxels(void* stream = nullptr) const {
+        uint64_t c = 0;
+        check(cr_volume_active_voxels(h_, &c, stream), ctx_->handle());
+        return c;
+    }
+    // level k, x-fastest, native element type (u8 / float / binary16)
+    void readLevel(int level, void* dst, size_t bytes, void* stream = nullptr) const {
+        check(cr_volume_read_level(h_, level, dst, bytes, stream), ctx_->handle());
     }
 
     cr_volume handle() const { return h_; }
@@ -122,6 +136,11 @@ struct Sun {
     vec3 position{5.0f, 20.0f, -5.0f};  // src/main.cpp:37
     vec3 direction{0.0f, 1.0f, 0.0f};
     bool directional = false;
+    // sprite (src/main.cpp:40-43)
+    vec3 innerColor{1.0f, 1.0f, 1.0f};
+    vec3 outerColor{1.0f, 1.0f, 0.0f};
+    float innerRadius = 1.0f;
+    float outerRadius = 2.0f;
     cr_light c() const {
         cr_light l{};
         for (int a = 0; a < 3; ++a) {
@@ -133,12 +152,29 @@ struct Sun {
     }
 };
 
+// Camera::update (src/Camera.cpp:59-60): V = lookAt(position, lookAt, up),
+// P = perspective(fovy, aspect, znear, zfar).  The reference passes fovy = 45
+// (glm takes it as radians) and aspect = (float)(Window::width /
+// Window::height), an integer division; aspect 0 here means width / height.
+// setMatrices() takes P and V as the reference uploads them instead
+// (column-major, src/Shaders/ConeTraceShader.cpp:64-69); position stays the
+// point billboards are sorted from.
 struct Camera {
     vec3 position{0.0f, 0.0f, 3.0f};
     vec3 lookAt{0.0f, 0.0f, 0.0f};
     vec3 up{0.0f, 1.0f, 0.0f};
     float fovy = 1.0471976f;
     float znear = 0.01f, zfar = 2500.0f;  // src/Camera.cpp:59
+    float aspect = 0.0f;
+    bool useMatrices = false;
+    std::array<float, 16> P{}, V{};
+    void setMatrices(const float* p, const float* v) {
+        for (int i = 0; i < 16; ++i) {
+            P[i] = p[i];
+            V[i] = v[i];
+        }
+        useMatrices = true;
+    }
     cr_camera c() const {
         cr_camera k{};
         for (int a = 0; a < 3; ++a) {
@@ -149,7 +185,31 @@ struct Camera {
         k.fovy = fovy;
         k.znear = znear;
         k.zfar = zfar;
+        k.aspect = aspect;
+        k.use_matrices = useMatrices ? 1 : 0;
+        for (int i = 0; i < 16; ++i) {
+            k.P[i] = P[i];
+            k.V[i] = V[i];
+        }
         return k;
+    }
+};
+
+// SunShader::render (src/Shaders/SunShader.cpp:7-42, drawn after the clear
+// and before the clouds, src/main.cpp:112-116): the sprite is composited by
+// the volume's next cone trace (GL-blend composite only).
+class SunShader {
+  public:
+    void render(CloudVolume* volume, const Sun& sun, bool enabled = true) const {
+        cr_sun_sprite q{};
+        q.enabled = enabled ? 1 : 0;
+        for (int a = 0; a < 3; ++a) {
+            q.inner_color[a] = sun.innerColor[a];
+            q.outer_color[a] = sun.outerColor[a];
+        }
+        q.inner_radius = sun.innerRadius;
+        q.outer_radius = sun.outerRadius;
+        check(cr_volume_set_sun_sprite(volume->handle(), &q), volume->context());
     }
 };
 
@@ -181,6 +241,40 @@ class ConeTraceShader {
     bool quantizeRGBA8 = true;
     bool compensated = false;
     std::array<float, 4> clearColor{0.2f, 0.3f, 0.5f, 1.0f};  // src/main.cpp:112
+    // noise march (src/Shaders/ConeTraceShader.hpp:14-25, 35-36); the map's
+    // alpha bytes come from noiseSeed in place of rand()
+    bool doNoiseSample = true;
+    uint32_t noiseSeed = 0;
+    float stepSize = 0.01f;
+    float noiseOpacity = 4.0f;
+    int numOctaves = 4;
+    float freqStep = 3.0f;
+    float persStep = 0.5f;
+    float adjustSize = 40.0f;
+    int minNoiseSteps = 2;
+    int maxNoiseSteps = 8;
+    float minNoiseColor = 0.2f;
+    float noiseColorScale = 0.45f;
+    vec3 windVel{0.01f, 0.0f, 0.0f};
+    float runTime = 0.0f;  // Window::runTime: octaveOffsets = windVel * runTime
+
+    cr_noise_march noiseParams() const {
+        cr_noise_march m{};
+        m.enabled = doNoiseSample ? 1 : 0;
+        m.seed = noiseSeed;
+        m.step_size = stepSize;
+        m.opacity = noiseOpacity;
+        m.octaves = numOctaves;
+        m.freq_step = freqStep;
+        m.pers_step = persStep;
+        m.adjust_size = adjustSize;
+        m.min_steps = minNoiseSteps;
+        m.max_steps = maxNoiseSteps;
+        m.min_color = minNoiseColor;
+        m.color_scale = noiseColorScale;
+        for (int a = 0; a < 3; ++a) m.offsets[a] = windVel[a] * runTime;
+        return m;
+    }
 
     cr_cone_params params() const {
         cr_cone_params p{};
@@ -205,6 +299,8 @@ class ConeTraceShader {
         cr_cone_params p = params();
         cr_light l = sun.c();
         cr_camera k = camera.c();
+        cr_noise_march m = noiseParams();
+        check(cr_volume_set_noise_march(volume->handle(), &m), volume->context());
         check(cr_cone_trace(volume->handle(), &l, &k, &p, d_image, width, height, pitch, pixelFormat, stream),
               volume->context());
     }
@@ -216,6 +312,7 @@ class ConeTraceShader {
         cr_cone_params p = params();
         cr_light l = sun.c();
         cr_camera c = camera.c();
+        check(cr_volume_set_noise_march(volume->handle(), nullptr), volume->context());
         check(cr_cone_trace_learned(volume->handle(), &l, &c, &p, mlp, k, errBound, d_image, width, height, pitch,
                                     pixelFormat, stream),
               volume->context());

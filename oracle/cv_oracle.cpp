@@ -63,9 +63,13 @@ typedef struct {
     float eye[3];
     float target[3];
     float up[3];
-    float fovy;         // radians
+    float fovy;         // radians (glm::perspective: only tan(fovy / 2) is used)
     float znear;
     float zfar;
+    float aspect;       // 0: W / H
+    int use_matrices;   // 1: P and V (column-major) replace target/up/fovy/aspect/znear/zfar
+    float P[16];
+    float V[16];
 } cvo_camera;
 
 typedef struct {
@@ -961,6 +965,7 @@ struct CamSetup {
     float V[16];
     Basis bb;
     float kx, ky;     // tan(fovy/2) * aspect, tan(fovy/2)
+    float znear, zfar;
     double Vd[16];
     Vec3T<double> Sd, Ud, Bd;
     double kxd, kyd;
@@ -988,6 +993,56 @@ void look_at_d(const float e[3], const float t[3], const float u[3], double M[16
     S->x = sx; S->y = sy; S->z = sz;
     U->x = ux; U->y = uy; U->z = uz;
     B->x = -fx; B->y = -fy; B->z = -fz;
+}
+
+// The camera as ConeTraceShader uploads it (src/Shaders/ConeTraceShader.cpp:64-69,
+// src/Camera.cpp:59-60): V = lookAt(eye, target, up) and glm::perspective's
+// tanHalfFovy = tan(fovy / 2) (float), the x scale tanHalfFovy * aspect; or V
+// and P given as matrices, the scales then 1 / P[0][0] and 1 / P[1][1] and the
+// depth range n = P[3][2] / (P[2][2] - 1), f = P[3][2] / (P[2][2] + 1) (the
+// inverse of glm::perspective's P[2][2] = -(f + n) / (f - n),
+// P[3][2] = -2 f n / (f - n)).  Returns false on an unusable camera.
+// Pinned by: test_oracle_camera_hemisphere (start points on the camera-facing
+// hemisphere), test_oracle_camera_matrices (matrices equal to lookAt /
+// perspective give the same image), test_oracle_reference_projection.
+bool camera_setup(const cvo_camera* cam, int W, int H, CamSetup* cs) {
+    if (cam->use_matrices) {
+        const float* P = cam->P;
+        static const int zero[] = {1, 2, 3, 4, 6, 7, 8, 9, 12, 13, 15};
+        for (int i : zero)
+            if (P[i] != 0.0f) return false;
+        if (P[11] != -1.0f || !(P[0] > 0.0f) || !(P[5] > 0.0f)) return false;
+        for (int i = 0; i < 16; ++i) { cs->V[i] = cam->V[i]; cs->Vd[i] = (double)cam->V[i]; }
+        cs->kx = 1.0f / P[0];
+        cs->ky = 1.0f / P[5];
+        cs->znear = P[14] / (P[10] - 1.0f);
+        cs->zfar = P[14] / (P[10] + 1.0f);
+        if (!(cs->znear > 0.0f) || !(cs->zfar > cs->znear)) return false;
+        cs->kxd = 1.0 / (double)P[0];
+        cs->kyd = 1.0 / (double)P[5];
+        const double* M = cs->Vd;
+        cs->Sd = Vec3T<double>{M[0], M[4], M[8]};
+        cs->Ud = Vec3T<double>{M[1], M[5], M[9]};
+        double bl = std::sqrt(M[2] * M[2] + M[6] * M[6] + M[10] * M[10]);
+        cs->Bd = Vec3T<double>{M[2] / bl, M[6] / bl, M[10] / bl};
+    } else {
+        look_at(v3(cam->eye[0], cam->eye[1], cam->eye[2]), v3(cam->target[0], cam->target[1], cam->target[2]),
+                v3(cam->up[0], cam->up[1], cam->up[2]), cs->V);
+        float tanHalfFovy = std::tan(cam->fovy / 2.0f);   // glm::perspective
+        if (!(tanHalfFovy > 0.0f)) return false;
+        float aspect = cam->aspect > 0.0f ? cam->aspect : (float)W / (float)H;
+        cs->kx = tanHalfFovy * aspect;
+        cs->ky = tanHalfFovy;
+        cs->znear = cam->znear;
+        cs->zfar = cam->zfar;
+        look_at_d(cam->eye, cam->target, cam->up, cs->Vd, &cs->Sd, &cs->Ud, &cs->Bd);
+        double tanHalfD = std::tan((double)cam->fovy / 2.0);
+        double aspectd = cam->aspect > 0.0f ? (double)cam->aspect : (double)W / (double)H;
+        cs->kxd = tanHalfD * aspectd;
+        cs->kyd = tanHalfD;
+    }
+    cs->bb = billboard_basis(cs->V);
+    return true;
 }
 
 inline uint8_t q8(float v) {  // GL float -> UNORM8 conversion (round to nearest)
@@ -1107,6 +1162,15 @@ template <class T> void noise3d(const int8_t* m, const cvo_noise_march* nm, NV3<
 
 template <class T> inline T sat(T x) { return std::fmin(std::fmax(x, (T)0), (T)1); }
 
+// Parameters the noise march accepts: iSteps = min(len / stepSize, max - min)
+// + min and the march divides by iSteps - 1 (res/conetrace_frag.glsl:154-158),
+// so min = max = 1 would divide by zero; max >= 2 keeps iSteps - 1 > 0 for
+// every kept fragment (its chord length is positive).
+bool noise_march_valid(const cvo_noise_march* nm) {
+    return nm->octaves >= 1 && nm->octaves <= 16 && nm->min_steps >= 1 && nm->max_steps >= nm->min_steps &&
+           nm->max_steps <= 64 && nm->max_steps >= 2 && nm->step_size > 0.0f && nm->adjust_size != 0.0f;
+}
+
 // The noise branch of main() (res/conetrace_frag.glsl:136-174) for one
 // fragment: fp = fragPos, c = center, r = radius, (tu, tv) = vertPos.xy (the
 // fragment's offset in billboard units, fragTex = (vertPos.xy + 1) / 2),
@@ -1194,6 +1258,7 @@ static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, con
                            const cvo_sun_sprite* sun = nullptr) {
     if (!vol || W <= 0 || H <= 0 || !cp || cp->steps < 0 || row_step < 1) return -1;
     const bool use_nm = nm && nm->enabled;
+    if (use_nm && !noise_march_valid(nm)) return -1;
     std::vector<int8_t> nmap(use_nm ? kNoiseDim * kNoiseDim * kNoiseDim * 4 : 0);
     if (use_nm) noise_map_bytes(nm->seed, nmap.data());
     Grid g = make_grid(vol);
@@ -1209,17 +1274,7 @@ static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, con
     StepTab<double> st64 = make_steps<double>(cp, L, N);
 
     CamSetup cs;
-    look_at(v3(cam->eye[0], cam->eye[1], cam->eye[2]), v3(cam->target[0], cam->target[1], cam->target[2]),
-            v3(cam->up[0], cam->up[1], cam->up[2]), cs.V);
-    cs.bb = billboard_basis(cs.V);
-    float tanHalfFovy = std::tan(cam->fovy / 2.0f);   // glm::perspective
-    float aspect = (float)W / (float)H;
-    cs.kx = tanHalfFovy * aspect;
-    cs.ky = tanHalfFovy;
-    look_at_d(cam->eye, cam->target, cam->up, cs.Vd, &cs.Sd, &cs.Ud, &cs.Bd);
-    double tanHalfD = std::tan((double)cam->fovy / 2.0);
-    cs.kxd = tanHalfD * ((double)W / (double)H);
-    cs.kyd = tanHalfD;
+    if (!camera_setup(cam, W, H, &cs)) return -1;
     // noise march view ray: normalize(vec3(V[0][2], V[1][2], V[2][2])) (res/conetrace_frag.glsl:138)
     V3 ray32 = normalize(v3(cs.V[2], cs.V[6], cs.V[10]));
     NV3<float> nray = {ray32.x, ray32.y, ray32.z};
@@ -1264,7 +1319,7 @@ static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, con
         double svy = ((cs.Vd[1] * sp.x + cs.Vd[5] * sp.y) + cs.Vd[9] * sp.z) + cs.Vd[13];
         double dzd = -(((cs.Vd[2] * sp.x + cs.Vd[6] * sp.y) + cs.Vd[10] * sp.z) + cs.Vd[14]);
         const float r0 = sun->inner_radius, r1 = sun->outer_radius;
-        if (dz >= cam->znear && dz <= cam->zfar) {
+        if (dz >= cs.znear && dz <= cs.zfar) {
             for (int py = std::max(row0, 0); py < std::min(row1, H); ++py) {
                 float ry = pixel_ndc(py, H) * cs.ky;
                 double ryd = ((2.0 * ((double)py + 0.5)) / (double)H - 1.0) * cs.kyd;
@@ -1328,7 +1383,7 @@ static int cone_trace_impl(const cvo_volume* vol, const void* levels_native, con
         o.s = radius[j];
         o.cv = view_point(cs.V, o.c);
         float dz = -o.cv.z;
-        o.live = (o.s > 0.0f) && (dz >= cam->znear && dz <= cam->zfar);
+        o.live = (o.s > 0.0f) && (dz >= cs.znear && dz <= cs.zfar);
         if (o.live) {
             o.box = pixel_box(o.cv.x, o.cv.y, o.s, cs.kx * dz, cs.ky * dz, 0.0f, 0.0f, W, H);
             o.live = !o.box.empty;
@@ -1526,6 +1581,25 @@ void cvo_noise_sample(uint32_t seed, const float* pts, int64_t n, float* out32, 
 // one fragment of the noise branch (FP32), for tests: in[0..2] fragPos,
 // in[3..5] center, in[6] radius, in[7..8] vertPos.xy, in[9..11] view ray;
 // out = (col, alpha, steps, kept)
+int cvo_noise_march_valid(const cvo_noise_march* nm) { return noise_march_valid(nm) ? 1 : 0; }
+
+// The camera as the cone trace sets it up, for tests: out[0..15] V,
+// out[16..17] the projection scales (kx, ky), out[18..19] znear, zfar,
+// out[20..28] the billboard basis S, U, B, out[29..31] the noise march's
+// view ray normalize(V[0][2], V[1][2], V[2][2]) (res/conetrace_frag.glsl:138).
+// Returns -1 on an unusable camera.
+int cvo_camera_setup(const cvo_camera* cam, int W, int H, float* out) {
+    CamSetup cs;
+    if (!camera_setup(cam, W, H, &cs)) return -1;
+    for (int i = 0; i < 16; ++i) out[i] = cs.V[i];
+    out[16] = cs.kx; out[17] = cs.ky; out[18] = cs.znear; out[19] = cs.zfar;
+    const V3 b[3] = {cs.bb.S, cs.bb.U, cs.bb.B};
+    for (int k = 0; k < 3; ++k) { out[20 + 3 * k] = b[k].x; out[21 + 3 * k] = b[k].y; out[22 + 3 * k] = b[k].z; }
+    V3 ray = normalize(v3(cs.V[2], cs.V[6], cs.V[10]));
+    out[29] = ray.x; out[30] = ray.y; out[31] = ray.z;
+    return 0;
+}
+
 void cvo_noise_fragment(const cvo_noise_march* nm, const float* in, float* out) {
     std::vector<int8_t> m(kNoiseDim * kNoiseDim * kNoiseDim * 4);
     noise_map_bytes(nm->seed, m.data());
@@ -1597,6 +1671,15 @@ int cvo_num_threads(void) {
     return omp_get_max_threads();
 #else
     return 1;
+#endif
+}
+
+// threads of the following parallel regions (n >= 1)
+void cvo_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+#else
+    (void)n;
 #endif
 }
 

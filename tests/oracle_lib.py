@@ -30,7 +30,8 @@ class cvo_light(C.Structure):
 
 class cvo_camera(C.Structure):
     _fields_ = [("eye", F3), ("target", F3), ("up", F3), ("fovy", C.c_float), ("znear", C.c_float),
-                ("zfar", C.c_float)]
+                ("zfar", C.c_float), ("aspect", C.c_float), ("use_matrices", C.c_int), ("P", C.c_float * 16),
+                ("V", C.c_float * 16)]
 
 
 class cvo_noise_march(C.Structure):
@@ -104,11 +105,16 @@ def lib():
         L.cvo_noise_map.argtypes = [C.c_uint32, vp]
         L.cvo_noise_sample.argtypes = [C.c_uint32, vp, C.c_int64, vp, vp]
         L.cvo_noise_fragment.argtypes = [P(cvo_noise_march), vp, vp]
+        L.cvo_noise_march_valid.argtypes = [P(cvo_noise_march)]
+        L.cvo_noise_march_valid.restype = C.c_int
+        L.cvo_camera_setup.argtypes = [P(cvo_camera), C.c_int, C.c_int, vp]
+        L.cvo_camera_setup.restype = C.c_int
         L.cvo_step_table.argtypes = [P(cvo_cone), C.c_int, C.c_int, vp, vp, vp, vp]
         L.cvo_sample.argtypes = [P(cvo_volume), vp, vp, C.c_float, vp, C.c_int64, vp, vp]
         L.cvo_trace_one.argtypes = [P(cvo_volume), vp, vp, P(cvo_cone), P(C.c_float), P(C.c_float), vp, vp,
                                     P(C.c_int32)]
         L.cvo_num_threads.restype = C.c_int
+        L.cvo_set_num_threads.argtypes = [C.c_int]
         L.cvo_segments.argtypes = [P(cvo_volume), vp, P(cvo_cone), P(cvo_light), C.c_int, vp, C.c_int64, vp, vp, vp]
         L.cvo_segments.restype = C.c_int
         L.cvo_mlp_param_count.restype = C.c_int
@@ -155,6 +161,12 @@ def camera_of(c) -> cvo_camera:
     k = cvo_camera()
     k.eye, k.target, k.up = f3(c.eye), f3(c.target), f3(c.up)
     k.fovy, k.znear, k.zfar = float(c.fovy), float(c.znear), float(c.zfar)
+    k.aspect = float(getattr(c, "aspect", 0.0))
+    P, V = getattr(c, "P", None), getattr(c, "V", None)
+    if P is not None and V is not None:
+        k.use_matrices = 1
+        k.P = (C.c_float * 16)(*[float(x) for x in np.asarray(P, np.float32).reshape(16)])
+        k.V = (C.c_float * 16)(*[float(x) for x in np.asarray(V, np.float32).reshape(16)])
     return k
 
 
@@ -274,6 +286,19 @@ def cone_trace(scene, nat, f64, order, cam, row0=0, row1=None, want_f64=True, ro
         rc = lib().cvo_cone_trace(*args)
     assert rc == 0
     return img32, img64, st
+
+
+def camera_setup(cam, W: int, H: int):
+    """(V[16], kx, ky, znear, zfar, S, U, B, noise ray) of the oracle's camera setup, or None."""
+    out = np.zeros(32, np.float32)
+    if lib().cvo_camera_setup(C.byref(camera_of(cam)), int(W), int(H), ptr(out)) != 0:
+        return None
+    return dict(V=out[:16], kx=out[16], ky=out[17], znear=out[18], zfar=out[19], S=out[20:23], U=out[23:26],
+                B=out[26:29], ray=out[29:32])
+
+
+def noise_march_valid(m) -> bool:
+    return bool(lib().cvo_noise_march_valid(C.byref(noise_march_of(m))))
 
 
 def noise_map(seed: int) -> np.ndarray:

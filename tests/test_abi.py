@@ -29,14 +29,14 @@ def test_library_exports_every_declared_symbol(cv):
 
 
 def test_abi_version(cv):
-    assert cv.lib().cr_abi_version() == 2
+    assert cv.lib().cr_abi_version() == 3
 
 
 def test_struct_layouts_match_header(cv):
     from cloud_renderer_amd import _abi
     # sizes of the plain-C parameter blocks (int/float fields, no padding surprises)
     assert C.sizeof(_abi.cr_light) == 7 * 4
-    assert C.sizeof(_abi.cr_camera) == 12 * 4
+    assert C.sizeof(_abi.cr_camera) == (12 + 2 + 32) * 4
     assert C.sizeof(_abi.cr_cone_params) == 15 * 4
     assert C.sizeof(_abi.cr_stats) == 5 * 8 + 8 * 4 + 6 * 8
     assert C.sizeof(_abi.cr_batch_config) == 3 * 4
@@ -105,3 +105,19 @@ def test_cpp_facade_compiles():
     p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", str(hpp.parent), "-x", "c++",
                         "-"], input=src, text=True, capture_output=True)
     assert p.returncode == 0, p.stderr
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not Path("/opt/rocm/include/hip/hip_runtime_api.h").exists(),
+                    reason="no host C++ compiler or HIP headers")
+def test_cpp_frame_program_links(cv, tmp_path):
+    """The reference-frame program (tests/cpp/reference_frame.cpp) compiles
+    against include/cloudvox.hpp and links against libcloudvox.so: every
+    facade call it makes resolves to an exported C symbol."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("cr_build", REPO / "cloud-renderer_amd" / "build.py")
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    out = tmp_path / "reference_frame"
+    p = subprocess.run(b.cpp_frame_command(out), capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    assert out.exists()

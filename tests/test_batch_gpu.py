@@ -168,3 +168,52 @@ def test_batch_after_single_traces_on_its_volumes(cv, gpu_ctx):
         log("[batch] runs after single traces on a batch volume equal the per-scene images")
     finally:
         b.close()
+
+
+def test_batched64_full_size(cv, gpu_ctx):
+    """The batched workload at its native sizes: the 64 scenes of
+    batched64(0) (volume sides 64..256, 256..8192 spheres, images up to
+    1024^2) rendered by one batch graph equal the per-scene path bit for bit,
+    and four of them -- the largest (N = 256, 1024^2, 5986 spheres), the one
+    with most spheres, a mid-size one and the smallest -- equal the oracle's
+    FP32 pass on every 8th image row bit for bit and its FP64 replay within
+    1e-3."""
+    import torch
+    import oracle_lib as O
+    from gpu_common import assert_bit_equal, psnr
+
+    scenes = cv.scenes.batched64(0)
+    log(f"[batch64] {len(scenes)} scenes, {sum(s.W * s.H for s in scenes)} pixels")
+    want = [_single(cv, gpu_ctx, sc) for sc in scenes]
+    log("[batch64] per-scene images done")
+    b, entries = _batch(cv, gpu_ctx, scenes)
+    try:
+        b.run()
+        torch.cuda.synchronize()
+        _check(entries, want, "batched64 full")
+        b.run()
+        torch.cuda.synchronize()
+        _check(entries, want, "batched64 full, second replay")
+        log("[batch64] two graph runs equal the per-scene images")
+    finally:
+        b.close()
+        for e in entries:
+            e[0].close()
+    size = [s.N ** 3 * s.W * s.H for s in scenes]
+    picks = [int(np.argmax(size)), int(np.argmax([s.S for s in scenes])), int(np.argsort(size)[len(size) // 2]),
+             int(np.argmin(size))]
+    step = 8
+    for i in dict.fromkeys(picks):
+        sc = scenes[i]
+        log(f"[batch64] scene {i}: N={sc.N} S={sc.S} {sc.W}x{sc.H} against the oracle, every {step}th row")
+        vr = O.voxelize(sc)
+        nat, f64, _, _ = O.build_levels(sc, vr.shell, vr.fill)
+        cam = sc.cameras[0]
+        order = O.sort_boards(sc, cam.eye)
+        i32, i64, _ = O.cone_trace(sc, nat, f64, order, cam, row_step=step)
+        rows = np.arange(0, sc.H, step)
+        assert_bit_equal(f"scene {i} rows (FP32)", want[i][rows], i32[rows])
+        err = float(np.abs(want[i][rows].astype(np.float64) - i64[rows]).max())
+        log(f"[batch64] scene {i}: rows bit-exact, max-abs vs FP64 {err:.2e}, "
+            f"PSNR {psnr(want[i][rows], i64[rows]):.1f} dB")
+        assert err <= 1e-3

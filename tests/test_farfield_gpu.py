@@ -214,11 +214,11 @@ def test_trained_network_gains_over_truncation(cv, gpu_ctx):
     """What the trained network adds: against the plain march, the learned
     trace (no fallback) must beat the trace truncated after k = 32 samples by
     >= 10 dB and reach >= 40 dB; at the 2e-3 error bound it must stay >= 60 dB.
-    Also scored against the oracle's FP64 plain march on every 8th row."""
+    Also scored against the oracle's FP64 plain march on every 8th row.
+    The frame is the learned workload at its full size (256^3, 1920x1080)."""
     from cloud_renderer_amd import farfield as ff
-    sc = cv.scenes.scaled(cv.scenes.plain256(0), 480, 270)
-    sc = replace(sc, light_w=1920, light_h=1080)
-    log("[farfield] trained network on a held-out plain256 frame at 480x270")
+    sc = cv.scenes.learned256(0)
+    log("[farfield] trained network on a held-out 256^3 frame at 1920x1080")
     mlp = cv.FarFieldMLP(gpu_ctx, _weights(cv))
     res = ff.evaluate(cv, mlp, sc, 32, [2e-3, 1e30], context=gpu_ctx, repeats=1)
     mlp.close()
@@ -245,3 +245,30 @@ def test_trained_network_gains_over_truncation(cv, gpu_ctx):
     assert p_free >= p_trunc + 10.0
     assert p_spec >= 60.0
     assert p_free64 >= 40.0
+
+
+def test_learned_fp16_against_fp64(cv, gpu_ctx):
+    """The learned + FP16 combination of the learned workload (256^3 volume
+    stored as RGBA16F with compensated accumulation, 1920x1080, k = 32, no
+    fallback) against the oracle's FP64 plain march on every 8th row:
+    PSNR >= 40 dB, and the plain FP16 march of the same volume within 2e-3."""
+    from cloud_renderer_amd import farfield as ff
+    base = cv.scenes.learned256(0)
+    sc = replace(base, name="learned256_fp16", fmt=2, cone=replace(base.cone, compensated=1))
+    log("[farfield] learned + FP16 at 256^3, 1920x1080")
+    mlp = cv.FarFieldMLP(gpu_ctx, _weights(cv))
+    res = ff.evaluate(cv, mlp, sc, 32, [1e30], context=gpu_ctx, repeats=1)
+    mlp.close()
+    plain = res["plain"][0]
+    learned, _ = res[1e30]
+    vr = O.voxelize(sc)
+    nat, f64, _, _ = O.build_levels(sc, vr.shell, vr.fill)
+    order = O.sort_boards(sc, sc.cameras[0].eye)
+    i32, i64, _ = O.cone_trace(sc, nat, f64, order, sc.cameras[0], row_step=8)
+    rows = np.arange(0, sc.H, 8)
+    assert np.array_equal(plain[rows].view(np.uint32), i32[rows].view(np.uint32)), "plain FP16 rows vs oracle FP32"
+    err_plain = float(np.abs(plain[rows].astype(np.float64) - i64[rows]).max())
+    p_learned = ff.psnr(learned[rows], i64[rows])
+    log(f"[farfield] FP16 plain max-abs vs FP64 {err_plain:.2e}; learned + FP16 vs FP64 {p_learned:.2f} dB")
+    assert err_plain <= 2e-3
+    assert p_learned >= 40.0

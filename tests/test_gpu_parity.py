@@ -2,11 +2,23 @@
 
 Bars (stated per check):
   * occupancy (lit shell / chord fill state per voxel), its count and hash,
-    the pass-1 position map, every mip level and the draw order: bit-exact;
+    activeVoxels (level-0 texels with r != 0), the pass-1 position map, every
+    mip level and the draw order: bit-exact;
   * the FP32 image: bit-exact against the oracle's FP32 pass (same op order);
   * the FP32 image against the oracle's FP64 replay: max-abs <= 1e-3 on
     every channel (FP32 storage), <= 2e-3 for RGBA16F storage, whose
-    rounding the FP64 replay does not share (PSNR is reported as well).
+    rounding the FP64 replay does not share (PSNR is reported as well);
+  * R8 volumes (reference semantics): the FP64 replay runs over the R8
+    levels as stored (byte / 255), so 1e-3 judges the arithmetic; the replay
+    over the unrounded mip means is reported beside it and, where the
+    fragment alpha does not depend on the cone (noise march on), bounded by
+    what the R8 rounding of the mips can change (r8_mip_bound);
+  * GL blending into an RGBA8 framebuffer rounds after every blend, which no
+    FP64 replay shares: those images are bit-exact against the FP32 pass and
+    their FP64 accuracy is judged on a float-blend twin case.
+Every reference-semantics case listed as informative keeps its weighted
+cone sums below 1 (no pixel saturates), so a wrong sample, weight or light
+direction changes the image.
 """
 from dataclasses import replace
 
@@ -14,7 +26,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from gpu_common import assert_bit_equal, log, psnr, run_gpu, run_oracle
+from gpu_common import assert_bit_equal, glm_look_at, glm_perspective, log, psnr, r8_mip_bound, run_gpu, run_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -63,8 +75,51 @@ def _scenes(cv):
         "npot96_plain": replace(sc.scaled(sc.plain256(1), 320, 180), N=96, levels=7),
         "ties": tie_scene(sc),
         "ties_ref": sc.reference_mode(tie_scene(sc)),
+        # reference semantics in a range that carries information: with
+        # vctDownScaling 16 no weighted sum reaches 1 (float blend over a
+        # zero clear; then the RGBA8 framebuffer over the reference clear)
+        "small32_ref_ds16": _ref_ds(sc, base, 16.0, quantize=0, clear=(0.0, 0.0, 0.0, 0.0)),
+        "small32_ref_ds16_q8": _ref_ds(sc, base, 16.0, quantize=1),
+        # off-origin volume with bounds of its own, fluffiness != 1, camera
+        # and point sun placed around it
+        "small32_ref_offset_fluffy": _offset_fluffy(sc, base),
+        # the reference application's default frame (src/main.cpp defaults)
+        "refdef": sc.reference_default(0),
+        "refdef_float": sc.reference_default(0, quantize_rgba8=0),
+        "refdef_fluffy": sc.reference_default(1, fluffiness=0.6, run_time=2.5, quantize_rgba8=0),
+        "refdef_nonoise_ds8": _refdef_nonoise(sc),
+        # the same frame with P and V handed over as matrices
+        "refdef_matrices": _with_matrices(sc.reference_default(2, quantize_rgba8=0)),
     }
     return cases
+
+
+def _ref_ds(sc, base, ds, quantize, clear=None):
+    r = sc.reference_mode(base)
+    cone = replace(r.cone, down_scaling=ds, quantize_rgba8=quantize)
+    if clear is not None:
+        cone = replace(cone, clear_color=clear)
+    return replace(r, cone=cone)
+
+
+def _offset_fluffy(sc, base):
+    r = _ref_ds(sc, base, 16.0, quantize=0)
+    pos = (10.0, -3.0, 2.0)
+    cam = sc.Camera(eye=(10.6, -2.7, 5.5), target=pos)
+    return replace(r, position=pos, bounds_lo=(-1.2, -0.9, -1.1), bounds_hi=(1.0, 1.3, 0.9), fluffiness=0.8,
+                   cameras=[cam], sun_position=(13.0, 6.0, -1.0))
+
+
+def _refdef_nonoise(sc):
+    r = sc.reference_default(0, noise_march=False, quantize_rgba8=0)
+    return replace(r, cone=replace(r.cone, down_scaling=8.0))
+
+
+def _with_matrices(scene):
+    c = scene.cameras[0]
+    P = glm_perspective(c.fovy, c.aspect or scene.W / scene.H, c.znear, c.zfar)
+    V = glm_look_at(c.eye, c.target, c.up)
+    return replace(scene, cameras=[replace(c, P=P, V=V)])
 
 
 def _unit3(v):
@@ -96,7 +151,13 @@ def tie_scene(sc):
 
 CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_rgba32f", "small32_ref_float_blend",
          "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_ref_noise_march",
-         "small32_ftb_noise_march", "small32_ref_sun", "small32_blend_sun_march", "small32_comp_blend", "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref"]
+         "small32_ftb_noise_march", "small32_ref_sun", "small32_blend_sun_march", "small32_comp_blend",
+         "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref", "small32_ref_ds16", "small32_ref_ds16_q8",
+         "small32_ref_offset_fluffy", "refdef", "refdef_float", "refdef_fluffy", "refdef_nonoise_ds8",
+         "refdef_matrices"]
+# reference-semantics cases whose pixels must stay below saturation
+INFORMATIVE = {"small32_ref_ds16", "small32_ref_ds16_q8", "small32_ref_offset_fluffy", "refdef", "refdef_float",
+               "refdef_fluffy", "refdef_nonoise_ds8", "refdef_matrices"}
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -112,16 +173,42 @@ def test_parity(cv, gpu_ctx, case):
     for k, (gl, ol) in enumerate(zip(g.levels, o.levels)):
         assert_bit_equal(f"level {k}", gl, ol)
     assert_bit_equal("draw order", g.order, o.order)
+    assert g.active == o.active, "activeVoxels"
     # image: FP32 bit-exact against the oracle's FP32 pass
     assert_bit_equal("image fp32", g.image, o.img32)
-    # and within tolerance of the FP64 replay
-    tol = TOL_F16 if scene.fmt == 2 else TOL_F32
-    if scene.cone.quantize_rgba8 and scene.cone.composite == 0:
-        tol = 2.0 / 255.0 * 8  # RGBA8 framebuffer rounding after every blend (not in the FP64 replay)
-    err = float(np.abs(g.image.astype(np.float64) - o.img64).max())
-    log(f"[parity] case={case} max_abs_vs_fp64={err:.3e} psnr={psnr(g.image, o.img64):.1f} dB "
-        f"fragments={g.stats['fragments']} samples={g.stats['samples']}")
-    assert err <= tol, f"max-abs {err} vs FP64 replay exceeds {tol}"
+    check_fp64(case, scene, g, o)
+
+
+def check_fp64(case, scene, g, o):
+    """The FP32 image against the oracle's FP64 replay (bars in the module docstring)."""
+    img = g.image.astype(np.float64)
+    err = float(np.abs(img - o.img64).max())
+    quantized = scene.cone.quantize_rgba8 and scene.cone.composite == 0
+    msg = (f"[parity] case={case} max_abs_vs_fp64={err:.3e} psnr={psnr(g.image, o.img64):.1f} dB "
+           f"fragments={g.stats['fragments']} samples={g.stats['samples']}")
+    if scene.fmt == 0:
+        err_s = float(np.abs(img - o.img64_stored).max())
+        msg += f" max_abs_vs_fp64_stored_r8={err_s:.3e}"
+        if scene.noise_march is not None and not quantized:
+            bound = r8_mip_bound(scene)
+            msg += f" r8_mip_bound={bound:.3e}"
+            assert err <= bound + TOL_F32, f"max-abs {err} vs the unrounded FP64 replay exceeds {bound} + 1e-3"
+        if not quantized:
+            assert err_s <= TOL_F32, f"max-abs {err_s} vs the FP64 replay over the stored R8 levels exceeds 1e-3"
+    elif not quantized:
+        tol = TOL_F16 if scene.fmt == 2 else TOL_F32
+        assert err <= tol, f"max-abs {err} vs FP64 replay exceeds {tol}"
+    if case in INFORMATIVE:
+        clear = np.asarray(scene.cone.clear_color, np.float32)
+        if quantized:  # the RGBA8 framebuffer holds the clear colour rounded to 8 bits
+            clear = (np.floor(np.clip(clear, 0, 1) * np.float32(255) + np.float32(0.5)) / np.float32(255)).astype(
+                np.float32)
+        covered = np.any(g.image != clear, axis=-1)
+        sat = int(np.count_nonzero((g.image[..., 0] >= 1.0) & covered))
+        msg += f" saturated_px={sat}/{int(np.count_nonzero(covered))}"
+        assert np.count_nonzero(covered) > 100, "the case covers too few pixels to carry information"
+        assert sat <= 0.05 * np.count_nonzero(covered), "more than 5% of the covered pixels saturate"
+    log(msg)
 
 
 def test_determinism(cv, gpu_ctx):
@@ -292,3 +379,37 @@ def test_noise_march_map_and_limits(cv, gpu_ctx):
     want = run_gpu(cv, base, ctx=gpu_ctx, read_levels=False).image
     assert_bit_equal("noise march off", plain, want)
     assert not np.array_equal(with_nm, plain)
+
+
+def test_runtime_refusals_camera_and_noise_steps(cv, gpu_ctx):
+    """The runtime refuses what the oracle refuses: min = max = 1 noise steps
+    (iSteps - 1 = 0 divides), a projection that is not a symmetric
+    perspective, and a degenerate lookAt; fovy = 45 (radians, as the
+    reference passes it) is accepted."""
+    import torch
+    sc = cv.scenes
+    scene = replace(sc.reference_default(0, W=64, H=36), noise_march=sc.NoiseMarch(min_steps=1, max_steps=1))
+    log("[refusals] noise steps 1..1, bad P, degenerate lookAt")
+    vol, sun, cam, vox, tr = cv.scene_objects(scene, 0, context=gpu_ctx)
+    try:
+        vox.voxelize(vol, sun)
+        img = torch.zeros((scene.H, scene.W, 4), dtype=torch.float32, device="cuda")
+        with pytest.raises(cv.CloudVoxError) as e:
+            tr.coneTrace(vol, sun, cam, img, scene.W, scene.H)
+        assert e.value.code == cv._abi.CR_EINVAL and "max_steps >= 2" in str(e.value)
+        tr.doNoiseSample = False
+        tr.coneTrace(vol, sun, cam, img, scene.W, scene.H)      # fovy 45, aspect 1: accepted
+        P = glm_perspective(45.0, 1.0, 0.01, 2500.0)
+        V = glm_look_at(cam.eye, cam.target, cam.up)
+        Q = P.copy()
+        Q[8] = 0.25
+        bad = cv.Camera(eye=cam.eye, P=Q, V=V)
+        with pytest.raises(cv.CloudVoxError) as e:
+            tr.coneTrace(vol, sun, bad, img, scene.W, scene.H)
+        assert e.value.code == cv._abi.CR_EUNSUPPORTED
+        with pytest.raises(cv.CloudVoxError) as e:
+            tr.coneTrace(vol, sun, cv.Camera(eye=(0, 0, 0), target=(0, 0, 0)), img, scene.W, scene.H)
+        assert e.value.code == cv._abi.CR_EINVAL
+        torch.cuda.synchronize()
+    finally:
+        vol.close()
