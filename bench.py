@@ -438,7 +438,7 @@ def main() -> int:
     value = W * H * args.steps / r["secs"] / 1e6
     ms_per_step = r["secs"] * 1e3 / args.steps
     # dominant kernel: the longest kernel-only span
-    kernels = {"cone": "k_cone", "light_raster": "k_light_raster", "splat": "k_light_splat",
+    kernels = {"cone": "k_cone_tab", "light_raster": "k_light_raster", "splat": "k_light_splat",
                "mip": "k_build_fused + k_mip_tail"}
     dom = max(kernels, key=lambda k: r["stage_ms"][k])
     dom_ms = r["stage_ms"][dom]

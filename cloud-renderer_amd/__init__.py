@@ -59,9 +59,9 @@ def _f3(v):
 class Context:
     """One HIP device (cr_context)."""
 
-    def __init__(self, device: int = 0, profiling: bool = False):
+    def __init__(self, device: int = 0, profiling: bool = False, flags: int = 0):
         h = C.c_void_p()
-        _check(lib().cr_context_create(int(device), 0, C.byref(h)))
+        _check(lib().cr_context_create(int(device), int(flags), C.byref(h)))
         self.handle = h
         self.device = device
         if profiling:
@@ -230,6 +230,24 @@ class CloudVolume:
         _check(lib().cr_volume_read_draw_order(self.handle, out.ctypes.data_as(C.c_void_p), out.shape[0],
                                                _stream_handle(stream)), self.ctx.handle)
         return out[:self.billboards.count]
+
+    def set_draw_order(self, mode: int) -> None:
+        """_abi.CR_ORDER_SORTED (cone traces sort first) or CR_ORDER_UPLOAD (draw in upload order)."""
+        _check(lib().cr_volume_set_draw_order(self.handle, int(mode)), self.ctx.handle)
+
+    def sortBoards(self, point, stream=None) -> np.ndarray:
+        """CloudVolume::sortBoards(point) (src/CloudVolume.cpp:65-82): permutes
+        billboards.positions / scales in place, far to near from point, as the
+        selection sort leaves them (equal distances included), and returns the
+        permutation (new index -> old index).  The next update() uploads them."""
+        n = self.billboards.count
+        order = np.empty(max(n, 1), np.int32)
+        _check(lib().cr_volume_sort_boards(self.handle, _f3(point), order.ctypes.data_as(C.c_void_p), order.shape[0],
+                                           _stream_handle(stream)), self.ctx.handle)
+        order = order[:n]
+        self.billboards = Billboards(np.ascontiguousarray(self.billboards.positions[order]),
+                                     np.ascontiguousarray(self.billboards.scales[order]))
+        return order
 
     def close(self) -> None:
         if getattr(self, "handle", None):

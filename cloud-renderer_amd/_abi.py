@@ -32,6 +32,11 @@ CR_CONE_FTB_DENSITY = 2
 CR_COMPOSITE_GL_BLEND = 0
 CR_COMPOSITE_FTB = 1
 
+CR_CONTEXT_DIRECT_ADDRESSING = 1
+
+CR_ORDER_SORTED = 0
+CR_ORDER_UPLOAD = 1
+
 CR_PIXEL_RGBA32F = 0
 CR_PIXEL_RGBA8 = 1
 
@@ -119,6 +124,8 @@ SIGNATURES = {
     "cr_volume_read_occupancy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "cr_volume_read_position_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "cr_volume_read_draw_order": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "cr_volume_set_draw_order": (C.c_int, [C.c_void_p, C.c_int]),
+    "cr_volume_sort_boards": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p, C.c_int, C.c_void_p]),
     "cr_mlp_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "cr_mlp_destroy": (C.c_int, [C.c_void_p]),
     "cr_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),

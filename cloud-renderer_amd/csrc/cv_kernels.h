@@ -14,6 +14,8 @@ constexpr int kMaxLevels = 16;
 constexpr int kMaxSteps = 256;
 constexpr int kTile = 8;           // default camera tile: 8x8 pixels = one wavefront
 constexpr int kLdsSortMax = 16384; // billboards sorted in one workgroup's LDS
+constexpr int kBatchSortMax = 8192; // billboards a batched LDS sort takes (64 KiB of keys)
+constexpr int kFillChunk = 4096;    // elements per workgroup of the segmented fill
 constexpr int kChordLdsWords = 4096; // largest LDS bitmap (bricks) per pass-1 tile for chord-fill aggregation
 constexpr int kLightTile = 32;       // pass-1 work item: 32 x 32 light pixels of one billboard
 constexpr unsigned kConeWaves = 6144; // persistent cone-trace wavefronts (256 CUs x 24)
@@ -172,6 +174,11 @@ void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s);
 void launch_cam_setup(const GridDesc& g, const CameraView& cv, const float4* boards, int S, float4* sph_world,
                       float4* sph_view, int4* sph_tiles, unsigned long long* keys, hipStream_t s);
 size_t sort_scratch_words(int S);
+// keys of a sort far to near from `point` alone (CloudVolume::sortBoards), and
+// the identity order (billboards drawn in upload order)
+void launch_sort_keys(const GridDesc& g, const float4* boards, int S, const float point[3], unsigned long long* keys,
+                      hipStream_t s);
+void launch_iota(int32_t* order, int S, hipStream_t s);
 int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s);
 // camera tiles in the order the cone waves take them (heaviest first)
 struct TileQueue {
@@ -198,6 +205,66 @@ struct BatchScene {
     int tile0;  // first global tile of the scene
 };
 
+// One scene of a batch's voxelization and camera pass, as the batched stage
+// kernels read it (device array, one entry per scene)
+struct BatchVox {
+    GridDesc g;
+    LightView lv;
+    VolumeDesc vd;
+    CameraView cv;
+    const float4* boards;
+    int S;
+    int chord;   // chord fill (rgba mode)
+    int words;   // LDS bitmap words of the pass-1 raster
+    int4* lbox;
+    int* counts;
+    int* offsets;
+    RasterItem* items;
+    int items_cap;
+    unsigned long long* keys;  // pass-1 depth keys
+    unsigned long long* shell;
+    unsigned long long* fill;
+    void* levels;
+    uint8_t* zflags;
+    int nlev;       // levels of the fused build
+    uint32_t rpa;   // fused build regions per axis
+    int tail_k;     // first level of the one-workgroup tail build (L: none)
+    float4* sph_world;
+    float4* sph_view;
+    int4* sph_tiles;
+    unsigned long long* sortkeys;
+    int32_t* order;
+    int* tie;
+    int order_mode;  // CR_ORDER_*
+    unsigned long long* masks;
+    int mask_words;  // mask words per camera tile
+    int* tile_count;
+};
+// A batched stage launch: workgroup b belongs to group member m with
+// first[m] <= b < first[m + 1] (first[n] = total), which is scene scene[m].
+struct BatchStage {
+    const int* first;
+    const int* scene;
+    int n;
+};
+// one launch per stage over every scene of the batch that the batched path
+// takes (format shared: FMT of the fused build); see batch_vox_eligible
+enum BatchStageId { kBsBoxes, kBsScan, kBsExpand, kBsPrep, kBsRaster, kBsSplat, kBsBuild, kBsTail, kBsCam,
+                    kBsSortIota, kBsSort, kBsBin, kBsCount };
+void launch_batch_stage(int stage, int variant, const BatchVox* scenes, const BatchStage& st, int total_blocks,
+                        hipStream_t s);
+// scenes the batched stages handle: sides multiple of 16, fused build up to
+// level 4 and every further level in the tail, at most 16384 billboards
+bool batch_vox_eligible(const GridDesc& g, const VolumeDesc& vd, int S);
+// launch shape of a stage for one scene: workgroups, and the variant (raster
+// <P2, WORDS> and sort width) that groups scenes into one launch
+void batch_stage_shape(int stage, const BatchVox& b, int* blocks, int* variant);
+// segmented fill: one launch over many (pointer, count, value) segments
+void launch_fill_segments(const FillSeg* d_segs, const int* d_first, int nseg, int total_blocks, hipStream_t s);
+int fill_segment_blocks(uint64_t n);
+// fills the kernel-derived fields of b (raster bitmap words, fused build shape)
+void batch_vox_complete(BatchVox& b);
+
 // batched cone trace: work items are global tiles in `order` (heavy first);
 // waves_per_group waves per workgroup, `groups` workgroups striding over the
 // items (groups = 0: one workgroup per waves_per_group items)
@@ -209,6 +276,22 @@ void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, 
 void launch_bin_counts(const int4* sph_tiles, const int32_t* order, int S, int tiles_x, int words,
                        unsigned long long* masks, int* tile_count, hipStream_t s);
 void launch_tile_order(const int* tile_count, int ntiles, int* scratch, int* tile_order, hipStream_t s);
+
+// LDS tables of the table-addressed cone kernel: per level, word offsets of
+// three arrays indexed by floor(u) + 1 for the in-volume range
+// floor(u) in [-1, n - 1]: the byte offset of the clamped x corner c0, and
+// the byte-offset pairs of the clamped y and z corners (c0, c1), all relative
+// to the level's first texel.  words: table words; wpg: waves per workgroup.
+struct ConeTab {
+    uint32_t x[kMaxLevels], y[kMaxLevels], z[kMaxLevels];
+    uint32_t words;
+    int levels;
+    int wpg;
+};
+// fills t for vd (format 1 or 2); false when a level's bytes reach 2^32 (the
+// tables hold 32-bit byte offsets) or the format has no paired slots
+bool cone_tab_plan(const VolumeDesc& vd, ConeTab& t);
+size_t cone_tab_lds_bytes(const ConeTab& t);
 
 // arguments of one cone-trace launch
 struct ConeLaunch {
@@ -228,6 +311,7 @@ struct ConeLaunch {
     MlpDev mlp;
     int learned;
     const float4* nmap;  // noise-march map (32^3 decoded RGBA8_SNORM texels) or null: no noise march
+    int table_addressing = 1;  // front-to-back RGBA traces: corner offsets from per-level LDS tables
 };
 void launch_cone(const ConeLaunch& a, hipStream_t s);
 void launch_segments(const GridDesc& g, const VolumeDesc& vd, const ConeDesc& cd, const void* levels,

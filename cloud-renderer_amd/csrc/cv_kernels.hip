@@ -274,17 +274,21 @@ __device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv,
 // kernels (256-thread workgroups): they find room on CUs that the cone trace
 // of the previous frame keeps busy, where one 1024-thread workgroup waited
 // for a whole CU to drain.
-__global__ void __launch_bounds__(256) k_light_boxes(GridDesc g, LightView lv, const float4* __restrict__ boards,
-                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void k_light_boxes_body(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts, int bid, int nblk) {
+    const int j = bid * 256 + threadIdx.x;
     if (j >= S) return;
     int4 box;
     counts[j] = light_box(g, lv, boards[j], &box);
     lbox[j] = box;
 }
+__global__ void __launch_bounds__(256) k_light_boxes(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts) {
+    k_light_boxes_body(g, lv, boards, S, lbox, counts, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // one workgroup: 4096 counts per round, 16 consecutive per thread
-__global__ void __launch_bounds__(256) k_light_scan(const int* __restrict__ counts, int S, int* __restrict__ offsets) {
+__device__ __forceinline__ void k_light_scan_body(const int* __restrict__ counts, int S, int* __restrict__ offsets) {
     __shared__ int wsum[4];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int carry = 0;
@@ -322,6 +326,9 @@ __global__ void __launch_bounds__(256) k_light_scan(const int* __restrict__ coun
     }
     if (tid == 0) offsets[S] = carry;
 }
+__global__ void __launch_bounds__(256) k_light_scan(const int* __restrict__ counts, int S, int* __restrict__ offsets) {
+    k_light_scan_body(counts, S, offsets);
+}
 
 // The brick box of the chord slab behind a tile (chord fill only): the world
 // box of c + du*S + dv*U + t*B, du/dv over the tile's view-plane rectangle
@@ -358,21 +365,25 @@ __device__ void slab_box(const GridDesc& g, const LightView& lv, const LightSphe
 
 // one thread per billboard enumerates its items (tiles in row order):
 // (billboard, tile) into the first two words of each
-__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ offsets, int S,
-                                                      RasterItem* __restrict__ items, int cap) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void k_light_expand_body(const int* __restrict__ offsets, int S,
+                                                      RasterItem* __restrict__ items, int cap, int bid, int nblk) {
+    const int j = bid * 256 + threadIdx.x;
     if (j >= S) return;
     const int o0 = offsets[j], c = offsets[j + 1] - o0;
     for (int t = 0; t < c && o0 + t < cap; ++t) *(int2*)&items[o0 + t] = make_int2(j, t);
 }
+__global__ void __launch_bounds__(256) k_light_expand(const int* __restrict__ offsets, int S,
+                                                      RasterItem* __restrict__ items, int cap) {
+    k_light_expand_body(offsets, S, items, cap, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // one thread per item: the tile's rectangle, the billboard's light-view data
 // and the slab box
-__global__ void __launch_bounds__(256) k_light_prep(GridDesc g, LightView lv, const float4* __restrict__ boards,
+__device__ __forceinline__ void k_light_prep_body(GridDesc g, LightView lv, const float4* __restrict__ boards,
                                                     const int4* __restrict__ lbox, const int* __restrict__ total_ptr,
-                                                    RasterItem* __restrict__ items, int cap, int chord, int words) {
+                                                    RasterItem* __restrict__ items, int cap, int chord, int words, int bid, int nblk) {
     const int n = min(*total_ptr, cap);
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = bid * 256 + threadIdx.x; i < n; i += nblk * 256) {
     const int2 jt = *(const int2*)&items[i];
     const int j = jt.x, t = jt.y;
     const LightSphere o = light_sphere(g, lv, boards[j]);
@@ -391,6 +402,11 @@ __global__ void __launch_bounds__(256) k_light_prep(GridDesc g, LightView lv, co
     items[i] = it;
     }
 }
+__global__ void __launch_bounds__(256) k_light_prep(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                    const int4* __restrict__ lbox, const int* __restrict__ total_ptr,
+                                                    RasterItem* __restrict__ items, int cap, int chord, int words) {
+    k_light_prep_body(g, lv, boards, lbox, total_ptr, items, cap, chord, words, (int)blockIdx.x, (int)gridDim.x);
+}
 
 constexpr int kTilePixels = kLightTile * kLightTile;
 
@@ -398,12 +414,12 @@ constexpr int kTilePixels = kLightTile * kLightTile;
 // the index map, and an exact k * stepSize chord counter); WORDS: size of the
 // LDS chord bitmap (bigger volumes have bigger slabs behind a tile)
 template <bool P2, int WORDS>
-__global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, int chord,
+__device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, int chord,
                                                       const int* __restrict__ total_ptr,
                                                       const RasterItem* __restrict__ items, int cap,
                                                       unsigned long long* __restrict__ keys,
                                                       unsigned long long* __restrict__ fillbits,
-                                                      unsigned long long* __restrict__ counters) {
+                                                      unsigned long long* __restrict__ counters, int bid, int nblk) {
     // Chord stores of one tile stay inside the sphere's slab behind that
     // tile: gather them in an LDS bitmap over the slab's brick box (1-voxel
     // margin for rounding) and flush non-zero words with one global atomic
@@ -428,13 +444,13 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
     // each (vector loads, so the LDS waits in between do not wait for them)
     // and the fields are read off those lanes at the next item; the last
     // item fetches itself again.
-    if ((int)blockIdx.x >= total) return;
+    if ((int)bid >= total) return;
     constexpr int kItemWords = (int)(sizeof(RasterItem) / 4);
     const int lane = tid & 63;
-    int item = blockIdx.x;
+    int item = bid;
     int iw = lane < kItemWords ? ((const int*)(items + item))[lane] : 0;
-    for (; item < total; item += gridDim.x) {
-    const int iwn = lane < kItemWords ? ((const int*)(items + min(item + (int)gridDim.x, total - 1)))[lane] : 0;
+    for (; item < total; item += nblk) {
+    const int iwn = lane < kItemWords ? ((const int*)(items + min(item + (int)nblk, total - 1)))[lane] : 0;
     auto field_i = [&](int k) { return __builtin_amdgcn_readlane(iw, k); };
     auto field_f = [&](int k) { return __int_as_float(__builtin_amdgcn_readlane(iw, k)); };
     const int j = field_i(0);
@@ -702,6 +718,15 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
         }
     }
 }
+template <bool P2, int WORDS>
+__global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, int chord,
+                                                      const int* __restrict__ total_ptr,
+                                                      const RasterItem* __restrict__ items, int cap,
+                                                      unsigned long long* __restrict__ keys,
+                                                      unsigned long long* __restrict__ fillbits,
+                                                      unsigned long long* __restrict__ counters) {
+    k_light_raster_body<P2, WORDS>(g, lv, chord, total_ptr, items, cap, keys, fillbits, counters, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // depth resolve + pass 2 (res/second_voxelize.glsl:34-51)
@@ -715,11 +740,11 @@ __global__ void __launch_bounds__(256) k_light_raster(GridDesc g, LightView lv, 
 constexpr int kSplatSlots = 512;
 constexpr uint32_t kSplatEmpty = 0xffffffffu;
 
-__global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, const float4* __restrict__ boards,
+__device__ __forceinline__ void k_light_splat_body(GridDesc g, LightView lv, const float4* __restrict__ boards,
                                                      const unsigned long long* __restrict__ keys,
                                                      float4* __restrict__ posmap,
                                                      unsigned long long* __restrict__ shellbits,
-                                                     unsigned long long* __restrict__ counters) {
+                                                     unsigned long long* __restrict__ counters, int bid, int nblk) {
     __shared__ uint32_t hkey[kSplatSlots];
     __shared__ unsigned long long hbits[kSplatSlots];
     const int t = threadIdx.x;
@@ -728,7 +753,7 @@ __global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, c
         __syncthreads();
     }
     const int tiles_x = (lv.W + 15) / 16;
-    const int px = (int)(blockIdx.x % tiles_x) * 16 + (t & 15), py = (int)(blockIdx.x / tiles_x) * 16 + (t >> 4);
+    const int px = (int)(bid % tiles_x) * 16 + (t & 15), py = (int)(bid / tiles_x) * 16 + (t >> 4);
     const bool inside = px < lv.W && py < lv.H;
     const size_t pix = (size_t)py * lv.W + px;
     const unsigned long long key = inside ? keys[pix] : ~0ull;
@@ -794,6 +819,13 @@ __global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, c
         for (int i = t; i < kSplatSlots; i += 256)
             if (hkey[i] != kSplatEmpty) atomicOr(&shellbits[hkey[i]], hbits[i]);
     }
+}
+__global__ void __launch_bounds__(256) k_light_splat(GridDesc g, LightView lv, const float4* __restrict__ boards,
+                                                     const unsigned long long* __restrict__ keys,
+                                                     float4* __restrict__ posmap,
+                                                     unsigned long long* __restrict__ shellbits,
+                                                     unsigned long long* __restrict__ counters) {
+    k_light_splat_body(g, lv, boards, keys, posmap, shellbits, counters, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1075,18 +1107,18 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
 // that its slots already hold zero returns at once; otherwise it writes and
 // records whether it wrote zeros.
 template <int FMT>
-__global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
+__device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
                                                      const unsigned long long* __restrict__ shellbits,
                                                      const unsigned long long* __restrict__ fillbits,
                                                      void* __restrict__ levels, uint8_t* __restrict__ zflags, int nlev,
-                                                     uint32_t rpa) {
+                                                     uint32_t rpa, int bid, int nblk) {
     static_assert(FMT == 1 || FMT == 2, "RGBA formats");
     __shared__ float2 stage[2048];  // [brick of the half region][Morton]
     __shared__ float2 l1[512];
     __shared__ float2 l2[64];
     __shared__ float2 l3[8];
     const uint32_t tid = threadIdx.x;
-    const uint32_t r = blockIdx.x;
+    const uint32_t r = bid;
     const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
     const bool last_x = rx + 1u == rpa;
     const uint32_t nb0 = g.nb0;
@@ -1233,6 +1265,14 @@ __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
         if (rx > 0u) store_half<FMT>(levels, brick_addr(vd.lv[4], rx - 1u, ry, rz), 1, p);
     }
 }
+template <int FMT>
+__global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
+                                                     const unsigned long long* __restrict__ shellbits,
+                                                     const unsigned long long* __restrict__ fillbits,
+                                                     void* __restrict__ levels, uint8_t* __restrict__ zflags, int nlev,
+                                                     uint32_t rpa) {
+    k_build_fused_body<FMT>(g, vd, shellbits, fillbits, levels, zflags, nlev, rpa, (int)blockIdx.x, (int)gridDim.x);
+}
 
 // R8 variant of the fused build (one byte per texel, no neighbour halves)
 __global__ void __launch_bounds__(256) k_build_fused_r8(GridDesc g, VolumeDesc vd,
@@ -1366,7 +1406,7 @@ __global__ void __launch_bounds__(256) k_mip(VolumeDesc vd, int k, void* __restr
 // most 4096 texels each), every level kept in LDS for the next one (level
 // k0 - 1 is read from memory).  Same sums, order and rounding as k_mip.
 constexpr int kTailTexels = 4096;
-__global__ void __launch_bounds__(1024) k_mip_tail(VolumeDesc vd, int k0, void* __restrict__ levels) {
+__device__ __forceinline__ void k_mip_tail_body(VolumeDesc vd, int k0, void* __restrict__ levels) {
     __shared__ float2 buf[2][kTailTexels];  // (r, a), or the R8 value in .x; linear x + n (y + n z)
     const int tid = threadIdx.x;
     int cur = 0;
@@ -1423,14 +1463,41 @@ __global__ void __launch_bounds__(1024) k_mip_tail(VolumeDesc vd, int k0, void* 
         __syncthreads();
     }
 }
+__global__ void __launch_bounds__(1024) k_mip_tail(VolumeDesc vd, int k0, void* __restrict__ levels) {
+    k_mip_tail_body(vd, k0, levels);
+}
 
 // ---------------------------------------------------------------------------
 // camera pass setup: world/view centres, draw-order key, tile box
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, const float4* __restrict__ boards, int S,
-                                                   float4* __restrict__ sph_world, float4* __restrict__ sph_view,
-                                                   int4* __restrict__ sph_tiles, unsigned long long* __restrict__ keys) {
+// sort key of billboard j for CloudVolume::sortBoards(point): the distance
+// glm::distance(position + p_j, point) (src/CloudVolume.cpp:69) with its bits
+// inverted above the index, so an ascending sort of keys is far to near with
+// the lower index first among equal distances
+__device__ __forceinline__ unsigned long long board_key(float px, float py, float pz, float cx, float cy, float cz,
+                                                        int j) {
+    float ex = px - cx, ey = py - cy, ez = pz - cz;
+    float dist = sqrtf((ex * ex + ey * ey) + ez * ez);
+    return ((unsigned long long)(~__float_as_uint(dist)) << 32) | (unsigned long long)(uint32_t)j;
+}
+
+__global__ void __launch_bounds__(256) k_sort_keys(GridDesc g, const float4* __restrict__ boards, int S, float px,
+                                                   float py, float pz, unsigned long long* __restrict__ keys) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= S) return;
+    float4 b = boards[j];
+    keys[j] = board_key(px, py, pz, g.vpos[0] + b.x, g.vpos[1] + b.y, g.vpos[2] + b.z, j);
+}
+
+__global__ void __launch_bounds__(256) k_iota(int32_t* __restrict__ order, int S) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < S) order[j] = j;
+}
+
+__device__ __forceinline__ void k_cam_setup_body(GridDesc g, CameraView cv, const float4* __restrict__ boards, int S,
+                                                   float4* __restrict__ sph_world, float4* __restrict__ sph_view,
+                                                   int4* __restrict__ sph_tiles, unsigned long long* __restrict__ keys, int bid, int nblk) {
+    int j = bid * blockDim.x + threadIdx.x;
     if (j >= S) return;
     float4 b = boards[j];
     float cx = g.vpos[0] + b.x, cy = g.vpos[1] + b.y, cz = g.vpos[2] + b.z, s = b.w;
@@ -1440,9 +1507,7 @@ __global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, co
     sph_world[j] = make_float4(cx, cy, cz, s);
     sph_view[j] = make_float4(vx, vy, vz, 0.0f);
     // glm::distance(position + p, camera) (src/CloudVolume.cpp:69)
-    float ex = cv.eye[0] - cx, ey = cv.eye[1] - cy, ez = cv.eye[2] - cz;
-    float dist = sqrtf((ex * ex + ey * ey) + ez * ez);
-    keys[j] = ((unsigned long long)(~__float_as_uint(dist)) << 32) | (unsigned long long)(uint32_t)j;
+    keys[j] = board_key(cv.eye[0], cv.eye[1], cv.eye[2], cx, cy, cz, j);
     float dz = -vz;
     int4 tiles = make_int4(1, 0, 1, 0);  // empty
     if ((s > 0.0f) && (dz >= cv.znear && dz <= cv.zfar)) {
@@ -1460,6 +1525,11 @@ __global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, co
         }
     }
     sph_tiles[j] = tiles;
+}
+__global__ void __launch_bounds__(256) k_cam_setup(GridDesc g, CameraView cv, const float4* __restrict__ boards, int S,
+                                                   float4* __restrict__ sph_world, float4* __restrict__ sph_view,
+                                                   int4* __restrict__ sph_tiles, unsigned long long* __restrict__ keys) {
+    k_cam_setup_body(g, cv, boards, S, sph_world, sph_view, sph_tiles, keys, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1646,7 +1716,7 @@ __device__ __forceinline__ void wave_bitonic(unsigned long long (&v)[E], int lan
 // (~distance bits << 32 | index), distinct for distinct billboards, so the
 // ascending order is the draw order up to runs of equal distances.
 template <int E>
-__global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S,
+__device__ __forceinline__ void k_sort_lds_body(const unsigned long long* __restrict__ keys_in, int S,
                                                    int32_t* __restrict__ order, int* __restrict__ tie_flag) {
     extern __shared__ unsigned long long sk[];
     __shared__ int tie;
@@ -1731,6 +1801,11 @@ __global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __r
     replay_ties_block(S, s_idx, s_D, placed, order);
     if (tid == 0) *tie_flag = 1;
 }
+template <int E>
+__global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S,
+                                                   int32_t* __restrict__ order, int* __restrict__ tie_flag) {
+    k_sort_lds_body<E>(keys_in, S, order, tie_flag);
+}
 
 // global-memory bitonic stage for more billboards than one LDS holds
 __global__ void k_bitonic_global(unsigned long long* __restrict__ keys, int npad, int j, int k) {
@@ -1774,10 +1849,10 @@ __global__ void k_sort_replay_global(int S, const uint32_t* __restrict__ g_idx, 
 // wave per billboard, lanes striding over its tile box (a near billboard
 // covers ~1000 tiles, which one thread would walk alone)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles, const int32_t* __restrict__ order, int S,
+__device__ __forceinline__ void k_bin_body(const int4* __restrict__ sph_tiles, const int32_t* __restrict__ order, int S,
                                              int tiles_x, int words, unsigned long long* __restrict__ masks,
-                                             int* __restrict__ tile_count) {
-    const int r = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+                                             int* __restrict__ tile_count, int bid, int nblk) {
+    const int r = bid * 4 + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= S) return;
     const int4 t = sph_tiles[order[r]];
@@ -1791,6 +1866,11 @@ __global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles,
         atomicOr(&masks[((size_t)ty * tiles_x + tx) * words + w], bit);
         atomicAdd(&tile_count[ty * tiles_x + tx], 1);
     }
+}
+__global__ void __launch_bounds__(256) k_bin(const int4* __restrict__ sph_tiles, const int32_t* __restrict__ order, int S,
+                                             int tiles_x, int words, unsigned long long* __restrict__ masks,
+                                             int* __restrict__ tile_count) {
+    k_bin_body(sph_tiles, order, S, tiles_x, words, masks, tile_count, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Longest job first: tiles are bucketed by log2(number of binned billboards)
@@ -1971,6 +2051,66 @@ __device__ __forceinline__ float2 grey_filter(const GreyTaps& T) {
     return o;
 }
 
+// Table-addressed grey fetch (the cone kernel with per-level LDS tables):
+// the corners and loads of grey_fetch, their byte offsets read from the
+// level's tables (ConeTab) instead of computed from Morton offsets.  tx, ty,
+// tz point at the entries of floor(u) = 0; lvl is the level's first byte.
+template <int FMT>
+__device__ __forceinline__ void grey_fetch_tab(const char* __restrict__ lvl, const uint32_t* tx, const uint2* ty,
+                                               const uint2* tz, float fn, float sx, float sy, float sz, GreyTaps& T) {
+    static_assert(FMT == 1 || FMT == 2, "RGBA formats");
+    const float ux = fmaf(sx, fn, -0.5f), uy = fmaf(sy, fn, -0.5f), uz = fmaf(sz, fn, -0.5f);
+    int ix, iy, iz;  // floor(u) in [-1, n - 1] for (sx, sy, sz) in [0, 1]^3
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(ix) : "v"(ux));
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(iy) : "v"(uy));
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(iz) : "v"(uz));
+    T.ax = ix < 0 ? 0.0f : __builtin_amdgcn_fractf(ux);  // as grey_fetch: texel 0 below the first centre
+    T.ay = __builtin_amdgcn_fractf(uy);
+    T.az = __builtin_amdgcn_fractf(uz);
+    const uint32_t X = tx[ix];
+    const uint2 Y = ty[iy], Z = tz[iz];
+    const uint32_t o0 = X + Y.x + Z.x, o1 = X + Y.y + Z.x, o2 = X + Y.x + Z.y, o3 = X + Y.y + Z.y;
+    if constexpr (FMT == 1) {
+        const float4 q0 = *(const float4*)(lvl + o0), q1 = *(const float4*)(lvl + o1);
+        const float4 q2 = *(const float4*)(lvl + o2), q3 = *(const float4*)(lvl + o3);
+        T.t[0] = make_float2(q0.x, q0.y); T.t[1] = make_float2(q0.z, q0.w);
+        T.t[2] = make_float2(q1.x, q1.y); T.t[3] = make_float2(q1.z, q1.w);
+        T.t[4] = make_float2(q2.x, q2.y); T.t[5] = make_float2(q2.z, q2.w);
+        T.t[6] = make_float2(q3.x, q3.y); T.t[7] = make_float2(q3.z, q3.w);
+    } else {
+        const uint2 q0 = *(const uint2*)(lvl + o0), q1 = *(const uint2*)(lvl + o1);
+        const uint2 q2 = *(const uint2*)(lvl + o2), q3 = *(const uint2*)(lvl + o3);
+        auto h2 = [](uint32_t u) { return make_float2(h2f(u & 0xffffu), h2f(u >> 16)); };
+        T.t[0] = h2(q0.x); T.t[1] = h2(q0.y);
+        T.t[2] = h2(q1.x); T.t[3] = h2(q1.y);
+        T.t[4] = h2(q2.x); T.t[5] = h2(q2.y);
+        T.t[6] = h2(q3.x); T.t[7] = h2(q3.y);
+    }
+}
+
+// One level's tables and first byte, as the table-addressed march holds them
+struct TabLevel {
+    const char* lvl;
+    const uint32_t* tx;
+    const uint2* ty;
+    const uint2* tz;
+};
+// the LDS tables of a workgroup and their per-level offsets (kernel argument)
+struct TabRef {
+    const uint32_t* lds;
+    const ConeTab* t;
+};
+template <int FMT>
+__device__ __forceinline__ TabLevel tab_level(const TabRef& tr, const void* vol, const LevelDesc& L, int k) {
+    constexpr uint64_t sb = FMT == 1 ? 16u : 8u;
+    TabLevel r;
+    r.lvl = (const char*)vol + L.base * sb;
+    r.tx = tr.lds + tr.t->x[k] + 1;
+    r.ty = (const uint2*)(tr.lds + tr.t->y[k]) + 1;
+    r.tz = (const uint2*)(tr.lds + tr.t->z[k]) + 1;
+    return r;
+}
+
 __device__ __forceinline__ int step_level(const ConeDesc& cd, int i) {
     return (int)((cd.llev4[i >> 2] >> (8 * (i & 3))) & 0xffu);
 }
@@ -2025,10 +2165,11 @@ struct FtbState {
 
 // RGBA volumes are grey (r = g = b), so only r and a are filtered --
 // bit-identical to filtering all four channels.  KAHAN: compensated sums.
-template <int FMT, bool KAHAN>
+// TAB: corner offsets from the workgroup's LDS tables (bit-identical loads).
+template <int FMT, bool KAHAN, bool TAB>
 __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
                                            float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
-                                           int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
+                                           int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo, const TabRef& tr) {
     if (i0 >= i1) return 0;
     // a step's scalars are loaded one step ahead, its level descriptors only
     // when the level changes (at most once per level over a march)
@@ -2036,6 +2177,7 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
     uint32_t lw_n = cd.llev4[i0 >> 2];  // the level's word, shifted at its step
     int cur = -1;
     LevelDesc LA{}, LB{};
+    TabLevel TA{}, TB{};
     for (int i = i0; i < i1; ++i) {
         const float h = h_n, fr = f_n;
         const int lv = (int)((lw_n >> (8 * (i & 3))) & 0xffu);
@@ -2045,8 +2187,13 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
             lw_n = cd.llev4[(i + 1) >> 2];
         }
         if (lv != cur) {
+            const int lb = lv + 1 < kMaxLevels ? lv + 1 : lv;
             LA = vd.lv[lv];
-            LB = vd.lv[lv + 1 < kMaxLevels ? lv + 1 : lv];
+            LB = vd.lv[lb];
+            if constexpr (TAB) {
+                TA = tab_level<FMT>(tr, vol, LA, lv);
+                TB = tab_level<FMT>(tr, vol, LB, lb);
+            }
             cur = lv;
             // wait here (scalar loads complete out of order, so a later wait
             // for them would also wait for the step prefetch above)
@@ -2055,7 +2202,23 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
         float sx = fmaf(h, dx, p0x), sy = fmaf(h, dy, p0y), sz = fmaf(h, dz, p0z);
         if (!(sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f)) return 1;
         bool two;
-        float4 sm = sample_levels<FMT, 2>(vol, LA, LB, fr, sx, sy, sz, &two);
+        float4 sm;
+        if constexpr (TAB) {
+            // sample_levels<FMT, 2> with table addressing
+            two = (__float_as_uint(fr) << 1) != 0u;
+            GreyTaps A, B;
+            grey_fetch_tab<FMT>(TA.lvl, TA.tx, TA.ty, TA.tz, LA.fn, sx, sy, sz, A);
+            if (two) grey_fetch_tab<FMT>(TB.lvl, TB.tx, TB.ty, TB.tz, LB.fn, sx, sy, sz, B);
+            float2 a = grey_filter(A);
+            if (two) {
+                float2 b = grey_filter(B);
+                a.x = lerpF(a.x, b.x, fr);
+                a.y = lerpF(a.y, b.y, fr);
+            }
+            sm = make_float4(a.x, a.x, a.x, a.y);
+        } else {
+            sm = sample_levels<FMT, 2>(vol, LA, LB, fr, sx, sy, sz, &two);
+        }
         ntwo += two ? 1u : 0u;
         float al;
         if (cd.cone_mode == 1) {
@@ -2079,12 +2242,14 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
     }
     return 0;
 }
-template <int FMT>
+template <int FMT, bool TAB = false>
 __device__ __forceinline__ int march_ftb(const VolumeDesc& vd, const void* __restrict__ vol, const ConeDesc& cd,
                                          float p0x, float p0y, float p0z, float dx, float dy, float dz, int i0,
-                                         int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo) {
-    return cd.compensated ? march_ftb_t<FMT, true>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dz, i0, i1, st, nsamp, ntwo)
-                          : march_ftb_t<FMT, false>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dz, i0, i1, st, nsamp, ntwo);
+                                         int i1, FtbState& st, uint32_t& nsamp, uint32_t& ntwo,
+                                         const TabRef& tr = TabRef{nullptr, nullptr}) {
+    return cd.compensated
+               ? march_ftb_t<FMT, true, TAB>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dz, i0, i1, st, nsamp, ntwo, tr)
+               : march_ftb_t<FMT, false, TAB>(vd, vol, cd, p0x, p0y, p0z, dx, dy, dz, i0, i1, st, nsamp, ntwo, tr);
 }
 
 // Far-field features after k samples: last sample position (2x - 1), cone
@@ -2302,7 +2467,7 @@ constexpr int kConeChunk = 256;
 // two-level sampling branch) stays uniform across the wave, and in the
 // learned kernels the lanes reach step k together: their cones form the
 // wave's 64-row MLP batch.
-template <int FMT, int CONE, int COMP, int LEARNED, int NM = 0>
+template <int FMT, int CONE, int COMP, int LEARNED, int NM = 0, bool TAB = false>
 __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& vd, const CameraView& cv,
                                           const ConeDesc& cd, const void* __restrict__ vol,
                                           const float4* __restrict__ sph_world, const float4* __restrict__ sph_view,
@@ -2311,7 +2476,8 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                                           void* __restrict__ image, const MlpDev& mlp, uint16_t* xs,
                                           unsigned long long* cover, int* id, int tile, int lane, uint32_t& nfrag,
                                           uint32_t& nsamp, uint32_t& ntwo, uint32_t& nrows, uint32_t& nfall,
-                                          uint32_t& nevals, const float4* __restrict__ nmap = nullptr) {
+                                          uint32_t& nevals, const float4* __restrict__ nmap = nullptr,
+                                          const TabRef& tr = TabRef{nullptr, nullptr}) {
     const int tx = tile % cv.tiles_x, ty = tile / cv.tiles_x;
     const int px = tx * cv.tile_w + (lane & (cv.tile_w - 1)), py = ty * cv.tile_h + (lane >> cv.tile_shift);
     const bool valid = px < cv.W && py < cv.H;
@@ -2453,8 +2619,8 @@ __device__ __forceinline__ void cone_tile(const GridDesc& g, const VolumeDesc& v
                     s0 = s1 = s2 = s3 = clamp01(color);
                     s0u = color;
                 } else {
-                    const int rs = march_ftb<FMT>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0,
-                                                        kk_steps, st, nsamp, ntwo);
+                    const int rs = march_ftb<FMT, TAB>(vd, vol, cd, f.p0x, f.p0y, f.p0z, f.dx, f.dy, f.dz, 0,
+                                                             kk_steps, st, nsamp, ntwo, tr);
                     alive = LEARNED && rs == 0 && kk_steps < cd.steps;
                 }
             }
@@ -2562,6 +2728,68 @@ __global__ void __launch_bounds__(64, LEARNED ? 3 : (NM ? 4 : 6)) k_cone(GridDes
                 atomicAdd(&counters[7], e1);
                 atomicAdd(&counters[8], (unsigned long long)nevals);
             }
+        }
+    }
+}
+
+// Build a workgroup's LDS tables (ConeTab): for level k of side n and
+// j = floor(u) + 1 in [0, n], the byte offsets (from the level's first
+// texel) of the clamped corners c0 = max(j - 1, 0) and c1 = min(j, n - 1):
+// x of c0 (the slot holds x and x + 1), y and z of (c0, c1).
+template <int FMT>
+__device__ void build_cone_tabs(const VolumeDesc& vd, const ConeTab& t, uint32_t* lds) {
+    constexpr uint32_t sb = FMT == 1 ? 16u : 8u;
+    for (int k = 0; k < t.levels; ++k) {
+        const LevelDesc L = vd.lv[k];
+        const uint32_t n = L.n, srow = L.nb << 6, sslab = (L.nb * L.nb) << 6;
+        uint32_t* tx = lds + t.x[k];
+        uint2* ty = (uint2*)(lds + t.y[k]);
+        uint2* tz = (uint2*)(lds + t.z[k]);
+        for (uint32_t j = threadIdx.x; j <= n; j += blockDim.x) {
+            const uint32_t c0 = j > 0u ? j - 1u : 0u, c1 = min(j, n - 1u);
+            tx[j] = morton_off_x(c0) * sb;
+            ty[j] = make_uint2(morton_off_y(c0, srow) * sb, morton_off_y(c1, srow) * sb);
+            tz[j] = make_uint2(morton_off_z(c0, sslab) * sb, morton_off_z(c1, sslab) * sb);
+        }
+    }
+    __syncthreads();
+}
+
+// The front-to-back cone trace with table addressing: t.wpg waves per
+// workgroup share the LDS tables of every level (built once per
+// workgroup); wave w of workgroup b shades tile tile_order[b * wpg + w].
+// Dynamic LDS: the tables (t.words words), then per wave the gather lists
+// (kConeChunk coverage masks and ids).
+template <int FMT, int COMP>
+__global__ void __launch_bounds__(512, 6) k_cone_tab(GridDesc g, VolumeDesc vd, CameraView cv, ConeDesc cd, ConeTab t,
+                                                     const void* __restrict__ vol,
+                                                     const float4* __restrict__ sph_world,
+                                                     const float4* __restrict__ sph_view,
+                                                     const int32_t* __restrict__ order,
+                                                     const unsigned long long* __restrict__ masks, int words,
+                                                     void* __restrict__ image, unsigned long long* __restrict__ counters,
+                                                     const int* __restrict__ tile_order, int ntiles) {
+    extern __shared__ __align__(16) uint32_t dyn[];
+    build_cone_tabs<FMT>(vd, t, dyn);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int item = blockIdx.x * t.wpg + wave;
+    if (item >= ntiles) return;
+    unsigned long long* cover = (unsigned long long*)(dyn + t.words) + (size_t)wave * kConeChunk;
+    int* id = (int*)((unsigned long long*)(dyn + t.words) + (size_t)t.wpg * kConeChunk) + (size_t)wave * kConeChunk;
+    const int tile = tile_order ? tile_order[item] : item;
+    uint32_t nfrag = 0, nsamp = 0, ntwo = 0, nrows = 0, nfall = 0, nevals = 0;
+    const MlpDev none{nullptr, nullptr, nullptr};
+    const TabRef tr{dyn, &t};
+    cone_tile<FMT, 1, COMP, 0, 0, true>(g, vd, cv, cd, vol, sph_world, sph_view, order, masks, words, image, none,
+                                        nullptr, cover, id, tile, lane, nfrag, nsamp, ntwo, nrows, nfall, nevals,
+                                        nullptr, tr);
+    if (counters) {
+        unsigned long long a = wave_sum_u64(nfrag), b = wave_sum_u64(nsamp), c = wave_sum_u64(ntwo);
+        if (lane == 0) {
+            atomicAdd(&counters[3], a);
+            atomicAdd(&counters[4], b);
+            atomicAdd(&counters[5], c);
         }
     }
 }
@@ -2883,6 +3111,18 @@ static int launch_sort_lds(const unsigned long long* keys, int S, int32_t* order
     return 1024 * E;
 }
 
+void launch_sort_keys(const GridDesc& g, const float4* boards, int S, const float point[3], unsigned long long* keys,
+                      hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks_for(S, 256)), dim3(256), 0, s, g, boards, S, point[0], point[1],
+                       point[2], keys);
+}
+
+void launch_iota(int32_t* order, int S, hipStream_t s) {
+    if (S <= 0) return;
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(S, 256)), dim3(256), 0, s, order, S);
+}
+
 int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s) {
     if (S <= 0) return 0;
     int npad = 2;
@@ -2989,11 +3229,51 @@ void launch_cone_batch(const BatchScene* scenes, const int* tile0, int nscenes, 
     else go(std::integral_constant<int, 2>{});
 }
 
+// dynamic LDS of k_cone_tab: the tables and, per wave, kConeChunk coverage
+// masks (8 bytes) and billboard ids (4 bytes)
+static size_t cone_tab_lds_bytes_for(const ConeTab& t, int wpg) {
+    return (size_t)t.words * 4u + (size_t)wpg * kConeChunk * (8u + 4u);
+}
+size_t cone_tab_lds_bytes(const ConeTab& t) { return cone_tab_lds_bytes_for(t, t.wpg); }
+
+bool cone_tab_plan(const VolumeDesc& vd, ConeTab& t) {
+    t = ConeTab{};
+    if (vd.format != 1 && vd.format != 2) return false;
+    const uint64_t sb = vd.format == 1 ? 16u : 8u;
+    uint32_t w = 0;
+    t.levels = vd.L;
+    for (int k = 0; k < vd.L; ++k) {
+        const LevelDesc& L = vd.lv[k];
+        const uint64_t bytes = (uint64_t)L.nb * L.nb * L.nb * 64u * sb;
+        if (bytes >= (1ull << 32)) return false;
+        const uint32_t n1 = L.n + 1u;
+        w = (w + 1u) & ~1u;  // the pair tables are read as uint2
+        t.y[k] = w; w += 2u * n1;
+        t.z[k] = w; w += 2u * n1;
+        t.x[k] = w; w += n1;
+    }
+    t.words = (w + 3u) & ~3u;  // the gather lists that follow are 16-byte aligned
+    // workgroups of 4 waves while four of them fit beside the VGPR-bound 6
+    // waves per SIMD, else 8 (the tables are shared by more waves)
+    t.wpg = cone_tab_lds_bytes_for(t, 4) * 6 <= 160u * 1024u ? 4 : 8;
+    return cone_tab_lds_bytes(t) <= 64u * 1024u;
+}
+
 // The noise-march kernels (NM = 1) are separate instantiations, so the plain
 // kernels keep their register budget.
 template <int FMT, int CONE, int COMP, int LEARNED>
 static void launch_cone_t(const ConeLaunch& a, hipStream_t s) {
     unsigned ntiles = (unsigned)(a.cv.tiles_x * a.cv.tiles_y);
+    if constexpr (!LEARNED && CONE == 1 && FMT != 0) {
+        ConeTab t;
+        if (!a.nmap && a.table_addressing && cone_tab_plan(a.vd, t)) {
+            const unsigned grid = (ntiles + (unsigned)t.wpg - 1u) / (unsigned)t.wpg;
+            hipLaunchKernelGGL((k_cone_tab<FMT, COMP>), dim3(grid), dim3(64 * t.wpg), cone_tab_lds_bytes(t), s, a.g,
+                               a.vd, a.cv, a.cd, t, a.levels, a.sph_world, a.sph_view, a.order, a.masks, a.words,
+                               a.image, a.counters, a.tile_order, (int)ntiles);
+            return;
+        }
+    }
     if constexpr (!LEARNED) {
         if (a.nmap) {
             hipLaunchKernelGGL((k_cone<FMT, CONE, COMP, 0, 1>), dim3(ntiles), dim3(64), 0, s, a.g, a.vd, a.cv, a.cd,
@@ -3059,6 +3339,220 @@ void launch_debrick(const VolumeDesc& vd, int k, const void* levels, void* out, 
     n = n * n * n;
     hipLaunchKernelGGL(k_debrick, dim3(blocks_for(n, 256)), dim3(256), 0, s, vd.lv[k], (int)texel_bytes(vd.format),
                        (const uint8_t*)levels, (uint8_t*)out);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched stage kernels: one launch per stage over the scenes of a batch.
+// Workgroup b finds its group member m by binary search over the stage's
+// first-workgroup table and runs the single-scene kernel's body on scene
+// scene[m] with its own workgroup index b - first[m] and count
+// first[m + 1] - first[m]: the same code and the same workgroup indices as
+// the scene's own launch, so every output is the same.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int stage_member(const BatchStage& st, int b) {
+    int lo = 0, hi = st.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (st.first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+struct StageSlot {
+    int sc, bid, nblk;
+};
+__device__ __forceinline__ StageSlot stage_slot(const BatchStage& st) {
+    const int m = stage_member(st, (int)blockIdx.x);
+    StageSlot r;
+    r.sc = __builtin_amdgcn_readfirstlane(st.scene[m]);
+    r.bid = (int)blockIdx.x - __builtin_amdgcn_readfirstlane(st.first[m]);
+    r.nblk = __builtin_amdgcn_readfirstlane(st.first[m + 1]) - __builtin_amdgcn_readfirstlane(st.first[m]);
+    return r;
+}
+
+__global__ void __launch_bounds__(256) kb_fill(const FillSeg* __restrict__ segs, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const FillSeg f = segs[q.sc];
+    const uint64_t e0 = (uint64_t)q.bid * kFillChunk, e1 = min(f.n, e0 + kFillChunk);
+    for (uint64_t e = e0 + threadIdx.x; e < e1; e += 256) f.p[e] = f.value;
+}
+__global__ void __launch_bounds__(256) kb_light_boxes(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_light_boxes_body(B.g, B.lv, B.boards, B.S, B.lbox, B.counts, q.bid, q.nblk);
+}
+__global__ void __launch_bounds__(256) kb_light_scan(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_light_scan_body(B.counts, B.S, B.offsets);
+}
+__global__ void __launch_bounds__(256) kb_light_expand(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_light_expand_body(B.offsets, B.S, B.items, (int)min(B.items_cap, 0x7fffffff), q.bid, q.nblk);
+}
+__global__ void __launch_bounds__(256) kb_light_prep(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_light_prep_body(B.g, B.lv, B.boards, B.lbox, B.offsets + B.S, B.items, (int)min(B.items_cap, 0x7fffffff), B.chord,
+                      B.words, q.bid, q.nblk);
+}
+template <bool P2, int WORDS>
+__global__ void __launch_bounds__(256) kb_light_raster(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_light_raster_body<P2, WORDS>(B.g, B.lv, B.chord, B.offsets + B.S, B.items, (int)min(B.items_cap, 0x7fffffff),
+                                   B.keys, B.fill, nullptr, q.bid, q.nblk);
+}
+__global__ void __launch_bounds__(256) kb_light_splat(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_light_splat_body(B.g, B.lv, B.boards, B.keys, nullptr, B.shell, nullptr, q.bid, q.nblk);
+}
+template <int FMT>
+__global__ void __launch_bounds__(256) kb_build_fused(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_build_fused_body<FMT>(B.g, B.vd, B.shell, B.vd.mode == 1 ? B.fill : nullptr, B.levels, B.zflags, B.nlev, B.rpa,
+                            q.bid, q.nblk);
+}
+__global__ void __launch_bounds__(1024) kb_mip_tail(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_mip_tail_body(B.vd, B.tail_k, B.levels);
+}
+__global__ void __launch_bounds__(256) kb_cam_setup(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_cam_setup_body(B.g, B.cv, B.boards, B.S, B.sph_world, B.sph_view, B.sph_tiles, B.sortkeys, q.bid, q.nblk);
+}
+__global__ void __launch_bounds__(256) kb_iota(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    const int j = q.bid * 256 + (int)threadIdx.x;
+    if (j < B.S) B.order[j] = j;
+}
+template <int E>
+__global__ void __launch_bounds__(1024) kb_sort_lds(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_sort_lds_body<E>(B.sortkeys, B.S, B.order, B.tie);
+}
+__global__ void __launch_bounds__(256) kb_bin(const BatchVox* __restrict__ sc, BatchStage st) {
+    const StageSlot q = stage_slot(st);
+    const BatchVox& B = sc[q.sc];
+    k_bin_body(B.sph_tiles, B.order, B.S, B.cv.tiles_x, B.mask_words, B.masks, B.tile_count, q.bid, q.nblk);
+}
+
+int fill_segment_blocks(uint64_t n) { return (int)((n + kFillChunk - 1) / kFillChunk); }
+
+void launch_fill_segments(const FillSeg* d_segs, const int* d_first, int nseg, int total_blocks, hipStream_t s) {
+    if (total_blocks <= 0) return;
+    const BatchStage st{d_first, d_first + nseg + 1, nseg};
+    hipLaunchKernelGGL(kb_fill, dim3((unsigned)total_blocks), dim3(256), 0, s, d_segs, st);
+}
+
+bool batch_vox_eligible(const GridDesc& g, const VolumeDesc& vd, int S) {
+    if (vd.format != 1 && vd.format != 2) return false;
+    if (g.N % 16 != 0 || S > kBatchSortMax) return false;
+    for (int k = 5; k < vd.L; ++k)
+        if (vd.lv[k].n * vd.lv[k].n * vd.lv[k].n > (uint32_t)kTailTexels) return false;
+    return true;
+}
+
+static int sort_width(int S) {
+    int npad = 1024;
+    while (npad < S) npad <<= 1;
+    return npad / 1024;
+}
+
+void batch_stage_shape(int stage, const BatchVox& b, int* blocks, int* variant) {
+    const int S = b.S;
+    *variant = 0;
+    *blocks = 0;
+    switch (stage) {
+        case kBsBoxes:
+        case kBsExpand:
+        case kBsCam:
+            *blocks = S > 0 ? (S + 255) / 256 : 0;
+            break;
+        case kBsScan:
+            *blocks = S > 0 ? 1 : 0;
+            break;
+        case kBsPrep:
+            *blocks = S > 0 ? (int)std::min<size_t>(((size_t)b.items_cap + 255) / 256, 2048) : 0;
+            break;
+        case kBsRaster: {
+            *blocks = S > 0 ? (int)std::min<size_t>((size_t)b.items_cap, 4096) : 0;
+            const bool p2 = b.g.pow2_range && b.g.pow2_N && b.g.pow2_step;
+            *variant = p2 ? (raster_words(b.g) == 2048 ? 0 : 1) : 2;
+            break;
+        }
+        case kBsSplat:
+            *blocks = (int)(((b.lv.W + 15) / 16) * ((b.lv.H + 15) / 16));
+            break;
+        case kBsBuild:
+            *blocks = (int)(b.rpa * b.rpa * b.rpa);
+            break;
+        case kBsTail:
+            *blocks = b.tail_k < b.vd.L ? 1 : 0;
+            break;
+        case kBsSortIota:
+            *blocks = (S > 0 && b.order_mode == 1) ? (S + 255) / 256 : 0;
+            break;
+        case kBsSort:
+            *blocks = (S > 0 && b.order_mode != 1) ? 1 : 0;
+            *variant = sort_width(S);
+            break;
+        case kBsBin:
+            *blocks = S > 0 ? (S + 3) / 4 : 0;
+            break;
+        default:
+            break;
+    }
+}
+
+void batch_vox_complete(BatchVox& b) {
+    b.words = raster_words(b.g);
+    b.rpa = (uint32_t)b.g.N / 16u;
+    b.nlev = b.vd.L < 5 ? b.vd.L : 5;
+    b.tail_k = b.nlev;
+}
+
+void launch_batch_stage(int stage, int variant, const BatchVox* scenes, const BatchStage& st, int total_blocks,
+                        hipStream_t s) {
+    if (total_blocks <= 0) return;
+    const dim3 grid((unsigned)total_blocks);
+    switch (stage) {
+        case kBsBoxes: hipLaunchKernelGGL(kb_light_boxes, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsScan: hipLaunchKernelGGL(kb_light_scan, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsExpand: hipLaunchKernelGGL(kb_light_expand, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsPrep: hipLaunchKernelGGL(kb_light_prep, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsRaster:
+            if (variant == 0) hipLaunchKernelGGL((kb_light_raster<true, 2048>), grid, dim3(256), 0, s, scenes, st);
+            else if (variant == 1) hipLaunchKernelGGL((kb_light_raster<true, 4096>), grid, dim3(256), 0, s, scenes, st);
+            else hipLaunchKernelGGL((kb_light_raster<false, 4096>), grid, dim3(256), 0, s, scenes, st);
+            break;
+        case kBsSplat: hipLaunchKernelGGL(kb_light_splat, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsBuild:
+            if (variant == 1) hipLaunchKernelGGL(kb_build_fused<1>, grid, dim3(256), 0, s, scenes, st);
+            else hipLaunchKernelGGL(kb_build_fused<2>, grid, dim3(256), 0, s, scenes, st);
+            break;
+        case kBsTail: hipLaunchKernelGGL(kb_mip_tail, grid, dim3(1024), 0, s, scenes, st); break;
+        case kBsCam: hipLaunchKernelGGL(kb_cam_setup, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsSortIota: hipLaunchKernelGGL(kb_iota, grid, dim3(256), 0, s, scenes, st); break;
+        case kBsSort: {
+            const size_t bytes = (size_t)1024 * variant * sizeof(unsigned long long);
+            if (variant == 1) hipLaunchKernelGGL(kb_sort_lds<1>, grid, dim3(1024), bytes, s, scenes, st);
+            else if (variant == 2) hipLaunchKernelGGL(kb_sort_lds<2>, grid, dim3(1024), bytes, s, scenes, st);
+            else if (variant == 4) hipLaunchKernelGGL(kb_sort_lds<4>, grid, dim3(1024), bytes, s, scenes, st);
+            else hipLaunchKernelGGL(kb_sort_lds<8>, grid, dim3(1024), bytes, s, scenes, st);
+            break;
+        }
+        case kBsBin: hipLaunchKernelGGL(kb_bin, grid, dim3(256), 0, s, scenes, st); break;
+        default: break;
+    }
 }
 
 }  // namespace cv

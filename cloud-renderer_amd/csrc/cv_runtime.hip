@@ -27,6 +27,7 @@ using namespace cv;
 
 struct cr_context_s {
     int device = 0;
+    unsigned flags = 0;  // CR_CONTEXT_*
     std::string err;
     int profiling = 0;
     cr_stats stats{};  // level 1: last call; level 2: sums since the last cr_get_stats
@@ -128,6 +129,14 @@ struct cr_volume_s {
     unsigned long long* d_sort = nullptr;
     int32_t* d_order = nullptr;
     int* d_tie = nullptr;
+    int order_mode = CR_ORDER_SORTED;
+    // cr_volume_sort_boards: keys and sort scratch, order, tie flag
+    unsigned long long* d_xsort = nullptr;
+    size_t xsort_cap = 0;
+    int32_t* d_xorder = nullptr;
+    size_t xorder_cap = 0;
+    int* d_xtie = nullptr;
+    size_t xtie_cap = 0;
     int* d_tiles = nullptr;  // binning / tile-queue scratch
     size_t tiles_cap = 0;
     int sph_cap = 0;
@@ -412,8 +421,8 @@ const char* cr_last_error(cr_context ctx) {
 }
 
 int cr_context_create(int device, unsigned flags, cr_context* out) {
-    (void)flags;
     if (!out) return fail(nullptr, CR_EINVAL, "cr_context_create: out is NULL");
+    if (flags & ~CR_CONTEXT_DIRECT_ADDRESSING) return fail(nullptr, CR_EINVAL, "cr_context_create: unknown flags");
     *out = nullptr;
     if (device < 0) return fail(nullptr, CR_EINVAL, "cr_context_create: negative device");
     int n = 0;
@@ -426,6 +435,7 @@ int cr_context_create(int device, unsigned flags, cr_context* out) {
     cr_context ctx = new (std::nothrow) cr_context_s();
     if (!ctx) return fail(nullptr, CR_ENOMEM, "cr_context_create: host allocation failed");
     ctx->device = device;
+    ctx->flags = flags;
     *out = ctx;
     return CR_OK;
 }
@@ -577,7 +587,7 @@ int cr_volume_destroy(cr_volume v) {
                     v->cam_spare.sph_world, v->cam_spare.sph_view, v->cam_spare.sph_tiles, v->cam_spare.sort,
                     v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks,
                     v->light_spare.lbox, v->light_spare.lcounts, v->light_spare.litems,
-                    v->light_spare.keys, v->d_nmap};
+                    v->light_spare.keys, v->d_nmap, v->d_xsort, v->d_xorder, v->d_xtie};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -1189,10 +1199,12 @@ static int cone_prepare(cr_volume v, const cr_light* light, const cr_camera* cam
 }
 
 // camera setup + draw order (kernels only)
+// (CR_ORDER_UPLOAD: the billboards are drawn in upload order, nothing sorted)
 static void cone_enqueue_sort(cr_volume v, const ConePlan& plan, StageEvents* ev, hipStream_t s) {
     launch_cam_setup(v->g, plan.cv, v->d_boards, plan.S, v->d_sph_world, v->d_sph_view, v->d_sph_tiles, v->d_sort, s);
     if (ev) ev->mark(1);
-    launch_sort(v->d_sort, plan.S, v->d_order, v->d_tie, s);
+    if (v->order_mode == CR_ORDER_UPLOAD) launch_iota(v->d_order, plan.S, s);
+    else launch_sort(v->d_sort, plan.S, v->d_order, v->d_tie, s);
     if (ev) ev->mark(2);
 }
 
@@ -1288,6 +1300,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     cl.counters = counters;
     cl.tile_order = tq.order;
     cl.learned = lo ? 1 : 0;
+    cl.table_addressing = (ctx->flags & CR_CONTEXT_DIRECT_ADDRESSING) ? 0 : 1;
     if (lo) cl.mlp = mlp_dev(lo->mlp);
     if (v->sun_on) {
         // SunShader::render before the clouds: centre Sun::position, clipped
@@ -1429,7 +1442,7 @@ int cr_volume_active_voxels(cr_volume v, uint64_t* count, void* stream) {
     return CR_OK;
 }
 
-static int scratch(cr_volume v, size_t bytes) {
+static int readback_scratch(cr_volume v, size_t bytes) {
     size_t cap = v->scratch_cap;
     int rc = ensure(v->ctx, &v->d_scratch, bytes, &cap);
     v->scratch_cap = cap;
@@ -1445,7 +1458,7 @@ int cr_volume_read_level(cr_volume v, int level, void* host_dst, size_t bytes, v
     if (bytes < need) return fail(ctx, CR_EINVAL, "cr_volume_read_level: destination too small");
     int rc = set_device(ctx);
     if (rc) return rc;
-    if ((rc = scratch(v, need))) return rc;
+    if ((rc = readback_scratch(v, need))) return rc;
     hipStream_t s = (hipStream_t)stream;
     launch_debrick(v->vd, level, v->d_levels, v->d_scratch, s);
     CR_HIP(ctx, hipGetLastError());
@@ -1461,7 +1474,7 @@ int cr_volume_read_occupancy(cr_volume v, uint8_t* host_dst, size_t bytes, void*
     if (bytes < need) return fail(ctx, CR_EINVAL, "cr_volume_read_occupancy: destination too small");
     int rc = set_device(ctx);
     if (rc) return rc;
-    if ((rc = scratch(v, need))) return rc;
+    if ((rc = readback_scratch(v, need))) return rc;
     hipStream_t s = (hipStream_t)stream;
     launch_read_state(v->g, v->d_shell, v->mode == CR_MODE_RGBA ? v->d_fill : nullptr, v->d_scratch, s);
     CR_HIP(ctx, hipGetLastError());
@@ -1486,6 +1499,38 @@ int cr_volume_read_position_map(cr_volume v, float* host_dst, size_t bytes, void
     launch_light_splat(v->g, v->last_lv, v->d_boards, v->d_keys, v->d_posmap, nullptr, nullptr, s);
     CR_HIP(ctx, hipGetLastError());
     CR_HIP(ctx, hipMemcpyAsync(host_dst, v->d_posmap, need, hipMemcpyDeviceToHost, s));
+    CR_HIP(ctx, hipStreamSynchronize(s));
+    return CR_OK;
+}
+
+int cr_volume_set_draw_order(cr_volume v, int order_mode) {
+    if (!v) return fail(nullptr, CR_EINVAL, "cr_volume_set_draw_order: vol is NULL");
+    if (order_mode != CR_ORDER_SORTED && order_mode != CR_ORDER_UPLOAD)
+        return fail(v->ctx, CR_EINVAL, "cr_volume_set_draw_order: mode must be CR_ORDER_SORTED or CR_ORDER_UPLOAD");
+    if (order_mode != v->order_mode) v->gen += 1;  // batch graphs bake the order in
+    v->order_mode = order_mode;
+    return CR_OK;
+}
+
+// CloudVolume::sortBoards(point) on the device, in buffers of its own (a
+// queued cone trace keeps reading its camera pair's draw order)
+int cr_volume_sort_boards(cr_volume v, const float point[3], int32_t* host_order, int count, void* stream) {
+    if (!v || !point || (!host_order && count > 0))
+        return fail(v ? v->ctx : nullptr, CR_EINVAL, "cr_volume_sort_boards: NULL argument");
+    cr_context ctx = v->ctx;
+    if (!finite3(point)) return fail(ctx, CR_EINVAL, "cr_volume_sort_boards: non-finite point");
+    if (count < v->S) return fail(ctx, CR_EINVAL, "cr_volume_sort_boards: count smaller than the billboard count");
+    if (v->S == 0) return CR_OK;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure(ctx, &v->d_xsort, sort_scratch_words(v->S), &v->xsort_cap))) return rc;
+    if ((rc = ensure(ctx, &v->d_xorder, (size_t)v->S, &v->xorder_cap))) return rc;
+    if ((rc = ensure(ctx, &v->d_xtie, (size_t)16, &v->xtie_cap))) return rc;  // 64 bytes
+    hipStream_t s = (hipStream_t)stream;
+    launch_sort_keys(v->g, v->d_boards, v->S, point, v->d_xsort, s);
+    launch_sort(v->d_xsort, v->S, v->d_xorder, v->d_xtie, s);
+    CR_HIP(ctx, hipGetLastError());
+    CR_HIP(ctx, hipMemcpyAsync(host_order, v->d_xorder, (size_t)v->S * 4, hipMemcpyDeviceToHost, s));
     CR_HIP(ctx, hipStreamSynchronize(s));
     return CR_OK;
 }
@@ -1593,15 +1638,14 @@ int cr_mlp_forward(cr_mlp m, const float* d_x, int n, float* d_y, void* stream) 
     return CR_OK;
 }
 
-// forward + backward over n rows; gradient partials per workgroup, reduced
-// in a fixed order (deterministic), optionally followed by an Adam step
-static int mlp_step(cr_mlp m, const float* d_x, const float* d_t, int n, float lr, int apply, hipStream_t s) {
+// forward + backward over n rows into the gradient partials of ceil(n / 64)
+// workgroups (each writes all of its own), reduced in a fixed order
+// (deterministic), optionally followed by an Adam step; kernels only
+static int mlp_launch_step(cr_mlp m, const float* d_x, const float* d_t, int n, float lr, int apply, float* partial,
+                           float* loss_part, hipStream_t s) {
     cr_context ctx = m->ctx;
     const int nwg = mlp_train_groups(n);
-    int rc = ensure(ctx, &m->d_partial, (size_t)nwg * kMlpParamsPad, &m->partial_cap);
-    if (rc) return rc;
-    if ((rc = ensure(ctx, &m->d_loss_part, (size_t)nwg, &m->loss_cap))) return rc;
-    launch_mlp_train(mlp_dev(m), d_x, d_t, n, m->d_partial, m->d_loss_part, s);
+    launch_mlp_train(mlp_dev(m), d_x, d_t, n, partial, loss_part, s);
     AdamStep st{};
     st.apply = apply;
     st.lr = lr;
@@ -1615,10 +1659,20 @@ static int mlp_step(cr_mlp m, const float* d_x, const float* d_t, int n, float l
     } else {
         st.bc1 = st.bc2 = 1.0f;
     }
-    launch_mlp_adam(m->d_partial, nwg, m->d_loss_part, m->d_master, m->d_m1, m->d_m2, m->d_grad, m->d_loss, st, s);
+    launch_mlp_adam(partial, nwg, loss_part, m->d_master, m->d_m1, m->d_m2, m->d_grad, m->d_loss, st, s);
     if (apply) launch_mlp_pack(m->d_master, m->d_fwd, m->d_bwd, m->d_bias, s);
     CR_HIP(ctx, hipGetLastError());
     return CR_OK;
+}
+
+// mlp_launch_step with partials in buffers of the cr_mlp, grown on demand
+static int mlp_step(cr_mlp m, const float* d_x, const float* d_t, int n, float lr, int apply, hipStream_t s) {
+    cr_context ctx = m->ctx;
+    const int nwg = mlp_train_groups(n);
+    int rc = ensure(ctx, &m->d_partial, (size_t)nwg * kMlpParamsPad, &m->partial_cap);
+    if (rc) return rc;
+    if ((rc = ensure(ctx, &m->d_loss_part, (size_t)nwg, &m->loss_cap))) return rc;
+    return mlp_launch_step(m, d_x, d_t, n, lr, apply, m->d_partial, m->d_loss_part, s);
 }
 
 int cr_mlp_gradient(cr_mlp m, const float* d_x, const float* d_t, int n, float* host_grad, float* host_loss,
@@ -1646,6 +1700,8 @@ int cr_mlp_train_step(cr_mlp m, const float* d_x, const float* d_t, int n, float
     return mlp_step(m, d_x, d_t, n, lr, 1, (hipStream_t)stream);
 }
 
+// cr_mlp_train_step with the partials in this device's `scratch`: no
+// allocation (the pointer was looked up when the context was created)
 int cr_mlp_last_loss(cr_mlp m, float* loss, void* stream) {
     if (!m || !loss) return fail(m ? m->ctx : nullptr, CR_EINVAL, "cr_mlp_last_loss: NULL argument");
     cr_context ctx = m->ctx;
@@ -1687,6 +1743,20 @@ struct cr_batch_s {
     std::vector<uint64_t> graph_gen;
     std::vector<const void*> graph_ptrs;  // per scene: the camera and work-list buffers the graph was captured with
     int total_tiles = 0;
+    // batched stage launches (scenes batch_vox_eligible takes): per-scene
+    // descriptors, the stages' first-workgroup and scene tables, fill segments
+    BatchVox* d_vox = nullptr;
+    size_t vox_cap = 0;
+    int* d_stage = nullptr;
+    size_t stage_cap = 0;
+    FillSeg* d_segs = nullptr;
+    size_t segs_cap = 0;
+    struct StageLaunch {
+        int stage, variant, first_off, n, blocks;
+    };
+    std::vector<StageLaunch> launches;
+    std::vector<char> batched;  // per scene: voxelization and camera pass in the batched stages
+    int fill_off = 0, nseg = 0, fill_blocks = 0;
 };
 
 namespace {
@@ -1741,9 +1811,102 @@ int batch_prepare(cr_batch b, std::vector<VoxPlan>& vp, std::vector<ConePlan>& c
     }
     b->total_tiles = total;
     int rc;
+    // batched stages: descriptors of the scenes they take, then per stage
+    // (and launch variant) the members' first workgroups and scene indices
+    std::vector<BatchVox> hv(n);
+    b->batched.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        cr_volume v = b->items[i].volume;
+        if (!batch_vox_eligible(v->g, v->vd, v->S)) continue;
+        b->batched[i] = 1;
+        BatchVox& X = hv[i];
+        X.g = v->g;
+        X.lv = vp[i].lv;
+        X.vd = v->vd;
+        X.cv = cp[i].cv;
+        X.boards = v->d_boards;
+        X.S = v->S;
+        X.chord = v->mode == CR_MODE_RGBA ? 1 : 0;
+        X.lbox = vp[i].ls.lbox;
+        X.counts = vp[i].ls.counts;
+        X.offsets = vp[i].ls.offsets;
+        X.items = vp[i].ls.items;
+        X.items_cap = (int)std::min<size_t>(vp[i].ls.items_cap, 0x7fffffff);
+        X.keys = v->d_keys;
+        X.shell = v->d_shell_s[0];
+        X.fill = v->d_fill_s[0];
+        X.levels = v->d_lset[0];
+        X.zflags = v->d_zflags[0];
+        X.sph_world = v->d_sph_world;
+        X.sph_view = v->d_sph_view;
+        X.sph_tiles = v->d_sph_tiles;
+        X.sortkeys = v->d_sort;
+        X.order = v->d_order;
+        X.tie = v->d_tie;
+        X.order_mode = v->order_mode;
+        X.masks = v->d_masks;
+        X.mask_words = cp[i].words;
+        batch_vox_complete(X);  // tile_count: once d_counts is sized below
+    }
+    if ((rc = ensure(ctx, &b->d_counts, (size_t)total + 2, &b->counts_cap))) return rc;
+    for (size_t i = 0; i < n; ++i)
+        if (b->batched[i]) hv[i].tile_count = b->d_counts + tile0[i];
+    std::vector<int> ints;
+    b->launches.clear();
+    for (int st = 0; st < kBsCount; ++st) {
+        // group the scenes by launch variant, in scene order
+        std::vector<std::pair<int, std::pair<int, int>>> mem;  // (variant, (scene, blocks))
+        for (size_t i = 0; i < n; ++i) {
+            if (!b->batched[i]) continue;
+            int blocks = 0, var = 0;
+            batch_stage_shape(st, hv[i], &blocks, &var);
+            if (st == kBsBuild) var = b->format;
+            if (blocks > 0) mem.push_back({var, {(int)i, blocks}});
+        }
+        std::stable_sort(mem.begin(), mem.end(),
+                         [](const auto& a, const auto& c) { return a.first < c.first; });
+        for (size_t a = 0; a < mem.size();) {
+            size_t e = a;
+            while (e < mem.size() && mem[e].first == mem[a].first) ++e;
+            cr_batch_s::StageLaunch L{st, mem[a].first, (int)ints.size(), (int)(e - a), 0};
+            int acc = 0;
+            for (size_t k = a; k < e; ++k) { ints.push_back(acc); acc += mem[k].second.second; }
+            ints.push_back(acc);
+            for (size_t k = a; k < e; ++k) ints.push_back(mem[k].second.first);
+            L.blocks = acc;
+            b->launches.push_back(L);
+            a = e;
+        }
+    }
+    // the fills: depth keys, occupancy words and tile masks of the batched
+    // scenes, and the tile counts of every scene
+    std::vector<FillSeg> segs;
+    for (size_t i = 0; i < n; ++i) {
+        if (!b->batched[i]) continue;
+        cr_volume v = b->items[i].volume;
+        segs.push_back({(uint64_t*)v->d_keys, (uint64_t)vp[i].npx, ~0ull});
+        segs.push_back({(uint64_t*)v->d_shell_s[0], (uint64_t)v->nbricks0, 0ull});
+        segs.push_back({(uint64_t*)v->d_fill_s[0], (uint64_t)v->nbricks0, 0ull});
+        segs.push_back({(uint64_t*)v->d_masks, (uint64_t)cp[i].nmask, 0ull});
+    }
+    segs.push_back({(uint64_t*)b->d_counts, (uint64_t)(total + 1) / 2, 0ull});
+    b->fill_off = (int)ints.size();
+    b->nseg = (int)segs.size();
+    {
+        int acc = 0;
+        for (const FillSeg& f : segs) { ints.push_back(acc); acc += fill_segment_blocks(f.n); }
+        ints.push_back(acc);
+        for (int k = 0; k < b->nseg; ++k) ints.push_back(k);
+        b->fill_blocks = acc;
+    }
+    if ((rc = ensure(ctx, &b->d_vox, n, &b->vox_cap))) return rc;
+    if ((rc = ensure(ctx, &b->d_stage, ints.size(), &b->stage_cap))) return rc;
+    if ((rc = ensure(ctx, &b->d_segs, segs.size(), &b->segs_cap))) return rc;
+    CR_HIP(ctx, hipMemcpy(b->d_vox, hv.data(), n * sizeof(BatchVox), hipMemcpyHostToDevice));
+    CR_HIP(ctx, hipMemcpy(b->d_stage, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+    CR_HIP(ctx, hipMemcpy(b->d_segs, segs.data(), segs.size() * sizeof(FillSeg), hipMemcpyHostToDevice));
     if ((rc = ensure(ctx, &b->d_scenes, n, &b->scenes_cap))) return rc;
     if ((rc = ensure(ctx, &b->d_tile0, n, &b->tile0_cap))) return rc;
-    if ((rc = ensure(ctx, &b->d_counts, (size_t)total + 2, &b->counts_cap))) return rc;
     if ((rc = ensure(ctx, &b->d_order, (size_t)total, &b->order_cap))) return rc;
     if ((rc = ensure(ctx, &b->d_oscratch, 64 + (size_t)total, &b->oscratch_cap))) return rc;
     if (!b->d_counters) {
@@ -1757,24 +1920,39 @@ int batch_prepare(cr_batch b, std::vector<VoxPlan>& vp, std::vector<ConePlan>& c
 
 // kernels only: voxelize every scene, camera setup / sort / bin every scene,
 // one heavy-first order over all tiles, one batched cone trace
+// kernels only.  Scenes the batched stages take: one segmented fill, then
+// one launch per voxelization stage and per camera-pass stage over all of
+// them; the others: their own per-scene launches.
 void batch_enqueue(cr_batch b, const std::vector<VoxPlan>& vp, const std::vector<ConePlan>& cp,
                    unsigned long long* counters, hipEvent_t* ev, hipStream_t s) {
     const size_t n = b->items.size();
     if (ev) (void)hipEventRecord(ev[0], s);
     if (counters) launch_fill_u64((uint64_t*)counters, 9, 0, s);
+    launch_fill_segments(b->d_segs, b->d_stage + b->fill_off, b->nseg, b->fill_blocks, s);  // tile counts included
+    auto stages = [&](bool camera) {
+        for (const auto& L : b->launches) {
+            if ((L.stage >= kBsCam) != camera) continue;
+            const BatchStage st{b->d_stage + L.first_off, b->d_stage + L.first_off + L.n + 1, L.n};
+            launch_batch_stage(L.stage, L.variant, b->d_vox, st, L.blocks, s);
+        }
+    };
+    stages(false);
     for (size_t i = 0; i < n; ++i) {
+        if (b->batched[i]) continue;
         cr_volume v = b->items[i].volume;
         voxelize_enqueue(v, vp[i], nullptr, nullptr, s, nullptr, s, s, 0, nullptr);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
-    launch_fill_u64((uint64_t*)b->d_counts, (uint64_t)(b->total_tiles + 1) / 2, 0, s);
+    stages(true);
     int t0 = 0;
     for (size_t i = 0; i < n; ++i) {
         cr_volume v = b->items[i].volume;
-        cone_enqueue_sort(v, cp[i], nullptr, s);
-        launch_fill_u64((uint64_t*)v->d_masks, cp[i].nmask, 0, s);
-        launch_bin_counts(v->d_sph_tiles, v->d_order, cp[i].S, cp[i].cv.tiles_x, cp[i].words, v->d_masks,
-                          b->d_counts + t0, s);
+        if (!b->batched[i]) {
+            cone_enqueue_sort(v, cp[i], nullptr, s);
+            launch_fill_u64((uint64_t*)v->d_masks, cp[i].nmask, 0, s);
+            launch_bin_counts(v->d_sph_tiles, v->d_order, cp[i].S, cp[i].cv.tiles_x, cp[i].words, v->d_masks,
+                              b->d_counts + t0, s);
+        }
         t0 += (int)cp[i].ntiles;
     }
     launch_tile_order(b->d_counts, b->total_tiles, b->d_oscratch, b->d_order, s);
@@ -1807,6 +1985,10 @@ bool graph_current(cr_batch b) {
 
 int batch_capture(cr_batch b) {
     cr_context ctx = b->ctx;
+    // a run of the previous graph may still read the descriptors and tables
+    // batch_prepare rewrites (captures are rare: new configuration, billboards
+    // or buffers)
+    CR_HIP(ctx, hipDeviceSynchronize());
     drop_graph(b);
     std::vector<VoxPlan> vp;
     std::vector<ConePlan> cp;
@@ -1883,7 +2065,8 @@ int cr_batch_destroy(cr_batch b) {
     if (!b) return CR_OK;
     (void)hipSetDevice(b->ctx->device);
     drop_graph(b);
-    void* ptrs[] = {b->d_scenes, b->d_tile0, b->d_counts, b->d_order, b->d_oscratch, b->d_counters};
+    void* ptrs[] = {b->d_scenes, b->d_tile0, b->d_counts, b->d_order, b->d_oscratch, b->d_counters,
+                    b->d_vox,    b->d_stage, b->d_segs};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -1940,6 +2123,7 @@ int cr_batch_run(cr_batch b, void* stream) {
         // eager run with counters and phase times
         std::vector<VoxPlan> vp;
         std::vector<ConePlan> cp;
+        CR_HIP(ctx, hipDeviceSynchronize());  // earlier runs may still read what batch_prepare rewrites
         if ((rc = batch_prepare(b, vp, cp))) return rc;
         hipEvent_t ev[4];
         for (int i = 0; i < 4; ++i) CR_HIP(ctx, hipEventCreate(&ev[i]));

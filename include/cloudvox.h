@@ -154,6 +154,9 @@ typedef struct {
 } cr_stats;
 
 /* ---- context ---------------------------------------------------------- */
+/* context flags */
+#define CR_CONTEXT_DIRECT_ADDRESSING 1u /* front-to-back cone traces compute every sample's texel addresses
+                                           instead of reading per-level offset tables from LDS (same images) */
 int cr_context_create(int device, unsigned flags, cr_context *out);
 int cr_context_destroy(cr_context ctx);
 const char *cr_last_error(cr_context ctx);
@@ -277,6 +280,28 @@ int cr_volume_read_occupancy(cr_volume vol, uint8_t *host_dst, size_t bytes, voi
 int cr_volume_read_position_map(cr_volume vol, float *host_dst, size_t bytes, void *stream);
 /* draw order of the last cr_cone_trace: order[r] = billboard drawn r-th (far to near) */
 int cr_volume_read_draw_order(cr_volume vol, int32_t *host_dst, int count, void *stream);
+
+/* ---- draw order as the reference keeps it ------------------------------ */
+/* CR_ORDER_SORTED (default): cr_cone_trace sorts the billboards far to near
+ * from the camera position first (CloudVolume::sortBoards) and draws them in
+ * that order.  CR_ORDER_UPLOAD: cr_cone_trace draws them in upload order and
+ * sorts nothing.  The reference does the latter: ConeTraceShader::coneTrace
+ * sorts its CPU arrays in place (src/Shaders/ConeTraceShader.cpp:20,
+ * src/CloudVolume.cpp:65-82) while its draw uses the buffers CloudVolume::
+ * update uploaded at the start of the frame, so frame f draws (and voxelizes)
+ * in the order sorted at frame f - 1, and each selection sort starts from
+ * the previous order, which decides equal distances.  A caller reproduces
+ * that with CR_ORDER_UPLOAD, cr_volume_sort_boards after the trace and the
+ * permuted arrays uploaded by the next cr_volume_set_billboards. */
+#define CR_ORDER_SORTED 0
+#define CR_ORDER_UPLOAD 1
+int cr_volume_set_draw_order(cr_volume vol, int order_mode);
+/* CloudVolume::sortBoards(point) (src/CloudVolume.cpp:65-82) on the device:
+ * host_order[r] = upload index of the billboard the selection sort puts at
+ * position r (far to near from point; equal distances exactly as the
+ * selection sort leaves them from the upload order).  count >= the number of
+ * billboards.  Synchronous; touches no buffer a queued cone trace reads. */
+int cr_volume_sort_boards(cr_volume vol, const float point[3], int32_t *host_order, int count, void *stream);
 
 /* ---- learned far field ------------------------------------------------ */
 /* No counterpart in the reference (it marches every cone to its end,

@@ -111,6 +111,22 @@ class CloudVolume {
         check(cr_volume_active_voxels(h_, &c, stream), ctx_->handle());
         return c;
     }
+    // CR_ORDER_SORTED (traces sort first) or CR_ORDER_UPLOAD (draw in upload order, as the reference's
+    // draw call uses the buffers update() uploaded)
+    void setDrawOrder(int mode) { check(cr_volume_set_draw_order(h_, mode), ctx_->handle()); }
+    // CloudVolume::sortBoards (src/CloudVolume.cpp:65-82): billboards permuted in place, far to near
+    // from point, equal distances as the selection sort leaves them; update() uploads the new order
+    void sortBoards(const vec3& point, void* stream = nullptr) {
+        const int n = billboards.count();
+        std::vector<int32_t> order((size_t)(n > 0 ? n : 1));
+        check(cr_volume_sort_boards(h_, point.data(), order.data(), (int)order.size(), stream), ctx_->handle());
+        Billboards sorted;
+        for (int r = 0; r < n; ++r) {
+            sorted.positions.push_back(billboards.positions[(size_t)order[(size_t)r]]);
+            sorted.scales.push_back(billboards.scales[(size_t)order[(size_t)r]]);
+        }
+        billboards = sorted;
+    }
     // level k, x-fastest, native element type (u8 / float / binary16)
     void readLevel(int level, void* dst, size_t bytes, void* stream = nullptr) const {
         check(cr_volume_read_level(h_, level, dst, bytes, stream), ctx_->handle());

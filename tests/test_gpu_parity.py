@@ -413,3 +413,71 @@ def test_runtime_refusals_camera_and_noise_steps(cv, gpu_ctx):
         torch.cuda.synchronize()
     finally:
         vol.close()
+
+
+@pytest.mark.parametrize("case", ["small32", "small32_fp16_noise", "npot96_plain", "small32_comp_blend", "plain256",
+                                  "fp16_512"])
+def test_table_addressing_matches_direct(cv, gpu_ctx, case):
+    """The front-to-back cone trace reads its corner offsets from per-level
+    LDS tables (4-wave workgroups, 8 at 512^3); a context created with
+    CR_CONTEXT_DIRECT_ADDRESSING computes them per sample.  Same images bit
+    for bit, at the small sizes and at the 256^3 / 512^3 workloads."""
+    sc = cv.scenes
+    scenes = {"plain256": sc.plain256(0), "fp16_512": sc.fp16_512(0)}
+    scene = scenes[case] if case in scenes else _scenes(cv)[case]
+    log(f"[tables] {case}: N={scene.N} {scene.W}x{scene.H} fmt={scene.fmt}, table vs direct addressing")
+    direct = cv.Context(0, flags=cv._abi.CR_CONTEXT_DIRECT_ADDRESSING)
+    try:
+        a = run_gpu(cv, scene, ctx=gpu_ctx, read_levels=False)
+        b = run_gpu(cv, scene, ctx=direct, read_levels=False)
+    finally:
+        direct.close()
+    assert_bit_equal("image (tables vs direct)", a.image, b.image)
+
+
+def test_reference_draw_order_frames(cv, gpu_ctx):
+    """The reference keeps its billboard arrays sorted in place
+    (src/CloudVolume.cpp:65-82) after the frame's draw call was set up, so
+    frame f voxelizes and draws in the order sorted at frame f - 1 and each
+    selection sort starts from the previous order.  With CR_ORDER_UPLOAD and
+    CloudVolume.sortBoards after each trace, three frames of the tie scene
+    under a moving camera equal the oracle run the same way (sort from the
+    current arrays, permute, upload): images, occupancy and every
+    permutation bit for bit, and the orders differ from fresh sorts."""
+    import torch
+    sc = cv.scenes
+    scene = sc.reference_mode(tie_scene(sc))
+    scene = replace(scene, cone=replace(scene.cone, down_scaling=16.0, quantize_rgba8=0))
+    eyes = [(0.0, 0.0, 3.0), (0.4, 0.3, 2.9), (-0.5, 0.1, 2.8)]
+    log("[draw order] tie scene, 3 frames, upload order + in-place sort")
+    vol, sun, _, vox, tr = cv.scene_objects(scene, 0, context=gpu_ctx)
+    xyz, scales = scene.xyz.copy(), scene.scales.copy()
+    try:
+        vol.set_draw_order(cv._abi.CR_ORDER_UPLOAD)
+        for f, eye in enumerate(eyes):
+            cam = cv.Camera(eye=eye)
+            vox.voxelize(vol, sun)
+            img = torch.zeros((scene.H, scene.W, 4), dtype=torch.float32, device="cuda")
+            tr.coneTrace(vol, sun, cam, img, scene.W, scene.H)
+            torch.cuda.synchronize()
+            got_img, got_state = img.cpu().numpy(), vol.read_occupancy()
+            perm = vol.sortBoards(eye)
+            vol.update()
+            # the oracle, run the same way on the current arrays
+            cur = replace(scene, xyz=xyz, scales=scales, cameras=[sc.Camera(eye=eye)])
+            vr = O.voxelize(cur)
+            nat, f64, _, _ = O.build_levels(cur, vr.shell, vr.fill)
+            ident = np.arange(cur.S, dtype=np.int32)
+            i32, _, _ = O.cone_trace(cur, nat, f64, ident, cur.cameras[0])
+            want_perm = O.sort_boards(cur, eye)
+            assert_bit_equal(f"frame {f} occupancy", got_state, vr.state())
+            assert_bit_equal(f"frame {f} image", got_img, i32)
+            assert np.array_equal(perm, want_perm), f"frame {f} permutation"
+            if f > 0:
+                fresh = O.sort_boards(replace(cur, xyz=scene.xyz, scales=scene.scales), eye)
+                log(f"[draw order] frame {f}: permuted arrays vs a fresh sort from upload order: "
+                    f"{int(np.count_nonzero(scene.xyz[fresh][:, 0] != xyz[want_perm][:, 0]))} positions differ")
+            xyz, scales = xyz[want_perm], scales[want_perm]
+            assert np.array_equal(vol.billboards.positions, xyz)
+    finally:
+        vol.close()
