@@ -500,6 +500,15 @@ class FarFieldMLP:
         _check(lib().cr_mlp_train_step(self.handle, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
                                        int(x.shape[0]), float(lr), _stream_handle(stream)), self.ctx.handle)
 
+    def train_step_static(self, x, t, lr: float, stream=None) -> None:
+        """train_step without allocation (cr_mlp_train_step_static): at most
+        CR_MLP_STATIC_ROWS rows; calls overlapping on one GPU race."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        _check(lib().cr_mlp_train_step_static(self.handle, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                              int(x.shape[0]), float(lr), _stream_handle(stream)), self.ctx.handle)
+
     def last_loss(self, stream=None) -> float:
         import torch
         loss = C.c_float()

@@ -96,6 +96,7 @@ class cr_batch_config(C.Structure):
 CR_MLP_INPUTS = 17
 CR_MLP_OUTPUTS = 5
 CR_MLP_PARAMS = 13957
+CR_MLP_STATIC_ROWS = 16384
 
 
 # name -> (restype, argtypes); every symbol declared in include/cloudvox.h
@@ -135,6 +136,7 @@ SIGNATURES = {
                                   C.POINTER(C.c_float), C.c_void_p]),
     "cr_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_void_p]),
     "cr_mlp_last_loss": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
+    "cr_mlp_train_step_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_void_p]),
     "cr_volume_segments": (C.c_int, [C.c_void_p, C.POINTER(cr_light), C.POINTER(cr_cone_params), C.c_int,
                                      C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cr_batch_create": (C.c_int, [C.c_void_p, C.POINTER(cr_batch_item), C.c_int, C.POINTER(C.c_void_p)]),

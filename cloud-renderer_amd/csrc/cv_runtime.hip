@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -28,6 +29,7 @@ using namespace cv;
 struct cr_context_s {
     int device = 0;
     unsigned flags = 0;  // CR_CONTEXT_*
+    float* static_scratch = nullptr;  // this device's copy of `scratch` (cr_mlp_train_step_static)
     std::string err;
     int profiling = 0;
     cr_stats stats{};  // level 1: last call; level 2: sums since the last cr_get_stats
@@ -420,6 +422,35 @@ const char* cr_last_error(cr_context ctx) {
     return g_last_error.c_str();
 }
 
+// Gradient and loss partials of cr_mlp_train_step_static: one fixed array per
+// GPU (the device copy of this file-scope variable).  Its host-side name is
+// not a device address; host code uses the pointer hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+namespace {
+constexpr int kStaticGroups = (CR_MLP_STATIC_ROWS + 63) / 64;
+constexpr size_t kStaticPartials = (size_t)kStaticGroups * kMlpParamsPad;
+}  // namespace
+static __device__ __align__(256) float scratch[kStaticPartials + kStaticGroups];
+
+static std::mutex g_scratch_mu;
+static std::vector<float*> g_scratch_dev;  // per device ordinal
+
+// the current device's copy of `scratch` (device = the current device)
+static int static_scratch_of(int device, float** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, CR_EDEVICE, "hipGetSymbolAddress(scratch): %s",
+                        e != hipSuccess ? hipGetErrorString(e) : "null address");
+        g_scratch_dev[device] = (float*)p;
+    }
+    *out = g_scratch_dev[device];
+    return CR_OK;
+}
+
 int cr_context_create(int device, unsigned flags, cr_context* out) {
     if (!out) return fail(nullptr, CR_EINVAL, "cr_context_create: out is NULL");
     if (flags & ~CR_CONTEXT_DIRECT_ADDRESSING) return fail(nullptr, CR_EINVAL, "cr_context_create: unknown flags");
@@ -432,10 +463,14 @@ int cr_context_create(int device, unsigned flags, cr_context* out) {
     if (device >= n) return fail(nullptr, CR_EINVAL, "cr_context_create: device index out of range");
     e = hipSetDevice(device);
     if (e != hipSuccess) return fail(nullptr, CR_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    float* sp = nullptr;
+    int rc = static_scratch_of(device, &sp);
+    if (rc) return rc;
     cr_context ctx = new (std::nothrow) cr_context_s();
     if (!ctx) return fail(nullptr, CR_ENOMEM, "cr_context_create: host allocation failed");
     ctx->device = device;
     ctx->flags = flags;
+    ctx->static_scratch = sp;
     *out = ctx;
     return CR_OK;
 }
@@ -1702,6 +1737,23 @@ int cr_mlp_train_step(cr_mlp m, const float* d_x, const float* d_t, int n, float
 
 // cr_mlp_train_step with the partials in this device's `scratch`: no
 // allocation (the pointer was looked up when the context was created)
+int cr_mlp_train_step_static(cr_mlp m, const float* d_x, const float* d_t, int n, float lr, void* stream) {
+    if (!m) return fail(nullptr, CR_EINVAL, "cr_mlp_train_step_static: mlp is NULL");
+    cr_context ctx = m->ctx;
+    if (!d_x || !d_t) return fail(ctx, CR_EINVAL, "cr_mlp_train_step_static: NULL array");
+    if (n < 1 || n > CR_MLP_STATIC_ROWS) {
+        char msg[128];
+        std::snprintf(msg, sizeof(msg), "n = %d outside [1, CR_MLP_STATIC_ROWS = %d]", n, CR_MLP_STATIC_ROWS);
+        return fail(ctx, CR_EINVAL, "cr_mlp_train_step_static: %s", msg);
+    }
+    if (!(lr >= 0.0f) || !std::isfinite(lr)) return fail(ctx, CR_EINVAL, "cr_mlp_train_step_static: bad learning rate");
+    if (!ctx->static_scratch) return fail(ctx, CR_ESTATE, "cr_mlp_train_step_static: no scratch array on this device");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    float* partial = ctx->static_scratch;
+    return mlp_launch_step(m, d_x, d_t, n, lr, 1, partial, partial + kStaticPartials, (hipStream_t)stream);
+}
+
 int cr_mlp_last_loss(cr_mlp m, float* loss, void* stream) {
     if (!m || !loss) return fail(m ? m->ctx : nullptr, CR_EINVAL, "cr_mlp_last_loss: NULL argument");
     cr_context ctx = m->ctx;

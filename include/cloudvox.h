@@ -335,6 +335,20 @@ int cr_mlp_gradient(cr_mlp mlp, const float *d_x, const float *d_t, int n, float
                     void *stream);
 int cr_mlp_train_step(cr_mlp mlp, const float *d_x, const float *d_t, int n, float lr, void *stream);
 int cr_mlp_last_loss(cr_mlp mlp, float *loss, void *stream);
+/* No counterpart in the reference (it trains nothing); for callers that
+ * cannot allocate: code under HIP graph capture, training loops that forbid
+ * hipMalloc.  cr_mlp_train_step without allocation: the per-workgroup
+ * gradient and loss partials of the step live in a fixed device array of the
+ * library (CR_MLP_STATIC_ROWS / 64 workgroups x 13960 floats, about 14 MiB;
+ * one copy per GPU, held until the process ends) instead of buffers that
+ * cr_mlp_train_step grows with hipMalloc.  n must be in
+ * [1, CR_MLP_STATIC_ROWS] (CR_EINVAL otherwise).  Same kernels and the same
+ * result as cr_mlp_train_step.  Every call on one GPU shares that array
+ * (every call writes all the partials it reads, so nothing needs clearing):
+ * calls that overlap on one GPU -- on other streams, from other cr_mlp
+ * objects or contexts -- race. */
+#define CR_MLP_STATIC_ROWS 16384
+int cr_mlp_train_step_static(cr_mlp mlp, const float *d_x, const float *d_t, int n, float lr, void *stream);
 
 /* Training rows from the plain march: for start points d_p0[n][3] (texture
  * space) the front-to-back cone toward the directional light is marched as
