@@ -64,6 +64,9 @@ struct VolumeDesc {
     int noise_octaves;
     uint32_t noise_seed;
     float noise_base_freq, noise_lacunarity, noise_gain;
+    // the density noise of every level-0 texel (level-0 slot order; binary16
+    // bits for RGBA16F volumes, float for RGBA32F), or null: computed in the build
+    const void* ntex;
 };
 
 struct CameraView {
@@ -168,6 +171,9 @@ void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* bo
 // region of the level set, 1 when everything the region's build writes holds
 // zero; regions that stay empty then write nothing
 size_t build_zero_flags(const GridDesc& g, const VolumeDesc& vd);
+// bytes of the density noise texture of an RGBA volume with noise (0: none)
+size_t noise_tex_bytes(const GridDesc& g, const VolumeDesc& vd);
+void launch_noise_tex(const GridDesc& g, const VolumeDesc& vd, void* ntex, hipStream_t s);
 void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned long long* shellbits,
                          const unsigned long long* fillbits, void* levels, uint8_t* zflags, hipStream_t s);
 void launch_mip(const VolumeDesc& vd, int k, void* levels, hipStream_t s);

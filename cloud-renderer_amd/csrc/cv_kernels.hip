@@ -160,6 +160,20 @@ __device__ __forceinline__ float cv_exp2_neg(float x) {
 
 // 1 << (b mod 64): v_lshlrev_b64 reads only the low 6 bits of its count, so
 // callers pass bit indices with unmasked high bits
+// bit index x | y << 2 | z << 4 of a voxel in its brick's fill word from the
+// voxel coordinates, exact in the low 6 bits (all a 64-bit shift reads): two
+// bit-field inserts instead of two masks, two shifts and an or
+__device__ __forceinline__ uint32_t fill_bit_index(uint32_t ix, uint32_t iy, uint32_t iz) {
+    const uint32_t xy = (ix & 3u) | ((iy << 2) & ~3u);
+    return (xy & 15u) | ((iz << 4) & ~15u);
+}
+// min of two floats that are never NaN: one v_min_f32 (fminf also quiets
+// signalling NaNs, which costs a canonicalising instruction per call)
+__device__ __forceinline__ float min_no_nan(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 __device__ __forceinline__ unsigned long long bit64_mod(uint32_t b) {
     unsigned long long r;
     asm("v_lshlrev_b64 %0, %1, 1" : "=v"(r) : "v"(b));
@@ -409,6 +423,7 @@ __global__ void __launch_bounds__(256) k_light_prep(GridDesc g, LightView lv, co
 }
 
 constexpr int kTilePixels = kLightTile * kLightTile;
+constexpr int kRasterThreads = 256;  // workgroup size of the raster kernels
 
 // P2: power-of-two extents (multiplications instead of IEEE divisions in
 // the index map, and an exact k * stepSize chord counter); WORDS: size of the
@@ -428,13 +443,18 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
     // With a power-of-two stepSize the chord loop counter i += stepSize is
     // exactly k * stepSize: covered fragments are queued in LDS with their
     // sample counts and the chords are walked from the queue, so no lane
-    // idles on a pixel outside the disc.
+    // idles on a pixel outside the disc.  The queue is ordered by sample
+    // count, longest first (a counting sort over 128 buckets), so the 64
+    // chords a wave walks together have nearly the same length.
     static_assert(WORDS <= 4096, "the flush decodes word indices below 4096 with float products");
+    static_assert(kTilePixels % kRasterThreads == 0, "whole pixel rounds");
+    constexpr int kRounds = kTilePixels / kRasterThreads;
     __shared__ unsigned long long lbits[WORDS];
     __shared__ float f_sx[kTilePixels], f_sy[kTilePixels], f_sz[kTilePixels];
     __shared__ int f_n[kTilePixels];
     __shared__ float h_x[kLightTile], h_y[kLightTile];  // the tile's pixel centres on the view plane
     __shared__ int nfr_s;
+    __shared__ int q_start[128];  // chord-length buckets: counts, then first queue slots
     const int tid = threadIdx.x;
     const int total = min(*total_ptr, cap);
     const bool flat = P2 && chord;
@@ -467,8 +487,9 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
         nbx = field_i(8); nby = field_i(9); nbz = field_i(10);
         use_lds = field_i(11) != 0;
         if (use_lds)
-            for (int w = tid; w < nbx * nby * nbz; w += blockDim.x) lbits[w] = 0ull;
+            for (int w = tid; w < nbx * nby * nbz; w += kRasterThreads) lbits[w] = 0ull;
         if (tid == 0) nfr_s = 0;
+        if (flat && tid < 128) q_start[tid] = 0;
     }
     if (tid < kLightTile) h_x[tid] = light_hx(lv, x0 + tid);
     else if (tid < 2 * kLightTile) h_y[tid - kLightTile] = light_hy(lv, y0 + tid - kLightTile);
@@ -488,7 +509,16 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
     // pass over the tile's pixels: depth keys; chords queued (flat) or walked
     const int tw = x1 - x0 + 1, th = y1 - y0 + 1;
     const float inv_tw = field_f(18);
-    for (int p = tid; p < tw * th; p += blockDim.x) {
+    // flat: this thread's queued chords (one per pixel round) and their
+    // ranks in their length bucket, placed once the buckets are counted
+    float q_sx[kRounds], q_sy[kRounds], q_sz[kRounds];
+    int q_n[kRounds], q_rank[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) { q_sx[r] = q_sy[r] = q_sz[r] = 0.0f; q_n[r] = 0; q_rank[r] = 0; }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int p = tid + r * kRasterThreads;
+        if (p >= tw * th) break;
         // p / tw as a float product: (p + 0.5) / tw lies at least 1 / (2 tw)
         // from an integer, far above the product's error for p < 1024
         const int ry = (int)(((float)p + 0.5f) * inv_tw), rx = p - ry * tw;
@@ -506,9 +536,9 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                 // k * stepSize < lim  <=>  k < lim / stepSize (exact: power-of-two scaling)
                 const int n = lim > 0.0f ? (int)ceilf(lim * g.inv_step) : 0;
                 if (n > 0) {
-                    const int slot = atomicAdd(&nfr_s, 1);
-                    f_sx[slot] = sx; f_sy[slot] = sy; f_sz[slot] = sz;
-                    f_n[slot] = n;
+                    q_sx[r] = sx; q_sy[r] = sy; q_sz[r] = sz;
+                    q_n[r] = n;
+                    q_rank[r] = atomicAdd(&q_start[min(n, 127)], 1);
                     nchord += (unsigned long long)n;
                 }
             } else {
@@ -543,9 +573,33 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
     }
     if (flat) {
         __syncthreads();
+        // bucket b's first slot = the count of the longer buckets (b' > b)
+        if (tid < 64) {
+            const int b0 = 127 - 2 * tid, b1 = 126 - 2 * tid;
+            const int h0 = q_start[b0], h1 = q_start[b1];
+            const int v = h0 + h1;
+            int incl = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o, 64);
+                if (tid >= o) incl += t;
+            }
+            q_start[b0] = incl - v;
+            q_start[b1] = incl - v + h0;
+            if (tid == 63) nfr_s = incl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            if (q_n[r] > 0) {
+                const int slot = q_start[min(q_n[r], 127)] + q_rank[r];
+                f_sx[slot] = q_sx[r]; f_sy[slot] = q_sy[r]; f_sz[slot] = q_sz[r];
+                f_n[slot] = q_n[r];
+            }
+        }
+        __syncthreads();
         // every lane walks one queued chord: no lane idles on a pixel
-        // outside the disc, and neighbouring queue slots (neighbouring
-        // pixels) have similar chord lengths
+        // outside the disc, and the 64 chords of a wave (consecutive slots
+        // of the length-sorted queue) have nearly the same length
         const int nfr = nfr_s;
         const float fN = (float)g.N;
         const float Bx = lv.B[0], By = lv.B[1], Bz = lv.B[2];
@@ -615,13 +669,13 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                     uint32_t ix, iy, iz;
                     voxel(0.0f, ix, iy, iz);
                     cur_w = word(ix, iy, iz);
-                    cur_b = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
+                    cur_b = bit64_mod(fill_bit_index(ix, iy, iz));
                     float iacc = 0.0f;
                     for (int k = 1; k < nmax; ++k) {
                         iacc += step;
-                        voxel(fminf(iacc, ilast), ix, iy, iz);
+                        voxel(min_no_nan(iacc, ilast), ix, iy, iz);
                         const int w = word(ix, iy, iz);
-                        const unsigned long long m = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
+                        const unsigned long long m = bit64_mod(fill_bit_index(ix, iy, iz));
                         if (w != cur_w) {
                             atomicOr(&lbits[cur_w], cur_b);
                             cur_w = w;
@@ -644,7 +698,7 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                     const uint32_t ux = (ix >> 2) - ubx, uy = (iy >> 2) - uby, uz = (iz >> 2) - ubz;
                     const bool inbox = (ux < unx) & (uy < uny) & (uz < unz);
                     const int w = (int)(__umul24(uz, unxy) + (__umul24(uy, unx) + ux));  // used when inbox
-                    const unsigned long long m = bit64_mod((ix & 3u) | ((iy & 3u) << 2) | (iz << 4));
+                    const unsigned long long m = bit64_mod(fill_bit_index(ix, iy, iz));
                     if (in && !inbox)  // outside the slab's box (not expected): straight to HBM
                         atomicOr(&fillbits[((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2)], m);
                     const bool take = in && inbox;
@@ -1106,7 +1160,7 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
 // zflags, a region whose occupancy words are all zero and whose flag says
 // that its slots already hold zero returns at once; otherwise it writes and
 // records whether it wrote zeros.
-template <int FMT>
+template <int FMT, bool TEX>
 __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
                                                      const unsigned long long* __restrict__ shellbits,
                                                      const unsigned long long* __restrict__ fillbits,
@@ -1158,17 +1212,32 @@ __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
         float nz[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) nz[c] = 1.0f;
-        if (noisy && occ)
-            fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), nz);
+        if (noisy && occ) {
+            // the texture holds the same values fbm_block8 computes
+            const uint64_t t0 = (uint64_t)brick * 64u + q * 8u;  // texel index in the texture
+            if (TEX && FMT == 2) {
+                const uint4 h = ((const uint4*)vd.ntex)[t0 / 8u];
+                const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+                for (int c = 0; c < 8; ++c) nz[c] = h2f((hw[c >> 1] >> (16 * (c & 1))) & 0xffffu);
+            } else if (TEX) {
+                const float4 a = ((const float4*)vd.ntex)[t0 / 4u], b = ((const float4*)vd.ntex)[t0 / 4u + 1u];
+                nz[0] = a.x; nz[1] = a.y; nz[2] = a.z; nz[3] = a.w;
+                nz[4] = b.x; nz[5] = b.y; nz[6] = b.z; nz[7] = b.w;
+            } else {
+                fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), nz);
+            }
+        }
         float2 acc = make_float2(0.0f, 0.0f);
+        float2 val[8];
 #pragma unroll
         for (uint32_t c = 0; c < 8; ++c) {
             const uint32_t m = q * 8u + c;
-            const float2 v = level0_ra<FMT>(vd, ((sh >> m) & 1ull) != 0,
-                                            ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull) != 0,
-                                            nz[c]);
-            stage[tid * 8u + c] = v;
-            acc = add2(acc, v);
+            val[c] = level0_ra<FMT>(vd, ((sh >> m) & 1ull) != 0,
+                                    ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull) != 0,
+                                    nz[c]);
+            stage[tid * 8u + c] = val[c];
+            acc = add2(acc, val[c]);
         }
         if (nlev > 1) {
             const uint32_t x1 = 2u * lbx + (q & 1u), y1 = 2u * lby + ((q >> 1) & 1u), z1 = 2u * lbz + (q >> 2);
@@ -1184,17 +1253,30 @@ __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
                             stage[(sbl << 6) | morton6(0u, ey & 3u, ez & 3u)]);
         }
         // 2048 staged texels = 32 bricks of this half region, 64 per store
-        for (uint32_t i = tid; i < 2048u; i += 256u) {
-            const uint32_t sbl = i >> 6, sm = i & 63u;
-            const uint32_t sbx = sbl & 3u, sby = (sbl >> 2) & 3u, sbz = sbl >> 4;
+        // instruction (lane = Morton slot: coalesced).  Texel i = tid + 256 k
+        // is slot sm = tid & 63 of staged brick w4 + 4 k (w4 = tid >> 6): the
+        // slot, its x column and so the neighbour's slot are the same for all
+        // k, and the brick advances by one row (y) or one z-layer.
+        {
+            const uint32_t sm = tid & 63u, w4 = tid >> 6;  // w4 = the staged brick's x (0..3)
             uint32_t tx, ty, tz;
             demorton6(sm, &tx, &ty, &tz);
-            const uint32_t sbrick = ((4u * rz + 2u * rep + sbz) * nb0 + 4u * ry + sby) * nb0 + 4u * rx + sbx;
-            const uint64_t idx = vd.lv[0].base + (uint64_t)sbrick * 64u + sm;
-            if (tx < 3u) store_pair<FMT>(levels, idx, stage[i], stage[(i & ~63u) | morton6(tx + 1u, ty, tz)]);
-            else if (sbx < 3u) store_pair<FMT>(levels, idx, stage[i], stage[((sbl + 1u) << 6) | morton6(0u, ty, tz)]);
-            else if (last_x) store_pair<FMT>(levels, idx, stage[i], stage[i]);
-            else store_half<FMT>(levels, idx, 0, stage[i]);
+            const uint32_t nsm = morton6(tx < 3u ? tx + 1u : 0u, ty, tz);
+            // +x neighbour: same brick, next brick in x, the voxel itself (the
+            // volume's last column), or none here (the next region writes it)
+            const int nbk = tx < 3u ? 0 : (w4 < 3u ? 1 : (last_x ? 2 : 3));
+            const uint64_t b0 = ((uint64_t)(4u * rz + 2u * rep) * nb0 + 4u * ry) * nb0 + 4u * rx + w4;
+            const uint64_t zstep = (uint64_t)nb0 * nb0;
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                const uint32_t sbl = w4 + 4u * k;
+                const uint64_t idx = vd.lv[0].base + (b0 + (k >> 2) * zstep + (k & 3u) * nb0) * 64u + sm;
+                const float2 own = stage[tid + 256u * k];
+                if (nbk == 0) store_pair<FMT>(levels, idx, own, stage[(sbl << 6) | nsm]);
+                else if (nbk == 1) store_pair<FMT>(levels, idx, own, stage[((sbl + 1u) << 6) | nsm]);
+                else if (nbk == 2) store_pair<FMT>(levels, idx, own, own);
+                else store_half<FMT>(levels, idx, 0, own);
+            }
         }
         __syncthreads();  // the next half reuses stage
     }
@@ -1265,13 +1347,58 @@ __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
         if (rx > 0u) store_half<FMT>(levels, brick_addr(vd.lv[4], rx - 1u, ry, rz), 1, p);
     }
 }
-template <int FMT>
+// TEX: the density noise comes from vd.ntex (non-null when the volume has
+// noise), else it is evaluated here
+template <int FMT, bool TEX>
 __global__ void __launch_bounds__(256) k_build_fused(GridDesc g, VolumeDesc vd,
                                                      const unsigned long long* __restrict__ shellbits,
                                                      const unsigned long long* __restrict__ fillbits,
                                                      void* __restrict__ levels, uint8_t* __restrict__ zflags, int nlev,
                                                      uint32_t rpa) {
-    k_build_fused_body<FMT>(g, vd, shellbits, fillbits, levels, zflags, nlev, rpa, (int)blockIdx.x, (int)gridDim.x);
+    k_build_fused_body<FMT, TEX>(g, vd, shellbits, fillbits, levels, zflags, nlev, rpa, (int)blockIdx.x,
+                                 (int)gridDim.x);
+}
+
+// The density noise texture: fbm_block8 of every 2x2x2 block of level-0
+// texels (one thread per block, the 8 Morton-consecutive slots of a brick),
+// stored in level-0 slot order (brick * 64 + Morton index, from 0); RGBA16F
+// volumes keep the binary16 rounding,
+// which is all their level 0 uses (a = 1 * noise, rounded to half).
+template <int FMT>
+__global__ void __launch_bounds__(256) k_noise_tex(GridDesc g, VolumeDesc vd, void* __restrict__ out,
+                                                   uint64_t nblocks) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= nblocks) return;
+    const uint32_t brick = (uint32_t)(i >> 3), q = (uint32_t)(i & 7u);
+    const uint32_t nb = g.nb0;
+    const uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
+    const int x0 = (int)(bx * 4u + 2u * (q & 1u)), y0 = (int)(by * 4u + 2u * ((q >> 1) & 1u));
+    const int z0 = (int)(bz * 4u + 2u * (q >> 2));
+    float nz[8];
+    fbm_block8(vd, g.N, x0, y0, z0, nz);
+    if (FMT == 2) {
+        uint32_t hw[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hw[k] = f2h(nz[2 * k]) | (f2h(nz[2 * k + 1]) << 16);
+        ((uint4*)out)[i] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+    } else {
+        ((float4*)out)[2 * i] = make_float4(nz[0], nz[1], nz[2], nz[3]);
+        ((float4*)out)[2 * i + 1] = make_float4(nz[4], nz[5], nz[6], nz[7]);
+    }
+}
+
+size_t noise_tex_bytes(const GridDesc& g, const VolumeDesc& vd) {
+    if (vd.mode != 1 || vd.noise_octaves <= 0 || (vd.format != 1 && vd.format != 2) || g.N % 16 != 0) return 0;
+    const uint64_t nb = g.nb0;
+    return (size_t)(nb * nb * nb * 64u) * (vd.format == 1 ? 4u : 2u);
+}
+
+void launch_noise_tex(const GridDesc& g, const VolumeDesc& vd, void* ntex, hipStream_t s) {
+    const uint64_t nb = g.nb0;
+    const uint64_t nblocks = nb * nb * nb * 8u;
+    const dim3 grid((unsigned)((nblocks + 255) / 256));
+    if (vd.format == 2) hipLaunchKernelGGL(k_noise_tex<2>, grid, dim3(256), 0, s, g, vd, ntex, nblocks);
+    else hipLaunchKernelGGL(k_noise_tex<1>, grid, dim3(256), 0, s, g, vd, ntex, nblocks);
 }
 
 // R8 variant of the fused build (one byte per texel, no neighbour halves)
@@ -3053,14 +3180,21 @@ void launch_build_levels(const GridDesc& g, const VolumeDesc& vd, const unsigned
         const uint32_t rpa = (uint32_t)g.N / 16u;
         const int nlev = vd.L < 5 ? vd.L : 5;
         const dim3 grid(rpa * rpa * rpa);
+        const bool eval = vd.mode == 1 && vd.noise_octaves > 0 && !vd.ntex;  // noise without its texture
         if (vd.format == 0)
             hipLaunchKernelGGL(k_build_fused_r8, grid, dim3(256), 0, s, g, vd, shellbits, levels, nlev, rpa);
+        else if (vd.format == 1 && eval)
+            hipLaunchKernelGGL((k_build_fused<1, false>), grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
+                               zflags, nlev, rpa);
         else if (vd.format == 1)
-            hipLaunchKernelGGL(k_build_fused<1>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, zflags,
-                               nlev, rpa);
+            hipLaunchKernelGGL((k_build_fused<1, true>), grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
+                               zflags, nlev, rpa);
+        else if (eval)
+            hipLaunchKernelGGL((k_build_fused<2, false>), grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
+                               zflags, nlev, rpa);
         else
-            hipLaunchKernelGGL(k_build_fused<2>, grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels, zflags,
-                               nlev, rpa);
+            hipLaunchKernelGGL((k_build_fused<2, true>), grid, dim3(256), 0, s, g, vd, shellbits, fillbits, levels,
+                               zflags, nlev, rpa);
         first_generic = nlev;
     } else {
         uint32_t nbricks = g.nb0 * g.nb0 * g.nb0;
@@ -3410,12 +3544,12 @@ __global__ void __launch_bounds__(256) kb_light_splat(const BatchVox* __restrict
     const BatchVox& B = sc[q.sc];
     k_light_splat_body(B.g, B.lv, B.boards, B.keys, nullptr, B.shell, nullptr, q.bid, q.nblk);
 }
-template <int FMT>
+template <int FMT, bool TEX>
 __global__ void __launch_bounds__(256) kb_build_fused(const BatchVox* __restrict__ sc, BatchStage st) {
     const StageSlot q = stage_slot(st);
     const BatchVox& B = sc[q.sc];
-    k_build_fused_body<FMT>(B.g, B.vd, B.shell, B.vd.mode == 1 ? B.fill : nullptr, B.levels, B.zflags, B.nlev, B.rpa,
-                            q.bid, q.nblk);
+    k_build_fused_body<FMT, TEX>(B.g, B.vd, B.shell, B.vd.mode == 1 ? B.fill : nullptr, B.levels, B.zflags, B.nlev,
+                                 B.rpa, q.bid, q.nblk);
 }
 __global__ void __launch_bounds__(1024) kb_mip_tail(const BatchVox* __restrict__ sc, BatchStage st) {
     const StageSlot q = stage_slot(st);
@@ -3494,6 +3628,7 @@ void batch_stage_shape(int stage, const BatchVox& b, int* blocks, int* variant) 
             break;
         case kBsBuild:
             *blocks = (int)(b.rpa * b.rpa * b.rpa);
+            *variant = b.vd.format | (b.vd.mode == 1 && b.vd.noise_octaves > 0 && !b.vd.ntex ? 4 : 0);
             break;
         case kBsTail:
             *blocks = b.tail_k < b.vd.L ? 1 : 0;
@@ -3536,8 +3671,11 @@ void launch_batch_stage(int stage, int variant, const BatchVox* scenes, const Ba
             break;
         case kBsSplat: hipLaunchKernelGGL(kb_light_splat, grid, dim3(256), 0, s, scenes, st); break;
         case kBsBuild:
-            if (variant == 1) hipLaunchKernelGGL(kb_build_fused<1>, grid, dim3(256), 0, s, scenes, st);
-            else hipLaunchKernelGGL(kb_build_fused<2>, grid, dim3(256), 0, s, scenes, st);
+            // variant: format | 4 when the noise is evaluated (no texture)
+            if (variant == 1) hipLaunchKernelGGL((kb_build_fused<1, true>), grid, dim3(256), 0, s, scenes, st);
+            else if (variant == 5) hipLaunchKernelGGL((kb_build_fused<1, false>), grid, dim3(256), 0, s, scenes, st);
+            else if (variant == 6) hipLaunchKernelGGL((kb_build_fused<2, false>), grid, dim3(256), 0, s, scenes, st);
+            else hipLaunchKernelGGL((kb_build_fused<2, true>), grid, dim3(256), 0, s, scenes, st);
             break;
         case kBsTail: hipLaunchKernelGGL(kb_mip_tail, grid, dim3(1024), 0, s, scenes, st); break;
         case kBsCam: hipLaunchKernelGGL(kb_cam_setup, grid, dim3(256), 0, s, scenes, st); break;

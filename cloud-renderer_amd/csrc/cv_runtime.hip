@@ -30,6 +30,16 @@ struct cr_context_s {
     int device = 0;
     unsigned flags = 0;  // CR_CONTEXT_*
     float* static_scratch = nullptr;  // this device's copy of `scratch` (cr_mlp_train_step_static)
+    // The context's three side streams, shared by its volumes and created
+    // with the context.  The runtime backs streams with a few hardware queues
+    // (four by default) and streams beyond that share one, which serialises
+    // them; with the caller's stream these four need no more.  lside takes
+    // the short latency-bound chains: the clear and the pass-1 work list
+    // (built ahead of the raster) and the camera pass (setup, sort, binning);
+    // vside the raster and the splat; bside the level build, which overlaps
+    // the previous frame's cone trace on the caller's stream.
+    hipStream_t lside = nullptr, vside = nullptr, bside = nullptr;
+    hipEvent_t ev_side[3] = {nullptr, nullptr, nullptr};  // ordering points on them (cr_batch_run)
     std::string err;
     int profiling = 0;
     cr_stats stats{};  // level 1: last call; level 2: sums since the last cr_get_stats
@@ -88,6 +98,8 @@ struct cr_volume_s {
     // stream, the end of the work enqueued while set i was current.
     void* d_lset[2] = {nullptr, nullptr};
     uint8_t* d_zflags[2] = {nullptr, nullptr};  // per set: regions whose slots hold zero (fused build)
+    void* d_ntex = nullptr;  // density noise texture (vd.ntex), rebuilt by cr_volume_set_noise
+    size_t ntex_cap = 0;
     size_t zflags_bytes = 0;
     int cur = 0;
     bool lset_failed = false;
@@ -115,12 +127,12 @@ struct cr_volume_s {
     size_t lbox_cap = 0, lcounts_cap = 0, litems_cap = 0;
     // consecutive overlapped voxelizations alternate between the work-list
     // and depth-key buffers above and light_spare (lset_i: which pair is
-    // above); the clear and the work list run on lside ahead of the frame's
-    // raster, ev_items marks them done, ev_lfree[i] the end of the last
-    // passes that read pair i
+    // above); the clear and the work list run on the context's lside ahead of
+    // the frame's raster, ev_items marks them done, ev_lfree[i] the end of
+    // the last passes that read pair i
     LightBufs light_spare;
     int lset_i = 0;
-    hipStream_t lside = nullptr;
+    hipStream_t lside = nullptr;  // the context's, once used
     hipEvent_t ev_items = nullptr;
     hipEvent_t ev_lfree[2] = {nullptr, nullptr};
     bool lfree_valid[2] = {false, false};
@@ -163,18 +175,19 @@ struct cr_volume_s {
     // camera stream after the caller's stream when cam_sync_needed.
     CamBufs cam_spare;
     int cset = 0;
-    hipStream_t side = nullptr;
+    hipStream_t side = nullptr;  // the context's lside, once a trace has used it
     hipEvent_t ev_cam = nullptr, ev_csync = nullptr;
     hipEvent_t ev_cfree[2] = {nullptr, nullptr};
     bool cfree_valid[2] = {false, false};
     bool cam_sync_needed = true;
-    // voxelization streams of cr_voxelize: clear, pass 1 and pass 2 run on
-    // vside, the level build on bside, both into the set that is not current,
-    // overlapping the cone trace that reads the current set; the caller's
-    // stream waits for the build (ev_built).  ev_bfree[i]: the end of the
-    // last build that read occupancy set i.  ev_sync orders the streams after
-    // the caller's stream when s_sync_needed (first call, after a serial call
-    // or a batch).
+    // cr_voxelize: pass 1 and pass 2 run on the context's voxelization
+    // stream (vside, once used), into the occupancy set that is not current,
+    // overlapping the cone trace that reads the current set; the level build
+    // follows on the context's bside after the splat (ev_splat), into the
+    // level set that is not current, and the caller's stream waits for it
+    // (ev_built).  ev_bfree[i]: the end of the last build that read occupancy
+    // set i.  ev_sync orders the side streams after the caller's stream when
+    // s_sync_needed (first call, after a serial call or a batch).
     hipStream_t vside = nullptr, bside = nullptr;
     hipEvent_t ev_splat = nullptr, ev_built = nullptr, ev_sync = nullptr;
     hipEvent_t ev_bfree[2] = {nullptr, nullptr};
@@ -471,6 +484,15 @@ int cr_context_create(int device, unsigned flags, cr_context* out) {
     ctx->device = device;
     ctx->flags = flags;
     ctx->static_scratch = sp;
+    if ((e = hipStreamCreateWithFlags(&ctx->lside, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&ctx->vside, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&ctx->bside, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->ev_side[0], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->ev_side[1], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&ctx->ev_side[2], hipEventDisableTiming)) != hipSuccess) {
+        cr_context_destroy(ctx);
+        return fail(nullptr, CR_EDEVICE, "cr_context_create: hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+    }
     *out = ctx;
     return CR_OK;
 }
@@ -478,6 +500,13 @@ int cr_context_create(int device, unsigned flags, cr_context* out) {
 int cr_context_destroy(cr_context ctx) {
     if (!ctx) return CR_EINVAL;
     (void)hipSetDevice(ctx->device);
+    for (hipStream_t st : {ctx->lside, ctx->vside, ctx->bside}) {
+        if (!st) continue;
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    for (hipEvent_t e : ctx->ev_side)
+        if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->pool) (void)hipEventDestroy(e);
     delete ctx;
     return CR_OK;
@@ -548,6 +577,7 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
     v->vd.format = format;
     v->vd.mode = mode;
     v->vd.noise_octaves = 0;
+    v->vd.ntex = nullptr;
     v->levels_bytes = base * texel_bytes(format);
     // grid: world bounds = position + bounds (src/Shaders/VoxelizeShader.cpp:124-126);
     // voxelSize = (bounds.y - bounds.x) / dimension (src/CloudVolume.cpp:50-54)
@@ -607,11 +637,8 @@ int cr_volume_create(cr_context ctx, int dim, const float bounds_lo[3], const fl
 int cr_volume_destroy(cr_volume v) {
     if (!v) return CR_EINVAL;
     if (v->ctx) (void)hipSetDevice(v->ctx->device);
-    for (hipStream_t st : {v->side, v->vside, v->bside, v->lside}) {
-        if (!st) continue;
-        (void)hipStreamSynchronize(st);
-        (void)hipStreamDestroy(st);
-    }
+    for (hipStream_t st : {v->side, v->vside, v->bside, v->lside})  // the context's streams: drain, keep
+        if (st) (void)hipStreamSynchronize(st);
     for (hipEvent_t e : {v->ev_cam, v->ev_csync, v->ev_cfree[0], v->ev_cfree[1], v->ev_splat, v->ev_built, v->ev_sync, v->ev_release[0],
                          v->ev_release[1], v->ev_bfree[0], v->ev_bfree[1], v->ev_items, v->ev_lfree[0],
                          v->ev_lfree[1]})
@@ -622,7 +649,7 @@ int cr_volume_destroy(cr_volume v) {
                     v->cam_spare.sph_world, v->cam_spare.sph_view, v->cam_spare.sph_tiles, v->cam_spare.sort,
                     v->cam_spare.order, v->cam_spare.tie, v->cam_spare.tiles, v->cam_spare.masks,
                     v->light_spare.lbox, v->light_spare.lcounts, v->light_spare.litems,
-                    v->light_spare.keys, v->d_nmap, v->d_xsort, v->d_xorder, v->d_xtie};
+                    v->light_spare.keys, v->d_nmap, v->d_xsort, v->d_xorder, v->d_xtie, v->d_ntex};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete v;
@@ -634,12 +661,39 @@ int cr_volume_set_noise(cr_volume v, int octaves, uint32_t seed, float base_freq
     if (octaves < 0 || octaves > 16) return fail(v->ctx, CR_EINVAL, "cr_volume_set_noise: octaves must be in [0, 16]");
     if (!std::isfinite(base_freq) || !std::isfinite(lacunarity) || !std::isfinite(gain))
         return fail(v->ctx, CR_EINVAL, "cr_volume_set_noise: non-finite parameter");
+    cr_context ctx = v->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    // queued builds read the descriptor's texture: let them finish first
+    CR_HIP(ctx, hipDeviceSynchronize());
     v->vd.noise_octaves = octaves;
     v->vd.noise_seed = seed;
     v->vd.noise_base_freq = base_freq;
     v->vd.noise_lacunarity = lacunarity;
     v->vd.noise_gain = gain;
+    v->vd.ntex = nullptr;
     v->gen += 1;
+    // The noise is a function of the voxel and these parameters only: the
+    // build reads it from a texture computed here (2 or 4 bytes per texel)
+    // instead of evaluating the octaves per frame.  Without memory for it the
+    // build evaluates them.
+    const size_t bytes = noise_tex_bytes(v->g, v->vd);
+    if (bytes == 0) return CR_OK;
+    if (bytes > v->ntex_cap) {
+        if (v->d_ntex) (void)hipFree(v->d_ntex);
+        v->d_ntex = nullptr;
+        v->ntex_cap = 0;
+        if (hipMalloc(&v->d_ntex, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            v->d_ntex = nullptr;
+            return CR_OK;
+        }
+        v->ntex_cap = bytes;
+    }
+    launch_noise_tex(v->g, v->vd, v->d_ntex, nullptr);
+    CR_HIP(ctx, hipGetLastError());
+    CR_HIP(ctx, hipDeviceSynchronize());
+    v->vd.ntex = v->d_ntex;
     return CR_OK;
 }
 
@@ -916,20 +970,21 @@ int cr_voxelize(cr_volume v, const cr_light* light, int light_w, int light_h, vo
     cr_context ctx = v->ctx;
     hipStream_t s = (hipStream_t)stream;
     unsigned long long* counters = ctx->profiling == 1 ? v->d_counters : nullptr;
-    // Clear, pass 1 and pass 2 (on the voxelization stream) and the build
-    // (on the build stream) write only the keys, the work list and the
-    // occupancy and level sets that are not current, so they overlap the cone
-    // trace that reads the current sets; the passes wait for the end of the
-    // build that last read their occupancy set, the build for the end of the
-    // work enqueued while its level set was last current, and the caller's
-    // stream waits for the build.  Without room for second sets, or in counter
-    // runs, everything runs on the caller's stream.
+    // Clear and work list (on the context's work-list stream), pass 1 and
+    // pass 2 (on its voxelization stream) write only the keys, the work list
+    // and the occupancy set that is not current, so they overlap the cone
+    // trace that reads the current sets; they wait for the end of the build
+    // that last read their occupancy set.  The build (on the build stream)
+    // follows the splat and the end of the work enqueued on the caller's
+    // stream while its level set was current, and the caller's stream waits
+    // for the build.  Without room for second sets, or in counter runs,
+    // everything runs on the caller's stream.
     const bool overlap = ctx->profiling != 1;
     if (overlap) {
         if (!v->vside) {
-            CR_HIP(ctx, hipStreamCreateWithFlags(&v->vside, hipStreamNonBlocking));
-            CR_HIP(ctx, hipStreamCreateWithFlags(&v->bside, hipStreamNonBlocking));
-            CR_HIP(ctx, hipStreamCreateWithFlags(&v->lside, hipStreamNonBlocking));
+            v->vside = ctx->vside;
+            v->bside = ctx->bside;
+            v->lside = ctx->lside;
             for (hipEvent_t* e : {&v->ev_splat, &v->ev_built, &v->ev_sync, &v->ev_release[0], &v->ev_release[1],
                                   &v->ev_bfree[0], &v->ev_bfree[1], &v->ev_items, &v->ev_lfree[0], &v->ev_lfree[1]})
                 CR_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
@@ -1295,7 +1350,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     hipStream_t cs = s;
     if (overlap) {
         if (!v->side) {
-            CR_HIP(ctx, hipStreamCreateWithFlags(&v->side, hipStreamNonBlocking));
+            v->side = ctx->lside;
             for (hipEvent_t* e : {&v->ev_cam, &v->ev_csync, &v->ev_cfree[0], &v->ev_cfree[1]})
                 CR_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
         }
@@ -1912,7 +1967,6 @@ int batch_prepare(cr_batch b, std::vector<VoxPlan>& vp, std::vector<ConePlan>& c
             if (!b->batched[i]) continue;
             int blocks = 0, var = 0;
             batch_stage_shape(st, hv[i], &blocks, &var);
-            if (st == kBsBuild) var = b->format;
             if (blocks > 0) mem.push_back({var, {(int)i, blocks}});
         }
         std::stable_sort(mem.begin(), mem.end(),
@@ -2152,22 +2206,16 @@ int cr_batch_run(cr_batch b, void* stream) {
     for (const auto& it : b->items)
         if (it.volume->nm_on || it.volume->sun_on)
             return fail(ctx, CR_EUNSUPPORTED, "cr_batch_run: the noise march and the sun sprite are not batched");
-    // the scenes' buffers are shared with their single cone traces: wait for
-    // any camera pass still running on a volume's camera-side stream
+    // the scenes' buffers are shared with their single voxelizations and
+    // cone traces: wait for whatever is still queued on the context's side
+    // streams (every volume's side work runs there)
+    for (int k = 0; k < 3; ++k) {
+        hipStream_t st = k == 0 ? ctx->lside : k == 1 ? ctx->vside : ctx->bside;
+        CR_HIP(ctx, hipEventRecord(ctx->ev_side[k], st));
+        CR_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_side[k], 0));
+    }
     for (const auto& it : b->items) {
         cr_volume v = it.volume;
-        if (v->side) {
-            CR_HIP(ctx, hipEventRecord(v->ev_cam, v->side));
-            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_cam, 0));
-        }
-        if (v->vside) {
-            CR_HIP(ctx, hipEventRecord(v->ev_splat, v->vside));
-            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_splat, 0));
-            CR_HIP(ctx, hipEventRecord(v->ev_sync, v->bside));
-            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_sync, 0));
-            CR_HIP(ctx, hipEventRecord(v->ev_items, v->lside));
-            CR_HIP(ctx, hipStreamWaitEvent(s, v->ev_items, 0));
-        }
         if (v->cset != 0) swap_cam(v);  // batches use camera pair 0
         if (v->lset_i != 0) swap_light(v);  // and work-list pair 0
     }
