@@ -136,7 +136,8 @@ struct RasterItem {
     float vx, vy;             // centre in light view
     float inv_tw;             // 1 / (x1 - x0 + 1)
     float inv_x, inv_xy;      // 1 / nbx, 1 / (nbx * nby) (bitmap flush decode)
-    int pad[3];
+    int slice, nslices;       // depth slice of the chords behind the tile (light_slices)
+    int pad;
 };
 static_assert(sizeof(RasterItem) == 96, "work items are 96 bytes");
 
@@ -157,7 +158,46 @@ struct FillSeg {
     uint64_t value;
 };
 void launch_fill_multi(const FillSeg* segs, int nseg, hipStream_t s);  // up to 4 segments, one launch
-size_t light_items_bound(int S, float max_radius, const LightView& lv);
+size_t light_items_bound(int S, float max_radius, const LightView& lv, const GridDesc& g, int chord);
+
+// LDS bitmap words of the raster kernel for this grid (its WORDS)
+__host__ __device__ inline int raster_words(const GridDesc& g) {
+    const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
+    return p2 && g.nb0 <= 64 ? 2048 : 4096;
+}
+// The chord walk of a pass-1 item runs from the queue (the flat walk) when the
+// grid has power-of-two extents and the chord fill is on.
+__host__ __device__ inline bool raster_flat(const GridDesc& g, int chord) {
+    return chord && g.pow2_range && g.pow2_N && g.pow2_step;
+}
+// Depth slices of a billboard's flat-walk items: the smallest power of two
+// K <= 32 for which the brick box of a full tile's chord slab cut into K
+// equal depth ranges fits the raster's LDS bitmap (a bound of the box
+// slab_box computes: extent + 2 voxels of margin, +2 bricks of alignment);
+// 1 when the chords are long enough that a slice's first sample index
+// would not fit 16 bits.  Slices of one tile are separate work items.
+__host__ __device__ inline int light_slices_box(const GridDesc& g, const LightView& lv, float s, int words) {
+    if (!(s > 0.0f)) return 1;
+    const float wu = fminf((float)kLightTile * (2.0f * lv.lsx / (float)lv.W), 2.0f * s);
+    const float wv = fminf((float)kLightTile * (2.0f * lv.lsy / (float)lv.H), 2.0f * s);
+    for (int K = 1; K < 32; K <<= 1) {
+        const float dt = 2.0f * s / (float)K;
+        long long w = 1;
+        for (int a = 0; a < 3; ++a) {
+            const float ext = (wu * fabsf(lv.S[a]) + wv * fabsf(lv.U[a])) + dt * fabsf(lv.B[a]);
+            const float vox = g.range[a] / (float)g.N;
+            long long nb = (long long)((ext / vox + 2.0f) * 0.25f) + 2;
+            if (nb > (long long)g.nb0) nb = g.nb0;
+            w *= nb;
+        }
+        if (w <= (long long)words) return K;
+    }
+    return 32;
+}
+__host__ __device__ inline int light_slices(const GridDesc& g, const LightView& lv, float s, int words) {
+    if (2.0f * s / g.stepSize >= 65535.0f) return 1;
+    return light_slices_box(g, lv, s, words);  // non-decreasing in s: the work-list bound takes the largest s
+}
 // the pass-1 work list (boxes, scan, items); launch_light_raster reads it
 void launch_light_items(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
                         const LightScratch& ls, hipStream_t s);

@@ -257,8 +257,10 @@ __device__ __forceinline__ bool light_fragment(const LightView& lv, const LightS
 // Work decomposition of pass 1: every billboard's pixel box is cut into
 // kLightTile x kLightTile tiles; one workgroup rasterizes one (billboard, tile)
 // item, so large spheres spread over many CUs instead of forming the tail.
-// light_box: the billboard's conservative pixel box and its tile count.
-__device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv, float4 board, int4* out) {
+// light_box: the billboard's conservative pixel box and its item count (tiles
+// times the depth slices of a flat chord walk, light_slices).
+__device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv, float4 board, int flat, int words,
+                                         int4* out) {
     LightSphere o = light_sphere(g, lv, board);
     int4 box = make_int4(1, 1, 0, 0);
     int cnt = 0;
@@ -276,6 +278,7 @@ __device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv,
             box.y = (int)fmaxf(fy0, 0.0f);
             box.w = (int)fminf(fy1, (float)(lv.H - 1));
             cnt = ((box.z - box.x) / kLightTile + 1) * ((box.w - box.y) / kLightTile + 1);
+            if (flat) cnt *= light_slices(g, lv, o.s, words);
         }
     }
     *out = box;
@@ -289,16 +292,17 @@ __device__ __forceinline__ int light_box(const GridDesc& g, const LightView& lv,
 // of the previous frame keeps busy, where one 1024-thread workgroup waited
 // for a whole CU to drain.
 __device__ __forceinline__ void k_light_boxes_body(GridDesc g, LightView lv, const float4* __restrict__ boards,
-                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts, int bid, int nblk) {
+                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts, int chord,
+                                                     int bid, int nblk) {
     const int j = bid * 256 + threadIdx.x;
     if (j >= S) return;
     int4 box;
-    counts[j] = light_box(g, lv, boards[j], &box);
+    counts[j] = light_box(g, lv, boards[j], raster_flat(g, chord) ? 1 : 0, raster_words(g), &box);
     lbox[j] = box;
 }
 __global__ void __launch_bounds__(256) k_light_boxes(GridDesc g, LightView lv, const float4* __restrict__ boards,
-                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts) {
-    k_light_boxes_body(g, lv, boards, S, lbox, counts, (int)blockIdx.x, (int)gridDim.x);
+                                                     int S, int4* __restrict__ lbox, int* __restrict__ counts, int chord) {
+    k_light_boxes_body(g, lv, boards, S, lbox, counts, chord, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // one workgroup: 4096 counts per round, 16 consecutive per thread
@@ -344,12 +348,22 @@ __global__ void __launch_bounds__(256) k_light_scan(const int* __restrict__ coun
     k_light_scan_body(counts, S, offsets);
 }
 
+// Depth (along the light direction, from the billboard plane) where depth
+// slice m of K starts: -s + m * (2s / K), s for m = K.  The raster splits each
+// chord's samples between slices at these bounds, computed the same way for
+// both slices that share one, so every sample falls in exactly one slice.
+__host__ __device__ __forceinline__ float slice_bound(float s, int m, int K) {
+    return m >= K ? s : -s + (float)m * ((2.0f * s) / (float)K);
+}
+
 // The brick box of the chord slab behind a tile (chord fill only): the world
 // box of c + du*S + dv*U + t*B, du/dv over the tile's view-plane rectangle
-// (within the disc), |t| <= s.  Reciprocal products instead of divisions:
-// the box keeps a 1-voxel margin, and the raster stores samples outside it
-// directly, so it only has to be close.
-__device__ void slab_box(const GridDesc& g, const LightView& lv, const LightSphere& o, int words, RasterItem& it) {
+// (within the disc), t in [ta, tb] (the item's depth slice, [-s, s] for a
+// whole chord).  Reciprocal products instead of divisions: the box keeps a
+// 1-voxel margin, and the raster stores samples outside it directly, so it
+// only has to be close.
+__device__ void slab_box(const GridDesc& g, const LightView& lv, const LightSphere& o, float ta, float tb, int words,
+                         RasterItem& it) {
     const float kW = 2.0f / (float)lv.W, kH = 2.0f / (float)lv.H;
     float du0 = fmaxf(((((float)it.x0 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, -o.s);
     float du1 = fminf(((((float)it.x1 + 0.5f) * kW - 1.0f) * lv.lsx + lv.lox) - o.vx, o.s);
@@ -360,9 +374,9 @@ __device__ void slab_box(const GridDesc& g, const LightView& lv, const LightSphe
     for (int a = 0; a < 3; ++a) {
         const float kvox = (float)g.N * g.inv_range[a];
         float su0 = du0 * lv.S[a], su1 = du1 * lv.S[a], sv0 = dv0 * lv.U[a], sv1 = dv1 * lv.U[a];
-        float tb = o.s * fabsf(lv.B[a]);
-        float wlo = c3[a] + fminf(su0, su1) + fminf(sv0, sv1) - tb;
-        float whi = c3[a] + fmaxf(su0, su1) + fmaxf(sv0, sv1) + tb;
+        const float t0 = ta * lv.B[a], t1 = tb * lv.B[a];
+        float wlo = c3[a] + fminf(su0, su1) + fminf(sv0, sv1) + fminf(t0, t1);
+        float whi = c3[a] + fmaxf(su0, su1) + fmaxf(sv0, sv1) + fmaxf(t0, t1);
         float l0 = (wlo - g.lo[a]) * kvox - 1.0f;
         float l1 = (whi - g.lo[a]) * kvox + 1.0f;
         float f0 = fmaxf(floorf(l0 * 0.25f), 0.0f), f1 = fminf(floorf(l1 * 0.25f), (float)(g.nb0 - 1));
@@ -399,12 +413,17 @@ __device__ __forceinline__ void k_light_prep_body(GridDesc g, LightView lv, cons
     const int n = min(*total_ptr, cap);
     for (int i = bid * 256 + threadIdx.x; i < n; i += nblk * 256) {
     const int2 jt = *(const int2*)&items[i];
-    const int j = jt.x, t = jt.y;
+    const int j = jt.x;
     const LightSphere o = light_sphere(g, lv, boards[j]);
+    // item t of the billboard: tile t / K, depth slice t % K (light_box's K)
+    const int K = raster_flat(g, chord) ? light_slices(g, lv, o.s, words) : 1;
+    const int t = jt.y / K, slice = jt.y - t * K;
     const int4 box = lbox[j];
     const int ntx = (box.z - box.x) / kLightTile + 1;
     RasterItem it{};
     it.j = j;
+    it.slice = slice;
+    it.nslices = K;
     it.x0 = box.x + (t % ntx) * kLightTile;
     it.y0 = box.y + (t / ntx) * kLightTile;
     it.x1 = min(it.x0 + kLightTile - 1, box.z);
@@ -412,7 +431,7 @@ __device__ __forceinline__ void k_light_prep_body(GridDesc g, LightView lv, cons
     it.cx = o.cx; it.cy = o.cy; it.cz = o.cz; it.s = o.s;
     it.vx = o.vx; it.vy = o.vy;
     it.inv_tw = 1.0f / (float)(it.x1 - it.x0 + 1);
-    if (chord) slab_box(g, lv, o, words, it);
+    if (chord) slab_box(g, lv, o, slice_bound(o.s, slice, K), slice_bound(o.s, slice + 1, K), words, it);
     items[i] = it;
     }
 }
@@ -482,6 +501,9 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
     o.live = true;
     int bx0 = 0, by0 = 0, bz0 = 0, nbx = 0, nby = 0, nbz = 0;
     bool use_lds = false;
+    // depth slice of the item (flat walk): its share of every chord's samples;
+    // slice 0 also writes the depth keys
+    const int slice = field_i(21), nsl = field_i(22);
     if (chord) {
         bx0 = field_i(5); by0 = field_i(6); bz0 = field_i(7);
         nbx = field_i(8); nby = field_i(9); nbz = field_i(10);
@@ -512,9 +534,12 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
     // flat: this thread's queued chords (one per pixel round) and their
     // ranks in their length bucket, placed once the buckets are counted
     float q_sx[kRounds], q_sy[kRounds], q_sz[kRounds];
-    int q_n[kRounds], q_rank[kRounds];
+    int q_n[kRounds], q_k0[kRounds], q_rank[kRounds];
 #pragma unroll
-    for (int r = 0; r < kRounds; ++r) { q_sx[r] = q_sy[r] = q_sz[r] = 0.0f; q_n[r] = 0; q_rank[r] = 0; }
+    for (int r = 0; r < kRounds; ++r) {
+        q_sx[r] = q_sy[r] = q_sz[r] = 0.0f;
+        q_n[r] = 0; q_k0[r] = 0; q_rank[r] = 0;
+    }
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         const int p = tid + r * kRasterThreads;
@@ -525,7 +550,7 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
         const int px = x0 + rx, py = y0 + ry;
         float fx, fy, fz, sc;
         if (!light_fragment_at(lv, o, h_x[rx], h_y[ry], &fx, &fy, &fz, &sc)) continue;
-        ++nfrag;
+        if (slice == 0) ++nfrag;
         float dist = o.s * sc;  // radius * sphereContrib
         if (chord) {
             // res/first_voxelize.glsl:53-58: start = fragPos - dir * dist;
@@ -535,11 +560,26 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
             if (flat) {
                 // k * stepSize < lim  <=>  k < lim / stepSize (exact: power-of-two scaling)
                 const int n = lim > 0.0f ? (int)ceilf(lim * g.inv_step) : 0;
-                if (n > 0) {
+                // this slice's samples k in [k0, k1): the first sample index
+                // whose depth -dist + k * stepSize reaches the slice bound
+                int k0 = 0, k1 = n;
+                if (nsl > 1) {
+                    auto kb = [&](int m) {
+                        if (m <= 0) return 0;
+                        if (m >= nsl) return n;
+                        const int k = (int)ceilf((slice_bound(o.s, m, nsl) + dist) * g.inv_step);
+                        return min(max(k, 0), n);
+                    };
+                    k0 = kb(slice);
+                    k1 = kb(slice + 1);
+                }
+                const int m = k1 - k0;
+                if (m > 0) {
                     q_sx[r] = sx; q_sy[r] = sy; q_sz[r] = sz;
-                    q_n[r] = n;
-                    q_rank[r] = atomicAdd(&q_start[min(n, 127)], 1);
-                    nchord += (unsigned long long)n;
+                    q_n[r] = m;
+                    q_k0[r] = k0;
+                    q_rank[r] = atomicAdd(&q_start[min(m, 127)], 1);
+                    nchord += (unsigned long long)m;
                 }
             } else {
                 // consecutive samples of one chord mostly share a brick word:
@@ -562,13 +602,15 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                 flush(cbx, cby, cbz, cur_b);
             }
         }
-        float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
-        float ex = wx - lv.np[0], ey = wy - lv.np[1], ez = wz - lv.np[2];
-        float d = sqrtf((ex * ex + ey * ey) + ez * ez) / lv.clip;
-        d = fminf(fmaxf(d, 0.0f), 1.0f);
-        if (d < 1.0f) {
-            unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned long long)(uint32_t)j;
-            atomicMin(&keys[(size_t)py * lv.W + px], key);
+        if (slice == 0) {
+            float wx = fx + lv.B[0] * dist, wy = fy + lv.B[1] * dist, wz = fz + lv.B[2] * dist;
+            float ex = wx - lv.np[0], ey = wy - lv.np[1], ez = wz - lv.np[2];
+            float d = sqrtf((ex * ex + ey * ey) + ez * ez) / lv.clip;
+            d = fminf(fmaxf(d, 0.0f), 1.0f);
+            if (d < 1.0f) {
+                unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned long long)(uint32_t)j;
+                atomicMin(&keys[(size_t)py * lv.W + px], key);
+            }
         }
     }
     if (flat) {
@@ -593,7 +635,8 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
             if (q_n[r] > 0) {
                 const int slot = q_start[min(q_n[r], 127)] + q_rank[r];
                 f_sx[slot] = q_sx[r]; f_sy[slot] = q_sy[r]; f_sz[slot] = q_sz[r];
-                f_n[slot] = q_n[r];
+                // sliced items: first sample index above the count (both < 2^16, light_slices)
+                f_n[slot] = nsl > 1 ? (q_k0[r] << 16) | q_n[r] : q_n[r];
             }
         }
         __syncthreads();
@@ -624,7 +667,10 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
             // is active, and the inner walk's trip count is wave-uniform
             const int f = min(f0 + (tid & 63), nfr - 1);
             const float sx = f_sx[f], sy = f_sy[f], sz = f_sz[f];
-            const int n = f_n[f];
+            const int fn_ = f_n[f];
+            const int n = nsl > 1 ? (fn_ & 0xffff) : fn_;  // samples k0 .. k0 + n - 1 of the chord
+            const int k0 = nsl > 1 ? (fn_ >> 16) : 0;
+            const float i0 = (float)k0 * step;  // exact: power-of-two step
             if (use_lds) {
                 // the LDS word of the current brick, its bits OR-ed in a
                 // register: a brick change costs one LDS atomic
@@ -636,8 +682,9 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                 // chords skip the per-sample bounds tests.
                 bool inner;
                 {
-                    const float ie = (float)(n - 1) * g.stepSize;
-                    const float ax = (sx - g.lo[0]) * kx, ay = (sy - g.lo[1]) * ky, az = (sz - g.lo[2]) * kz;
+                    const float ie = (float)(k0 + n - 1) * g.stepSize;
+                    const float ax = ((sx + Bx * i0) - g.lo[0]) * kx, ay = ((sy + By * i0) - g.lo[1]) * ky;
+                    const float az = ((sz + Bz * i0) - g.lo[2]) * kz;
                     const float bx = ((sx + Bx * ie) - g.lo[0]) * kx, by = ((sy + By * ie) - g.lo[1]) * ky;
                     const float bz = ((sz + Bz * ie) - g.lo[2]) * kz;
                     auto ok = [&](float l, uint32_t b0, uint32_t nb) {
@@ -656,7 +703,7 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                     // step k at i = min(k, n - 1) * stepSize: a lane past its
                     // last sample repeats it.  i is accumulated: with a
                     // power-of-two step every partial sum is exact.
-                    const float ilast = (float)(n - 1) * step;
+                    const float ilast = (float)(k0 + n - 1) * step;
                     auto voxel = [&](float i, uint32_t& ix, uint32_t& iy, uint32_t& iz) {
                         ix = (uint32_t)(int)((skx + Bkx * i) - lkx);
                         iy = (uint32_t)(int)((sky + Bky * i) - lky);
@@ -667,10 +714,10 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                         return (int)(__umul24(iz >> 2, unxy) + (__umul24(iy >> 2, unx) + ((ix >> 2) - wbase)));
                     };
                     uint32_t ix, iy, iz;
-                    voxel(0.0f, ix, iy, iz);
+                    voxel(i0, ix, iy, iz);
                     cur_w = word(ix, iy, iz);
                     cur_b = bit64_mod(fill_bit_index(ix, iy, iz));
-                    float iacc = 0.0f;
+                    float iacc = i0;
                     for (int k = 1; k < nmax; ++k) {
                         iacc += step;
                         voxel(min_no_nan(iacc, ilast), ix, iy, iz);
@@ -686,7 +733,7 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                     atomicOr(&lbits[cur_w], cur_b);
                     continue;
                 }
-                float i = 0.0f;
+                float i = i0;
                 for (int k = 0; k < n; ++k, i += step) {
                     const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
                     const float lx = __builtin_amdgcn_fmed3f((qx - g.lo[0]) * kx, -2.0f, fN);
@@ -716,7 +763,7 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
             // bricks per axis), bits of the current word OR-ed in a register
             uint32_t cur_key = 0xffffffffu;
             unsigned long long cur_b = 0ull;
-            for (int k = 0; k < n; ++k) {
+            for (int k = k0; k < k0 + n; ++k) {
                 const float i = (float)k * g.stepSize;
                 const float qx = sx + Bx * i, qy = sy + By * i, qz = sz + Bz * i;
                 // voxel_brick_bit with power-of-two extents, branch free
@@ -3110,18 +3157,13 @@ void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t value, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, s, p, n, value);
 }
 
-// LDS bitmap words of the raster kernel for this grid
-static int raster_words(const GridDesc& g) {
-    const bool p2 = g.pow2_range && g.pow2_N && g.pow2_step;
-    return p2 && g.nb0 <= 64 ? 2048 : 4096;
-}
 
 void launch_light_items(const GridDesc& g, const LightView& lv, int mode, const float4* boards, int S,
                         const LightScratch& ls, hipStream_t s) {
     if (S <= 0) return;
     const int cap = (int)std::min<size_t>(ls.items_cap, 0x7fffffff);
     const dim3 bgrid((unsigned)((S + 255) / 256));
-    hipLaunchKernelGGL(k_light_boxes, bgrid, dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts);
+    hipLaunchKernelGGL(k_light_boxes, bgrid, dim3(256), 0, s, g, lv, boards, S, ls.lbox, ls.counts, mode == 1 ? 1 : 0);
     hipLaunchKernelGGL(k_light_scan, dim3(1), dim3(256), 0, s, ls.counts, S, ls.offsets);
     hipLaunchKernelGGL(k_light_expand, bgrid, dim3(256), 0, s, ls.offsets, S, ls.items, cap);
     const dim3 igrid((unsigned)std::min<size_t>((ls.items_cap + 255) / 256, 2048));
@@ -3148,14 +3190,16 @@ void launch_light_raster(const GridDesc& g, const LightView& lv, int mode, const
                            ls.items, cap, keys, fillbits, counters);
 }
 
-size_t light_items_bound(int S, float max_radius, const LightView& lv) {
+size_t light_items_bound(int S, float max_radius, const LightView& lv, const GridDesc& g, int chord) {
+    // depth slices multiply the items; a billboard's slice count grows with its radius
+    const size_t kmax = raster_flat(g, chord) ? (size_t)light_slices_box(g, lv, max_radius, raster_words(g)) : 1;
     // widest pixel box of a billboard of radius max_radius: the box spans
     // s * W / lsx pixels plus at most 5 from floor/ceil and the 1-pixel margins
     double wx = 2.0 * max_radius * lv.W / (2.0 * lv.lsx) + 6.0;
     double wy = 2.0 * max_radius * lv.H / (2.0 * lv.lsy) + 6.0;
     size_t tx = std::min<size_t>((size_t)(wx / kLightTile) + 2, (size_t)(lv.W / kLightTile) + 1);
     size_t ty = std::min<size_t>((size_t)(wy / kLightTile) + 2, (size_t)(lv.H / kLightTile) + 1);
-    return std::max<size_t>((size_t)S * tx * ty, 1);
+    return std::max<size_t>((size_t)S * tx * ty * kmax, 1);
 }
 
 void launch_light_splat(const GridDesc& g, const LightView& lv, const float4* boards, const unsigned long long* keys,
@@ -3514,7 +3558,7 @@ __global__ void __launch_bounds__(256) kb_fill(const FillSeg* __restrict__ segs,
 __global__ void __launch_bounds__(256) kb_light_boxes(const BatchVox* __restrict__ sc, BatchStage st) {
     const StageSlot q = stage_slot(st);
     const BatchVox& B = sc[q.sc];
-    k_light_boxes_body(B.g, B.lv, B.boards, B.S, B.lbox, B.counts, q.bid, q.nblk);
+    k_light_boxes_body(B.g, B.lv, B.boards, B.S, B.lbox, B.counts, B.chord, q.bid, q.nblk);
 }
 __global__ void __launch_bounds__(256) kb_light_scan(const BatchVox* __restrict__ sc, BatchStage st) {
     const StageSlot q = stage_slot(st);

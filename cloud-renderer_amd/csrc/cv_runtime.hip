@@ -883,7 +883,7 @@ static int voxelize_prepare(cr_volume v, const cr_light* light, int light_w, int
     v->last_lv = plan.lv;  // the position map is rebuilt from the keys on request
     v->vox_gen = v->gen;
     plan.ls = LightScratch{};
-    plan.nitems = v->S > 0 ? light_items_bound(v->S, v->max_radius, plan.lv) : 0;
+    plan.nitems = v->S > 0 ? light_items_bound(v->S, v->max_radius, plan.lv, v->g, v->mode == CR_MODE_RGBA ? 1 : 0) : 0;
     return CR_OK;
 }
 
