@@ -73,6 +73,9 @@ def _scenes(cv):
         "small32_rect": replace(base, W=200, H=120, light_w=130, light_h=210),
         "npot24": replace(base, N=24, levels=5),
         "npot96_plain": replace(sc.scaled(sc.plain256(1), 320, 180), N=96, levels=7),
+        # spheres whose chord slabs overflow the raster's LDS bitmap at 128^3:
+        # pass 1 cuts their tiles into depth slices (light_slices K = 2..4)
+        "sliced128": _sliced(sc, base),
         "ties": tie_scene(sc),
         "ties_ref": sc.reference_mode(tie_scene(sc)),
         # reference semantics in a range that carries information: with
@@ -92,6 +95,14 @@ def _scenes(cv):
         "refdef_matrices": _with_matrices(sc.reference_default(2, quantize_rgba8=0)),
     }
     return cases
+
+
+def _sliced(sc, base):
+    rng = np.random.default_rng(11)
+    xyz = rng.uniform(-0.35, 0.35, size=(6, 3)).astype(np.float32)
+    r = rng.uniform(0.35, 0.6, size=6).astype(np.float32)
+    return replace(base, name="sliced128", N=128, levels=8, xyz=xyz, scales=r, sun_dir=_unit3((1.0, 0.8, 0.5)),
+                   W=256, H=256, light_w=256, light_h=256)
 
 
 def _ref_ds(sc, base, ds, quantize, clear=None):
