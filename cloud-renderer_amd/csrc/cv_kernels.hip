@@ -167,6 +167,30 @@ __device__ __forceinline__ uint32_t fill_bit_index(uint32_t ix, uint32_t iy, uin
     const uint32_t xy = (ix & 3u) | ((iy << 2) & ~3u);
     return (xy & 15u) | ((iz << 4) & ~15u);
 }
+// Rotated bit layout of the pass-1 LDS words: voxel (ix, iy, iz) takes bit
+// (ix + 4 iy + 16 iz) mod 64 (two shift-adds; bit64_mod keeps the low 6
+// bits).  Inside brick (X, Y, Z), ix = 4 X + a etc., that is
+// (lin6(a, b, c) + 4 X + 16 Y) mod 64: a bijection of the brick's voxels,
+// the canonical word rotated left by (4 X + 16 Y) mod 64 (rot_to_lin6 undoes it).
+__device__ __forceinline__ uint32_t rot_bit_index(uint32_t ix, uint32_t iy, uint32_t iz) {
+    return ix + (iy << 2) + (iz << 4);
+}
+__device__ __forceinline__ unsigned long long rot_to_lin6(unsigned long long w, uint32_t X, uint32_t Y) {
+    const uint32_t r = (4u * X + 16u * Y) & 63u;
+    return r ? (w >> r) | (w << (64u - r)) : w;
+}
+// a * b + c with 24-bit unsigned factors (c and the sum mod 2^32); b and c
+// uniform
+__device__ __forceinline__ uint32_t mad_u24_vsv(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ uint32_t mad_u24_v8s(uint32_t a, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, 8, %2" : "=v"(r) : "v"(a), "s"(c));
+    return r;
+}
 // min of two floats that are never NaN: one v_min_f32 (fminf also quiets
 // signalling NaNs, which costs a canonicalising instruction per call)
 __device__ __forceinline__ float min_no_nan(float a, float b) {
@@ -709,28 +733,32 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                         iy = (uint32_t)(int)((sky + Bky * i) - lky);
                         iz = (uint32_t)(int)((skz + Bkz * i) - lkz);
                     };
-                    // word of the brick in the box; lin6 bit (the shift drops iz's high bits)
-                    auto word = [&](uint32_t ix, uint32_t iy, uint32_t iz) {
-                        return (int)(__umul24(iz >> 2, unxy) + (__umul24(iy >> 2, unx) + ((ix >> 2) - wbase)));
+                    // byte offset of the brick's word in the box,
+                    // ((Z * unxy + Y * unx + X) - wbase) * 8 mod 2^32 (factors
+                    // below 2^24); lin6 bit (the shift drops iz's high bits)
+                    const uint32_t unxy8 = unxy * 8u, unx8 = unx * 8u, nwbase8 = 0u - wbase * 8u;
+                    auto waddr = [&](uint32_t ix, uint32_t iy, uint32_t iz) {
+                        return mad_u24_vsv(iz >> 2, unxy8, mad_u24_vsv(iy >> 2, unx8, mad_u24_v8s(ix >> 2, nwbase8)));
                     };
+                    auto lword = [&](uint32_t a) { return (unsigned long long*)((char*)lbits + a); };
                     uint32_t ix, iy, iz;
                     voxel(i0, ix, iy, iz);
-                    cur_w = word(ix, iy, iz);
-                    cur_b = bit64_mod(fill_bit_index(ix, iy, iz));
+                    uint32_t cur_a = waddr(ix, iy, iz);
+                    cur_b = bit64_mod(rot_bit_index(ix, iy, iz));
                     float iacc = i0;
                     for (int k = 1; k < nmax; ++k) {
                         iacc += step;
                         voxel(min_no_nan(iacc, ilast), ix, iy, iz);
-                        const int w = word(ix, iy, iz);
-                        const unsigned long long m = bit64_mod(fill_bit_index(ix, iy, iz));
-                        if (w != cur_w) {
-                            atomicOr(&lbits[cur_w], cur_b);
-                            cur_w = w;
+                        const uint32_t a = waddr(ix, iy, iz);
+                        const unsigned long long m = bit64_mod(rot_bit_index(ix, iy, iz));
+                        if (a != cur_a) {
+                            atomicOr(lword(cur_a), cur_b);
+                            cur_a = a;
                             cur_b = 0ull;
                         }
                         cur_b |= m;
                     }
-                    atomicOr(&lbits[cur_w], cur_b);
+                    atomicOr(lword(cur_a), cur_b);
                     continue;
                 }
                 float i = i0;
@@ -745,9 +773,10 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                     const uint32_t ux = (ix >> 2) - ubx, uy = (iy >> 2) - uby, uz = (iz >> 2) - ubz;
                     const bool inbox = (ux < unx) & (uy < uny) & (uz < unz);
                     const int w = (int)(__umul24(uz, unxy) + (__umul24(uy, unx) + ux));  // used when inbox
-                    const unsigned long long m = bit64_mod(fill_bit_index(ix, iy, iz));
                     if (in && !inbox)  // outside the slab's box (not expected): straight to HBM
-                        atomicOr(&fillbits[((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2)], m);
+                        atomicOr(&fillbits[((iz >> 2) * g.nb0 + (iy >> 2)) * g.nb0 + (ix >> 2)],
+                                 bit64_mod(fill_bit_index(ix, iy, iz)));
+                    const unsigned long long m = bit64_mod(rot_bit_index(ix, iy, iz));  // LDS: rotated layout
                     const bool take = in && inbox;
                     if (take && w != cur_w) {
                         if (cur_w >= 0) atomicOr(&lbits[cur_w], cur_b);
@@ -803,6 +832,8 @@ __device__ __forceinline__ void k_light_raster_body(GridDesc g, LightView lv, in
                 const int q = (int)(fw * inv_x), lz = (int)(fw * inv_xy);
                 const int lx = w - q * nbx, ly = q - lz * nby;
                 uint32_t gw = ((uint32_t)(lz + bz0) * g.nb0 + (uint32_t)(ly + by0)) * g.nb0 + (uint32_t)(lx + bx0);
+                // the flat walk's words are in the rotated layout
+                if (flat) bits = rot_to_lin6(bits, (uint32_t)(lx + bx0), (uint32_t)(ly + by0));
                 atomicOr(&fillbits[gw], bits);
             }
         }
@@ -1186,6 +1217,58 @@ __global__ void __launch_bounds__(256) k_level0(GridDesc g, VolumeDesc vd, const
     store_pair_rt(levels, vd.format, idx, v, nbv);
 }
 
+// The 8 noise texels of one 2x2x2 block (texture index t0 .. t0 + 7, t0 a
+// multiple of 8): binary16 for RGBA16F volumes, float for RGBA32F.
+template <int FMT>
+struct NoiseRaw;
+template <>
+struct NoiseRaw<2> {
+    uint4 h;
+    __device__ __forceinline__ void load(const void* tex, uint64_t t0) { h = ((const uint4*)tex)[t0 / 8u]; }
+    __device__ __forceinline__ void unpack(float (&nz)[8]) const {
+        const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) nz[c] = h2f((hw[c >> 1] >> (16 * (c & 1))) & 0xffffu);
+    }
+};
+template <>
+struct NoiseRaw<1> {
+    float4 a, b;
+    __device__ __forceinline__ void load(const void* tex, uint64_t t0) {
+        a = ((const float4*)tex)[t0 / 4u];
+        b = ((const float4*)tex)[t0 / 4u + 1u];
+    }
+    __device__ __forceinline__ void unpack(float (&nz)[8]) const {
+        nz[0] = a.x; nz[1] = a.y; nz[2] = a.z; nz[3] = a.w;
+        nz[4] = b.x; nz[5] = b.y; nz[6] = b.z; nz[7] = b.w;
+    }
+};
+
+// A level-0 texel staged in LDS for the slot stores: RGBA16F as the binary16
+// bits of (r, a) in one word (r in the low half), RGBA32F as floats.
+template <int FMT>
+struct Staged {
+    using T = float2;
+    static __device__ __forceinline__ T pack(float2 v) { return v; }
+    static __device__ __forceinline__ void pair(void* levels, uint64_t idx, T own, T nb) {
+        ((float4*)levels)[idx] = make_float4(own.x, own.y, nb.x, nb.y);
+    }
+    static __device__ __forceinline__ void half(void* levels, uint64_t idx, int h, T v) {
+        ((float2*)levels)[2 * idx + h] = v;
+    }
+};
+template <>
+struct Staged<2> {
+    using T = uint32_t;
+    static __device__ __forceinline__ T pack(float2 v) { return f2h(v.x) | (f2h(v.y) << 16); }
+    static __device__ __forceinline__ void pair(void* levels, uint64_t idx, T own, T nb) {
+        ((uint2*)levels)[idx] = make_uint2(own, nb);
+    }
+    static __device__ __forceinline__ void half(void* levels, uint64_t idx, int h, T v) {
+        ((uint32_t*)levels)[2 * idx + h] = v;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // fused level 0 + mips 1..4 for one 16^3 region of level 0 (N % 16 == 0)
 //
@@ -1214,77 +1297,100 @@ __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
                                                      void* __restrict__ levels, uint8_t* __restrict__ zflags, int nlev,
                                                      uint32_t rpa, int bid, int nblk) {
     static_assert(FMT == 1 || FMT == 2, "RGBA formats");
-    __shared__ float2 stage[2048];  // [brick of the half region][Morton]
+    using ST = Staged<FMT>;
+    __shared__ typename ST::T stage[2048];  // [brick of the half region][Morton]
     __shared__ float2 l1[512];
     __shared__ float2 l2[64];
     __shared__ float2 l3[8];
+    __shared__ unsigned long long s_words[128];  // shell words of the region's 64 bricks, then fill words
     const uint32_t tid = threadIdx.x;
     const uint32_t r = bid;
     const uint32_t rx = r % rpa, ry = (r / rpa) % rpa, rz = r / (rpa * rpa);
     const bool last_x = rx + 1u == rpa;
     const uint32_t nb0 = g.nb0;
+    // threads 0..63 load the shell words of the region's 64 bricks, 64..127
+    // the fill words (brick lb = x + 4 y + 16 z inside the region)
+    unsigned long long w = 0ull;
+    if (tid < 128u) {
+        const uint32_t lb = tid & 63u;
+        const uint32_t brick = ((4u * rz + (lb >> 4)) * nb0 + 4u * ry + ((lb >> 2) & 3u)) * nb0 + 4u * rx + (lb & 3u);
+        if (tid < 64u) w = shellbits[brick];
+        else if (fillbits) w = fillbits[brick];
+        s_words[tid] = w;
+    }
     if (zflags) {
-        // threads 0..63: shell words of the region's 64 bricks, 64..127: fill words
-        unsigned long long w = 0ull;
-        if (tid < 128u) {
-            const uint32_t lb = tid & 63u;
-            const uint32_t brick = ((4u * rz + (lb >> 4)) * nb0 + 4u * ry + ((lb >> 2) & 3u)) * nb0 + 4u * rx + (lb & 3u);
-            if (tid < 64u) w = shellbits[brick];
-            else if (fillbits) w = fillbits[brick];
-        }
         const bool stored_zero = zflags[r] != 0;
         const bool any = __syncthreads_or(w != 0ull) != 0;
         if (!any && stored_zero) return;
         if (tid == 0) zflags[r] = any ? 0 : 1;
+    } else {
+        __syncthreads();
     }
     const bool noisy = vd.mode != 0 && vd.noise_octaves > 0;
+    // Each thread owns one 2x2x2 block of texels per half region: Morton
+    // slots q*8 .. q*8+7 of brick lb, i.e. the block at brick offset
+    // 2 * (q & 1, (q >> 1) & 1, q >> 2).  Its occupancy and its noise texels
+    // (the texture holds the values fbm_block8 computes) are fetched for both
+    // halves before either is built, so the two texture reads overlap.
+    // Texel c of a block (c = dx + 2 dy + 4 dz, the Morton order) has shell
+    // bit q*8 + c of the brick's shell word and fill bit lin6(ox + dx, oy + dy,
+    // oz + dz) = base + dx + 4 dy + 16 dz of its fill word: both gathered into
+    // one byte each, bit c.
+    uint32_t shb2[2], fb2[2];
+    NoiseRaw<FMT> nraw[2];
+#pragma unroll
+    for (uint32_t rep = 0; rep < 2; ++rep) {
+        const uint32_t grp = tid + rep * 256u;
+        const uint32_t lb = grp >> 3, q = grp & 7u;
+        const unsigned long long sh = s_words[lb], fl = s_words[64u + lb];
+        const uint32_t ox = 2u * (q & 1u), oy = 2u * ((q >> 1) & 1u), oz = 2u * (q >> 2);
+        const uint32_t fw = (uint32_t)(fl >> (ox + 4u * oy + 16u * oz));
+        shb2[rep] = (uint32_t)(sh >> (q * 8u)) & 0xffu;
+        fb2[rep] = (fw & 3u) | ((fw >> 2) & 0xcu) | ((fw >> 12) & 0x30u) | ((fw >> 14) & 0xc0u);
+        const uint32_t occ = shb2[rep] | fb2[rep];
+        nraw[rep] = NoiseRaw<FMT>{};
+        if (TEX && noisy && occ) {
+            const uint32_t brick = ((4u * rz + (lb >> 4)) * nb0 + 4u * ry + ((lb >> 2) & 3u)) * nb0 + 4u * rx + (lb & 3u);
+            nraw[rep].load(vd.ntex, (uint64_t)brick * 64u + q * 8u);
+        }
+    }
+#pragma unroll
     for (uint32_t rep = 0; rep < 2; ++rep) {
         const uint32_t grp = tid + rep * 256u;
         const uint32_t lb = grp >> 3, q = grp & 7u;
         const uint32_t lbx = lb & 3u, lby = (lb >> 2) & 3u, lbz = lb >> 4;
         const uint32_t gbx = 4u * rx + lbx, gby = 4u * ry + lby, gbz = 4u * rz + lbz;
-        const uint32_t brick = (gbz * nb0 + gby) * nb0 + gbx;
-        const unsigned long long sh = shellbits[brick];
-        const unsigned long long fl = fillbits ? fillbits[brick] : 0ull;
-        // the thread's 8 texels (Morton q*8 .. q*8+7) are the 2x2x2 block at
-        // brick offset 2 * (q & 1, (q >> 1) & 1, q >> 2)
         const uint32_t ox = 2u * (q & 1u), oy = 2u * ((q >> 1) & 1u), oz = 2u * (q >> 2);
-        uint32_t occ = 0;
-#pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            const uint32_t m = q * 8u + c;
-            const bool on = ((sh >> m) & 1ull) || ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull);
-            occ |= on ? (1u << c) : 0u;
-        }
-        float nz[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) nz[c] = 1.0f;
-        if (noisy && occ) {
-            // the texture holds the same values fbm_block8 computes
-            const uint64_t t0 = (uint64_t)brick * 64u + q * 8u;  // texel index in the texture
-            if (TEX && FMT == 2) {
-                const uint4 h = ((const uint4*)vd.ntex)[t0 / 8u];
-                const uint32_t hw[4] = {h.x, h.y, h.z, h.w};
-#pragma unroll
-                for (int c = 0; c < 8; ++c) nz[c] = h2f((hw[c >> 1] >> (16 * (c & 1))) & 0xffffu);
-            } else if (TEX) {
-                const float4 a = ((const float4*)vd.ntex)[t0 / 4u], b = ((const float4*)vd.ntex)[t0 / 4u + 1u];
-                nz[0] = a.x; nz[1] = a.y; nz[2] = a.z; nz[3] = a.w;
-                nz[4] = b.x; nz[5] = b.y; nz[6] = b.z; nz[7] = b.w;
-            } else {
-                fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), nz);
-            }
-        }
+        const uint32_t shb = shb2[rep], fb = fb2[rep];
         float2 acc = make_float2(0.0f, 0.0f);
-        float2 val[8];
+        if constexpr (FMT == 2 && TEX) {
+            // binary16 bits straight from the texture: r and a are 0, 1 or a
+            // texture value, all exact halves, so level0_ra's rounding to half
+            // is the identity
+            const uint32_t hw[4] = {nraw[rep].h.x, nraw[rep].h.y, nraw[rep].h.z, nraw[rep].h.w};
 #pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            const uint32_t m = q * 8u + c;
-            val[c] = level0_ra<FMT>(vd, ((sh >> m) & 1ull) != 0,
-                                    ((fl >> lin6(ox + (c & 1u), oy + ((c >> 1) & 1u), oz + (c >> 2))) & 1ull) != 0,
-                                    nz[c]);
-            stage[tid * 8u + c] = val[c];
-            acc = add2(acc, val[c]);
+            for (uint32_t c = 0; c < 8; ++c) {
+                const bool shell = (shb >> c) & 1u, fill = (fb >> c) & 1u;
+                const bool on = vd.mode == 0 ? shell : (shell || fill);
+                const uint32_t nbits = (hw[c >> 1] >> (16u * (c & 1u))) & 0xffffu;
+                const uint32_t ab = on ? (noisy ? nbits : 0x3c00u) : 0u;
+                stage[tid * 8u + c] = (shell ? 0x3c00u : 0u) | (ab << 16);
+                acc = add2(acc, make_float2(shell ? 1.0f : 0.0f, h2f(ab)));
+            }
+        } else {
+            float nz[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) nz[c] = 1.0f;
+            if (noisy && (shb | fb)) {
+                if (TEX) nraw[rep].unpack(nz);
+                else fbm_block8(vd, g.N, (int)(gbx * 4u + ox), (int)(gby * 4u + oy), (int)(gbz * 4u + oz), nz);
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < 8; ++c) {
+                const float2 v = level0_ra<FMT>(vd, (shb >> c) & 1u, (fb >> c) & 1u, nz[c]);
+                stage[tid * 8u + c] = ST::pack(v);
+                acc = add2(acc, v);
+            }
         }
         if (nlev > 1) {
             const uint32_t x1 = 2u * lbx + (q & 1u), y1 = 2u * lby + ((q >> 1) & 1u), z1 = 2u * lbz + (q >> 2);
@@ -1296,8 +1402,8 @@ __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
             const uint32_t ey = tid & 15u, ez = tid >> 4;
             const uint32_t sbl = ((ez >> 2) << 4) | ((ey >> 2) << 2);
             const uint32_t pbrick = ((4u * rz + 2u * rep + (ez >> 2)) * nb0 + 4u * ry + (ey >> 2)) * nb0 + 4u * rx - 1u;
-            store_half<FMT>(levels, vd.lv[0].base + (uint64_t)pbrick * 64u + morton6(3u, ey & 3u, ez & 3u), 1,
-                            stage[(sbl << 6) | morton6(0u, ey & 3u, ez & 3u)]);
+            ST::half(levels, vd.lv[0].base + (uint64_t)pbrick * 64u + morton6(3u, ey & 3u, ez & 3u), 1,
+                     stage[(sbl << 6) | morton6(0u, ey & 3u, ez & 3u)]);
         }
         // 2048 staged texels = 32 bricks of this half region, 64 per store
         // instruction (lane = Morton slot: coalesced).  Texel i = tid + 256 k
@@ -1308,21 +1414,28 @@ __device__ __forceinline__ void k_build_fused_body(GridDesc g, VolumeDesc vd,
             const uint32_t sm = tid & 63u, w4 = tid >> 6;  // w4 = the staged brick's x (0..3)
             uint32_t tx, ty, tz;
             demorton6(sm, &tx, &ty, &tz);
-            const uint32_t nsm = morton6(tx < 3u ? tx + 1u : 0u, ty, tz);
-            // +x neighbour: same brick, next brick in x, the voxel itself (the
-            // volume's last column), or none here (the next region writes it)
-            const int nbk = tx < 3u ? 0 : (w4 < 3u ? 1 : (last_x ? 2 : 3));
+            // +x neighbour: the same staged brick, the next one in x, the voxel
+            // itself (the volume's last column), or none here (the region's
+            // last column: the next region writes that half)
+            const bool edge = tx == 3u && w4 == 3u;
+            const bool half = edge && !last_x;
+            const uint32_t nb = edge ? sm : morton6(tx < 3u ? tx + 1u : 0u, ty, tz) + (tx == 3u ? 64u : 0u);
             const uint64_t b0 = ((uint64_t)(4u * rz + 2u * rep) * nb0 + 4u * ry) * nb0 + 4u * rx + w4;
-            const uint64_t zstep = (uint64_t)nb0 * nb0;
+            // slot offsets of the next brick row (y) and layer (z): uniform
+            const uint64_t rowS = (uint64_t)nb0 * 64u, zS = (uint64_t)nb0 * nb0 * 64u;
+            const uint64_t slot0 = vd.lv[0].base + b0 * 64u + sm;
+            typename ST::T own[8], nbv[8];
 #pragma unroll
             for (uint32_t k = 0; k < 8u; ++k) {
-                const uint32_t sbl = w4 + 4u * k;
-                const uint64_t idx = vd.lv[0].base + (b0 + (k >> 2) * zstep + (k & 3u) * nb0) * 64u + sm;
-                const float2 own = stage[tid + 256u * k];
-                if (nbk == 0) store_pair<FMT>(levels, idx, own, stage[(sbl << 6) | nsm]);
-                else if (nbk == 1) store_pair<FMT>(levels, idx, own, stage[((sbl + 1u) << 6) | nsm]);
-                else if (nbk == 2) store_pair<FMT>(levels, idx, own, own);
-                else store_half<FMT>(levels, idx, 0, own);
+                own[k] = stage[tid + 256u * k];
+                nbv[k] = stage[((w4 + 4u * k) << 6) + nb];
+            }
+            if (half) {
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; ++k) ST::half(levels, slot0 + (k & 3u) * rowS + (k >> 2) * zS, 0, own[k]);
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; ++k) ST::pair(levels, slot0 + (k & 3u) * rowS + (k >> 2) * zS, own[k], nbv[k]);
             }
         }
         __syncthreads();  // the next half reuses stage
