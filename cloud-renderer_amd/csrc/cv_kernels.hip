@@ -2282,9 +2282,22 @@ __device__ __forceinline__ float trilinear_r(const void* __restrict__ vol, const
 // Grey sampling split into fetch and filter, so a two-level sample issues
 // the loads of both footprints before waiting for any of them.
 struct GreyTaps {
-    float2 t[8];  // (r, a) at x0y0z0, x1y0z0, x0y1z0, x1y1z0, x0y0z1, x1y0z1, x0y1z1, x1y1z1
+    float2 t[8];  // RGBA32F: (r, a) at x0y0z0, x1y0z0, x0y1z0, x1y1z0, x0y0z1, x1y0z1, x0y1z1, x1y1z1
+    uint2 h[4];   // RGBA16F: the four slots (x0y0z0, x0y1z0, x0y0z1, x0y1z1) as loaded
     float ax, ay, az;
 };
+// (float)b - (float)a of the low / high binary16 halves of two words, rounded
+// once: 1 * b + (-a) on v_fma_mix_f32 (the conversions are exact)
+__device__ __forceinline__ float hdiff_lo(uint32_t b, uint32_t a) {
+    float r;
+    asm("v_fma_mix_f32 %0, 1.0, %1, -%2 op_sel_hi:[0,1,1]" : "=v"(r) : "v"(b), "v"(a));
+    return r;
+}
+__device__ __forceinline__ float hdiff_hi(uint32_t b, uint32_t a) {
+    float r;
+    asm("v_fma_mix_f32 %0, 1.0, %1, -%2 op_sel:[0,1,1] op_sel_hi:[0,1,1]" : "=v"(r) : "v"(b), "v"(a));
+    return r;
+}
 // (sx, sy, sz) in [0, 1]^3: callers stop a cone before it leaves the volume
 template <int FMT>
 __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const LevelDesc& L, float sx, float sy,
@@ -2312,12 +2325,7 @@ __device__ __forceinline__ void grey_fetch(const void* __restrict__ vol, const L
         T.t[6] = make_float2(q3.x, q3.y); T.t[7] = make_float2(q3.z, q3.w);
     } else {
         const uint2* v = (const uint2*)vol + L.base;
-        const uint2 q0 = v[X0 + Y0 + Z0], q1 = v[X0 + Y1 + Z0], q2 = v[X0 + Y0 + Z1], q3 = v[X0 + Y1 + Z1];
-        auto h2 = [](uint32_t u) { return make_float2(h2f(u & 0xffffu), h2f(u >> 16)); };
-        T.t[0] = h2(q0.x); T.t[1] = h2(q0.y);
-        T.t[2] = h2(q1.x); T.t[3] = h2(q1.y);
-        T.t[4] = h2(q2.x); T.t[5] = h2(q2.y);
-        T.t[6] = h2(q3.x); T.t[7] = h2(q3.y);
+        T.h[0] = v[X0 + Y0 + Z0]; T.h[1] = v[X0 + Y1 + Z0]; T.h[2] = v[X0 + Y0 + Z1]; T.h[3] = v[X0 + Y1 + Z1];
     }
 }
 // lerps along x, then y, then z (GL's trilinear order)
@@ -2336,6 +2344,27 @@ __device__ __forceinline__ float2 grey_filter(const GreyTaps& T) {
         o.y = lerpF(c0, c1, T.az);
     }
     return o;
+}
+// RGBA16F: the x lerps straight from the halves, lerp(a, b, w) =
+// fma(w, b - a, a) with b - a and the fma on v_fma_mix_f32 (the same
+// roundings as on converted floats); then y and z as grey_filter
+__device__ __forceinline__ float2 grey_filter_h(const GreyTaps& T) {
+    float cr[4], ca[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t a = T.h[k].x, b = T.h[k].y;  // (r, a) at x0 and at x1
+        cr[k] = fmaf(T.ax, hdiff_lo(b, a), h2f(a & 0xffffu));
+        ca[k] = fmaf(T.ax, hdiff_hi(b, a), h2f(a >> 16));
+    }
+    float2 o;
+    o.x = lerpF(lerpF(cr[0], cr[1], T.ay), lerpF(cr[2], cr[3], T.ay), T.az);
+    o.y = lerpF(lerpF(ca[0], ca[1], T.ay), lerpF(ca[2], ca[3], T.ay), T.az);
+    return o;
+}
+template <int FMT>
+__device__ __forceinline__ float2 grey_filter_f(const GreyTaps& T) {
+    if constexpr (FMT == 2) return grey_filter_h(T);
+    else return grey_filter(T);
 }
 
 // Table-addressed grey fetch (the cone kernel with per-level LDS tables):
@@ -2365,13 +2394,8 @@ __device__ __forceinline__ void grey_fetch_tab(const char* __restrict__ lvl, con
         T.t[4] = make_float2(q2.x, q2.y); T.t[5] = make_float2(q2.z, q2.w);
         T.t[6] = make_float2(q3.x, q3.y); T.t[7] = make_float2(q3.z, q3.w);
     } else {
-        const uint2 q0 = *(const uint2*)(lvl + o0), q1 = *(const uint2*)(lvl + o1);
-        const uint2 q2 = *(const uint2*)(lvl + o2), q3 = *(const uint2*)(lvl + o3);
-        auto h2 = [](uint32_t u) { return make_float2(h2f(u & 0xffffu), h2f(u >> 16)); };
-        T.t[0] = h2(q0.x); T.t[1] = h2(q0.y);
-        T.t[2] = h2(q1.x); T.t[3] = h2(q1.y);
-        T.t[4] = h2(q2.x); T.t[5] = h2(q2.y);
-        T.t[6] = h2(q3.x); T.t[7] = h2(q3.y);
+        T.h[0] = *(const uint2*)(lvl + o0); T.h[1] = *(const uint2*)(lvl + o1);
+        T.h[2] = *(const uint2*)(lvl + o2); T.h[3] = *(const uint2*)(lvl + o3);
     }
 }
 
@@ -2416,9 +2440,9 @@ __device__ __forceinline__ float4 sample_levels(const void* __restrict__ vol, co
         GreyTaps A, B;
         grey_fetch<FMT>(vol, LA, sx, sy, sz, A);
         if (tw) grey_fetch<FMT>(vol, LB, sx, sy, sz, B);  // in flight with A
-        float2 a = grey_filter(A);
+        float2 a = grey_filter_f<FMT>(A);
         if (tw) {
-            float2 b = grey_filter(B);
+            float2 b = grey_filter_f<FMT>(B);
             a.x = lerpF(a.x, b.x, f);
             a.y = lerpF(a.y, b.y, f);
         }
@@ -2496,9 +2520,9 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
             GreyTaps A, B;
             grey_fetch_tab<FMT>(TA.lvl, TA.tx, TA.ty, TA.tz, LA.fn, sx, sy, sz, A);
             if (two) grey_fetch_tab<FMT>(TB.lvl, TB.tx, TB.ty, TB.tz, LB.fn, sx, sy, sz, B);
-            float2 a = grey_filter(A);
+            float2 a = grey_filter_f<FMT>(A);
             if (two) {
-                float2 b = grey_filter(B);
+                float2 b = grey_filter_f<FMT>(B);
                 a.x = lerpF(a.x, b.x, fr);
                 a.y = lerpF(a.y, b.y, fr);
             }
@@ -2565,7 +2589,7 @@ __device__ __forceinline__ void farfield_features(const VolumeDesc& vd, const vo
         if (sx >= 0.0f && sx <= 1.0f && sy >= 0.0f && sy <= 1.0f && sz >= 0.0f && sz <= 1.0f) {
             GreyTaps T;
             grey_fetch<FMT>(vol, vd.lv[cd.feat_lvl[j]], sx, sy, sz, T);
-            const float2 t = grey_filter(T);
+            const float2 t = grey_filter_f<FMT>(T);
             dens = t.y;
             light = t.x;
         }
