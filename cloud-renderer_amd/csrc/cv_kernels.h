@@ -93,6 +93,7 @@ struct ConeDesc {
     float alpha_stop;
     float kscale;          // sigma * log2(e) / N
     int cone_mode;
+    int count;             // march: count samples and two-level samples (counter runs only)
     int composite;
     int quantize;
     int compensated;

@@ -2515,8 +2515,11 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
         bool two;
         float4 sm;
         if constexpr (TAB) {
-            // sample_levels<FMT, 2> with table addressing
-            two = (__float_as_uint(fr) << 1) != 0u;
+            // sample_levels<FMT, 2> with table addressing; f != 0 tested on
+            // the scalar bits (fr is the same for every lane)
+            uint32_t fr2;
+            asm("s_lshl_b32 %0, %1, 1" : "=s"(fr2) : "s"(__builtin_amdgcn_readfirstlane(__float_as_uint(fr))));
+            two = fr2 != 0u;
             GreyTaps A, B;
             grey_fetch_tab<FMT>(TA.lvl, TA.tx, TA.ty, TA.tz, LA.fn, sx, sy, sz, A);
             if (two) grey_fetch_tab<FMT>(TB.lvl, TB.tx, TB.ty, TB.tz, LB.fn, sx, sy, sz, B);
@@ -2530,7 +2533,13 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
         } else {
             sm = sample_levels<FMT, 2>(vol, LA, LB, fr, sx, sy, sz, &two);
         }
-        ntwo += two ? 1u : 0u;
+        if (cd.count) {
+            // counter runs only; the empty asm keeps this a scalar branch
+            // (no per-step selects in the timed kernels)
+            asm volatile("");
+            ++nsamp;
+            ntwo += two ? 1u : 0u;
+        }
         float al;
         if (cd.cone_mode == 1) {
             al = sm.w;
@@ -2548,7 +2557,6 @@ __device__ __forceinline__ int march_ftb_t(const VolumeDesc& vd, const void* __r
             st.c = fmaf(wgt, sm.x, st.c);
             st.a = st.a + wgt;
         }
-        ++nsamp;
         if (st.a >= cd.alpha_stop) return 2;
     }
     return 0;

@@ -1107,6 +1107,7 @@ static void fill_cone_desc(cr_volume v, const cr_light* light, const cr_cone_par
     cd.alpha_stop = p->alpha_stop;
     cd.kscale = (p->sigma * (float)1.4426950408889634) / (float)v->N;
     cd.cone_mode = p->cone_mode;
+    cd.count = 1;
     cd.composite = p->composite;
     cd.quantize = (p->composite == CR_COMPOSITE_GL_BLEND && p->quantize_rgba8) ? 1 : 0;
     cd.compensated = p->compensated ? 1 : 0;
@@ -1380,6 +1381,7 @@ static int cone_trace_impl(cr_volume v, const cr_light* light, const cr_camera* 
     cl.vd = v->vd;
     cl.cv = plan.cv;
     cl.cd = plan.cd;
+    cl.cd.count = counters ? 1 : 0;  // the march counts are read only from counter runs
     cl.levels = v->d_levels;
     cl.sph_world = v->d_sph_world;
     cl.sph_view = v->d_sph_view;
