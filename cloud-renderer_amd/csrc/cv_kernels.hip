@@ -173,7 +173,10 @@ __device__ __forceinline__ uint32_t fill_bit_index(uint32_t ix, uint32_t iy, uin
 // (lin6(a, b, c) + 4 X + 16 Y) mod 64: a bijection of the brick's voxels,
 // the canonical word rotated left by (4 X + 16 Y) mod 64 (rot_to_lin6 undoes it).
 __device__ __forceinline__ uint32_t rot_bit_index(uint32_t ix, uint32_t iy, uint32_t iz) {
-    return ix + (iy << 2) + (iz << 4);
+    uint32_t t, r;  // two shift-adds (the compiler otherwise emits two shifts and an add3)
+    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(t) : "v"(iy), "v"(ix));
+    asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(r) : "v"(iz), "v"(t));
+    return r;
 }
 __device__ __forceinline__ unsigned long long rot_to_lin6(unsigned long long w, uint32_t X, uint32_t Y) {
     const uint32_t r = (4u * X + 16u * Y) & 63u;
