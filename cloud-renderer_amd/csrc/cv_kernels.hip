@@ -1999,18 +1999,18 @@ __device__ __forceinline__ void wave_bitonic(unsigned long long (&v)[E], int lan
     }
 }
 
-// Sort of npad = NT * E keys (the first S real, the rest ~0) in one
-// workgroup of NT threads: each wave bitonic-sorts its 64 * E keys in
-// registers, then log2(NT / 64) merge rounds pair the sorted runs through LDS
-// (merge path: a binary search gives each thread its E outputs).  The keys
-// are (~distance bits << 32 | index), distinct for distinct billboards, so
-// the ascending order is the draw order up to runs of equal distances.
-template <int E, int NT = 1024>
+// Sort of npad = 1024 * E keys (the first S real, the rest ~0) in one
+// workgroup of 1024: each wave bitonic-sorts its 64 * E keys in registers,
+// then four merge rounds pair the 16 sorted runs through LDS (merge path:
+// a binary search gives each thread its E outputs).  The keys are
+// (~distance bits << 32 | index), distinct for distinct billboards, so the
+// ascending order is the draw order up to runs of equal distances.
+template <int E>
 __device__ __forceinline__ void k_sort_lds_body(const unsigned long long* __restrict__ keys_in, int S,
                                                    int32_t* __restrict__ order, int* __restrict__ tie_flag) {
     extern __shared__ unsigned long long sk[];
     __shared__ int tie;
-    constexpr int npad = NT * E;
+    constexpr int npad = 1024 * E;
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     if (tid == 0) tie = 0;
     {
@@ -2091,10 +2091,10 @@ __device__ __forceinline__ void k_sort_lds_body(const unsigned long long* __rest
     replay_ties_block(S, s_idx, s_D, placed, order);
     if (tid == 0) *tie_flag = 1;
 }
-template <int E, int NT>
-__global__ void __launch_bounds__(NT) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S,
-                                                 int32_t* __restrict__ order, int* __restrict__ tie_flag) {
-    k_sort_lds_body<E, NT>(keys_in, S, order, tie_flag);
+template <int E>
+__global__ void __launch_bounds__(1024) k_sort_lds(const unsigned long long* __restrict__ keys_in, int S,
+                                                   int32_t* __restrict__ order, int* __restrict__ tie_flag) {
+    k_sort_lds_body<E>(keys_in, S, order, tie_flag);
 }
 
 // global-memory bitonic stage for more billboards than one LDS holds
@@ -3420,24 +3420,21 @@ size_t sort_scratch_words(int S) {
     return 5 * npad;
 }
 
-// NT threads: up to 4096 keys sort in one 256-thread workgroup (16 keys per
-// thread), which finds room on a CU much sooner than 1024 threads when the
-// device is busy with the raster and the cone trace
-template <int E, int NT>
+template <int E>
 static int launch_sort_lds(const unsigned long long* keys, int S, int32_t* order, int* tie_flag, hipStream_t s) {
-    static_assert(NT * E <= kLdsSortMax, "one workgroup's LDS");
-    const int bytes = NT * E * (int)sizeof(unsigned long long);
+    static_assert(1024 * E <= kLdsSortMax, "one workgroup's LDS");
+    const int bytes = 1024 * E * (int)sizeof(unsigned long long);
     if (bytes > 64 * 1024) {
         static bool attr_set = false;  // raise the dynamic LDS limit once per instantiation
         if (!attr_set) {
-            if (hipFuncSetAttribute((const void*)k_sort_lds<E, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    bytes) != hipSuccess)
+            if (hipFuncSetAttribute((const void*)k_sort_lds<E>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+                hipSuccess)
                 return -1;
             attr_set = true;
         }
     }
-    hipLaunchKernelGGL((k_sort_lds<E, NT>), dim3(1), dim3(NT), (size_t)bytes, s, keys, S, order, tie_flag);
-    return NT * E;
+    hipLaunchKernelGGL(k_sort_lds<E>, dim3(1), dim3(1024), (size_t)bytes, s, keys, S, order, tie_flag);
+    return 1024 * E;
 }
 
 void launch_sort_keys(const GridDesc& g, const float4* boards, int S, const float point[3], unsigned long long* keys,
@@ -3459,11 +3456,11 @@ int launch_sort(unsigned long long* keys, int S, int32_t* order, int* tie_flag, 
     if (npad <= kLdsSortMax) {
         int r;
         switch (npad < 1024 ? 1024 : npad) {
-            case 1024: r = launch_sort_lds<4, 256>(keys, S, order, tie_flag, s); break;
-            case 2048: r = launch_sort_lds<8, 256>(keys, S, order, tie_flag, s); break;
-            case 4096: r = launch_sort_lds<16, 256>(keys, S, order, tie_flag, s); break;
-            case 8192: r = launch_sort_lds<16, 512>(keys, S, order, tie_flag, s); break;
-            default: r = launch_sort_lds<16, 1024>(keys, S, order, tie_flag, s); break;
+            case 1024: r = launch_sort_lds<1>(keys, S, order, tie_flag, s); break;
+            case 2048: r = launch_sort_lds<2>(keys, S, order, tie_flag, s); break;
+            case 4096: r = launch_sort_lds<4>(keys, S, order, tie_flag, s); break;
+            case 8192: r = launch_sort_lds<8>(keys, S, order, tie_flag, s); break;
+            default: r = launch_sort_lds<16>(keys, S, order, tie_flag, s); break;
         }
         if (r > 0) return r;  // else (LDS limit not raised): the global-memory sort below
     }
