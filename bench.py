@@ -547,7 +547,7 @@ def main() -> int:
                                   "counters_per_frame": {k: int(v) for k, v in rn["counters"].items()}}
     if not args.no_fp16:
         f16 = cv.scenes.fp16_512(args.seed)
-        k = max(4, args.steps // 4)
+        k = max(16, args.steps // 2)  # enough pipelined frames that the first and last do not weigh in
         log(f"[bench] fp16_512: N={f16.N} S={f16.S} {f16.W}x{f16.H} steps={k}")
         rf = run_gpu(cv, f16, k, 2, ctx, "fp16_512")
         out["fp16_512"] = {"desc": "512^3 RGBA16F volume with 4-octave value-noise density, 16384 clumped spheres, "
