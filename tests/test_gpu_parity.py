@@ -76,6 +76,12 @@ def _scenes(cv):
         # spheres whose chord slabs overflow the raster's LDS bitmap at 128^3:
         # pass 1 cuts their tiles into depth slices (light_slices K = 2..4)
         "sliced128": _sliced(sc, base),
+        # light along a volume axis: chords advance in one coordinate only
+        # (pass 1's brick words change along one axis, the slab boxes are thin)
+        "small32_sun_x": replace(base, sun_dir=_unit3((1.0, 0.0, 0.0))),
+        "small32_sun_negz": replace(base, sun_dir=_unit3((0.0, 0.0, -1.0))),
+        # the reference's lit-shell semantics stored as RGBA16F
+        "small32_ref_fp16": replace(sc.reference_mode(base), fmt=2),
         "ties": tie_scene(sc),
         "ties_ref": sc.reference_mode(tie_scene(sc)),
         # reference semantics in a range that carries information: with
@@ -163,7 +169,8 @@ def tie_scene(sc):
 CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_rgba32f", "small32_ref_float_blend",
          "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_ref_noise_march",
          "small32_ftb_noise_march", "small32_ref_sun", "small32_blend_sun_march", "small32_comp_blend",
-         "small32_rect", "npot24", "npot96_plain", "ties", "ties_ref", "small32_ref_ds16", "small32_ref_ds16_q8",
+         "small32_rect", "npot24", "npot96_plain", "sliced128", "small32_sun_x", "small32_sun_negz",
+         "small32_ref_fp16", "ties", "ties_ref", "small32_ref_ds16", "small32_ref_ds16_q8",
          "small32_ref_offset_fluffy", "refdef", "refdef_float", "refdef_fluffy", "refdef_nonoise_ds8",
          "refdef_matrices"]
 # reference-semantics cases whose pixels must stay below saturation
