@@ -82,6 +82,12 @@ def _scenes(cv):
         "small32_sun_negz": replace(base, sun_dir=_unit3((0.0, 0.0, -1.0))),
         # the reference's lit-shell semantics stored as RGBA16F
         "small32_ref_fp16": replace(sc.reference_mode(base), fmt=2),
+        # edge cases: no billboards, one, all outside the volume, an image
+        # and light map of a few pixels (tiles cut by every edge)
+        "empty": replace(base, xyz=np.zeros((0, 3), np.float32), scales=np.zeros(0, np.float32)),
+        "single": replace(base, xyz=base.xyz[:1].copy(), scales=base.scales[:1].copy()),
+        "outside": replace(base, xyz=(base.xyz + np.asarray([1.9, 0.0, 0.0], np.float32)).astype(np.float32)),
+        "tiny_img": replace(base, W=7, H=5, light_w=9, light_h=3),
         "ties": tie_scene(sc),
         "ties_ref": sc.reference_mode(tie_scene(sc)),
         # reference semantics in a range that carries information: with
@@ -170,7 +176,8 @@ CASES = ["small32", "small32_seed3", "small32_ftb", "small32_ref", "small32_ref_
          "small32_fp16_noise", "small32_fp32_noise_odd", "small32_fp16_noise_wide", "small32_ref_noise_march",
          "small32_ftb_noise_march", "small32_ref_sun", "small32_blend_sun_march", "small32_comp_blend",
          "small32_rect", "npot24", "npot96_plain", "sliced128", "small32_sun_x", "small32_sun_negz",
-         "small32_ref_fp16", "ties", "ties_ref", "small32_ref_ds16", "small32_ref_ds16_q8",
+         "small32_ref_fp16", "empty", "single", "outside", "tiny_img", "ties", "ties_ref", "small32_ref_ds16",
+         "small32_ref_ds16_q8",
          "small32_ref_offset_fluffy", "refdef", "refdef_float", "refdef_fluffy", "refdef_nonoise_ds8",
          "refdef_matrices"]
 # reference-semantics cases whose pixels must stay below saturation
